@@ -1,0 +1,188 @@
+#!/usr/bin/env python
+"""Benchmark: P1 kappa-Laplace assembly + Jacobi-PCG on one MI355X.
+
+Workload: 3D variable-coefficient Poisson -div(kappa grad u) = 1 on the unit
+cube, 128^3 Kuhn grid x 6 P1 tetrahedra (12,582,912 elements, 2,146,689
+nodes, nnz 31,802,497), kappa = exp(g) with g a seeded Gaussian random field
+(length 0.1, sigma 1, seed 0), zero Dirichlet.  A step is one assembly of the
+stiffness values from resident coordinates / kappa into the precomputed CSR
+pattern (FP64 geometry and accumulation, FP32 values).  The solve leg then
+condenses and runs Jacobi-PCG to relative residual 1e-8 (FP64 vectors).
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_info():
+    model, sockets = "unknown", set()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name") and model == "unknown":
+                model = line.split(":", 1)[1].strip()
+            if line.startswith("physical id"):
+                sockets.add(line.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    return model, max(1, len(sockets))
+
+
+def cpu_baseline(n_cpu: int, repeats: int = 3):
+    """Float64 oracle Map + Reduce (numpy, one thread) on an n_cpu^3 Kuhn grid."""
+    import numpy as np
+    from oracle import assemble as oa
+    from oracle import pattern as op
+    from tensormesh_amd.dataset import lognormal_kappa_cube
+    from tensormesh_amd.mesh import kuhn_cube_arrays
+    pts, conn, _ = kuhn_cube_arrays(n_cpu)
+    kappa = lognormal_kappa_cube(n_cpu, 0.1, 1.0, 0)
+    u, v, pe = op.pattern_and_slots(conn, len(pts))
+    times = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        ke = oa.element_matrices(pts, conn, "tetra", "kappa_laplace", kappa=kappa)
+        oa.reduce(pe, ke, len(u))
+        times.append(time.perf_counter() - t0)
+    t = sorted(times)[len(times) // 2]
+    model, sockets = cpu_info()
+    return {"value": round(len(conn) / t / 1e6, 4), "unit": "Melem/s", "cores": 1, "kind": "port",
+            "sample": f"float64 numpy oracle kappa-Laplace Map+Reduce on the {n_cpu}^3 Kuhn grid "
+                      f"({len(conn)} tets), median of {repeats}, pattern prebuilt",
+            "seconds_per_assembly": round(t, 4), "cpu": model, "sockets": sockets}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=40)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(see the sharding design in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import tensormesh_amd as tm
+    from tensormesh_amd.dataset import lognormal_kappa_cube
+    from tensormesh_amd.mesh import Mesh, kuhn_cube_arrays
+    from tensormesh_amd.sparse.ops import pcg_raw
+
+    dev = torch.device("cuda:0")
+    n = args.n
+    t0 = time.perf_counter()
+    pts, conn, bnd = kuhn_cube_arrays(n)
+    kappa_np = lognormal_kappa_cube(n, 0.1, 1.0, 0)
+    mesh = Mesh(torch.from_numpy(pts), {"tetra": torch.from_numpy(conn)}, {"is_boundary": torch.from_numpy(bnd)})
+    log(f"mesh {n}^3: {len(conn)} tets, {len(pts)} nodes ({time.perf_counter() - t0:.1f}s)")
+    t0 = time.perf_counter()
+    asm = tm.KappaLaplaceElementAssembler.from_mesh(mesh)
+    t_pattern = time.perf_counter() - t0
+    asm = asm.to(dev)
+    t0 = time.perf_counter()
+    plan = asm.topology.device_plan(dev)
+    t_plan = time.perf_counter() - t0
+    log(f"pattern {t_pattern:.1f}s (nnz {asm.nnz}), plan {t_plan:.1f}s "
+        f"({plan.n_blocks} blocks, {int(plan.info[2])} records, lds {int(plan.info[4])} B)")
+    kappa = torch.tensor(kappa_np, dtype=torch.float32, device=dev)
+    pd = {"kappa": kappa}
+    E, N, nnz, nb = len(conn), len(pts), asm.nnz, 4
+
+    for _ in range(args.warmup):
+        K = asm(point_data=pd)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream(dev)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        K = asm(point_data=pd)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    t_step = ev0.elapsed_time(ev1) / 1e3 / args.steps
+    log(f"assembly: {t_step * 1e6:.1f} us/step (events), {wall / args.steps * 1e6:.1f} us/step (wall)")
+    b_asm = 4 * nb * E + 24 * N + 4 * N + 4 * (N + 1) + 4 * nnz + 4 * nnz
+    achieved = b_asm / t_step / 1e9
+
+    traffic = None
+    tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "asm_traffic.json")
+    if os.path.exists(tpath) and n == 128:
+        try:
+            traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+
+    out = {
+        "metric": "assembly Melem/s + HBM GB/s; time-to-1e-8 residual, learned-FPCG vs Jacobi-PCG",
+        "value": round(E / t_step / 1e6, 2), "unit": "Melem/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(t_step * 1e3, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"3D variable-coefficient Poisson, {n}^3 Kuhn grid x 6 P1 tets, "
+                               f"kappa=exp(GRF l=0.1 sigma=1 seed 0), kappa-Laplace assembly into CSR",
+                   "elements": E, "nodes": N, "nnz": nnz, "values_stored": "f32",
+                   "geometry_and_accumulation": "f64", "setup_pattern_s": round(t_pattern, 2),
+                   "setup_plan_s": round(t_plan, 2)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(achieved / 8000.0, 4), "traffic": traffic,
+                     "algorithmic_bytes_per_launch": b_asm},
+    }
+
+    if not args.no_solve:
+        b = tm.const_node_assembler(1.0).from_mesh(mesh).to(dev)()
+        cond = tm.Condenser(mesh.boundary_mask.to(dev))
+        Ki, fi = cond(K, b)
+        csr = Ki.csr()
+        ni, nnzi = Ki.shape[0], Ki.nnz
+        # SpMV bandwidth
+        x = torch.rand(ni, dtype=torch.float64, device=dev)
+        from tensormesh_amd.sparse.ops import _spmv_raw
+        for _ in range(3):
+            _spmv_raw(csr, Ki.values, x)
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        for _ in range(20):
+            _spmv_raw(csr, Ki.values, x)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        t_spmv = ev0.elapsed_time(ev1) / 1e3 / 20
+        b_spmv = 4 * (ni + 1) + 8 * nnzi + 16 * ni
+        pcg_raw(csr, Ki.values, fi, rtol=1e-8, maxiter=50)           # warm-up (kernels, allocator)
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        xs, it, rel, ok = pcg_raw(csr, Ki.values, fi, rtol=1e-8, maxiter=20000)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        t_solve = ev0.elapsed_time(ev1) / 1e3
+        true_rel = ((fi.double() - Ki @ xs).norm() / fi.double().norm()).item()
+        b_iter = 8 * nnzi + 4 * (ni + 1) + 104 * ni
+        out["solve"] = {"method": "jacobi-pcg", "n_inner": ni, "nnz_inner": nnzi, "iterations": it,
+                        "converged": ok, "relres_recursive": rel, "relres_true_fp32_matrix": true_rel,
+                        "time_to_1e-8_s": round(t_solve, 4), "ms_per_iter": round(t_solve / max(it, 1) * 1e3, 4),
+                        "pcg_iter_GBps": round(b_iter / (t_solve / max(it, 1)) / 1e9, 1),
+                        "spmv_us": round(t_spmv * 1e6, 2), "spmv_GBps": round(b_spmv / t_spmv / 1e9, 1)}
+        log(f"solve: {it} its, {t_solve:.3f}s, true relres {true_rel:.2e}, spmv {t_spmv * 1e6:.1f} us")
+
+    if not args.no_cpu:
+        log(f"cpu baseline on {args.cpu_n}^3 ...")
+        out["cpu_baseline"] = cpu_baseline(args.cpu_n)
+        log(f"cpu baseline {out['cpu_baseline']['value']} Melem/s")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,157 @@
+/*
+ * tensormesh_amd C ABI: P1 finite-element assembly, condensation and Krylov
+ * solves on one MI355X (gfx950).
+ *
+ * Conventions
+ *  - Every function returns an int status: TM_OK (0) or a negative code;
+ *    tm_last_error() gives a message for the calling thread.  Nothing throws
+ *    across the ABI.
+ *  - "host" functions take host pointers and run on the CPU (setup work).
+ *    "device" functions take device pointers, enqueue work on the given HIP
+ *    stream (a hipStream_t passed as void*, NULL = default stream) and return
+ *    without synchronising unless stated otherwise.
+ *  - The caller owns every buffer; the library never frees caller memory.
+ *    Host-side plans are opaque handles with *_build / *_destroy.
+ *  - Index types: connectivity int64 (the reference keeps cells as int64,
+ *    tensormesh/mesh/mesh.py:126), CSR row_ptr/col_idx int32, element->slot
+ *    map int64 (the reference's ReduceProjector indices, projector.py:24-93).
+ */
+#ifndef TENSORMESH_AMD_H
+#define TENSORMESH_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TM_OK 0
+#define TM_ERR_ARG (-1)
+#define TM_ERR_HIP (-2)
+#define TM_ERR_LIMIT (-3)
+#define TM_ERR_ALLOC (-4)
+#define TM_ERR_NOTCONV (-5)
+
+/* Forms recognised by the assembly kernels. */
+#define TM_FORM_LAPLACE 0       /* scale * grad u . grad v                      */
+#define TM_FORM_KAPPA_LAPLACE 1 /* kappa(x) grad u . grad v, kappa nodal        */
+#define TM_FORM_MASS 2          /* scale * u v                                  */
+#define TM_LOAD_CONST 0         /* c * v                                        */
+#define TM_LOAD_NODAL 1         /* f(x) v, f nodal (interpolated at quadrature)  */
+
+/* dtype codes */
+#define TM_F32 0
+#define TM_F64 1
+
+const char* tm_last_error(void);
+int tm_version(void);
+
+/* ---------------------------------------------------------------- host --- */
+
+/* Canonical CSR pattern of all (conn[e,i], conn[e,j]) pairs: rows ascending,
+ * columns ascending and unique.  Replaces the SciPy dedup of
+ * tensormesh/assemble/element_assembler.py:838-857.  Two calls: the first
+ * fills row_ptr[N+1] (nnz = row_ptr[N]), the second col_idx[nnz]. */
+int tm_pattern_rowptr(const int64_t* conn, int64_t n_elem, int nb, int64_t n_points,
+                      int32_t* row_ptr);
+int tm_pattern_colidx(const int64_t* conn, int64_t n_elem, int nb, int64_t n_points,
+                      const int32_t* row_ptr, int32_t* col_idx);
+
+/* Element->edge map P_E[e*nb*nb + i*nb + j] = slot of (conn[e,i], conn[e,j]).
+ * Replaces element_assembler.py:858-866 (eids_csr[elem_u, elem_v]). */
+int tm_element_slots(const int64_t* conn, int64_t n_elem, int nb, const int32_t* row_ptr,
+                     const int32_t* col_idx, int64_t* slots);
+
+/* Block plan for the owner-computes assembly kernels (no reference
+ * counterpart: it is the precomputed form of the projector P_E,
+ * element_assembler.py:863-884).  Nodes are split by recursive coordinate
+ * bisection into blocks of at most max_rows owned rows such that a block's
+ * LDS footprint fits lds_budget bytes; entry_bytes is the accumulator size
+ * per CSR entry (8 scalar, 72 for 3x3 blocks), node_bytes the per-halo-node
+ * cache size, own_bytes the per-owned-row scratch. */
+typedef struct tm_plan tm_plan;
+int tm_plan_build(const double* points, int dim, const int64_t* conn, int64_t n_elem, int nb,
+                  int64_t n_points, const int32_t* row_ptr, const int32_t* col_idx,
+                  int max_rows, int lds_budget, int entry_bytes, int node_bytes, int own_bytes,
+                  int n_threads, tm_plan** out);
+/* info[0] n_blocks, [1] n_halo_total, [2] n_records, [3] n_runs,
+ * [4] max LDS bytes of one block, [5] max halo, [6] max records, [7] max acc */
+int tm_plan_info(const tm_plan* plan, int64_t* info);
+/* blk[n_blocks*8] = {halo_off, n_halo, n_own, rec_off, n_rec, run_off, n_run, n_acc};
+ * halo[n_halo_total] global node ids (owned rows first, each part ascending);
+ * meta[n_halo_total] = acc_offset | diag_rank << 16 (owned entries only);
+ * recs[n_records*4] packed element records; rec_elem[n_records] element ids;
+ * runs[n_runs*4] = {value_start, acc_start, length, 0}. */
+int tm_plan_export(const tm_plan* plan, int32_t* blk, int32_t* halo, uint32_t* meta,
+                   uint32_t* recs, int32_t* rec_elem, int32_t* runs);
+void tm_plan_destroy(tm_plan* plan);
+
+/* -------------------------------------------------------------- device --- */
+
+/* Device view of an exported plan. */
+typedef struct {
+  const int32_t* blk;
+  const int32_t* halo;
+  const uint32_t* meta;
+  const uint32_t* recs;
+  const int32_t* rec_elem;
+  const int32_t* runs;
+  int32_t n_blocks;
+  int32_t lds_bytes;
+} tm_plan_dev;
+
+/* Reference-element moments of the quadrature rule: wsum = sum_q w_q,
+ * W[c] = sum_q w_q phi_c(q), M[a][b] = sum_q w_q phi_a(q) phi_b(q). */
+typedef struct {
+  double wsum;
+  double W[4];
+  double M[16];
+} tm_quad;
+
+/* Stiffness / mass values in the canonical CSR order (ElementAssembler.__call__,
+ * element_assembler.py:243-466, for the forms above).  coords [N, dim] f64;
+ * coef [N] nodal (f32 or f64, NULL when unused); values [nnz] f32 or f64.
+ * Every CSR entry is written exactly once. */
+int tm_asm_p1_matrix(int form, int nb, int dim, const tm_plan_dev* plan, const double* coords,
+                     const void* coef, int coef_dtype, double scale, const tm_quad* quad,
+                     void* values, int value_dtype, void* stream);
+
+/* Load vector b[N] (f64) (NodeAssembler.__call__, node_assembler.py:242-409):
+ * TM_LOAD_CONST: c * int phi_a;  TM_LOAD_NODAL: int (sum_c f_c phi_c) phi_a. */
+int tm_asm_p1_load(int form, int nb, int dim, const tm_plan_dev* plan, const double* coords,
+                   const void* f, int f_dtype, double c, const tm_quad* quad, double* b,
+                   void* stream);
+
+/* Adjoint of the kappa-Laplace assembly: dkappa[N] += d(sum_s g_s K_s)/dkappa
+ * for g[nnz] (f32 or f64).  dkappa must be zeroed by the caller.  Reference
+ * gradient: autograd through element_assembler.py:402-466. */
+int tm_asm_p1_kappa_adjoint(int nb, int dim, const tm_plan_dev* plan, const double* coords,
+                            const void* g, int g_dtype, const tm_quad* quad, double* dkappa,
+                            void* stream);
+
+/* y = A x for CSR A (values f32 or f64), x and y f64.  Reference: SparseMatrix
+ * @ (matrix.py:172-176), SparseMVTorch (mm/torch_mm.py:32-48). */
+int tm_spmv(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const void* values,
+            int value_dtype, const double* x, double* y, void* stream);
+
+/* dst[i] = src[idx[i]] (4- or 8-byte elements): Condenser value gathers
+ * (condense.py:294-300). */
+int tm_gather(int64_t n, const int32_t* idx, const void* src, void* dst, int elem_bytes,
+              void* stream);
+
+/* Jacobi-PCG for SPD CSR A: stops when ||b - A x||_2 <= max(rtol ||b||_2, atol)
+ * (recursive residual) or after maxiter iterations.  x holds the initial guess
+ * on entry.  work must hold tm_pcg_work_size(n) bytes of device memory.
+ * Synchronises the stream every check_every iterations to test convergence;
+ * *iters and *relres receive the final count and ||r||/||b||.  Returns TM_OK
+ * or TM_ERR_NOTCONV (x still holds the last iterate). */
+int64_t tm_pcg_work_size(int64_t n);
+int tm_pcg_jacobi(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const void* values,
+                  int value_dtype, const double* b, double* x, void* work, double rtol,
+                  double atol, int64_t maxiter, int check_every, int64_t* iters, double* relres,
+                  void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

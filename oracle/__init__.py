@@ -1,0 +1,30 @@
+"""CPU oracle for the assembly + condensation + Krylov path (test infrastructure only).
+
+This package is a float64 CPU restatement of the reference algorithm and is
+used only as the checker: by ``tests/``, by ``__graft_entry__.smoke()`` and
+by the ``cpu_baseline`` leg of ``bench.py``.  The product package
+``tensormesh_amd`` never imports it.
+
+How each function is pinned (beyond restating the reference):
+
+* ``pattern.pattern_and_slots`` - SciPy canonical CSR exactly as
+  ``element_assembler.py:838-861``; pinned by brute-force pair sets on tiny
+  meshes, by the closed-form Kuhn-grid nnz count and by 15 entries per
+  interior row (tests/test_oracle.py).
+* ``assemble.*`` - einsum / inverse / det sequence of
+  ``element.py:958-966`` and ``transformation.py:1284-1290``, quadrature sum
+  ``element_assembler.py:196-221``, ``index_add_`` reduce
+  ``projector.py:95-130``; pinned by single-element known answers (unit
+  right triangle, Kuhn tetrahedron), ``K 1 = 0``, symmetry / PSD,
+  ``1^T M 1 = |Omega|``, ``b = M f`` and the O(h^2) convergence to the exact
+  solution of the 2D quickstart problem.
+* ``condense.*`` - ``condense.py:182-309, 388-458``; pinned by the 4x4
+  tridiagonal fixture of ``tests/operator/test_condenser.py:48-99`` and
+  dense block elimination.
+* ``solve.*`` - textbook Jacobi-PCG (Saad, Iterative Methods, Alg. 9.1) with
+  float64 reductions; pinned against ``scipy.sparse.linalg.spsolve`` and an
+  independently recomputed residual.
+* ``hyperelastic.*`` - neo-Hookean density of ``builtin.py:285-315`` with
+  closed-form P and dP/dF; pinned by central finite differences (h = 1e-6)
+  and by torch autograd of the energy, and psi(F = I) = 0.
+"""

@@ -1,0 +1,62 @@
+"""Float64 reference solvers for the condensed systems.
+
+* :func:`jacobi_pcg` - textbook preconditioned CG (Saad, *Iterative Methods
+  for Sparse Linear Systems*, 2nd ed., Alg. 9.1) with ``M = diag(A)``;
+  the diagonal follows ``torch_solve.py:24-59`` (missing diagonal -> 1).
+  Stops when ``||r_k||_2 <= rtol * ||b||_2``.
+* :func:`direct` - ``scipy.sparse.linalg.spsolve`` (SuperLU), the CPU
+  backend the reference's quickstart reports (``README.md:132``).
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.sparse
+import scipy.sparse.linalg
+
+
+def csr(values, row, col, n):
+    return scipy.sparse.csr_matrix((np.asarray(values, np.float64), (row, col)), shape=(n, n))
+
+
+def diagonal(A: scipy.sparse.csr_matrix):
+    d = np.ones(A.shape[0])
+    coo = A.tocoo()
+    m = coo.row == coo.col
+    d[coo.row[m]] = 0.0
+    np.add.at(d, coo.row[m], coo.data[m])
+    return d
+
+
+def jacobi_pcg(A, b, rtol=1e-8, maxiter=20000, x0=None):
+    """Return ``(x, iterations, relres_history)``."""
+    b = np.asarray(b, np.float64)
+    x = np.zeros_like(b) if x0 is None else np.array(x0, np.float64)
+    dinv = 1.0 / diagonal(A)
+    r = b - A @ x
+    z = dinv * r
+    p = z.copy()
+    rz = r @ z
+    bn = np.linalg.norm(b)
+    hist = [np.linalg.norm(r) / bn]
+    it = 0
+    while hist[-1] > rtol and it < maxiter:
+        q = A @ p
+        alpha = rz / (p @ q)
+        x += alpha * p
+        r -= alpha * q
+        z = dinv * r
+        rz_new = r @ z
+        p = z + (rz_new / rz) * p
+        rz = rz_new
+        it += 1
+        hist.append(np.linalg.norm(r) / bn)
+    return x, it, np.asarray(hist)
+
+
+def direct(A, b):
+    return scipy.sparse.linalg.spsolve(A.tocsc(), np.asarray(b, np.float64))
+
+
+def relres(A, x, b):
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(b - A @ np.asarray(x, np.float64)) / np.linalg.norm(b))

@@ -1,0 +1,132 @@
+"""ctypes binding of ``libtensormesh_amd.so`` (the C ABI declared in
+``include/tensormesh_amd.h``).
+
+torch is imported before the library is opened so that the library binds to
+the HIP runtime torch already loaded (same ``libamdhip64.so.7`` soname).  A
+missing library raises immediately: there is no alternative code path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+import torch
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libtensormesh_amd.so")
+
+TM_OK, TM_ERR_ARG, TM_ERR_HIP, TM_ERR_LIMIT, TM_ERR_ALLOC, TM_ERR_NOTCONV = 0, -1, -2, -3, -4, -5
+FORM_LAPLACE, FORM_KAPPA_LAPLACE, FORM_MASS = 0, 1, 2
+LOAD_CONST, LOAD_NODAL = 0, 1
+F32, F64 = 0, 1
+
+c_void_p = ctypes.c_void_p
+c_int = ctypes.c_int
+c_i64 = ctypes.c_int64
+c_double = ctypes.c_double
+
+
+class PlanDev(ctypes.Structure):
+    _fields_ = [("blk", c_void_p), ("halo", c_void_p), ("meta", c_void_p), ("recs", c_void_p),
+                ("rec_elem", c_void_p), ("runs", c_void_p), ("n_blocks", ctypes.c_int32),
+                ("lds_bytes", ctypes.c_int32)]
+
+
+class Quad(ctypes.Structure):
+    _fields_ = [("wsum", c_double), ("W", c_double * 4), ("M", c_double * 16)]
+
+
+class TMError(RuntimeError):
+    """Failure reported by the native library."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[tensormesh_amd error {code}] {msg}")
+        self.code = code
+
+
+class NotConverged(TMError):
+    pass
+
+
+_SIGS = {
+    "tm_last_error": (ctypes.c_char_p, []),
+    "tm_version": (c_int, []),
+    "tm_pattern_rowptr": (c_int, [c_void_p, c_i64, c_int, c_i64, c_void_p]),
+    "tm_pattern_colidx": (c_int, [c_void_p, c_i64, c_int, c_i64, c_void_p, c_void_p]),
+    "tm_element_slots": (c_int, [c_void_p, c_i64, c_int, c_void_p, c_void_p, c_void_p]),
+    "tm_plan_build": (c_int, [c_void_p, c_int, c_void_p, c_i64, c_int, c_i64, c_void_p, c_void_p,
+                              c_int, c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_void_p)]),
+    "tm_plan_info": (c_int, [c_void_p, c_void_p]),
+    "tm_plan_export": (c_int, [c_void_p] * 7),
+    "tm_plan_destroy": (None, [c_void_p]),
+    "tm_asm_p1_matrix": (c_int, [c_int, c_int, c_int, ctypes.POINTER(PlanDev), c_void_p, c_void_p, c_int,
+                                 c_double, ctypes.POINTER(Quad), c_void_p, c_int, c_void_p]),
+    "tm_asm_p1_load": (c_int, [c_int, c_int, c_int, ctypes.POINTER(PlanDev), c_void_p, c_void_p, c_int,
+                               c_double, ctypes.POINTER(Quad), c_void_p, c_void_p]),
+    "tm_asm_p1_kappa_adjoint": (c_int, [c_int, c_int, ctypes.POINTER(PlanDev), c_void_p, c_void_p, c_int,
+                                        ctypes.POINTER(Quad), c_void_p, c_void_p]),
+    "tm_spmv": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "tm_gather": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+    "tm_pcg_work_size": (c_i64, [c_i64]),
+    "tm_pcg_jacobi": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                              c_double, c_double, c_i64, c_int, ctypes.POINTER(c_i64),
+                              ctypes.POINTER(c_double), c_void_p]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded native library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with "
+                              f"`python -m tensormesh_amd.csrc.build`")
+        h = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = h
+    return _lib
+
+
+def declared_symbols():
+    return list(_SIGS)
+
+
+def check(status: int):
+    if status == TM_OK:
+        return
+    msg = lib().tm_last_error().decode(errors="replace")
+    if status == TM_ERR_NOTCONV:
+        raise NotConverged(status, msg)
+    if status == TM_ERR_ARG:
+        raise ValueError(msg)
+    raise TMError(status, msg)
+
+
+def ptr(t) -> c_void_p:
+    if t is None:
+        return c_void_p(None)
+    if isinstance(t, np.ndarray):
+        return c_void_p(t.ctypes.data)
+    return c_void_p(t.data_ptr())
+
+
+def stream_of(device: torch.device) -> c_void_p:
+    return c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def dtype_code(t: torch.Tensor) -> int:
+    if t.dtype == torch.float32:
+        return F32
+    if t.dtype == torch.float64:
+        return F64
+    raise TypeError(f"unsupported dtype {t.dtype} (float32 or float64 expected)")
+
+
+def require_gpu(t: torch.Tensor, what: str):
+    if not t.is_cuda:
+        raise RuntimeError(f"{what} must be on a GPU device for the HIP path, got {t.device}")

@@ -1,0 +1,196 @@
+"""Recognition of weak forms that the HIP kernels implement, and the generic path.
+
+The reference dispatches ``forward`` arguments by name
+(``tensormesh/assemble/element_assembler.py:332-365``,
+``node_assembler.py:341-366``) and evaluates it under nested ``vmap``.  A
+P1 kernel can only implement a fixed integrand, so ``forward`` is probed: it
+is evaluated on random per-quadrature-point arguments and compared with the
+candidate integrands below.  A match (constant ratio, relative spread
+<= 1e-12 over three draws) selects the kernel with that ratio as scale:
+
+======================================  =====================  ==============
+argument names (besides scalar_data)     integrand               kernel form
+======================================  =====================  ==============
+``gradu, gradv``                         s * gradu . gradv       LAPLACE
+``gradu, gradv, k`` (k nodal, 1-D)       s * k gradu . gradv     KAPPA_LAPLACE
+``u, v``                                 s * u v                 MASS
+``v``                  (load)            s * v                   LOAD_CONST
+``v, f`` (f nodal, 1-D) (load)           s * f v                 LOAD_NODAL
+======================================  =====================  ==============
+
+Any other ``forward`` runs through :func:`generic_matrix` /
+:func:`generic_load`: the same name-dispatch and ``vmap`` evaluation on the
+tensors' own device, with the element integrals scattered by ``index_add_``
+into the same CSR slots.  It is the general-form path, not the hot path.
+"""
+from __future__ import annotations
+
+import inspect
+from typing import Callable, Dict, Optional, Tuple
+
+import numpy as np
+import torch
+
+from .. import _lib
+from ..element import quadrature, shape_val, ref_grad
+
+_TOL = 1e-12
+
+
+def _probe(fn: Callable, names, build, ref) -> Optional[float]:
+    gen = torch.Generator().manual_seed(20240531)
+    ratios = []
+    try:
+        for _ in range(3):
+            args = build(gen)
+            out = fn(*[args[n] for n in names])
+            if not torch.is_tensor(out) or out.numel() != 1:
+                return None
+            e = float(ref(args))
+            ratios.append(float(out.reshape(())) / e)
+    except Exception:
+        return None
+    r0 = ratios[0]
+    if not np.isfinite(r0) or r0 == 0.0:
+        return None
+    if max(abs(r - r0) for r in ratios) > _TOL * abs(r0):
+        return None
+    return r0
+
+
+def _scalars(scalar_data):
+    return {k: (v if torch.is_tensor(v) else torch.tensor(v, dtype=torch.float64)) for k, v in scalar_data.items()}
+
+
+def classify_bilinear(fn: Callable, dim: int, point_data: Dict, scalar_data: Dict,
+                      element_data: Dict) -> Optional[Tuple[int, float, Optional[str]]]:
+    """Return ``(form, scale, coefficient key)`` or ``None``."""
+    names = list(inspect.signature(fn).parameters)
+    sc = _scalars(scalar_data)
+    rest = [n for n in names if n not in sc]
+    if any(n in element_data for n in rest):
+        return None
+
+    def rnd(gen, shape=()):
+        return torch.rand(shape, generator=gen, dtype=torch.float64) + 0.5
+
+    def build(extra):
+        def b(gen):
+            a = dict(sc)
+            for n in rest:
+                a[n] = rnd(gen, (dim,)) if n in ("gradu", "gradv") else rnd(gen)
+            return a
+        return b
+
+    s = set(rest)
+    if s == {"gradu", "gradv"}:
+        r = _probe(fn, names, build(None), lambda a: torch.dot(a["gradu"], a["gradv"]))
+        return None if r is None else (_lib.FORM_LAPLACE, r, None)
+    if len(rest) == 3 and {"gradu", "gradv"} <= s:
+        k = (s - {"gradu", "gradv"}).pop()
+        if k in point_data and point_data[k].dim() == 1:
+            r = _probe(fn, names, build(None), lambda a: a[k] * torch.dot(a["gradu"], a["gradv"]))
+            return None if r is None else (_lib.FORM_KAPPA_LAPLACE, r, k)
+        return None
+    if s == {"u", "v"}:
+        r = _probe(fn, names, build(None), lambda a: a["u"] * a["v"])
+        return None if r is None else (_lib.FORM_MASS, r, None)
+    return None
+
+
+def classify_linear(fn: Callable, point_data: Dict, scalar_data: Dict) -> Optional[Tuple[int, float, Optional[str]]]:
+    names = list(inspect.signature(fn).parameters)
+    sc = _scalars(scalar_data)
+    rest = [n for n in names if n not in sc]
+
+    def build(gen):
+        a = dict(sc)
+        for n in rest:
+            a[n] = torch.rand((), generator=gen, dtype=torch.float64) + 0.5
+        return a
+
+    if rest == ["v"]:
+        r = _probe(fn, names, build, lambda a: a["v"])
+        return None if r is None else (_lib.LOAD_CONST, r, None)
+    if len(rest) == 2 and "v" in rest:
+        k = [n for n in rest if n != "v"][0]
+        if k in point_data and point_data[k].dim() == 1:
+            r = _probe(fn, names, build, lambda a: a[k] * a["v"])
+            return None if r is None else (_lib.LOAD_NODAL, r, k)
+    return None
+
+
+# ---------------------------------------------------------------- generic path ---
+
+def geometry_torch(points: torch.Tensor, conn: torch.Tensor, element_type: str, order: int):
+    """``(shape_val [Q, nb], grad [E, Q, nb, d], jxw [E, Q])`` with torch ops
+    (``element.py:958-966``, ``transformation.py:1284-1290``)."""
+    w, q = quadrature(element_type, order)
+    dt, dev = points.dtype, points.device
+    phi = torch.as_tensor(shape_val(element_type, q), dtype=dt, device=dev)
+    G = torch.as_tensor(np.broadcast_to(ref_grad(element_type).T[None], (len(w),) + ref_grad(element_type).T.shape).copy(),
+                        dtype=dt, device=dev)                               # [Q, d, nb]
+    X = points[conn]                                                         # [E, nb, d]
+    J = torch.einsum("ebj,qib->eqij", X, G)
+    grad = torch.einsum("qib,eqji->eqbj", G, torch.linalg.inv(J))
+    jxw = torch.as_tensor(w, dtype=dt, device=dev)[None] * torch.linalg.det(J).abs()
+    return phi, grad, jxw
+
+
+def _args(names, phi, grad, point_data, element_data, scalar_data, conn, with_u: bool):
+    """Arguments and in_dims per level (element, quadrature, u, v) like the
+    reference's broadcast table (``element_assembler.py:332-346``)."""
+    args, dims = [], []
+    epd = {}
+    for n in names:
+        if n in ("u", "v"):
+            args.append(phi)
+            dims.append((None, 0, 0 if n == "u" else None, 0 if n == "v" else None))
+        elif n in ("gradu", "gradv"):
+            args.append(grad)
+            dims.append((0, 0, 0 if n == "gradu" else None, 0 if n == "gradv" else None))
+        elif n in point_data:
+            v = epd.setdefault(n, point_data[n][conn])
+            args.append(torch.einsum("eb...,qb->eq...", v, phi.to(v.dtype)))
+            dims.append((0, 0, None, None))
+        elif n.startswith("grad") and n[4:] in point_data:
+            v = epd.setdefault(n[4:], point_data[n[4:]][conn])
+            args.append(torch.einsum("eb...,eqbd->eq...d", v, grad.to(v.dtype)))
+            dims.append((0, 0, None, None))
+        elif n in element_data:
+            args.append(element_data[n])
+            dims.append((0, None, None, None))
+        elif n in scalar_data:
+            v = scalar_data[n]
+            args.append(v if torch.is_tensor(v) else torch.tensor(v))
+            dims.append((None, None, None, None))
+        else:
+            raise ValueError(f"{n} is not supported, please use `u`, `v`, `gradu`, `gradv` or more keys "
+                             f"provided by point_data, element_data or scalar_data")
+    if not with_u:
+        dims = [(e, q, v) for (e, q, u, v) in dims]
+    return args, dims
+
+
+def _vmapped(fn, dims, levels):
+    f = fn
+    for lv in reversed(range(levels)):
+        d = tuple(x[lv] for x in dims)
+        if all(x is None for x in d):
+            continue
+        f = torch.vmap(f, in_dims=d)
+    return f
+
+
+def generic_element_integrals(fn, points, conn, element_type, order, point_data, element_data, scalar_data,
+                              bilinear: bool):
+    names = list(inspect.signature(fn).parameters)
+    phi, grad, jxw = geometry_torch(points, conn, element_type, order)
+    args, dims = _args(names, phi, grad, point_data, element_data, scalar_data, conn, bilinear)
+    levels = 4 if bilinear else 3
+    elem_batched = any(d[0] is not None for d in dims)
+    if elem_batched:
+        out = _vmapped(fn, dims, levels)(*args)                 # [E, Q, nb(, nb), ...]
+        return torch.einsum("eq...,eq->e...", out, jxw.to(out.dtype))
+    out = _vmapped(fn, [d[1:] for d in dims], levels - 1)(*args)  # [Q, nb(, nb), ...]
+    return torch.einsum("q...,eq->e...", out, jxw.to(out.dtype))

@@ -1,0 +1,179 @@
+"""Mesh topology shared by the assemblers: CSR pattern, element slots, block plans.
+
+The pattern is the canonical CSR of all element node pairs (the reference's
+``edges``, ``tensormesh/assemble/element_assembler.py:838-857``), built on the
+host by the native library.  A *block plan* is the precomputed form of the
+reference's projector ``P_E`` (``:858-884``) for the owner-computes kernels:
+nodes are grouped into compact blocks by recursive coordinate bisection and
+each block lists the elements touching its rows with the in-row position of
+every element entry.  Plans depend only on the connectivity (the
+coordinates steer the partition, never the result), so one topology serves
+every assembler built on the same mesh.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Dict, Optional, Tuple
+
+import numpy as np
+import torch
+
+from .. import _lib
+from ..element import ELEMENT_DIM, ELEMENT_NB, quadrature_moments
+
+# Plan kinds: (accumulator bytes per CSR entry, LDS bytes per halo node,
+# LDS bytes per owned row).  "scalar" serves the matrix, load and adjoint
+# kernels of scalar problems.
+PLAN_KINDS = {"scalar": (8, 32, 16)}
+
+
+def _default_plan_params(kind: str, dim: int) -> Tuple[int, int]:
+    """(max owned rows per block, LDS budget in bytes)."""
+    budget = int(os.environ.get("TM_ASM_LDS_BUDGET", 64 * 1024 - 64))
+    if dim == 3:
+        rows = int(os.environ.get("TM_ASM_MAX_ROWS", 384))
+    else:
+        rows = int(os.environ.get("TM_ASM_MAX_ROWS_2D", 640))
+    return rows, budget
+
+
+class DevicePlan:
+    """Device copy of an exported block plan (keeps the tensors alive)."""
+
+    def __init__(self, host: Dict[str, np.ndarray], info: np.ndarray, device: torch.device):
+        self.tensors = {k: torch.from_numpy(v).to(device) for k, v in host.items()}
+        self.info = info
+        t = self.tensors
+        self.struct = _lib.PlanDev(
+            t["blk"].data_ptr(), t["halo"].data_ptr(), t["meta"].data_ptr(), t["recs"].data_ptr(),
+            t["rec_elem"].data_ptr(), t["runs"].data_ptr(), int(info[0]), int(info[4]))
+
+    @property
+    def n_blocks(self) -> int:
+        return int(self.info[0])
+
+    def bytes(self) -> int:
+        return sum(v.numel() * v.element_size() for v in self.tensors.values())
+
+
+class P1Topology:
+    """Pattern and plans of one P1 simplex connectivity (``triangle`` or ``tetra``)."""
+
+    def __init__(self, points: np.ndarray, conn: np.ndarray, n_points: int, element_type: str):
+        if element_type not in ELEMENT_NB:
+            raise NotImplementedError(f"element type {element_type!r}: the HIP path covers P1 "
+                                      f"'triangle' and 'tetra' meshes")
+        self.element_type = element_type
+        self.nb = ELEMENT_NB[element_type]
+        self.dim = ELEMENT_DIM[element_type]
+        self.n_points = int(n_points)
+        self.conn = np.ascontiguousarray(conn, dtype=np.int64)
+        assert self.conn.ndim == 2 and self.conn.shape[1] == self.nb, \
+            f"connectivity must be [n_elements, {self.nb}], got {self.conn.shape}"
+        self.n_elements = self.conn.shape[0]
+        self.points = np.ascontiguousarray(points, dtype=np.float64)
+        L = _lib.lib()
+        rp = np.empty(self.n_points + 1, dtype=np.int32)
+        _lib.check(L.tm_pattern_rowptr(_lib.ptr(self.conn), self.n_elements, self.nb, self.n_points,
+                                       _lib.ptr(rp)))
+        ci = np.empty(int(rp[-1]), dtype=np.int32)
+        _lib.check(L.tm_pattern_colidx(_lib.ptr(self.conn), self.n_elements, self.nb, self.n_points,
+                                       _lib.ptr(rp), _lib.ptr(ci)))
+        self.row_ptr = rp
+        self.col_idx = ci
+        self.nnz = int(rp[-1])
+        row = np.repeat(np.arange(self.n_points, dtype=np.int64), np.diff(rp))
+        self.edges = torch.from_numpy(np.stack([row, ci.astype(np.int64)], 0))
+        self._host_plans: Dict[tuple, tuple] = {}
+        self._dev_plans: Dict[tuple, DevicePlan] = {}
+        self._dev_csr: Dict[torch.device, Tuple[torch.Tensor, torch.Tensor]] = {}
+        self._slots: Optional[np.ndarray] = None
+
+    # ---- pattern ------------------------------------------------------------
+    def csr(self, device) -> Tuple[torch.Tensor, torch.Tensor]:
+        device = torch.device(device)
+        if device not in self._dev_csr:
+            self._dev_csr[device] = (torch.from_numpy(self.row_ptr).to(device),
+                                     torch.from_numpy(self.col_idx).to(device))
+        return self._dev_csr[device]
+
+    def element_slots(self) -> np.ndarray:
+        """The reference's ``P_E``: int64 ``[E * nb * nb]``, element-major then (i, j)."""
+        if self._slots is None:
+            s = np.empty(self.n_elements * self.nb * self.nb, dtype=np.int64)
+            _lib.check(_lib.lib().tm_element_slots(_lib.ptr(self.conn), self.n_elements, self.nb,
+                                                   _lib.ptr(self.row_ptr), _lib.ptr(self.col_idx),
+                                                   _lib.ptr(s)))
+            self._slots = s
+        return self._slots
+
+    # ---- plans ----------------------------------------------------------------
+    def host_plan(self, kind: str = "scalar", max_rows: Optional[int] = None,
+                  lds_budget: Optional[int] = None):
+        rows0, budget0 = _default_plan_params(kind, self.dim)
+        rows = int(max_rows or rows0)
+        budget = int(lds_budget or budget0)
+        key = (kind, rows, budget)
+        if key not in self._host_plans:
+            eb, nbytes, ob = PLAN_KINDS[kind]
+            L = _lib.lib()
+            h = ctypes.c_void_p()
+            _lib.check(L.tm_plan_build(_lib.ptr(self.points), self.dim, _lib.ptr(self.conn), self.n_elements,
+                                       self.nb, self.n_points, _lib.ptr(self.row_ptr), _lib.ptr(self.col_idx),
+                                       rows, budget, eb, nbytes, ob, 0, ctypes.byref(h)))
+            try:
+                info = np.zeros(8, dtype=np.int64)
+                _lib.check(L.tm_plan_info(h, _lib.ptr(info)))
+                nblk, nh, nr, nrun = (int(x) for x in info[:4])
+                arrs = dict(blk=np.empty(nblk * 8, np.int32), halo=np.empty(nh, np.int32),
+                            meta=np.empty(nh, np.uint32), recs=np.empty(nr * 4, np.uint32),
+                            rec_elem=np.empty(nr, np.int32), runs=np.empty(nrun * 4, np.int32))
+                _lib.check(L.tm_plan_export(h, *(_lib.ptr(arrs[k]) for k in
+                                                  ("blk", "halo", "meta", "recs", "rec_elem", "runs"))))
+            finally:
+                L.tm_plan_destroy(h)
+            self._host_plans[key] = (arrs, info)
+        return self._host_plans[key]
+
+    def device_plan(self, device, kind: str = "scalar", max_rows: Optional[int] = None,
+                    lds_budget: Optional[int] = None) -> DevicePlan:
+        device = torch.device(device)
+        rows0, budget0 = _default_plan_params(kind, self.dim)
+        key = (device, kind, int(max_rows or rows0), int(lds_budget or budget0))
+        if key not in self._dev_plans:
+            arrs, info = self.host_plan(kind, max_rows, lds_budget)
+            self._dev_plans[key] = DevicePlan(arrs, info, device)
+        return self._dev_plans[key]
+
+    def quad(self, order: int) -> _lib.Quad:
+        wsum, W, M = quadrature_moments(self.element_type, order)
+        q = _lib.Quad()
+        q.wsum = wsum
+        for i, v in enumerate(W):
+            q.W[i] = float(v)
+        nb = self.nb
+        for a in range(nb):
+            for b in range(nb):
+                q.M[a * nb + b] = float(M[a, b])
+        return q
+
+
+_KEEP: Dict[tuple, Tuple[P1Topology, torch.Tensor]] = {}
+
+
+def topology_for(points: torch.Tensor, conn: torch.Tensor, element_type: str) -> P1Topology:
+    """Topology of a connectivity tensor, shared by every assembler built on it."""
+    key = (conn.data_ptr(), tuple(conn.shape), str(conn.device), conn._version, points.shape[0])
+    hit = _KEEP.get(key)
+    if hit is None:
+        topo = P1Topology(points.detach().cpu().double().numpy(), conn.detach().cpu().numpy(),
+                          points.shape[0], element_type)
+        # the cache holds the connectivity tensor so its address stays unique
+        _KEEP[key] = (topo, conn)
+        return topo
+    return hit[0]
+
+
+def clear_topology_cache():
+    _KEEP.clear()

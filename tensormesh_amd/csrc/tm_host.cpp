@@ -1,0 +1,491 @@
+// Host-side setup for the P1 assembly path: canonical CSR pattern, element->slot
+// map and the owner-computes block plan.  Pure C++ (std::thread), no HIP.
+#include <algorithm>
+#include <atomic>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "tm_common.h"
+
+namespace tmx {
+static thread_local std::string g_err;
+void set_error(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+}  // namespace tmx
+
+extern "C" const char* tm_last_error(void) { return tmx::g_err.c_str(); }
+extern "C" int tm_version(void) { return 1; }
+
+namespace {
+
+int pick_threads(int req) {
+  if (req > 0) return req;
+  unsigned h = std::thread::hardware_concurrency();
+  return h == 0 ? 1 : (int)std::min(h, 16u);
+}
+
+// Runs f(begin, end) over [0, n) split into contiguous chunks on nthreads threads.
+template <class F>
+void parallel_for(int64_t n, int nthreads, F f) {
+  if (nthreads <= 1 || n < 8192) {
+    f((int64_t)0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  int64_t chunk = (n + nthreads - 1) / nthreads;
+  for (int t = 0; t < nthreads; ++t) {
+    int64_t b = t * chunk, e = std::min(n, b + chunk);
+    if (b >= e) break;
+    th.emplace_back([=, &f] { f(b, e); });
+  }
+  for (auto& x : th) x.join();
+}
+
+// Node -> incident elements (ascending element ids per node).
+struct Incidence {
+  std::vector<int64_t> ptr;
+  std::vector<int32_t> elem;
+};
+
+int build_incidence(const int64_t* conn, int64_t E, int nb, int64_t N, Incidence& inc) {
+  TM_REQUIRE(E >= 0 && N >= 0 && nb >= 1 && nb <= 4, TM_ERR_ARG,
+             "bad sizes: n_elem=%lld nb=%d n_points=%lld", (long long)E, nb, (long long)N);
+  TM_REQUIRE(E < (int64_t(1) << 31) && N < (int64_t(1) << 31), TM_ERR_LIMIT,
+             "mesh too large for int32 ids");
+  inc.ptr.assign(N + 1, 0);
+  for (int64_t k = 0; k < E * nb; ++k) {
+    int64_t g = conn[k];
+    TM_REQUIRE(g >= 0 && g < N, TM_ERR_ARG, "connectivity entry %lld = %lld out of [0, %lld)",
+               (long long)k, (long long)g, (long long)N);
+    inc.ptr[g + 1]++;
+  }
+  for (int64_t i = 0; i < N; ++i) inc.ptr[i + 1] += inc.ptr[i];
+  inc.elem.resize(inc.ptr[N]);
+  std::vector<int64_t> cur(inc.ptr.begin(), inc.ptr.end() - 1);
+  for (int64_t e = 0; e < E; ++e)
+    for (int a = 0; a < nb; ++a) inc.elem[cur[conn[e * nb + a]]++] = (int32_t)e;
+  return TM_OK;
+}
+
+// Sorted unique neighbours (including itself) of node i.
+inline void row_of(const Incidence& inc, const int64_t* conn, int nb, int64_t i,
+                   std::vector<int32_t>& buf) {
+  buf.clear();
+  for (int64_t k = inc.ptr[i]; k < inc.ptr[i + 1]; ++k) {
+    const int64_t* c = conn + (int64_t)inc.elem[k] * nb;
+    for (int a = 0; a < nb; ++a) buf.push_back((int32_t)c[a]);
+  }
+  std::sort(buf.begin(), buf.end());
+  buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
+}
+
+inline int32_t find_in_row(const int32_t* row_ptr, const int32_t* col_idx, int64_t r, int64_t c) {
+  const int32_t* b = col_idx + row_ptr[r];
+  const int32_t* e = col_idx + row_ptr[r + 1];
+  const int32_t* p = std::lower_bound(b, e, (int32_t)c);
+  if (p == e || *p != c) return -1;
+  return (int32_t)(p - b);
+}
+
+}  // namespace
+
+extern "C" int tm_pattern_rowptr(const int64_t* conn, int64_t E, int nb, int64_t N,
+                                 int32_t* row_ptr) {
+  Incidence inc;
+  int st = build_incidence(conn, E, nb, N, inc);
+  if (st) return st;
+  std::vector<int64_t> len(N);
+  parallel_for(N, pick_threads(0), [&](int64_t b, int64_t e) {
+    std::vector<int32_t> buf;
+    for (int64_t i = b; i < e; ++i) {
+      row_of(inc, conn, nb, i, buf);
+      len[i] = (int64_t)buf.size();
+    }
+  });
+  int64_t acc = 0;
+  row_ptr[0] = 0;
+  for (int64_t i = 0; i < N; ++i) {
+    acc += len[i];
+    TM_REQUIRE(acc < (int64_t(1) << 31), TM_ERR_LIMIT, "nnz exceeds int32 range");
+    row_ptr[i + 1] = (int32_t)acc;
+  }
+  return TM_OK;
+}
+
+extern "C" int tm_pattern_colidx(const int64_t* conn, int64_t E, int nb, int64_t N,
+                                 const int32_t* row_ptr, int32_t* col_idx) {
+  Incidence inc;
+  int st = build_incidence(conn, E, nb, N, inc);
+  if (st) return st;
+  std::atomic<int> bad{0};
+  parallel_for(N, pick_threads(0), [&](int64_t b, int64_t e) {
+    std::vector<int32_t> buf;
+    for (int64_t i = b; i < e; ++i) {
+      row_of(inc, conn, nb, i, buf);
+      if ((int64_t)buf.size() != (int64_t)(row_ptr[i + 1] - row_ptr[i])) {
+        bad = 1;
+        continue;
+      }
+      std::memcpy(col_idx + row_ptr[i], buf.data(), buf.size() * sizeof(int32_t));
+    }
+  });
+  TM_REQUIRE(!bad, TM_ERR_ARG, "row_ptr does not match this connectivity");
+  return TM_OK;
+}
+
+extern "C" int tm_element_slots(const int64_t* conn, int64_t E, int nb, const int32_t* row_ptr,
+                                const int32_t* col_idx, int64_t* slots) {
+  TM_REQUIRE(E >= 0 && nb >= 1 && nb <= 4, TM_ERR_ARG, "bad sizes");
+  std::atomic<int> bad{0};
+  parallel_for(E, pick_threads(0), [&](int64_t b, int64_t e) {
+    for (int64_t el = b; el < e; ++el) {
+      const int64_t* c = conn + el * nb;
+      for (int i = 0; i < nb; ++i) {
+        for (int j = 0; j < nb; ++j) {
+          int32_t p = find_in_row(row_ptr, col_idx, c[i], c[j]);
+          if (p < 0) {
+            bad = 1;
+            p = 0;
+          }
+          slots[(el * nb + i) * nb + j] = (int64_t)row_ptr[c[i]] + p;
+        }
+      }
+    }
+  });
+  TM_REQUIRE(!bad, TM_ERR_ARG, "pattern does not contain every element pair");
+  return TM_OK;
+}
+
+// ------------------------------------------------------------------ plan ----
+
+struct PlanBlock {
+  std::vector<int32_t> halo;
+  std::vector<uint32_t> meta;
+  std::vector<uint32_t> recs;
+  std::vector<int32_t> rec_elem;
+  std::vector<int32_t> runs;
+  int32_t n_own = 0, n_acc = 0, lds = 0;
+};
+
+struct tm_plan {
+  std::vector<int32_t> blk, halo, rec_elem, runs;
+  std::vector<uint32_t> meta, recs;
+  int64_t info[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+namespace {
+
+constexpr int kMaxHalo = 1024;  // 10-bit local ids in the element records
+constexpr int kMaxRank = 64;    // 6-bit in-row ranks
+constexpr int kSlack = 64;      // alignment padding of the kernel's LDS regions
+
+struct PlanCtx {
+  const double* pts;
+  int dim;
+  const int64_t* conn;
+  int64_t E, N;
+  int nb;
+  const int32_t* row_ptr;
+  const int32_t* col_idx;
+  const Incidence* inc;
+  int max_rows, budget, entry_bytes, node_bytes, own_bytes;
+};
+
+struct Scratch {
+  std::vector<uint32_t> estamp, nstamp;
+  std::vector<int32_t> local;
+  uint32_t tag = 0;
+  explicit Scratch(const PlanCtx& c) : estamp(c.E, 0), nstamp(c.N, 0), local(c.N, -1) {}
+};
+
+inline int64_t align16(int64_t x) { return (x + 15) & ~int64_t(15); }
+
+// Splits ids in place into two halves along the longest bounding-box axis;
+// returns the split position (left part gets ceil(parts/2)/parts of the ids).
+size_t rcb_split(const PlanCtx& c, std::vector<int32_t>& ids, size_t lo, size_t hi, int max_rows) {
+  double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+  for (size_t k = lo; k < hi; ++k)
+    for (int d = 0; d < c.dim; ++d) {
+      double v = c.pts[(int64_t)ids[k] * c.dim + d];
+      mn[d] = std::min(mn[d], v);
+      mx[d] = std::max(mx[d], v);
+    }
+  int ax = 0;
+  for (int d = 1; d < c.dim; ++d)
+    if (mx[d] - mn[d] > mx[ax] - mn[ax]) ax = d;
+  size_t n = hi - lo;
+  size_t parts = (n + max_rows - 1) / max_rows;
+  if (parts < 2) parts = 2;
+  size_t mid = lo + (size_t)((n * (parts / 2)) / parts);
+  if (mid == lo) mid = lo + 1;
+  const double* P = c.pts;
+  int dim = c.dim;
+  std::nth_element(ids.begin() + lo, ids.begin() + mid, ids.begin() + hi,
+                   [P, dim, ax](int32_t a, int32_t b) {
+                     double va = P[(int64_t)a * dim + ax], vb = P[(int64_t)b * dim + ax];
+                     return va < vb || (va == vb && a < b);
+                   });
+  return mid;
+}
+
+// Builds the block for the owned node set; on a capacity violation splits the
+// set and recurses.  Returns TM_OK or an error.
+int make_blocks(const PlanCtx& c, Scratch& s, std::vector<int32_t> owned,
+                std::vector<PlanBlock>& out) {
+  std::sort(owned.begin(), owned.end());
+  const int nown = (int)owned.size();
+  if (++s.tag == 0) {
+    std::fill(s.estamp.begin(), s.estamp.end(), 0);
+    std::fill(s.nstamp.begin(), s.nstamp.end(), 0);
+    s.tag = 1;
+  }
+  const uint32_t tag = s.tag;
+  std::vector<int32_t> elems;
+  int64_t n_acc = 0;
+  for (int a = 0; a < nown; ++a) {
+    int32_t g = owned[a];
+    s.nstamp[g] = tag;
+    s.local[g] = a;
+    n_acc += c.row_ptr[g + 1] - c.row_ptr[g];
+    for (int64_t k = c.inc->ptr[g]; k < c.inc->ptr[g + 1]; ++k) {
+      int32_t e = c.inc->elem[k];
+      if (s.estamp[e] != tag) {
+        s.estamp[e] = tag;
+        elems.push_back(e);
+      }
+    }
+  }
+  std::sort(elems.begin(), elems.end());
+  std::vector<int32_t> others;
+  for (int32_t e : elems)
+    for (int a = 0; a < c.nb; ++a) {
+      int32_t g = (int32_t)c.conn[(int64_t)e * c.nb + a];
+      if (s.nstamp[g] != tag) {
+        s.nstamp[g] = tag;
+        others.push_back(g);
+      }
+    }
+  std::sort(others.begin(), others.end());
+  const int64_t nh = nown + (int64_t)others.size();
+  const int64_t lds = align16(n_acc * c.entry_bytes) + align16(nh * c.node_bytes) +
+                      align16((int64_t)nown * c.own_bytes) + kSlack;
+  if (nh > kMaxHalo || n_acc > 65535 || lds > c.budget) {
+    TM_REQUIRE(nown > 1, TM_ERR_LIMIT,
+               "a single row does not fit the LDS budget (halo %lld, entries %lld, %lld bytes)",
+               (long long)nh, (long long)n_acc, (long long)lds);
+    std::vector<int32_t> ids(owned);
+    size_t mid = rcb_split(c, ids, 0, ids.size(), std::max(1, nown / 2));
+    std::vector<int32_t> left(ids.begin(), ids.begin() + mid), right(ids.begin() + mid, ids.end());
+    int st = make_blocks(c, s, std::move(left), out);
+    if (st) return st;
+    return make_blocks(c, s, std::move(right), out);
+  }
+  for (size_t k = 0; k < others.size(); ++k) s.local[others[k]] = nown + (int32_t)k;
+
+  PlanBlock B;
+  B.n_own = nown;
+  B.n_acc = (int32_t)n_acc;
+  B.lds = (int32_t)lds;
+  B.halo.reserve(nh);
+  B.halo.insert(B.halo.end(), owned.begin(), owned.end());
+  B.halo.insert(B.halo.end(), others.begin(), others.end());
+  B.meta.assign(nh, 0u);
+  uint32_t lrp = 0;
+  for (int a = 0; a < nown; ++a) {
+    int32_t g = owned[a];
+    int32_t dr = find_in_row(c.row_ptr, c.col_idx, g, g);
+    TM_REQUIRE(dr >= 0 && dr < 256, TM_ERR_LIMIT, "row %d has no diagonal or rank >= 256", g);
+    B.meta[a] = lrp | ((uint32_t)dr << 16);
+    lrp += (uint32_t)(c.row_ptr[g + 1] - c.row_ptr[g]);
+  }
+  // element records: 4 x 10-bit local ids, then for each owned vertex a the
+  // in-row ranks of the other vertices (6 bits each, a-major, b ascending)
+  B.recs.reserve(elems.size() * 4);
+  B.rec_elem.reserve(elems.size());
+  for (int32_t e : elems) {
+    const int64_t* ce = c.conn + (int64_t)e * c.nb;
+    int32_t l[4] = {0, 0, 0, 0};
+    for (int a = 0; a < c.nb; ++a) l[a] = s.local[ce[a]];
+    uint64_t lo = (uint64_t)l[0] | ((uint64_t)l[1] << 10) | ((uint64_t)l[2] << 20) |
+                  ((uint64_t)l[3] << 30);
+    uint64_t hi = 0;
+    for (int a = 0; a < c.nb; ++a) {
+      if (l[a] >= nown) continue;
+      int j = 0;
+      for (int b = 0; b < c.nb; ++b) {
+        if (b == a) continue;
+        int32_t r = find_in_row(c.row_ptr, c.col_idx, ce[a], ce[b]);
+        TM_REQUIRE(r >= 0 && r < kMaxRank, TM_ERR_LIMIT,
+                   "row %lld: column %lld missing or rank >= %d", (long long)ce[a],
+                   (long long)ce[b], kMaxRank);
+        int k = a * (c.nb - 1) + j++;
+        int bit = 40 + 6 * k;
+        if (bit < 64)
+          lo |= (uint64_t)r << bit;
+        else
+          hi |= (uint64_t)r << (bit - 64);
+      }
+    }
+    B.recs.push_back((uint32_t)lo);
+    B.recs.push_back((uint32_t)(lo >> 32));
+    B.recs.push_back((uint32_t)hi);
+    B.recs.push_back((uint32_t)(hi >> 32));
+    B.rec_elem.push_back(e);
+  }
+  // runs of consecutive owned rows -> contiguous value ranges
+  int a = 0;
+  uint32_t acc0 = 0;
+  while (a < nown) {
+    int b = a;
+    uint32_t len = (uint32_t)(c.row_ptr[owned[a] + 1] - c.row_ptr[owned[a]]);
+    while (b + 1 < nown && owned[b + 1] == owned[b] + 1) {
+      ++b;
+      len += (uint32_t)(c.row_ptr[owned[b] + 1] - c.row_ptr[owned[b]]);
+    }
+    B.runs.push_back(c.row_ptr[owned[a]]);
+    B.runs.push_back((int32_t)acc0);
+    B.runs.push_back((int32_t)len);
+    B.runs.push_back(owned[a]);
+    acc0 += len;
+    a = b + 1;
+  }
+  out.push_back(std::move(B));
+  return TM_OK;
+}
+
+}  // namespace
+
+extern "C" int tm_plan_build(const double* points, int dim, const int64_t* conn, int64_t E, int nb,
+                             int64_t N, const int32_t* row_ptr, const int32_t* col_idx,
+                             int max_rows, int lds_budget, int entry_bytes, int node_bytes,
+                             int own_bytes, int n_threads, tm_plan** out) {
+  TM_REQUIRE(out != nullptr, TM_ERR_ARG, "null output");
+  *out = nullptr;
+  TM_REQUIRE(dim >= 1 && dim <= 3 && nb >= 2 && nb <= 4 && max_rows >= 1, TM_ERR_ARG,
+             "bad plan parameters dim=%d nb=%d max_rows=%d", dim, nb, max_rows);
+  Incidence inc;
+  int st = build_incidence(conn, E, nb, N, inc);
+  if (st) return st;
+  PlanCtx c{points, dim, conn, E, N, nb, row_ptr, col_idx, &inc,
+            max_rows, lds_budget, entry_bytes, node_bytes, own_bytes};
+
+  // recursive coordinate bisection into leaves of <= max_rows nodes (DFS order)
+  std::vector<int32_t> ids(N);
+  std::iota(ids.begin(), ids.end(), 0);
+  std::vector<std::pair<size_t, size_t>> leaves, stack;
+  if (N > 0) stack.push_back({0, (size_t)N});
+  while (!stack.empty()) {
+    auto [lo, hi] = stack.back();
+    stack.pop_back();
+    if (hi - lo <= (size_t)max_rows) {
+      leaves.push_back({lo, hi});
+      continue;
+    }
+    size_t mid = rcb_split(c, ids, lo, hi, max_rows);
+    stack.push_back({mid, hi});
+    stack.push_back({lo, mid});
+  }
+
+  const int nt = std::max(1, std::min(pick_threads(n_threads), (int)std::max<size_t>(1, leaves.size())));
+  std::vector<std::vector<PlanBlock>> res(leaves.size());
+  std::atomic<size_t> next{0};
+  std::atomic<int> err{0};
+  std::string err_msg;
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; ++t) {
+    th.emplace_back([&] {
+      Scratch s(c);
+      for (;;) {
+        size_t k = next++;
+        if (k >= leaves.size() || err) break;
+        std::vector<int32_t> owned(ids.begin() + leaves[k].first, ids.begin() + leaves[k].second);
+        int r = make_blocks(c, s, std::move(owned), res[k]);
+        if (r && !err.exchange(r)) err_msg = tm_last_error();
+      }
+    });
+  }
+  for (auto& x : th) x.join();
+  if (err) {
+    tmx::set_error("%s", err_msg.c_str());
+    return err;
+  }
+
+  auto* P = new tm_plan();
+  int64_t nblk = 0, nh = 0, nr = 0, nrun = 0;
+  for (auto& v : res)
+    for (auto& b : v) {
+      ++nblk;
+      nh += b.halo.size();
+      nr += b.rec_elem.size();
+      nrun += b.runs.size() / 4;
+    }
+  if (nh >= (int64_t(1) << 31) || nr >= (int64_t(1) << 31)) {
+    delete P;
+    TM_REQUIRE(false, TM_ERR_LIMIT, "plan too large for int32 offsets");
+  }
+  P->blk.reserve(nblk * 8);
+  P->halo.reserve(nh);
+  P->meta.reserve(nh);
+  P->recs.reserve(nr * 4);
+  P->rec_elem.reserve(nr);
+  P->runs.reserve(nrun * 4);
+  int64_t max_lds = 0, max_halo = 0, max_rec = 0, max_acc = 0;
+  for (auto& v : res)
+    for (auto& b : v) {
+      int32_t n_rec = (int32_t)b.rec_elem.size();
+      int32_t n_run = (int32_t)(b.runs.size() / 4);
+      P->blk.insert(P->blk.end(),
+                    {(int32_t)P->halo.size(), (int32_t)b.halo.size(), b.n_own,
+                     (int32_t)P->rec_elem.size(), n_rec, (int32_t)(P->runs.size() / 4), n_run,
+                     b.n_acc});
+      P->halo.insert(P->halo.end(), b.halo.begin(), b.halo.end());
+      P->meta.insert(P->meta.end(), b.meta.begin(), b.meta.end());
+      P->recs.insert(P->recs.end(), b.recs.begin(), b.recs.end());
+      P->rec_elem.insert(P->rec_elem.end(), b.rec_elem.begin(), b.rec_elem.end());
+      P->runs.insert(P->runs.end(), b.runs.begin(), b.runs.end());
+      max_lds = std::max<int64_t>(max_lds, b.lds);
+      max_halo = std::max<int64_t>(max_halo, b.halo.size());
+      max_rec = std::max<int64_t>(max_rec, n_rec);
+      max_acc = std::max<int64_t>(max_acc, b.n_acc);
+    }
+  P->info[0] = nblk;
+  P->info[1] = nh;
+  P->info[2] = nr;
+  P->info[3] = nrun;
+  P->info[4] = max_lds;
+  P->info[5] = max_halo;
+  P->info[6] = max_rec;
+  P->info[7] = max_acc;
+  *out = P;
+  return TM_OK;
+}
+
+extern "C" int tm_plan_info(const tm_plan* p, int64_t* info) {
+  TM_REQUIRE(p && info, TM_ERR_ARG, "null plan");
+  std::memcpy(info, p->info, sizeof(p->info));
+  return TM_OK;
+}
+
+extern "C" int tm_plan_export(const tm_plan* p, int32_t* blk, int32_t* halo, uint32_t* meta,
+                              uint32_t* recs, int32_t* rec_elem, int32_t* runs) {
+  TM_REQUIRE(p, TM_ERR_ARG, "null plan");
+  if (blk) std::memcpy(blk, p->blk.data(), p->blk.size() * 4);
+  if (halo) std::memcpy(halo, p->halo.data(), p->halo.size() * 4);
+  if (meta) std::memcpy(meta, p->meta.data(), p->meta.size() * 4);
+  if (recs) std::memcpy(recs, p->recs.data(), p->recs.size() * 4);
+  if (rec_elem) std::memcpy(rec_elem, p->rec_elem.data(), p->rec_elem.size() * 4);
+  if (runs) std::memcpy(runs, p->runs.data(), p->runs.size() * 4);
+  return TM_OK;
+}
+
+extern "C" void tm_plan_destroy(tm_plan* p) { delete p; }

@@ -1,0 +1,1 @@
+from .grf import grf_lattice, lognormal_kappa_cube, lognormal_kappa_points

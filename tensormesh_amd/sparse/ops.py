@@ -1,0 +1,150 @@
+"""Autograd-aware device operators over CSR matrices (HIP kernels).
+
+* :class:`SpMV` - ``y = A x`` with ``dx = A^T g`` and ``dvalues = g[row] x[col]``
+  (the in-tree reference ``SparseMVTorch``, ``tensormesh/sparse/mm/torch_mm.py:32-48``).
+* :class:`PCGSolve` - Jacobi-PCG solve with the adjoint backward of
+  ``tensormesh/sparse/solve/torch_sla_solve.py:102-136``:
+  ``lambda = A^-T g``, ``db = lambda``, ``dvalues = -lambda[row] x[col]``.
+* :class:`Gather` - value gather used by condensation, scatter in backward.
+"""
+from __future__ import annotations
+
+import ctypes
+import warnings
+from dataclasses import dataclass
+from typing import Optional
+
+import torch
+
+from .. import _lib
+
+
+@dataclass
+class CSR:
+    """Device CSR view: int32 row pointers / columns, values in canonical order."""
+    row_ptr: torch.Tensor
+    col_idx: torch.Tensor
+    n_rows: int
+    n_cols: int
+
+    @property
+    def nnz(self) -> int:
+        return int(self.col_idx.numel())
+
+
+def _spmv_raw(csr: CSR, values: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
+    _lib.require_gpu(values, "matrix values")
+    x64 = x.detach().to(torch.float64).contiguous()
+    y = torch.empty(csr.n_rows, dtype=torch.float64, device=values.device)
+    _lib.check(_lib.lib().tm_spmv(csr.n_rows, _lib.ptr(csr.row_ptr), _lib.ptr(csr.col_idx),
+                                  _lib.ptr(values.detach().contiguous()), _lib.dtype_code(values),
+                                  _lib.ptr(x64), _lib.ptr(y), _lib.stream_of(values.device)))
+    return y
+
+
+def _transpose_csr(csr: CSR, values: torch.Tensor):
+    """Row-major CSR of A^T (torch sort; used only for non-symmetric adjoints)."""
+    rows = torch.repeat_interleave(torch.arange(csr.n_rows, device=values.device),
+                                   (csr.row_ptr[1:] - csr.row_ptr[:-1]).long())
+    cols = csr.col_idx.long()
+    key = cols * csr.n_rows + rows
+    order = torch.argsort(key)
+    counts = torch.bincount(cols, minlength=csr.n_cols)
+    rp = torch.zeros(csr.n_cols + 1, dtype=torch.int32, device=values.device)
+    rp[1:] = torch.cumsum(counts, 0).to(torch.int32)
+    return CSR(rp, rows[order].to(torch.int32), csr.n_cols, csr.n_rows), values.detach()[order]
+
+
+class SpMV(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, values, x, csr: CSR, row: torch.Tensor, col: torch.Tensor, symmetric: bool):
+        ctx.csr, ctx.symmetric = csr, symmetric
+        ctx.save_for_backward(values, x, row, col)
+        y = _spmv_raw(csr, values, x)
+        return y.to(torch.promote_types(values.dtype, x.dtype))
+
+    @staticmethod
+    def backward(ctx, g):
+        values, x, row, col = ctx.saved_tensors
+        gv = gx = None
+        if ctx.needs_input_grad[0]:
+            gv = (g[row] * x[col]).to(values.dtype)
+        if ctx.needs_input_grad[1]:
+            if ctx.symmetric:
+                gx = _spmv_raw(ctx.csr, values, g)
+            else:
+                csr_t, vt = _transpose_csr(ctx.csr, values)
+                gx = _spmv_raw(csr_t, vt, g)
+            gx = gx.to(x.dtype)
+        return gv, gx, None, None, None, None
+
+
+def pcg_raw(csr: CSR, values: torch.Tensor, b: torch.Tensor, x0: Optional[torch.Tensor] = None,
+            rtol: float = 1e-8, atol: float = 0.0, maxiter: int = 10000, check_every: int = 0,
+            raise_on_fail: bool = False):
+    """Jacobi-PCG on the device; returns ``(x [f64], iterations, relres, converged)``."""
+    _lib.require_gpu(values, "matrix values")
+    dev = values.device
+    n = csr.n_rows
+    b64 = b.detach().to(torch.float64).contiguous()
+    x = (torch.zeros(n, dtype=torch.float64, device=dev) if x0 is None
+         else x0.detach().to(torch.float64).clone().contiguous())
+    L = _lib.lib()
+    work = torch.empty(int(L.tm_pcg_work_size(n)) // 8 + 1, dtype=torch.float64, device=dev)
+    it = ctypes.c_int64(0)
+    rr = ctypes.c_double(0.0)
+    st = L.tm_pcg_jacobi(n, _lib.ptr(csr.row_ptr), _lib.ptr(csr.col_idx), _lib.ptr(values.detach().contiguous()),
+                         _lib.dtype_code(values), _lib.ptr(b64), _lib.ptr(x), _lib.ptr(work), float(rtol),
+                         float(atol), int(maxiter), int(check_every), ctypes.byref(it), ctypes.byref(rr),
+                         _lib.stream_of(dev))
+    converged = st == _lib.TM_OK
+    if not converged:
+        if st != _lib.TM_ERR_NOTCONV or raise_on_fail:
+            _lib.check(st)
+        warnings.warn(f"PCG did not converge: {L.tm_last_error().decode()}")
+    return x, int(it.value), float(rr.value), converged
+
+
+class PCGSolve(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, values, b, csr: CSR, row, col, rtol, atol, maxiter, info: dict):
+        x, it, rel, ok = pcg_raw(csr, values, b, None, rtol, atol, maxiter)
+        info.update(iterations=it, relres=rel, converged=ok)
+        ctx.csr, ctx.tol = csr, (rtol, atol, maxiter)
+        ctx.save_for_backward(values, x, row, col)
+        return x.to(torch.promote_types(values.dtype, b.dtype))
+
+    @staticmethod
+    def backward(ctx, g):
+        values, x, row, col = ctx.saved_tensors
+        rtol, atol, maxiter = ctx.tol
+        lam, _, _, _ = pcg_raw(ctx.csr, values, g, None, rtol, atol, maxiter)  # A symmetric: A^-T = A^-1
+        gv = (-lam[row] * x[col]).to(values.dtype) if ctx.needs_input_grad[0] else None
+        gb = lam.to(g.dtype) if ctx.needs_input_grad[1] else None
+        return gv, gb, None, None, None, None, None, None, None
+
+
+def gather_raw(src: torch.Tensor, idx: torch.Tensor) -> torch.Tensor:
+    _lib.require_gpu(src, "gather source")
+    src = src.contiguous()
+    out = torch.empty(idx.numel(), dtype=src.dtype, device=src.device)
+    _lib.check(_lib.lib().tm_gather(idx.numel(), _lib.ptr(idx), _lib.ptr(src), _lib.ptr(out),
+                                    src.element_size(), _lib.stream_of(src.device)))
+    return out
+
+
+class Gather(torch.autograd.Function):
+    """``out = src[idx]`` for a duplicate-free int32 index list."""
+
+    @staticmethod
+    def forward(ctx, src, idx, idx_long):
+        ctx.n = src.numel()
+        ctx.save_for_backward(idx_long)
+        return gather_raw(src, idx)
+
+    @staticmethod
+    def backward(ctx, g):
+        (idx_long,) = ctx.saved_tensors
+        out = torch.zeros(ctx.n, dtype=g.dtype, device=g.device)
+        out.index_copy_(0, idx_long, g)
+        return out, None, None

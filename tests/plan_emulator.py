@@ -1,0 +1,61 @@
+"""Float64 CPU emulation of the block-plan assembly kernel (test helper).
+
+Decodes the exported plan exactly as the HIP kernel does and checks every
+index against its buffer: local ids < halo size, accumulator offsets inside
+the block, each CSR value written by exactly one run.
+"""
+import numpy as np
+
+
+def decode(rec, nb):
+    lo = int(rec[0]) | (int(rec[1]) << 32)
+    hi = int(rec[2]) | (int(rec[3]) << 32)
+    loc = [(lo >> (10 * a)) & 1023 for a in range(nb)]
+
+    def rank(a, j):
+        bit = 40 + 6 * (a * (nb - 1) + j)
+        return (lo >> bit) & 63 if bit < 64 else (hi >> (bit - 64)) & 63
+    return loc, rank
+
+
+def emulate_matrix(topo, arrs, element_matrix):
+    """``element_matrix(e, x[nb, d]) -> [nb, nb]``; returns the CSR values."""
+    nb = topo.nb
+    blk = arrs["blk"].reshape(-1, 8)
+    recs = arrs["recs"].reshape(-1, 4)
+    runs = arrs["runs"].reshape(-1, 4)
+    vals = np.zeros(topo.nnz)
+    covered = np.zeros(topo.nnz, dtype=np.int64)
+    for b in range(blk.shape[0]):
+        hoff, nh, nown, roff, nrec, runoff, nrun, nacc = (int(v) for v in blk[b])
+        assert 0 < nh <= 1024 and 0 < nown <= nh
+        halo = arrs["halo"][hoff:hoff + nh]
+        meta = arrs["meta"][hoff:hoff + nown].astype(np.int64)
+        acc = np.zeros(nacc)
+        for r in range(nrec):
+            loc, rank = decode(recs[roff + r], nb)
+            assert max(loc) < nh
+            e = int(arrs["rec_elem"][roff + r])
+            assert list(halo[loc]) == list(topo.conn[e])
+            K = element_matrix(e, topo.points[halo[loc]])
+            for a in range(nb):
+                if loc[a] >= nown:
+                    continue
+                base, dr = int(meta[loc[a]] & 0xFFFF), int(meta[loc[a]] >> 16)
+                assert base + dr < nacc
+                acc[base + dr] += K[a, a]
+                j = 0
+                for bb in range(nb):
+                    if bb == a:
+                        continue
+                    k = rank(a, j)
+                    assert base + k < nacc
+                    acc[base + k] += K[a, bb]
+                    j += 1
+        for q in range(nrun):
+            vs, as_, ln, _ = (int(v) for v in runs[runoff + q])
+            assert as_ + ln <= nacc
+            vals[vs:vs + ln] = acc[as_:as_ + ln]
+            covered[vs:vs + ln] += 1
+    assert (covered == 1).all(), "every CSR value must be written by exactly one run"
+    return vals

@@ -1,0 +1,113 @@
+"""Host side of the native library (CPU only): exported symbols, bit-exact
+pattern and element slots against the SciPy oracle, and a float64 emulation
+of the block-plan kernel against the oracle matrices."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import assemble as oa
+from oracle import pattern as op
+from tensormesh_amd import _lib
+from tensormesh_amd.assemble.topology import P1Topology
+from tensormesh_amd.mesh import kuhn_cube_arrays, structured_rectangle_arrays
+from tests.plan_emulator import emulate_matrix
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "tensormesh_amd.h")).read()
+    declared = set(re.findall(r"^\s*(?:const char\*|int64_t|int|void)\s+(tm_\w+)\(", hdr, re.M))
+    assert declared, "no declarations parsed"
+    L = _lib.lib()
+    for name in declared:
+        assert hasattr(L, name), f"{name} missing from libtensormesh_amd.so"
+    assert declared == set(_lib.declared_symbols())
+    assert L.tm_version() >= 1
+
+
+def _meshes():
+    yield "tri", structured_rectangle_arrays(9, jitter=0.2, seed=5), "triangle"
+    yield "tet", kuhn_cube_arrays(4), "tetra"
+
+
+@pytest.mark.parametrize("name,arrays,et", list(_meshes()))
+def test_pattern_and_slots_bit_exact(name, arrays, et):
+    pts, conn, _ = arrays
+    topo = P1Topology(pts, conn, len(pts), et)
+    u, v, pe = op.pattern_and_slots(conn, len(pts))
+    assert topo.edges.dtype.is_floating_point is False
+    assert np.array_equal(topo.edges[0].numpy(), u) and np.array_equal(topo.edges[1].numpy(), v)
+    assert np.array_equal(topo.row_ptr, op.csr_from_edges(u, len(pts)))
+    assert np.array_equal(topo.element_slots(), pe)
+
+
+def test_pattern_with_isolated_node_and_shuffled_elements():
+    pts, conn, _ = kuhn_cube_arrays(3)
+    rng = np.random.default_rng(1)
+    conn = conn[rng.permutation(len(conn))]
+    conn = np.where(conn >= 10, conn + 1, conn)               # node 10 belongs to no element
+    pts = np.insert(pts, 10, [[9.0, 9.0, 9.0]], axis=0)
+    topo = P1Topology(pts, conn, len(pts), "tetra")
+    u, v, pe = op.pattern_and_slots(conn, len(pts))
+    assert np.array_equal(topo.edges[0].numpy(), u) and np.array_equal(topo.edges[1].numpy(), v)
+    assert topo.row_ptr[11] == topo.row_ptr[10]
+    assert np.array_equal(topo.element_slots(), pe)
+
+
+def test_bad_connectivity_raises():
+    pts, conn, _ = kuhn_cube_arrays(1)
+    conn = conn.copy()
+    conn[0, 0] = 99
+    with pytest.raises(ValueError, match="out of"):
+        P1Topology(pts, conn, len(pts), "tetra")
+
+
+def _kernel_element_matrix(topo, form, kappa, order=2):
+    w, phi, G = oa.rule(topo.element_type, order)
+    W = (w[:, None] * phi).sum(0)
+
+    def f(e, x):
+        J = x[1:] - x[0]
+        det = np.linalg.det(J)
+        Gi = np.linalg.inv(J)
+        g = np.concatenate([-Gi.sum(1, keepdims=True), Gi], 1).T
+        s = abs(det) * (w.sum() if form == "laplace" else (kappa[topo.conn[e]] * W).sum())
+        return s * g @ g.T
+    return f
+
+
+@pytest.mark.parametrize("name,arrays,et", list(_meshes()))
+@pytest.mark.parametrize("rows", [7, 33, 200])
+def test_block_plan_emulation_matches_oracle(name, arrays, et, rows):
+    pts, conn, _ = arrays
+    topo = P1Topology(pts, conn, len(pts), et)
+    arrs, info = topo.host_plan("scalar", max_rows=rows)
+    assert info[5] <= 1024 and info[4] <= 64 * 1024
+    kappa = np.exp(np.random.default_rng(3).standard_normal(len(pts)))
+    u, v, pe = op.pattern_and_slots(conn, len(pts))
+    for form in ("laplace", "kappa_laplace"):
+        got = emulate_matrix(topo, arrs, _kernel_element_matrix(topo, form, kappa))
+        ref = oa.reduce(pe, oa.element_matrices(pts, conn, et, form, kappa=kappa), len(u))
+        assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def test_block_plan_respects_lds_budget():
+    pts, conn, _ = kuhn_cube_arrays(6)
+    topo = P1Topology(pts, conn, len(pts), "tetra")
+    arrs, info = topo.host_plan("scalar", max_rows=10_000, lds_budget=24 * 1024)
+    assert info[4] <= 24 * 1024
+    blk = arrs["blk"].reshape(-1, 8)
+    assert blk[:, 2].sum() == len(pts)                         # every node owned exactly once
+    owned = np.concatenate([arrs["halo"][b[0]:b[0] + b[2]] for b in blk])
+    assert np.array_equal(np.sort(owned), np.arange(len(pts)))
+
+
+def test_plan_limit_error_is_reported():
+    pts, conn, _ = kuhn_cube_arrays(2)
+    topo = P1Topology(pts, conn, len(pts), "tetra")
+    with pytest.raises(_lib.TMError, match="single row"):
+        topo.host_plan("scalar", max_rows=8, lds_budget=256)
