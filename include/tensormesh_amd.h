@@ -81,7 +81,7 @@ int tm_plan_info(const tm_plan* plan, int64_t* info);
  * halo[n_halo_total] global node ids (owned rows first, each part ascending);
  * meta[n_halo_total] = acc_offset | diag_rank << 16 (owned entries only);
  * recs[n_records*4] packed element records; rec_elem[n_records] element ids;
- * runs[n_runs*4] = {value_start, acc_start, length, 0}. */
+ * runs[n_runs*4] = {value_start, acc_start, length, first_row}. */
 int tm_plan_export(const tm_plan* plan, int32_t* blk, int32_t* halo, uint32_t* meta,
                    uint32_t* recs, int32_t* rec_elem, int32_t* runs);
 void tm_plan_destroy(tm_plan* plan);
@@ -150,6 +150,77 @@ int tm_pcg_jacobi(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, con
                   int value_dtype, const double* b, double* x, void* work, double rtol,
                   double atol, int64_t maxiter, int check_every, int64_t* iters, double* relres,
                   void* stream);
+
+/* y = A x for a block-CSR matrix with 3x3 blocks: values[nnz*9], block k at
+ * values[9k..9k+8] row-major, on the scalar pattern (row_ptr, col_idx) of the
+ * nodes; x and y have 3*n_block_rows entries (dof = node*3 + component).  This is
+ * the storage order of SparseMatrix.from_block_coo (sparse/matrix.py:325-375). */
+int tm_spmv_bsr3(int64_t n_block_rows, const int32_t* row_ptr, const int32_t* col_idx, const void* values,
+                 int value_dtype, const double* x, double* y, void* stream);
+
+/* Jacobi-PCG on a 3x3 block-CSR matrix (point Jacobi on the block diagonals);
+ * semantics as tm_pcg_jacobi with n = 3*n_block_rows unknowns; work must hold
+ * tm_pcg_work_size(3*n_block_rows) bytes. */
+int tm_pcg_jacobi_bsr3(int64_t n_block_rows, const int32_t* row_ptr, const int32_t* col_idx,
+                       const void* values, int value_dtype, const double* b, double* x, void* work,
+                       double rtol, double atol, int64_t maxiter, int check_every, int64_t* iters,
+                       double* relres, void* stream);
+
+/* Compressible neo-Hookean P1 tetrahedra (NeoHookeanModel, assemble/builtin.py:220-334):
+ * for displacements u[N*3] (f64) and Young's modulus per element E_elem[E]
+ * (f32/f64; NULL = the constant E0), Poisson ratio nu, writes any of
+ *   kvals[nnz*9]  tangent 3x3 blocks on the scalar pattern (f32 or f64),
+ *   R[N*3]        residual (gradient of the energy w.r.t. u, f64),
+ *   *energy       += total strain energy (f64, caller zeroes it).
+ * NULL outputs are skipped.  plan must be built with entry_bytes 72,
+ * node_bytes 56, own_bytes 32.  The reference computes the residual by
+ * autograd of ElementAssembler.energy (element_assembler.py:468-680) and has
+ * no tangent kernel. */
+int tm_asm_nh(const tm_plan_dev* plan, const double* coords, const double* u, const void* E_elem, int E_dtype,
+              double E0, double nu, void* kvals, int kv_dtype, double* R, double* energy, void* stream);
+
+/* Matrix-free tangent action Y = K(u) lam (f64 [N*3], every entry written) and,
+ * when dE != NULL, dE[e] += lam . dR/dE_e (caller zeroes dE): the adjoint of
+ * the residual assembly w.r.t. nodal displacements and per-element E. */
+int tm_nh_apply(const tm_plan_dev* plan, const double* coords, const double* u, const void* E_elem, int E_dtype,
+                double E0, double nu, const double* lam, double* Y, double* dE, void* stream);
+
+/* Batched Jacobi-PCG over independent SPD systems stacked into one
+ * block-diagonal CSR (no reference counterpart: the reference solves varying
+ * meshes one at a time in a Python loop, docs batched_workflows.rst:247-270).
+ * Rows are cut into tiles [tile_row0[t], tile_row0[t+1]) that lie inside one
+ * system tile_mesh[t]; system k owns tiles [mesh_tile0[k], mesh_tile0[k+1]).
+ * Each system stops on its own relative residual rtol; x starts at zero.
+ * One asynchronous persistent launch (capturable in a hipGraph); lanes = lanes
+ * per row (2, 4, 8, 16), blocks = workgroups (<= 0: as many as are resident).
+ * Per system: mesh_status (1 converged, 2 breakdown, 3 max iterations),
+ * mesh_iters, mesh_relres. */
+typedef struct {
+  int64_t n_rows;
+  int32_t n_tiles;
+  int32_t n_mesh;
+  const int32_t* row_ptr;
+  const int32_t* col_idx;
+  const void* values;
+  int32_t value_dtype;
+  int32_t reserved;
+  const double* b;
+  double* x;
+  const int32_t* tile_row0;
+  const int32_t* tile_mesh;
+  const int32_t* mesh_tile0;
+  double rtol;
+  int64_t maxiter;
+  void* work;
+  int32_t* mesh_status;
+  int32_t* mesh_iters;
+  double* mesh_relres;
+} tm_batched_args;
+int64_t tm_batched_pcg_work_size(int64_t n_rows, int32_t n_tiles);
+int tm_batched_pcg(const tm_batched_args* args, int lanes, int blocks, void* stream);
+/* Copies the grid-barrier timeout flag (0 = every barrier completed) of the
+ * last launch on this work buffer into a device int32. */
+int tm_batched_timeout_flag(const tm_batched_args* args, int32_t* flag, void* stream);
 
 #ifdef __cplusplus
 }

@@ -36,6 +36,16 @@ class Quad(ctypes.Structure):
     _fields_ = [("wsum", c_double), ("W", c_double * 4), ("M", c_double * 16)]
 
 
+class BatchedArgs(ctypes.Structure):
+    _fields_ = [("n_rows", c_i64), ("n_tiles", ctypes.c_int32), ("n_mesh", ctypes.c_int32),
+                ("row_ptr", c_void_p), ("col_idx", c_void_p), ("values", c_void_p),
+                ("value_dtype", ctypes.c_int32), ("reserved", ctypes.c_int32), ("b", c_void_p),
+                ("x", c_void_p), ("tile_row0", c_void_p), ("tile_mesh", c_void_p),
+                ("mesh_tile0", c_void_p), ("rtol", c_double), ("maxiter", c_i64),
+                ("work", c_void_p), ("mesh_status", c_void_p), ("mesh_iters", c_void_p),
+                ("mesh_relres", c_void_p)]
+
+
 class TMError(RuntimeError):
     """Failure reported by the native library."""
 
@@ -67,6 +77,17 @@ _SIGS = {
                                         ctypes.POINTER(Quad), c_void_p, c_void_p]),
     "tm_spmv": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "tm_gather": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+    "tm_spmv_bsr3": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "tm_pcg_jacobi_bsr3": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                                   c_double, c_double, c_i64, c_int, ctypes.POINTER(c_i64),
+                                   ctypes.POINTER(c_double), c_void_p]),
+    "tm_asm_nh": (c_int, [ctypes.POINTER(PlanDev), c_void_p, c_void_p, c_void_p, c_int, c_double, c_double,
+                          c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "tm_nh_apply": (c_int, [ctypes.POINTER(PlanDev), c_void_p, c_void_p, c_void_p, c_int, c_double, c_double,
+                            c_void_p, c_void_p, c_void_p, c_void_p]),
+    "tm_batched_pcg_work_size": (c_i64, [c_i64, ctypes.c_int32]),
+    "tm_batched_pcg": (c_int, [ctypes.POINTER(BatchedArgs), c_int, c_int, c_void_p]),
+    "tm_batched_timeout_flag": (c_int, [ctypes.POINTER(BatchedArgs), c_void_p, c_void_p]),
     "tm_pcg_work_size": (c_i64, [c_i64]),
     "tm_pcg_jacobi": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                               c_double, c_double, c_i64, c_int, ctypes.POINTER(c_i64),

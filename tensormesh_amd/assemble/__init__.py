@@ -3,3 +3,4 @@ from .node_assembler import NodeAssembler
 from .builtin import (LaplaceElementAssembler, MassElementAssembler, KappaLaplaceElementAssembler,
                       SourceNodeAssembler, const_node_assembler, func_node_assembler)
 from .topology import P1Topology, topology_for, clear_topology_cache
+from .hyperelastic import NeoHookeanModel, BlockCondenser

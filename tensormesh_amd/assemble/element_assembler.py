@@ -210,6 +210,26 @@ class ElementAssembler(nn.Module):
                                           self.csr(dev), symmetric=True)
         return self._generic(fn, points, conn, point_data, element_data, scalar_data)
 
+    def energy(self, points: Optional[torch.Tensor] = None, func: Optional[Callable] = None,
+               point_data: Optional[Mapping[str, torch.Tensor]] = None, element_data=None,
+               scalar_data: Optional[Mapping[str, torch.Tensor]] = None, batch_size: int = -1):
+        """Total energy ``sum_e sum_q w_q |J| psi`` (``element_assembler.py:468-680``),
+        generic autograd-differentiable path; ``element_energy`` gives psi."""
+        assert isinstance(point_data, dict) or point_data is None, "point_data should be a dict"
+        point_data = dict(point_data or {})
+        etype = self.element_types[0]
+        element_data = dict(element_data or {})
+        if element_data and isinstance(next(iter(element_data.values())), dict):
+            element_data = {k: v[etype] for k, v in element_data.items()}
+        points = self.points if points is None else points
+        point_data["x"] = points
+        fn = self.element_energy if func is None else func
+        return forms.generic_energy(fn, points, self.elements[etype].to(points.device), etype,
+                                    self.quadrature_order, point_data, element_data, dict(scalar_data or {}))
+
+    def element_energy(self, **kwargs):
+        raise NotImplementedError("element_energy is not implemented")
+
     def _generic(self, fn, points, conn, point_data, element_data, scalar_data) -> SparseMatrix:
         integ = forms.generic_element_integrals(fn, points, conn.to(points.device), self.element_types[0],
                                                 self.quadrature_order, point_data, element_data, scalar_data,

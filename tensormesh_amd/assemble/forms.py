@@ -194,3 +194,39 @@ def generic_element_integrals(fn, points, conn, element_type, order, point_data,
         return torch.einsum("eq...,eq->e...", out, jxw.to(out.dtype))
     out = _vmapped(fn, [d[1:] for d in dims], levels - 1)(*args)  # [Q, nb(, nb), ...]
     return torch.einsum("q...,eq->e...", out, jxw.to(out.dtype))
+
+
+def generic_energy(fn, points, conn, element_type, order, point_data, element_data, scalar_data):
+    """``sum_e sum_q jxw psi(...)`` with the reference's energy broadcast rules
+    (``element_assembler.py:582-680``): point data and their gradients are
+    interpolated per quadrature point, element data is per element (or per
+    element and quadrature point when its second dimension equals Q)."""
+    names = list(inspect.signature(fn).parameters)
+    phi, grad, jxw = geometry_torch(points, conn, element_type, order)
+    Q = phi.shape[0]
+    args, edims, qdims = [], [], []
+    for n in names:
+        if n in element_data:
+            v = element_data[n]
+            args.append(v)
+            edims.append(0)
+            qdims.append(0 if (v.dim() >= 2 and v.shape[1] == Q) else None)
+        elif n in scalar_data:
+            v = scalar_data[n]
+            args.append(v if torch.is_tensor(v) else torch.tensor(v))
+            edims.append(None)
+            qdims.append(None)
+        elif n in point_data:
+            v = point_data[n][conn]
+            args.append(torch.einsum("eb...,qb->eq...", v, phi.to(v.dtype)))
+            edims.append(0)
+            qdims.append(0)
+        elif n.startswith("grad") and n[4:] in point_data:
+            v = point_data[n[4:]][conn]
+            args.append(torch.einsum("eb...,eqbd->eq...d", v, grad.to(v.dtype)))
+            edims.append(0)
+            qdims.append(0)
+        else:
+            raise ValueError(f"{n} is not supported for energy calculation.")
+    dens = torch.vmap(torch.vmap(fn, in_dims=tuple(qdims)), in_dims=tuple(edims))(*args)
+    return (dens * jxw.to(dens.dtype)).sum()

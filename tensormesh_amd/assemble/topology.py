@@ -25,12 +25,14 @@ from ..element import ELEMENT_DIM, ELEMENT_NB, quadrature_moments
 # Plan kinds: (accumulator bytes per CSR entry, LDS bytes per halo node,
 # LDS bytes per owned row).  "scalar" serves the matrix, load and adjoint
 # kernels of scalar problems.
-PLAN_KINDS = {"scalar": (8, 32, 16)}
+PLAN_KINDS = {"scalar": (8, 32, 16), "vector3": (72, 56, 32)}
 
 
 def _default_plan_params(kind: str, dim: int) -> Tuple[int, int]:
     """(max owned rows per block, LDS budget in bytes)."""
     budget = int(os.environ.get("TM_ASM_LDS_BUDGET", 64 * 1024 - 64))
+    if kind == "vector3":
+        return int(os.environ.get("TM_ASM_MAX_ROWS_VEC", 64)), budget
     if dim == 3:
         rows = int(os.environ.get("TM_ASM_MAX_ROWS", 384))
     else:

@@ -55,27 +55,44 @@ PcgWork carve_work(void* work, int64_t n) {
   return w;
 }
 
-template <typename VT, int S>
-__device__ __forceinline__ double row_dot(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
-                                          const VT* __restrict__ v, const double* __restrict__ x, int64_t row,
-                                          int sl) {
+// out[i] = sum over the block row of B[i][j] x[col*BS + j], reduced over the S
+// lanes that share the row (BS = 1: plain CSR; BS = 3: 3x3 blocks stored
+// row-major per CSR entry).
+template <typename VT, int S, int BS>
+__device__ __forceinline__ void row_dot(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                        const VT* __restrict__ v, const double* __restrict__ x, int64_t row,
+                                        int sl, double (&out)[BS]) {
   const int b = rp[row], e = rp[row + 1];
-  double s = 0.0;
-  for (int k = b + sl; k < e; k += S) s += (double)v[k] * x[ci[k]];
 #pragma unroll
-  for (int o = S / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, kWave);
-  return s;
+  for (int i = 0; i < BS; ++i) out[i] = 0.0;
+  for (int k = b + sl; k < e; k += S) {
+    const int64_t c = ci[k];
+    double xc[BS];
+#pragma unroll
+    for (int j = 0; j < BS; ++j) xc[j] = x[c * BS + j];
+#pragma unroll
+    for (int i = 0; i < BS; ++i)
+#pragma unroll
+      for (int j = 0; j < BS; ++j) out[i] += (double)v[(int64_t)k * BS * BS + i * BS + j] * xc[j];
+  }
+#pragma unroll
+  for (int i = 0; i < BS; ++i)
+#pragma unroll
+    for (int o = S / 2; o > 0; o >>= 1) out[i] += __shfl_xor(out[i], o, kWave);
 }
 
-template <typename VT, int S>
-__global__ __launch_bounds__(kThreads) void k_spmv(int64_t n, const int32_t* __restrict__ rp,
+template <typename VT, int S, int BS>
+__global__ __launch_bounds__(kThreads) void k_spmv(int64_t nrows, const int32_t* __restrict__ rp,
                                                   const int32_t* __restrict__ ci, const VT* __restrict__ v,
                                                   const double* __restrict__ x, double* __restrict__ y) {
   const int sl = threadIdx.x % S;
   const int64_t per = kThreads / S;
-  for (int64_t row = blockIdx.x * per + threadIdx.x / S; row < n; row += (int64_t)gridDim.x * per) {
-    const double s = row_dot<VT, S>(rp, ci, v, x, row, sl);
-    if (sl == 0) y[row] = s;
+  for (int64_t row = blockIdx.x * per + threadIdx.x / S; row < nrows; row += (int64_t)gridDim.x * per) {
+    double o[BS];
+    row_dot<VT, S, BS>(rp, ci, v, x, row, sl, o);
+    if (sl == 0)
+#pragma unroll
+      for (int i = 0; i < BS; ++i) y[row * BS + i] = o[i];
   }
 }
 
@@ -107,19 +124,26 @@ __device__ __forceinline__ bool finish_reduce(double (&local)[NS], double* part,
   return true;
 }
 
-template <typename VT>
-__global__ __launch_bounds__(kThreads) void k_diag_inv(int64_t n, const int32_t* __restrict__ rp,
+template <typename VT, int BS>
+__global__ __launch_bounds__(kThreads) void k_diag_inv(int64_t nrows, const int32_t* __restrict__ rp,
                                                       const int32_t* __restrict__ ci, const VT* __restrict__ v,
                                                       double* __restrict__ dinv) {
-  for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-    double d = 1.0;  // missing diagonal -> 1 (torch_solve.py:24-45)
+  for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < nrows; i += (int64_t)gridDim.x * kThreads) {
+    double d[BS];
+#pragma unroll
+    for (int c = 0; c < BS; ++c) d[c] = 1.0;  // missing diagonal -> 1 (torch_solve.py:24-45)
     bool found = false;
     for (int k = rp[i]; k < rp[i + 1]; ++k)
       if (ci[k] == i) {
-        d = found ? d + (double)v[k] : (double)v[k];
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+          const double vk = (double)v[(int64_t)k * BS * BS + c * BS + c];
+          d[c] = found ? d[c] + vk : vk;
+        }
         found = true;
       }
-    dinv[i] = 1.0 / d;
+#pragma unroll
+    for (int c = 0; c < BS; ++c) dinv[i * BS + c] = 1.0 / d[c];
   }
 }
 
@@ -150,19 +174,23 @@ __global__ __launch_bounds__(kThreads) void k_pcg_init(int64_t n, const double* 
   }
 }
 
-template <typename VT, int S>
-__global__ __launch_bounds__(kThreads) void k_pcg_spmv(int64_t n, const int32_t* __restrict__ rp,
+template <typename VT, int S, int BS>
+__global__ __launch_bounds__(kThreads) void k_pcg_spmv(int64_t nrows, const int32_t* __restrict__ rp,
                                                       const int32_t* __restrict__ ci, const VT* __restrict__ v,
                                                       PcgWork w) {
   if (w.f->status != 0) return;
   const int sl = threadIdx.x % S;
   const int64_t per = kThreads / S;
   double loc[1] = {0.0};
-  for (int64_t row = blockIdx.x * per + threadIdx.x / S; row < n; row += (int64_t)gridDim.x * per) {
-    const double s = row_dot<VT, S>(rp, ci, v, w.p, row, sl);
+  for (int64_t row = blockIdx.x * per + threadIdx.x / S; row < nrows; row += (int64_t)gridDim.x * per) {
+    double o[BS];
+    row_dot<VT, S, BS>(rp, ci, v, w.p, row, sl, o);
     if (sl == 0) {
-      w.q[row] = s;
-      loc[0] += s * w.p[row];
+#pragma unroll
+      for (int i = 0; i < BS; ++i) {
+        w.q[row * BS + i] = o[i];
+        loc[0] += o[i] * w.p[row * BS + i];
+      }
     }
   }
   double tot[1];
@@ -229,15 +257,17 @@ int grid_for(int64_t n, int per_block) {
   return (int)(g < 1 ? 1 : (g > kMaxBlocks ? kMaxBlocks : g));
 }
 
-template <typename VT, int S>
-int pcg_run(int64_t n, const int32_t* rp, const int32_t* ci, const VT* v, const double* b, double* x, PcgWork w,
+template <typename VT, int S, int BS>
+int pcg_run(int64_t nrows, const int32_t* rp, const int32_t* ci, const VT* v, const double* b, double* x, PcgWork w,
             double rtol, double atol, int64_t maxiter, int check_every, int64_t* iters, double* relres,
             hipStream_t st) {
+  const int64_t n = nrows * BS;
   const int gv = grid_for(n, kThreads * 4);
-  const int gs = grid_for(n, kThreads / S);
+  const int gs = grid_for(nrows, kThreads / S);
   hipLaunchKernelGGL(k_pcg_setmax, dim3(1), dim3(1), 0, st, w, (long long)maxiter);
-  hipLaunchKernelGGL((k_diag_inv<VT>), dim3(gv), dim3(kThreads), 0, st, n, rp, ci, v, w.dinv);
-  hipLaunchKernelGGL((k_spmv<VT, S>), dim3(gs), dim3(kThreads), 0, st, n, rp, ci, v, x, w.q);
+  hipLaunchKernelGGL((k_diag_inv<VT, BS>), dim3(grid_for(nrows, kThreads)), dim3(kThreads), 0, st, nrows, rp, ci, v,
+                     w.dinv);
+  hipLaunchKernelGGL((k_spmv<VT, S, BS>), dim3(gs), dim3(kThreads), 0, st, nrows, rp, ci, v, x, w.q);
   hipLaunchKernelGGL(k_pcg_init, dim3(gv), dim3(kThreads), 0, st, n, b, w, rtol * rtol, atol * atol);
   TM_HIP(hipGetLastError());
   PcgFlags hf{};
@@ -248,7 +278,7 @@ int pcg_run(int64_t n, const int32_t* rp, const int32_t* ci, const VT* v, const 
     TM_HIP(hipStreamSynchronize(st));
     if (hf.status != 0) break;
     for (int k = 0; k < chunk; ++k) {
-      hipLaunchKernelGGL((k_pcg_spmv<VT, S>), dim3(gs), dim3(kThreads), 0, st, n, rp, ci, v, w);
+      hipLaunchKernelGGL((k_pcg_spmv<VT, S, BS>), dim3(gs), dim3(kThreads), 0, st, nrows, rp, ci, v, w);
       hipLaunchKernelGGL(k_pcg_update, dim3(gv), dim3(kThreads), 0, st, n, x, w);
       hipLaunchKernelGGL(k_pcg_dir, dim3(gv), dim3(kThreads), 0, st, n, w);
     }
@@ -279,30 +309,30 @@ int lanes_for(int64_t n, int64_t nnz) {
   return 32;
 }
 
-template <typename VT>
+template <typename VT, int BS>
 int spmv_dispatch(int64_t n, int64_t nnz, const int32_t* rp, const int32_t* ci, const VT* v, const double* x,
                   double* y, hipStream_t st) {
   const int S = lanes_for(n, nnz);
-  const int g = grid_for(n, kThreads / S) ;
+  const int g = grid_for(n, kThreads / S);
   switch (S) {
-    case 4: hipLaunchKernelGGL((k_spmv<VT, 4>), dim3(g), dim3(kThreads), 0, st, n, rp, ci, v, x, y); break;
-    case 8: hipLaunchKernelGGL((k_spmv<VT, 8>), dim3(g), dim3(kThreads), 0, st, n, rp, ci, v, x, y); break;
-    case 16: hipLaunchKernelGGL((k_spmv<VT, 16>), dim3(g), dim3(kThreads), 0, st, n, rp, ci, v, x, y); break;
-    default: hipLaunchKernelGGL((k_spmv<VT, 32>), dim3(g), dim3(kThreads), 0, st, n, rp, ci, v, x, y); break;
+    case 4: hipLaunchKernelGGL((k_spmv<VT, 4, BS>), dim3(g), dim3(kThreads), 0, st, n, rp, ci, v, x, y); break;
+    case 8: hipLaunchKernelGGL((k_spmv<VT, 8, BS>), dim3(g), dim3(kThreads), 0, st, n, rp, ci, v, x, y); break;
+    case 16: hipLaunchKernelGGL((k_spmv<VT, 16, BS>), dim3(g), dim3(kThreads), 0, st, n, rp, ci, v, x, y); break;
+    default: hipLaunchKernelGGL((k_spmv<VT, 32, BS>), dim3(g), dim3(kThreads), 0, st, n, rp, ci, v, x, y); break;
   }
   TM_HIP(hipGetLastError());
   return TM_OK;
 }
 
-template <typename VT>
+template <typename VT, int BS>
 int pcg_dispatch(int64_t n, int64_t nnz, const int32_t* rp, const int32_t* ci, const VT* v, const double* b,
                  double* x, PcgWork w, double rtol, double atol, int64_t maxiter, int check_every,
                  int64_t* iters, double* relres, hipStream_t st) {
   switch (lanes_for(n, nnz)) {
-    case 4: return pcg_run<VT, 4>(n, rp, ci, v, b, x, w, rtol, atol, maxiter, check_every, iters, relres, st);
-    case 8: return pcg_run<VT, 8>(n, rp, ci, v, b, x, w, rtol, atol, maxiter, check_every, iters, relres, st);
-    case 16: return pcg_run<VT, 16>(n, rp, ci, v, b, x, w, rtol, atol, maxiter, check_every, iters, relres, st);
-    default: return pcg_run<VT, 32>(n, rp, ci, v, b, x, w, rtol, atol, maxiter, check_every, iters, relres, st);
+    case 4: return pcg_run<VT, 4, BS>(n, rp, ci, v, b, x, w, rtol, atol, maxiter, check_every, iters, relres, st);
+    case 8: return pcg_run<VT, 8, BS>(n, rp, ci, v, b, x, w, rtol, atol, maxiter, check_every, iters, relres, st);
+    case 16: return pcg_run<VT, 16, BS>(n, rp, ci, v, b, x, w, rtol, atol, maxiter, check_every, iters, relres, st);
+    default: return pcg_run<VT, 32, BS>(n, rp, ci, v, b, x, w, rtol, atol, maxiter, check_every, iters, relres, st);
   }
 }
 
@@ -316,8 +346,21 @@ extern "C" int tm_spmv(int64_t n, const int32_t* row_ptr, const int32_t* col_idx
   hipStream_t st = (hipStream_t)stream;
   TM_HIP(hipMemcpyAsync(&nnz_h, row_ptr + n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
   TM_HIP(hipStreamSynchronize(st));
-  if (value_dtype == TM_F32) return spmv_dispatch(n, nnz_h, row_ptr, col_idx, (const float*)values, x, y, st);
-  return spmv_dispatch(n, nnz_h, row_ptr, col_idx, (const double*)values, x, y, st);
+  if (value_dtype == TM_F32) return spmv_dispatch<float, 1>(n, nnz_h, row_ptr, col_idx, (const float*)values, x, y, st);
+  return spmv_dispatch<double, 1>(n, nnz_h, row_ptr, col_idx, (const double*)values, x, y, st);
+}
+
+extern "C" int tm_spmv_bsr3(int64_t n_block_rows, const int32_t* row_ptr, const int32_t* col_idx,
+                            const void* values, int value_dtype, const double* x, double* y, void* stream) {
+  TM_REQUIRE(n_block_rows >= 0 && row_ptr && col_idx && values && x && y, TM_ERR_ARG, "null argument");
+  if (n_block_rows == 0) return TM_OK;
+  int32_t nnz_h = 0;
+  hipStream_t st = (hipStream_t)stream;
+  TM_HIP(hipMemcpyAsync(&nnz_h, row_ptr + n_block_rows, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  TM_HIP(hipStreamSynchronize(st));
+  if (value_dtype == TM_F32)
+    return spmv_dispatch<float, 3>(n_block_rows, nnz_h, row_ptr, col_idx, (const float*)values, x, y, st);
+  return spmv_dispatch<double, 3>(n_block_rows, nnz_h, row_ptr, col_idx, (const double*)values, x, y, st);
 }
 
 extern "C" int tm_gather(int64_t n, const int32_t* idx, const void* src, void* dst, int elem_bytes,
@@ -350,8 +393,25 @@ extern "C" int tm_pcg_jacobi(int64_t n, const int32_t* row_ptr, const int32_t* c
   TM_HIP(hipStreamSynchronize(st));
   PcgWork w = carve_work(work, n);
   if (value_dtype == TM_F32)
-    return pcg_dispatch(n, nnz_h, row_ptr, col_idx, (const float*)values, b, x, w, rtol, atol, maxiter,
-                        check_every, iters, relres, st);
-  return pcg_dispatch(n, nnz_h, row_ptr, col_idx, (const double*)values, b, x, w, rtol, atol, maxiter,
-                      check_every, iters, relres, st);
+    return pcg_dispatch<float, 1>(n, nnz_h, row_ptr, col_idx, (const float*)values, b, x, w, rtol, atol, maxiter,
+                                  check_every, iters, relres, st);
+  return pcg_dispatch<double, 1>(n, nnz_h, row_ptr, col_idx, (const double*)values, b, x, w, rtol, atol, maxiter,
+                                 check_every, iters, relres, st);
+}
+
+extern "C" int tm_pcg_jacobi_bsr3(int64_t n_block_rows, const int32_t* row_ptr, const int32_t* col_idx,
+                                  const void* values, int value_dtype, const double* b, double* x, void* work,
+                                  double rtol, double atol, int64_t maxiter, int check_every, int64_t* iters,
+                                  double* relres, void* stream) {
+  TM_REQUIRE(n_block_rows > 0 && row_ptr && col_idx && values && b && x && work, TM_ERR_ARG, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  int32_t nnz_h = 0;
+  TM_HIP(hipMemcpyAsync(&nnz_h, row_ptr + n_block_rows, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  TM_HIP(hipStreamSynchronize(st));
+  PcgWork w = carve_work(work, 3 * n_block_rows);
+  if (value_dtype == TM_F32)
+    return pcg_dispatch<float, 3>(n_block_rows, nnz_h, row_ptr, col_idx, (const float*)values, b, x, w, rtol, atol,
+                                  maxiter, check_every, iters, relres, st);
+  return pcg_dispatch<double, 3>(n_block_rows, nnz_h, row_ptr, col_idx, (const double*)values, b, x, w, rtol, atol,
+                                 maxiter, check_every, iters, relres, st);
 }

@@ -38,6 +38,9 @@ class SparseMatrix:
         self._csr: Optional[CSR] = None
         self._perm = None          # canonical order of a non-canonical COO
         self.symmetric: Optional[bool] = None
+        # 3 when the COO is the block expansion of 3x3 blocks on a node pattern
+        # (from_block_coo order); then _csr is the node-level pattern
+        self.block_size: int = 1
         self.last_solve_info: dict = {}
 
     @classmethod
@@ -112,7 +115,7 @@ class SparseMatrix:
         m = SparseMatrix(values, self.row_indices if row is None else row,
                          self.col_indices if col is None else col, self.shape)
         if row is None and col is None and values.device == self.values.device:
-            m._csr, m._perm, m.symmetric = self._csr, self._perm, self.symmetric
+            m._csr, m._perm, m.symmetric, m.block_size = self._csr, self._perm, self.symmetric, self.block_size
         return m
 
     def to(self, *args, **kwargs) -> "SparseMatrix":
@@ -238,6 +241,9 @@ class SparseMatrix:
             return (self.to_sparse_coo() @ x.to(self.values.dtype))
         if x.dim() == 2:
             return torch.stack([self @ x[:, k] for k in range(x.shape[1])], 1)
+        if self.block_size != 1:
+            return SpMV.apply(self.values, x, self._csr, self.row_indices, self.col_indices,
+                              bool(self.symmetric), self.block_size)
         v, r, c = self._canonical()
         sym = bool(self.symmetric) if self.symmetric is not None else False
         return SpMV.apply(v, x, self.csr(), r, c, sym)
@@ -268,9 +274,13 @@ class SparseMatrix:
             raise ValueError(f"backend {backend!r} is not available for GPU tensors (use 'hip')")
         if method not in (None, "cg", "pcg", "jacobi-pcg", "auto"):
             raise ValueError(f"method {method!r} is not available on the GPU (Jacobi-PCG only)")
-        v, r, c = self._canonical()
         info: dict = {}
-        x = PCGSolve.apply(v, b.to(self.device), self.csr(), r, c, float(tol), float(atol), int(maxiter), info)
+        if self.block_size != 1:
+            x = PCGSolve.apply(self.values, b.to(self.device), self._csr, self.row_indices, self.col_indices,
+                               float(tol), float(atol), int(maxiter), info, self.block_size)
+        else:
+            v, r, c = self._canonical()
+            x = PCGSolve.apply(v, b.to(self.device), self.csr(), r, c, float(tol), float(atol), int(maxiter), info)
         self.last_solve_info = info
         if verbose:
             print(f"[tensormesh_amd] solve: n={self.shape[0]}, nnz={self.nnz}, dtype={self.dtype}, "

@@ -32,13 +32,17 @@ class CSR:
         return int(self.col_idx.numel())
 
 
-def _spmv_raw(csr: CSR, values: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
+def _spmv_raw(csr: CSR, values: torch.Tensor, x: torch.Tensor, block_size: int = 1) -> torch.Tensor:
+    """``y = A x``; with ``block_size`` 3 the CSR is over block rows and values hold
+    one row-major 3x3 block per entry."""
     _lib.require_gpu(values, "matrix values")
     x64 = x.detach().to(torch.float64).contiguous()
-    y = torch.empty(csr.n_rows, dtype=torch.float64, device=values.device)
-    _lib.check(_lib.lib().tm_spmv(csr.n_rows, _lib.ptr(csr.row_ptr), _lib.ptr(csr.col_idx),
-                                  _lib.ptr(values.detach().contiguous()), _lib.dtype_code(values),
-                                  _lib.ptr(x64), _lib.ptr(y), _lib.stream_of(values.device)))
+    y = torch.empty(csr.n_rows * block_size, dtype=torch.float64, device=values.device)
+    fn = _lib.lib().tm_spmv if block_size == 1 else _lib.lib().tm_spmv_bsr3
+    assert block_size in (1, 3), f"block size {block_size} not supported"
+    _lib.check(fn(csr.n_rows, _lib.ptr(csr.row_ptr), _lib.ptr(csr.col_idx),
+                  _lib.ptr(values.detach().contiguous()), _lib.dtype_code(values),
+                  _lib.ptr(x64), _lib.ptr(y), _lib.stream_of(values.device)))
     return y
 
 
@@ -57,10 +61,11 @@ def _transpose_csr(csr: CSR, values: torch.Tensor):
 
 class SpMV(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, values, x, csr: CSR, row: torch.Tensor, col: torch.Tensor, symmetric: bool):
-        ctx.csr, ctx.symmetric = csr, symmetric
+    def forward(ctx, values, x, csr: CSR, row: torch.Tensor, col: torch.Tensor, symmetric: bool,
+                block_size: int = 1):
+        ctx.csr, ctx.symmetric, ctx.bs = csr, symmetric, block_size
         ctx.save_for_backward(values, x, row, col)
-        y = _spmv_raw(csr, values, x)
+        y = _spmv_raw(csr, values, x, block_size)
         return y.to(torch.promote_types(values.dtype, x.dtype))
 
     @staticmethod
@@ -71,21 +76,22 @@ class SpMV(torch.autograd.Function):
             gv = (g[row] * x[col]).to(values.dtype)
         if ctx.needs_input_grad[1]:
             if ctx.symmetric:
-                gx = _spmv_raw(ctx.csr, values, g)
+                gx = _spmv_raw(ctx.csr, values, g, ctx.bs)
             else:
+                assert ctx.bs == 1, "transpose products of non-symmetric block matrices are not supported"
                 csr_t, vt = _transpose_csr(ctx.csr, values)
                 gx = _spmv_raw(csr_t, vt, g)
             gx = gx.to(x.dtype)
-        return gv, gx, None, None, None, None
+        return gv, gx, None, None, None, None, None
 
 
 def pcg_raw(csr: CSR, values: torch.Tensor, b: torch.Tensor, x0: Optional[torch.Tensor] = None,
             rtol: float = 1e-8, atol: float = 0.0, maxiter: int = 10000, check_every: int = 0,
-            raise_on_fail: bool = False):
+            raise_on_fail: bool = False, block_size: int = 1):
     """Jacobi-PCG on the device; returns ``(x [f64], iterations, relres, converged)``."""
     _lib.require_gpu(values, "matrix values")
     dev = values.device
-    n = csr.n_rows
+    n = csr.n_rows * block_size
     b64 = b.detach().to(torch.float64).contiguous()
     x = (torch.zeros(n, dtype=torch.float64, device=dev) if x0 is None
          else x0.detach().to(torch.float64).clone().contiguous())
@@ -93,7 +99,8 @@ def pcg_raw(csr: CSR, values: torch.Tensor, b: torch.Tensor, x0: Optional[torch.
     work = torch.empty(int(L.tm_pcg_work_size(n)) // 8 + 1, dtype=torch.float64, device=dev)
     it = ctypes.c_int64(0)
     rr = ctypes.c_double(0.0)
-    st = L.tm_pcg_jacobi(n, _lib.ptr(csr.row_ptr), _lib.ptr(csr.col_idx), _lib.ptr(values.detach().contiguous()),
+    fn = L.tm_pcg_jacobi if block_size == 1 else L.tm_pcg_jacobi_bsr3
+    st = fn(csr.n_rows, _lib.ptr(csr.row_ptr), _lib.ptr(csr.col_idx), _lib.ptr(values.detach().contiguous()),
                          _lib.dtype_code(values), _lib.ptr(b64), _lib.ptr(x), _lib.ptr(work), float(rtol),
                          float(atol), int(maxiter), int(check_every), ctypes.byref(it), ctypes.byref(rr),
                          _lib.stream_of(dev))
@@ -107,10 +114,10 @@ def pcg_raw(csr: CSR, values: torch.Tensor, b: torch.Tensor, x0: Optional[torch.
 
 class PCGSolve(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, values, b, csr: CSR, row, col, rtol, atol, maxiter, info: dict):
-        x, it, rel, ok = pcg_raw(csr, values, b, None, rtol, atol, maxiter)
+    def forward(ctx, values, b, csr: CSR, row, col, rtol, atol, maxiter, info: dict, block_size: int = 1):
+        x, it, rel, ok = pcg_raw(csr, values, b, None, rtol, atol, maxiter, block_size=block_size)
         info.update(iterations=it, relres=rel, converged=ok)
-        ctx.csr, ctx.tol = csr, (rtol, atol, maxiter)
+        ctx.csr, ctx.tol, ctx.bs = csr, (rtol, atol, maxiter), block_size
         ctx.save_for_backward(values, x, row, col)
         return x.to(torch.promote_types(values.dtype, b.dtype))
 
@@ -118,10 +125,11 @@ class PCGSolve(torch.autograd.Function):
     def backward(ctx, g):
         values, x, row, col = ctx.saved_tensors
         rtol, atol, maxiter = ctx.tol
-        lam, _, _, _ = pcg_raw(ctx.csr, values, g, None, rtol, atol, maxiter)  # A symmetric: A^-T = A^-1
+        lam, _, _, _ = pcg_raw(ctx.csr, values, g, None, rtol, atol, maxiter,
+                               block_size=ctx.bs)  # A symmetric: A^-T = A^-1
         gv = (-lam[row] * x[col]).to(values.dtype) if ctx.needs_input_grad[0] else None
         gb = lam.to(g.dtype) if ctx.needs_input_grad[1] else None
-        return gv, gb, None, None, None, None, None, None, None
+        return gv, gb, None, None, None, None, None, None, None, None
 
 
 def gather_raw(src: torch.Tensor, idx: torch.Tensor) -> torch.Tensor:
