@@ -56,7 +56,7 @@ class ElementAssembler(nn.Module):
         self.topology = topology
         self.quadrature_order = int(quadrature_order)
         self.elements = BufferDict({k: v.long() for k, v in elements.items()})
-        self.register_buffer("edges", topology.edges.to(points.device))
+        self.register_buffer("edges", topology.edges_on(points.device))
         self.register_buffer("points", points)
         self.element_types = list(elements.keys())
         self.dimension = ELEMENT_DIM[self.element_types[0]]
@@ -206,7 +206,8 @@ class ElementAssembler(nn.Module):
                     values = self._launch_matrix(kind, p64, coef, scale, self.value_dtype)
             else:
                 values = self._launch_matrix(kind, p64, None, scale, self.value_dtype)
-            return SparseMatrix._from_csr(values, self.edges[0], self.edges[1], (self.n_points, self.n_points),
+            e = self.topology.edges_on(dev)
+            return SparseMatrix._from_csr(values, e[0], e[1], (self.n_points, self.n_points),
                                           self.csr(dev), symmetric=True)
         return self._generic(fn, points, conn, point_data, element_data, scalar_data)
 
@@ -237,16 +238,17 @@ class ElementAssembler(nn.Module):
         slots = self.projector.to(points.device)
         E, nb = conn.shape
         n = self.n_points
+        edges = self.topology.edges_on(points.device)
         if integ.dim() == 3:
             vals = torch.zeros(self.topology.nnz, dtype=integ.dtype, device=points.device)
             vals = vals.index_add(0, slots, integ.reshape(-1))
-            return SparseMatrix._from_csr(vals, self.edges[0], self.edges[1], (n, n), self.csr(points.device)) \
-                if points.is_cuda else SparseMatrix(vals, self.edges[0], self.edges[1], (n, n))
+            return SparseMatrix._from_csr(vals, edges[0], edges[1], (n, n), self.csr(points.device)) \
+                if points.is_cuda else SparseMatrix(vals, edges[0], edges[1], (n, n))
         if integ.dim() == 5:
             H = integ.shape[-1]
             vals = torch.zeros((self.topology.nnz, H, H), dtype=integ.dtype, device=points.device)
             vals = vals.index_add(0, slots, integ.reshape(E * nb * nb, H, H))
-            return SparseMatrix.from_block_coo(vals, self.edges[0], self.edges[1], (n, n))
+            return SparseMatrix.from_block_coo(vals, edges[0], edges[1], (n, n))
         raise Exception(f"the shape of integral is supposed to be  1D or 3D, but got {integ.shape}")
 
     def __repr__(self):

@@ -90,9 +90,20 @@ class P1Topology:
         self._host_plans: Dict[tuple, tuple] = {}
         self._dev_plans: Dict[tuple, DevicePlan] = {}
         self._dev_csr: Dict[torch.device, Tuple[torch.Tensor, torch.Tensor]] = {}
+        self._dev_edges: Dict[torch.device, torch.Tensor] = {}
         self._slots: Optional[np.ndarray] = None
 
     # ---- pattern ------------------------------------------------------------
+    def edges_on(self, device) -> torch.Tensor:
+        """int64 ``[2, nnz]`` edges on ``device``, one shared tensor per device so every
+        matrix assembled on this mesh carries the same row/col storage."""
+        device = torch.device(device)
+        if device.type == "cpu":
+            return self.edges
+        if device not in self._dev_edges:
+            self._dev_edges[device] = self.edges.to(device)
+        return self._dev_edges[device]
+
     def csr(self, device) -> Tuple[torch.Tensor, torch.Tensor]:
         device = torch.device(device)
         if device not in self._dev_csr:

@@ -40,7 +40,7 @@ def grf_lattice(shape, spacing: float, length: float, sigma: float = 1.0,
     np.maximum(lam, 0.0, out=lam)
     rng = np.random.default_rng(seed)
     w = rng.standard_normal(big)
-    g = np.fft.irfftn(np.sqrt(lam) * np.fft.rfftn(w), s=big)
+    g = np.fft.irfftn(np.sqrt(lam) * np.fft.rfftn(w), s=big, axes=tuple(range(len(big))))
     sl = tuple(slice(0, s) for s in shape)
     return np.ascontiguousarray(g[sl])
 
