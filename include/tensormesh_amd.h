@@ -98,6 +98,8 @@ typedef struct {
   const int32_t* runs;
   int32_t n_blocks;
   int32_t lds_bytes;
+  int32_t threads; /* workgroup size of the matrix kernel: 256, 512 or 1024 (0 = 256) */
+  int32_t reserved;
 } tm_plan_dev;
 
 /* Reference-element moments of the quadrature rule: wsum = sum_q w_q,

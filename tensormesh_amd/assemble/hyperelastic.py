@@ -47,7 +47,7 @@ class _Residual(torch.autograd.Function):
         u, E_t = ctx.saved_tensors
         E = E_t if ctx.has_E else ctx.E0
         need_E = ctx.has_E and ctx.needs_input_grad[1]
-        y, dE = ctx.model._apply(u, E, g.contiguous().to(torch.float64), want_dE=need_E)
+        y, dE = ctx.model._tangent_action(u, E, g.contiguous().to(torch.float64), want_dE=need_E)
         gu = y.to(u.dtype) if ctx.needs_input_grad[0] else None
         gE = dE.to(E_t.dtype) if need_E else None
         return gu, gE, None, None
@@ -120,7 +120,7 @@ class NeoHookeanModel(ElementAssembler):
                                         _lib.ptr(R), _lib.ptr(en), _lib.stream_of(dev)))
         return kv, R, (en[0] if en is not None else None)
 
-    def _apply(self, u, E, lam, want_dE=False):
+    def _tangent_action(self, u, E, lam, want_dE=False):
         dev = u.device
         u64 = u.detach().to(torch.float64).contiguous()
         Et, Edt, E0 = _e_arg(E.detach().to(dev) if torch.is_tensor(E) else E)
@@ -152,7 +152,7 @@ class NeoHookeanModel(ElementAssembler):
         return _Residual.apply(u, None, self, float(E))
 
     def apply_tangent(self, u, lam, E=None) -> torch.Tensor:
-        return self._apply(u, self._resolve_E(E), lam.to(torch.float64).contiguous())[0]
+        return self._tangent_action(u, self._resolve_E(E), lam.to(torch.float64).contiguous())[0]
 
     def tangent(self, u, E=None, value_dtype: Optional[torch.dtype] = None) -> SparseMatrix:
         """Tangent ``dR/du`` as a ``[3N, 3N]`` block matrix (3x3 blocks per scalar edge)."""

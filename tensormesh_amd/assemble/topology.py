@@ -49,7 +49,8 @@ class DevicePlan:
         t = self.tensors
         self.struct = _lib.PlanDev(
             t["blk"].data_ptr(), t["halo"].data_ptr(), t["meta"].data_ptr(), t["recs"].data_ptr(),
-            t["rec_elem"].data_ptr(), t["runs"].data_ptr(), int(info[0]), int(info[4]))
+            t["rec_elem"].data_ptr(), t["runs"].data_ptr(), int(info[0]), int(info[4]),
+            int(os.environ.get("TM_ASM_THREADS", 256)), 0)
 
     @property
     def n_blocks(self) -> int:

@@ -303,10 +303,24 @@ int make_blocks(const PlanCtx& c, Scratch& s, std::vector<int32_t> owned,
   uint32_t lrp = 0;
   for (int a = 0; a < nown; ++a) {
     int32_t g = owned[a];
-    int32_t dr = find_in_row(c.row_ptr, c.col_idx, g, g);
-    TM_REQUIRE(dr >= 0 && dr < 256, TM_ERR_LIMIT, "row %d has no diagonal or rank >= 256", g);
+    const int32_t len = c.row_ptr[g + 1] - c.row_ptr[g];
+    int32_t dr = 0;  // a node in no element has an empty row
+    if (len > 0) {
+      dr = find_in_row(c.row_ptr, c.col_idx, g, g);
+      TM_REQUIRE(dr >= 0 && dr < 256, TM_ERR_LIMIT, "row %d has no diagonal or rank >= 256", g);
+    }
     B.meta[a] = lrp | ((uint32_t)dr << 16);
-    lrp += (uint32_t)(c.row_ptr[g + 1] - c.row_ptr[g]);
+    lrp += (uint32_t)len;
+  }
+  // Record order: element j*q + k (q = ceil(n/64)) goes to position 64*k + j,
+  // so the 64 lanes of a wave take elements q apart.  Neighbouring elements
+  // share nodes; spreading them keeps a wave's LDS atomics on distinct rows.
+  {
+    const size_t ne = elems.size(), qq = (ne + 63) / 64;
+    std::vector<std::pair<size_t, int32_t>> key(ne);
+    for (size_t i = 0; i < ne; ++i) key[i] = {(i % qq) * 64 + i / qq, elems[i]};
+    std::sort(key.begin(), key.end());
+    for (size_t i = 0; i < ne; ++i) elems[i] = key[i].second;
   }
   // element records: 4 x 10-bit local ids, then for each owned vertex a the
   // in-row ranks of the other vertices (6 bits each, a-major, b ascending)

@@ -11,6 +11,7 @@
 // not depend on arrival order).  Scalars live in device memory; kernels return
 // immediately once the device-side status flag is set, so the host only reads
 // the flag every few iterations.
+#include <cstdlib>
 #include <vector>
 
 #include "tm_device.h"
@@ -21,12 +22,12 @@ using tmx::kWave;
 constexpr int kThreads = 256;
 constexpr int kMaxBlocks = 1024;
 
-// Device-side PCG state (doubles), followed by integer flags.
+// Device-side PCG state.  Scalars are produced by one designated workgroup of
+// a kernel and only consumed by later kernels (kernel boundaries order them).
 struct PcgScalars {
-  double rz, pq, rr, alpha, beta, bb, tol2, relres;
+  double rz, rz_prev, bb, tol2, relres;
 };
 struct PcgFlags {
-  unsigned int counter[2];
   int status;  // 0 running, 1 converged, 2 breakdown (p.Ap <= 0), 3 max iterations
   int iter;
   long long maxiter;
@@ -37,7 +38,7 @@ struct PcgWork {
   double* p;
   double* q;
   double* dinv;
-  double* part;  // [3][kMaxBlocks]
+  double* part;  // [4][kMaxBlocks]: (p,q) | (r,z) | (r,r) | (b,b)
   PcgScalars* s;
   PcgFlags* f;
 };
@@ -50,78 +51,103 @@ PcgWork carve_work(void* work, int64_t n) {
   w.q = d + 2 * n;
   w.dinv = d + 3 * n;
   w.part = d + 4 * n;
-  w.s = (PcgScalars*)(w.part + 3 * kMaxBlocks);
+  w.s = (PcgScalars*)(w.part + 4 * kMaxBlocks);
   w.f = (PcgFlags*)(w.s + 1);
   return w;
 }
 
-// out[i] = sum over the block row of B[i][j] x[col*BS + j], reduced over the S
-// lanes that share the row (BS = 1: plain CSR; BS = 3: 3x3 blocks stored
-// row-major per CSR entry).
+constexpr int P_PQ = 0, P_RZ = 1, P_RR = 2, P_BB = 3;
+
+// out[i] += sum over entries of one block row, 4-deep unrolled so a lane keeps
+// several independent (col, value, x) load chains in flight.
 template <typename VT, int S, int BS>
-__device__ __forceinline__ void row_dot(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
-                                        const VT* __restrict__ v, const double* __restrict__ x, int64_t row,
-                                        int sl, double (&out)[BS]) {
+__device__ __forceinline__ void row_accum(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                          const VT* __restrict__ v, const double* __restrict__ x, int64_t row,
+                                          int sl, double (&out)[BS]) {
   const int b = rp[row], e = rp[row + 1];
+  int k = b + sl;
+  for (; k + 3 * S < e; k += 4 * S) {
+    int64_t c[4];
 #pragma unroll
-  for (int i = 0; i < BS; ++i) out[i] = 0.0;
-  for (int k = b + sl; k < e; k += S) {
+    for (int u = 0; u < 4; ++u) c[u] = ci[k + u * S];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int i = 0; i < BS; ++i)
+#pragma unroll
+        for (int j = 0; j < BS; ++j)
+          out[i] += (double)v[(int64_t)(k + u * S) * BS * BS + i * BS + j] * x[c[u] * BS + j];
+  }
+  for (; k < e; k += S) {
     const int64_t c = ci[k];
-    double xc[BS];
-#pragma unroll
-    for (int j = 0; j < BS; ++j) xc[j] = x[c * BS + j];
 #pragma unroll
     for (int i = 0; i < BS; ++i)
 #pragma unroll
-      for (int j = 0; j < BS; ++j) out[i] += (double)v[(int64_t)k * BS * BS + i * BS + j] * xc[j];
+      for (int j = 0; j < BS; ++j) out[i] += (double)v[(int64_t)k * BS * BS + i * BS + j] * x[c * BS + j];
   }
+}
+
+template <int S, int BS>
+__device__ __forceinline__ void sub_reduce(double (&o)[BS]) {
 #pragma unroll
   for (int i = 0; i < BS; ++i)
 #pragma unroll
-    for (int o = S / 2; o > 0; o >>= 1) out[i] += __shfl_xor(out[i], o, kWave);
+    for (int d = S / 2; d > 0; d >>= 1) o[i] += __shfl_xor(o[i], d, kWave);
 }
 
-template <typename VT, int S, int BS>
+// y = A x; each S-lane group handles R rows per trip (rows row, row + stride, ...).
+template <typename VT, int S, int BS, int R>
 __global__ __launch_bounds__(kThreads) void k_spmv(int64_t nrows, const int32_t* __restrict__ rp,
                                                   const int32_t* __restrict__ ci, const VT* __restrict__ v,
                                                   const double* __restrict__ x, double* __restrict__ y) {
   const int sl = threadIdx.x % S;
-  const int64_t per = kThreads / S;
-  for (int64_t row = blockIdx.x * per + threadIdx.x / S; row < nrows; row += (int64_t)gridDim.x * per) {
-    double o[BS];
-    row_dot<VT, S, BS>(rp, ci, v, x, row, sl, o);
-    if (sl == 0)
+  const int64_t stride = (int64_t)gridDim.x * (kThreads / S);
+  for (int64_t row = blockIdx.x * (kThreads / S) + threadIdx.x / S; row < nrows; row += stride * R) {
+    double o[R][BS];
 #pragma unroll
-      for (int i = 0; i < BS; ++i) y[row * BS + i] = o[i];
+    for (int r = 0; r < R; ++r) {
+#pragma unroll
+      for (int i = 0; i < BS; ++i) o[r][i] = 0.0;
+      if (row + r * stride < nrows) row_accum<VT, S, BS>(rp, ci, v, x, row + r * stride, sl, o[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      sub_reduce<S, BS>(o[r]);
+      if (sl == 0 && row + r * stride < nrows)
+#pragma unroll
+        for (int i = 0; i < BS; ++i) y[(row + r * stride) * BS + i] = o[r][i];
+    }
   }
 }
 
-// Last workgroup to arrive finishes the reduction of NS partial sums.
+// Sum of the first n entries of each of NS partial arrays, computed by wave 0 in
+// a fixed order (lane-strided, then a shuffle tree) and broadcast through LDS:
+// every workgroup of a kernel obtains bit-identical values.
 template <int NS>
-__device__ __forceinline__ bool finish_reduce(double (&local)[NS], double* part, unsigned int* counter,
-                                              double (&tot)[NS]) {
-  __shared__ double red[kThreads / kWave];
-  __shared__ bool last;
+__device__ __forceinline__ void sum_parts(const double* part, const int (&which)[NS], int n, double (&out)[NS]) {
+  __shared__ double sh[NS];
+  if (threadIdx.x < kWave) {
 #pragma unroll
-  for (int i = 0; i < NS; ++i) {
-    const double b = tmx::block_sum<kThreads>(local[i], red);
-    if (threadIdx.x == 0) part[i * kMaxBlocks + blockIdx.x] = b;
-  }
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    last = (t == gridDim.x - 1);
+    for (int i = 0; i < NS; ++i) {
+      double s = 0.0;
+      for (int k = threadIdx.x; k < n; k += kWave) s += part[which[i] * kMaxBlocks + k];
+      s = tmx::wave_sum(s);
+      if (threadIdx.x == 0) sh[i] = s;
+    }
   }
   __syncthreads();
-  if (!last) return false;
+#pragma unroll
+  for (int i = 0; i < NS; ++i) out[i] = sh[i];
+}
+
+template <int NS>
+__device__ __forceinline__ void write_parts(double* part, const int (&which)[NS], double (&loc)[NS]) {
+  __shared__ double red[kThreads / kWave];
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
-    double s = 0.0;
-    for (int k = threadIdx.x; k < (int)gridDim.x; k += kThreads) s += tmx::load_agent(part + i * kMaxBlocks + k);
-    tot[i] = tmx::block_sum<kThreads>(s, red);
+    const double b = tmx::block_sum<kThreads>(loc[i], red);
+    if (threadIdx.x == 0) part[which[i] * kMaxBlocks + blockIdx.x] = b;
   }
-  if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return true;
 }
 
 template <typename VT, int BS>
@@ -147,9 +173,8 @@ __global__ __launch_bounds__(kThreads) void k_diag_inv(int64_t nrows, const int3
   }
 }
 
-// r = b - q (q = A x0), p = D^-1 r, rz = (r, p), rr = (r, r), bb = (b, b)
-__global__ __launch_bounds__(kThreads) void k_pcg_init(int64_t n, const double* __restrict__ b, PcgWork w,
-                                                      double rtol2, double atol2) {
+// r = b - q (q = A x0), p = D^-1 r; partials (r,z), (r,r), (b,b)
+__global__ __launch_bounds__(kThreads) void k_pcg_init(int64_t n, const double* __restrict__ b, PcgWork w) {
   double loc[3] = {0.0, 0.0, 0.0};
   for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
     const double bi = b[i];
@@ -161,88 +186,111 @@ __global__ __launch_bounds__(kThreads) void k_pcg_init(int64_t n, const double* 
     loc[1] += ri * ri;
     loc[2] += bi * bi;
   }
-  double tot[3];
-  if (finish_reduce<3>(loc, w.part, &w.f->counter[0], tot) && threadIdx.x == 0) {
-    PcgScalars* s = w.s;
-    s->rz = tot[0];
-    s->rr = tot[1];
-    s->bb = tot[2];
-    s->tol2 = fmax(rtol2 * tot[2], atol2);
-    s->relres = tot[2] > 0.0 ? sqrt(tot[1] / tot[2]) : 0.0;
+  const int which[3] = {P_RZ, P_RR, P_BB};
+  write_parts<3>(w.part, which, loc);
+}
+
+__global__ __launch_bounds__(kThreads) void k_pcg_ctl0(PcgWork w, int nparts, double rtol2, double atol2,
+                                                      long long maxiter) {
+  const int which[3] = {P_RZ, P_RR, P_BB};
+  double t[3];
+  sum_parts<3>(w.part, which, nparts, t);
+  if (threadIdx.x == 0) {
+    w.s->rz = t[0];
+    w.s->rz_prev = t[0];
+    w.s->bb = t[2];
+    w.s->tol2 = fmax(rtol2 * t[2], atol2);
+    w.s->relres = t[2] > 0.0 ? sqrt(t[1] / t[2]) : 0.0;
     w.f->iter = 0;
-    w.f->status = (tot[1] <= s->tol2) ? 1 : 0;
+    w.f->maxiter = maxiter;
+    w.f->status = (t[1] <= w.s->tol2) ? 1 : 0;
   }
 }
 
+// K1: q = A p, partial (p, q)
 template <typename VT, int S, int BS>
 __global__ __launch_bounds__(kThreads) void k_pcg_spmv(int64_t nrows, const int32_t* __restrict__ rp,
                                                       const int32_t* __restrict__ ci, const VT* __restrict__ v,
                                                       PcgWork w) {
   if (w.f->status != 0) return;
+  constexpr int R = 2;
   const int sl = threadIdx.x % S;
-  const int64_t per = kThreads / S;
+  const int64_t stride = (int64_t)gridDim.x * (kThreads / S);
   double loc[1] = {0.0};
-  for (int64_t row = blockIdx.x * per + threadIdx.x / S; row < nrows; row += (int64_t)gridDim.x * per) {
-    double o[BS];
-    row_dot<VT, S, BS>(rp, ci, v, w.p, row, sl, o);
-    if (sl == 0) {
+  for (int64_t row = blockIdx.x * (kThreads / S) + threadIdx.x / S; row < nrows; row += stride * R) {
+    double o[R][BS];
 #pragma unroll
-      for (int i = 0; i < BS; ++i) {
-        w.q[row * BS + i] = o[i];
-        loc[0] += o[i] * w.p[row * BS + i];
-      }
+    for (int r = 0; r < R; ++r) {
+#pragma unroll
+      for (int i = 0; i < BS; ++i) o[r][i] = 0.0;
+      if (row + r * stride < nrows) row_accum<VT, S, BS>(rp, ci, v, w.p, row + r * stride, sl, o[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      sub_reduce<S, BS>(o[r]);
+      const int64_t rr = row + r * stride;
+      if (sl == 0 && rr < nrows)
+#pragma unroll
+        for (int i = 0; i < BS; ++i) {
+          w.q[rr * BS + i] = o[r][i];
+          loc[0] += o[r][i] * w.p[rr * BS + i];
+        }
     }
   }
-  double tot[1];
-  if (finish_reduce<1>(loc, w.part, &w.f->counter[0], tot) && threadIdx.x == 0) {
-    w.s->pq = tot[0];
-    if (!(tot[0] > 0.0)) {
-      w.f->status = 2;
-    } else {
-      w.s->alpha = w.s->rz / tot[0];
-    }
-  }
+  const int which[1] = {P_PQ};
+  write_parts<1>(w.part, which, loc);
 }
 
-__global__ __launch_bounds__(kThreads) void k_pcg_update(int64_t n, double* __restrict__ x, PcgWork w) {
+// K2: alpha = rz / pq, x += alpha p, r -= alpha q, partials (r, z), (r, r)
+__global__ __launch_bounds__(kThreads) void k_pcg_update(int64_t n, double* __restrict__ x, PcgWork w, int nparts) {
   if (w.f->status != 0) return;
-  const double alpha = w.s->alpha;
-  double loc[2] = {0.0, 0.0};
-  for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-    x[i] += alpha * w.p[i];
-    const double ri = w.r[i] - alpha * w.q[i];
-    w.r[i] = ri;
-    loc[0] += ri * w.dinv[i] * ri;
-    loc[1] += ri * ri;
+  const int which_in[1] = {P_PQ};
+  double pq[1];
+  sum_parts<1>(w.part, which_in, nparts, pq);
+  const double rz = w.s->rz;
+  const bool ok = pq[0] > 0.0;
+  const double alpha = ok ? rz / pq[0] : 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    w.s->rz_prev = rz;
+    if (!ok) w.f->status = 2;
   }
-  double tot[2];
-  if (finish_reduce<2>(loc, w.part, &w.f->counter[1], tot) && threadIdx.x == 0) {
-    PcgScalars* s = w.s;
-    s->beta = tot[0] / s->rz;
-    s->rz = tot[0];
-    s->rr = tot[1];
-    s->relres = s->bb > 0.0 ? sqrt(tot[1] / s->bb) : 0.0;
-    const int it = ++w.f->iter;
-    if (tot[1] <= s->tol2)
+  double loc[2] = {0.0, 0.0};
+  if (ok) {
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+      x[i] += alpha * w.p[i];
+      const double ri = w.r[i] - alpha * w.q[i];
+      w.r[i] = ri;
+      loc[0] += ri * w.dinv[i] * ri;
+      loc[1] += ri * ri;
+    }
+  }
+  const int which[2] = {P_RZ, P_RR};
+  write_parts<2>(w.part, which, loc);
+}
+
+// K3: beta = rz_new / rz_old, convergence test, p = D^-1 r + beta p
+__global__ __launch_bounds__(kThreads) void k_pcg_dir(int64_t n, PcgWork w, int nparts) {
+  if (w.f->status != 0) return;
+  const int which[2] = {P_RZ, P_RR};
+  double t[2];
+  sum_parts<2>(w.part, which, nparts, t);
+  const double rz_new = t[0], rr = t[1];
+  const double tol2 = w.s->tol2;
+  const bool conv = rr <= tol2;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const int it = w.f->iter + 1;
+    w.f->iter = it;
+    w.s->rz = rz_new;
+    w.s->relres = w.s->bb > 0.0 ? sqrt(rr / w.s->bb) : 0.0;
+    if (conv)
       w.f->status = 1;
     else if (it >= w.f->maxiter)
       w.f->status = 3;
   }
-}
-
-__global__ __launch_bounds__(kThreads) void k_pcg_dir(int64_t n, PcgWork w) {
-  if (w.f->status != 0) return;
-  const double beta = w.s->beta;
+  if (conv) return;
+  const double beta = rz_new / w.s->rz_prev;
   for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
     w.p[i] = w.dinv[i] * w.r[i] + beta * w.p[i];
-}
-
-__global__ void k_pcg_setmax(PcgWork w, long long maxiter) {
-  w.f->counter[0] = 0;
-  w.f->counter[1] = 0;
-  w.f->maxiter = maxiter;
-  w.f->status = 0;
-  w.f->iter = 0;
 }
 
 template <typename T>
@@ -262,13 +310,13 @@ int pcg_run(int64_t nrows, const int32_t* rp, const int32_t* ci, const VT* v, co
             double rtol, double atol, int64_t maxiter, int check_every, int64_t* iters, double* relres,
             hipStream_t st) {
   const int64_t n = nrows * BS;
-  const int gv = grid_for(n, kThreads * 4);
-  const int gs = grid_for(nrows, kThreads / S);
-  hipLaunchKernelGGL(k_pcg_setmax, dim3(1), dim3(1), 0, st, w, (long long)maxiter);
+  // one grid size for every PCG kernel: the partial arrays have G entries
+  const int G = grid_for(n, kThreads * 2);
   hipLaunchKernelGGL((k_diag_inv<VT, BS>), dim3(grid_for(nrows, kThreads)), dim3(kThreads), 0, st, nrows, rp, ci, v,
                      w.dinv);
-  hipLaunchKernelGGL((k_spmv<VT, S, BS>), dim3(gs), dim3(kThreads), 0, st, nrows, rp, ci, v, x, w.q);
-  hipLaunchKernelGGL(k_pcg_init, dim3(gv), dim3(kThreads), 0, st, n, b, w, rtol * rtol, atol * atol);
+  hipLaunchKernelGGL((k_spmv<VT, S, BS, 2>), dim3(G), dim3(kThreads), 0, st, nrows, rp, ci, v, x, w.q);
+  hipLaunchKernelGGL(k_pcg_init, dim3(G), dim3(kThreads), 0, st, n, b, w);
+  hipLaunchKernelGGL(k_pcg_ctl0, dim3(1), dim3(kThreads), 0, st, w, G, rtol * rtol, atol * atol, (long long)maxiter);
   TM_HIP(hipGetLastError());
   PcgFlags hf{};
   PcgScalars hs{};
@@ -278,9 +326,9 @@ int pcg_run(int64_t nrows, const int32_t* rp, const int32_t* ci, const VT* v, co
     TM_HIP(hipStreamSynchronize(st));
     if (hf.status != 0) break;
     for (int k = 0; k < chunk; ++k) {
-      hipLaunchKernelGGL((k_pcg_spmv<VT, S, BS>), dim3(gs), dim3(kThreads), 0, st, nrows, rp, ci, v, w);
-      hipLaunchKernelGGL(k_pcg_update, dim3(gv), dim3(kThreads), 0, st, n, x, w);
-      hipLaunchKernelGGL(k_pcg_dir, dim3(gv), dim3(kThreads), 0, st, n, w);
+      hipLaunchKernelGGL((k_pcg_spmv<VT, S, BS>), dim3(G), dim3(kThreads), 0, st, nrows, rp, ci, v, w);
+      hipLaunchKernelGGL(k_pcg_update, dim3(G), dim3(kThreads), 0, st, n, x, w, G);
+      hipLaunchKernelGGL(k_pcg_dir, dim3(G), dim3(kThreads), 0, st, n, w, G);
     }
     TM_HIP(hipGetLastError());
     if (check_every <= 0 && chunk < 256) chunk *= 2;
@@ -300,9 +348,16 @@ int pcg_run(int64_t nrows, const int32_t* rp, const int32_t* ci, const VT* v, co
   return TM_OK;
 }
 
-// Lanes per row: the smallest power of two covering the mean row length.
+// Lanes per row: TM_SPMV_LANES when set (4, 8, 16, 32), otherwise the smallest
+// power of two covering half the mean row length (each lane then keeps two or
+// more independent loads in flight).
 int lanes_for(int64_t n, int64_t nnz) {
-  const int64_t avg = n > 0 ? (nnz + n - 1) / n : 1;
+  static const int forced = [] {
+    const char* e = getenv("TM_SPMV_LANES");
+    return e ? atoi(e) : 0;
+  }();
+  if (forced == 4 || forced == 8 || forced == 16 || forced == 32) return forced;
+  const int64_t avg = n > 0 ? (nnz + 2 * n - 1) / (2 * n) : 1;
   if (avg <= 4) return 4;
   if (avg <= 8) return 8;
   if (avg <= 16) return 16;
@@ -313,12 +368,12 @@ template <typename VT, int BS>
 int spmv_dispatch(int64_t n, int64_t nnz, const int32_t* rp, const int32_t* ci, const VT* v, const double* x,
                   double* y, hipStream_t st) {
   const int S = lanes_for(n, nnz);
-  const int g = grid_for(n, kThreads / S);
+  const int g = grid_for(n, kThreads / S * 2);
   switch (S) {
-    case 4: hipLaunchKernelGGL((k_spmv<VT, 4, BS>), dim3(g), dim3(kThreads), 0, st, n, rp, ci, v, x, y); break;
-    case 8: hipLaunchKernelGGL((k_spmv<VT, 8, BS>), dim3(g), dim3(kThreads), 0, st, n, rp, ci, v, x, y); break;
-    case 16: hipLaunchKernelGGL((k_spmv<VT, 16, BS>), dim3(g), dim3(kThreads), 0, st, n, rp, ci, v, x, y); break;
-    default: hipLaunchKernelGGL((k_spmv<VT, 32, BS>), dim3(g), dim3(kThreads), 0, st, n, rp, ci, v, x, y); break;
+    case 4: hipLaunchKernelGGL((k_spmv<VT, 4, BS, 2>), dim3(g), dim3(kThreads), 0, st, n, rp, ci, v, x, y); break;
+    case 8: hipLaunchKernelGGL((k_spmv<VT, 8, BS, 2>), dim3(g), dim3(kThreads), 0, st, n, rp, ci, v, x, y); break;
+    case 16: hipLaunchKernelGGL((k_spmv<VT, 16, BS, 2>), dim3(g), dim3(kThreads), 0, st, n, rp, ci, v, x, y); break;
+    default: hipLaunchKernelGGL((k_spmv<VT, 32, BS, 2>), dim3(g), dim3(kThreads), 0, st, n, rp, ci, v, x, y); break;
   }
   TM_HIP(hipGetLastError());
   return TM_OK;
@@ -379,7 +434,7 @@ extern "C" int tm_gather(int64_t n, const int32_t* idx, const void* src, void* d
 }
 
 extern "C" int64_t tm_pcg_work_size(int64_t n) {
-  return (int64_t)sizeof(double) * (4 * n + 3 * kMaxBlocks) + (int64_t)sizeof(PcgScalars) +
+  return (int64_t)sizeof(double) * (4 * n + 4 * kMaxBlocks) + (int64_t)sizeof(PcgScalars) +
          (int64_t)sizeof(PcgFlags) + 64;
 }
 

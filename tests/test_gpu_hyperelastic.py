@@ -13,7 +13,7 @@ from tensormesh_amd.mesh import Mesh, kuhn_cube_arrays
 pytestmark = pytest.mark.gpu
 
 
-def _case(n=4, seed=0, amp=0.04):
+def _case(n=4, seed=0, amp=0.01):
     pts, conn, bnd = kuhn_cube_arrays(n)
     rng = np.random.default_rng(seed)
     pts = pts + (~bnd)[:, None] * rng.uniform(-0.1, 0.1, pts.shape) / n
