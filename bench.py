@@ -68,6 +68,8 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=40)
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="MiB of a plain device copy run before the timed region (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -101,6 +103,13 @@ def main():
     pd = {"kappa": kappa}
     E, N, nnz, nb = len(conn), len(pts), asm.nnz, 4
 
+    if args.calibrate_copy:
+        src = torch.empty(args.calibrate_copy << 18, dtype=torch.float32, device=dev).uniform_()
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calibrate_copy} MiB read + {args.calibrate_copy} MiB written")
+        del src, dst
     for _ in range(args.warmup):
         K = asm(point_data=pd)
     torch.cuda.synchronize()

@@ -239,10 +239,29 @@ __global__ __launch_bounds__(NT) void k_asm_matrix(PlanArgs p, const double* __r
     }
     bool own[NB];
     uint32_t m[NB];
+    bool all = true;
 #pragma unroll
     for (int a = 0; a < NB; ++a) {
       own[a] = l[a] < nown;
+      all = all && own[a];
       m[a] = own[a] ? L.ms[l[a]] : 0u;
+    }
+    if (__all(all)) {
+      // every vertex owned in every lane: no branches (records are sorted so
+      // that the block's interior elements form whole waves)
+      if (!ROWSUM) {
+#pragma unroll
+        for (int a = 0; a < NB; ++a) atomicAdd(L.acc + (m[a] & 0xffffu) + (m[a] >> 16), s * q.M[a * NB + a]);
+      }
+#pragma unroll
+      for (int a = 0; a < NB; ++a)
+#pragma unroll
+        for (int bb = a + 1; bb < NB; ++bb) {
+          const double kab = FORM == 2 ? s * q.M[a * NB + bb] : s * dotc<DIM>(c[a], c[bb]);
+          atomicAdd(L.acc + (m[a] & 0xffffu) + rec_rank<NB>(lo, hi, a, bb), kab);
+          atomicAdd(L.acc + (m[bb] & 0xffffu) + rec_rank<NB>(lo, hi, bb, a), kab);
+        }
+      continue;
     }
     if (!ROWSUM) {
 #pragma unroll

@@ -312,15 +312,33 @@ int make_blocks(const PlanCtx& c, Scratch& s, std::vector<int32_t> owned,
     B.meta[a] = lrp | ((uint32_t)dr << 16);
     lrp += (uint32_t)len;
   }
-  // Record order: element j*q + k (q = ceil(n/64)) goes to position 64*k + j,
-  // so the 64 lanes of a wave take elements q apart.  Neighbouring elements
-  // share nodes; spreading them keeps a wave's LDS atomics on distinct rows.
+  // Record order.  Elements whose vertices are all owned come first (a wave
+  // of them runs a branch-free path), then the rest grouped by owned mask.
+  // Inside a group, element j*q + k (q = ceil(n/64)) goes to position 64*k + j,
+  // so the 64 lanes of a wave take elements q apart: neighbouring elements
+  // share nodes, and spreading them keeps a wave's LDS atomics on distinct rows.
   {
-    const size_t ne = elems.size(), qq = (ne + 63) / 64;
-    std::vector<std::pair<size_t, int32_t>> key(ne);
-    for (size_t i = 0; i < ne; ++i) key[i] = {(i % qq) * 64 + i / qq, elems[i]};
-    std::sort(key.begin(), key.end());
-    for (size_t i = 0; i < ne; ++i) elems[i] = key[i].second;
+    std::vector<std::pair<uint32_t, int32_t>> bym(elems.size());
+    for (size_t i = 0; i < elems.size(); ++i) {
+      uint32_t mask = 0;
+      for (int a = 0; a < c.nb; ++a)
+        if (s.local[c.conn[(int64_t)elems[i] * c.nb + a]] < nown) mask |= 1u << a;
+      const uint32_t full = (1u << c.nb) - 1;
+      bym[i] = {mask == full ? 0u : 16u - (uint32_t)__builtin_popcount(mask) * 4u + mask, elems[i]};
+    }
+    std::stable_sort(bym.begin(), bym.end(),
+                     [](const auto& x, const auto& y) { return x.first < y.first; });
+    size_t g0 = 0;
+    while (g0 < bym.size()) {
+      size_t g1 = g0;
+      while (g1 < bym.size() && bym[g1].first == bym[g0].first) ++g1;
+      const size_t ne = g1 - g0, qq = (ne + 63) / 64;
+      std::vector<std::pair<size_t, int32_t>> key(ne);
+      for (size_t i = 0; i < ne; ++i) key[i] = {(i % qq) * 64 + i / qq, bym[g0 + i].second};
+      std::sort(key.begin(), key.end());
+      for (size_t i = 0; i < ne; ++i) elems[g0 + i] = key[i].second;
+      g0 = g1;
+    }
   }
   // element records: 4 x 10-bit local ids, then for each owned vertex a the
   // in-row ranks of the other vertices (6 bits each, a-major, b ascending)

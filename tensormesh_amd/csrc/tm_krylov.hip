@@ -19,7 +19,7 @@
 namespace {
 
 using tmx::kWave;
-constexpr int kThreads = 256;
+constexpr int kThreads = 512;
 constexpr int kMaxBlocks = 1024;
 
 // Device-side PCG state.  Scalars are produced by one designated workgroup of
@@ -87,6 +87,19 @@ __device__ __forceinline__ void row_accum(const int32_t* __restrict__ rp, const 
   }
 }
 
+// Rows of logical workgroup L: the contiguous range [L*chunk, (L+1)*chunk),
+// with L = XCD-remapped block id, so each XCD sweeps one contiguous slab of
+// rows and the x entries of neighbouring rows stay in its L2.
+struct RowChunk {
+  int64_t r0, r1;
+};
+__device__ __forceinline__ RowChunk row_chunk(int64_t nrows) {
+  const int L = tmx::xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t chunk = (nrows + gridDim.x - 1) / gridDim.x;
+  const int64_t r0 = (int64_t)L * chunk;
+  return RowChunk{r0, r0 + chunk < nrows ? r0 + chunk : nrows};
+}
+
 template <int S, int BS>
 __device__ __forceinline__ void sub_reduce(double (&o)[BS]) {
 #pragma unroll
@@ -101,19 +114,20 @@ __global__ __launch_bounds__(kThreads) void k_spmv(int64_t nrows, const int32_t*
                                                   const int32_t* __restrict__ ci, const VT* __restrict__ v,
                                                   const double* __restrict__ x, double* __restrict__ y) {
   const int sl = threadIdx.x % S;
-  const int64_t stride = (int64_t)gridDim.x * (kThreads / S);
-  for (int64_t row = blockIdx.x * (kThreads / S) + threadIdx.x / S; row < nrows; row += stride * R) {
+  constexpr int64_t stride = kThreads / S;
+  const RowChunk rc = row_chunk(nrows);
+  for (int64_t row = rc.r0 + threadIdx.x / S; row < rc.r1; row += stride * R) {
     double o[R][BS];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
 #pragma unroll
       for (int i = 0; i < BS; ++i) o[r][i] = 0.0;
-      if (row + r * stride < nrows) row_accum<VT, S, BS>(rp, ci, v, x, row + r * stride, sl, o[r]);
+      if (row + r * stride < rc.r1) row_accum<VT, S, BS>(rp, ci, v, x, row + r * stride, sl, o[r]);
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       sub_reduce<S, BS>(o[r]);
-      if (sl == 0 && row + r * stride < nrows)
+      if (sl == 0 && row + r * stride < rc.r1)
 #pragma unroll
         for (int i = 0; i < BS; ++i) y[(row + r * stride) * BS + i] = o[r][i];
     }
@@ -215,21 +229,22 @@ __global__ __launch_bounds__(kThreads) void k_pcg_spmv(int64_t nrows, const int3
   if (w.f->status != 0) return;
   constexpr int R = 2;
   const int sl = threadIdx.x % S;
-  const int64_t stride = (int64_t)gridDim.x * (kThreads / S);
+  constexpr int64_t stride = kThreads / S;
+  const RowChunk rc = row_chunk(nrows);
   double loc[1] = {0.0};
-  for (int64_t row = blockIdx.x * (kThreads / S) + threadIdx.x / S; row < nrows; row += stride * R) {
+  for (int64_t row = rc.r0 + threadIdx.x / S; row < rc.r1; row += stride * R) {
     double o[R][BS];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
 #pragma unroll
       for (int i = 0; i < BS; ++i) o[r][i] = 0.0;
-      if (row + r * stride < nrows) row_accum<VT, S, BS>(rp, ci, v, w.p, row + r * stride, sl, o[r]);
+      if (row + r * stride < rc.r1) row_accum<VT, S, BS>(rp, ci, v, w.p, row + r * stride, sl, o[r]);
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       sub_reduce<S, BS>(o[r]);
       const int64_t rr = row + r * stride;
-      if (sl == 0 && rr < nrows)
+      if (sl == 0 && rr < rc.r1)
 #pragma unroll
         for (int i = 0; i < BS; ++i) {
           w.q[rr * BS + i] = o[r][i];
@@ -356,8 +371,8 @@ int lanes_for(int64_t n, int64_t nnz) {
     const char* e = getenv("TM_SPMV_LANES");
     return e ? atoi(e) : 0;
   }();
-  if (forced == 4 || forced == 8 || forced == 16 || forced == 32) return forced;
-  const int64_t avg = n > 0 ? (nnz + 2 * n - 1) / (2 * n) : 1;
+  if (forced == 2 || forced == 4 || forced == 8 || forced == 16 || forced == 32) return forced;
+  const int64_t avg = n > 0 ? (nnz + 3 * n - 1) / (3 * n) : 1;
   if (avg <= 4) return 4;
   if (avg <= 8) return 8;
   if (avg <= 16) return 16;
@@ -370,6 +385,7 @@ int spmv_dispatch(int64_t n, int64_t nnz, const int32_t* rp, const int32_t* ci, 
   const int S = lanes_for(n, nnz);
   const int g = grid_for(n, kThreads / S * 2);
   switch (S) {
+    case 2: hipLaunchKernelGGL((k_spmv<VT, 2, BS, 2>), dim3(g), dim3(kThreads), 0, st, n, rp, ci, v, x, y); break;
     case 4: hipLaunchKernelGGL((k_spmv<VT, 4, BS, 2>), dim3(g), dim3(kThreads), 0, st, n, rp, ci, v, x, y); break;
     case 8: hipLaunchKernelGGL((k_spmv<VT, 8, BS, 2>), dim3(g), dim3(kThreads), 0, st, n, rp, ci, v, x, y); break;
     case 16: hipLaunchKernelGGL((k_spmv<VT, 16, BS, 2>), dim3(g), dim3(kThreads), 0, st, n, rp, ci, v, x, y); break;
@@ -384,6 +400,7 @@ int pcg_dispatch(int64_t n, int64_t nnz, const int32_t* rp, const int32_t* ci, c
                  double* x, PcgWork w, double rtol, double atol, int64_t maxiter, int check_every,
                  int64_t* iters, double* relres, hipStream_t st) {
   switch (lanes_for(n, nnz)) {
+    case 2: return pcg_run<VT, 2, BS>(n, rp, ci, v, b, x, w, rtol, atol, maxiter, check_every, iters, relres, st);
     case 4: return pcg_run<VT, 4, BS>(n, rp, ci, v, b, x, w, rtol, atol, maxiter, check_every, iters, relres, st);
     case 8: return pcg_run<VT, 8, BS>(n, rp, ci, v, b, x, w, rtol, atol, maxiter, check_every, iters, relres, st);
     case 16: return pcg_run<VT, 16, BS>(n, rp, ci, v, b, x, w, rtol, atol, maxiter, check_every, iters, relres, st);
