@@ -29,15 +29,21 @@ PLAN_KINDS = {"scalar": (8, 32, 16), "vector3": (72, 56, 32)}
 
 
 def _default_plan_params(kind: str, dim: int) -> Tuple[int, int]:
-    """(max owned rows per block, LDS budget in bytes)."""
-    budget = int(os.environ.get("TM_ASM_LDS_BUDGET", 64 * 1024 - 64))
+    """(max owned rows per block, LDS budget in bytes).
+
+    3D scalar: 160 rows in 40 KiB of LDS measured fastest on MI355X for the
+    128^3 Kuhn kappa-Laplace matrix (245 us vs 250-430 us for 224-512 rows
+    and 48-80 KiB): three such blocks of 512 lanes fit one CU, so the LDS
+    atomics of one block overlap the geometry of the others.
+    """
     if kind == "vector3":
-        return int(os.environ.get("TM_ASM_MAX_ROWS_VEC", 64)), budget
+        return (int(os.environ.get("TM_ASM_MAX_ROWS_VEC", 64)),
+                int(os.environ.get("TM_ASM_LDS_BUDGET", 64 * 1024 - 64)))
     if dim == 3:
-        rows = int(os.environ.get("TM_ASM_MAX_ROWS", 384))
-    else:
-        rows = int(os.environ.get("TM_ASM_MAX_ROWS_2D", 640))
-    return rows, budget
+        return (int(os.environ.get("TM_ASM_MAX_ROWS", 160)),
+                int(os.environ.get("TM_ASM_LDS_BUDGET", 40 * 1024)))
+    return (int(os.environ.get("TM_ASM_MAX_ROWS_2D", 640)),
+            int(os.environ.get("TM_ASM_LDS_BUDGET", 64 * 1024 - 64)))
 
 
 class DevicePlan:
@@ -50,7 +56,7 @@ class DevicePlan:
         self.struct = _lib.PlanDev(
             t["blk"].data_ptr(), t["halo"].data_ptr(), t["meta"].data_ptr(), t["recs"].data_ptr(),
             t["rec_elem"].data_ptr(), t["runs"].data_ptr(), int(info[0]), int(info[4]),
-            int(os.environ.get("TM_ASM_THREADS", 256)), 0)
+            int(os.environ.get("TM_ASM_THREADS", 512)), 0)
 
     @property
     def n_blocks(self) -> int:
