@@ -42,6 +42,7 @@ extern "C" {
 /* dtype codes */
 #define TM_F32 0
 #define TM_F64 1
+#define TM_BF16 2
 
 const char* tm_last_error(void);
 int tm_version(void);
@@ -223,6 +224,108 @@ int tm_batched_pcg(const tm_batched_args* args, int lanes, int blocks, void* str
 /* Copies the grid-barrier timeout flag (0 = every barrier completed) of the
  * last launch on this work buffer into a device int32. */
 int tm_batched_timeout_flag(const tm_batched_args* args, int32_t* flag, void* stream);
+
+/* ------------------------------------------- learned preconditioner --- */
+/* No reference counterpart: the reference has no learned preconditioner and
+ * no flexible CG (its Krylov methods are cg_py / bicgstab_py with Jacobi,
+ * sparse/solve/torch_solve.py:70-248).
+ *
+ * Features are node-major [n][64] BF16 (uint16_t bit patterns).  A level is a
+ * CSR graph with normalised values A^ = D^-1/2 A D^-1/2 (symmetric).  Layer
+ * weights: w_t = W^T as [64][128] BF16 for the forward, w = W as [128][64]
+ * BF16 for the backward, bias [64] FP32; channels < n_linear have identity
+ * activation, the rest ReLU.  stop (nullable) points at a device int: kernels
+ * return at once while it is non-zero (the flexible-CG status). */
+int tm_gnn_mp_forward(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values,
+                      const uint16_t* h, const uint16_t* w_t, const float* bias, int n_linear, uint16_t* out,
+                      uint16_t* save_msg, uint16_t* save_pre, const int* stop, void* stream);
+/* FP32 floats of the weight-gradient partial buffer one backward call needs. */
+int64_t tm_gnn_part_floats(void);
+/* g_in = dL/dh from g_out = dL/dh'; grad_w[128*64 + 64] += (dW | db).
+ * g_msg is [n][64] FP32 scratch. */
+int tm_gnn_mp_backward(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values,
+                       const float* g_out, const uint16_t* h, const uint16_t* msg, const uint16_t* pre,
+                       const uint16_t* w, int n_linear, float* g_in, float* g_msg, float* part, float* grad_w,
+                       void* stream);
+/* out[I] = sum over the children i of coarse node I of w_i h[i] (dtype TM_BF16 / TM_F32). */
+int tm_gnn_pool(int64_t n_coarse, const int32_t* child_ptr, const int32_t* child, const float* w, const void* h,
+                void* out, int dtype, const int* stop, void* stream);
+/* out[i] = skip[i] + w_i hc[agg_i] (skip nullable; out may alias skip). */
+int tm_gnn_unpool(int64_t n_fine, const int32_t* agg, const float* w, const void* hc, const void* skip, void* out,
+                  int dtype, const int* stop, void* stream);
+/* Per-workgroup partials of sum (d_i r_i)^2 (for the input scale). */
+int tm_gnn_sumsq(int64_t n, const double* r, const float* d, double* ss_part, int* nparts, void* stream);
+/* h0 = act([d r / s, lk] We + be), s = rms(d r) from the partials; writes s. */
+int tm_gnn_encode(int64_t n, const double* r, const float* d, const float* lk, const double* ss_part, int nparts,
+                  const float* we, const float* be, int n_linear, uint16_t* h0, double* scale, const int* stop,
+                  void* stream);
+int tm_gnn_encode_backward(int64_t n, const double* r, const float* d, const float* lk, const double* scale,
+                           const float* we, const float* be, int n_linear, const float* g_h0, float* part,
+                           float* grad, void* stream);
+/* y = h wd + bd (nullable output), z = s d y; with q, partials of (z, q)
+ * (their count goes to *zq_nparts when non-null). */
+int tm_gnn_decode(int64_t n, const uint16_t* h, const float* wd, const float* bd, const float* d,
+                  const double* scale, float* y, double* z, const double* q, double* zq_part, int* zq_nparts,
+                  const int* stop, void* stream);
+/* Inference fusions of the finest level (same values as the unfused kernels):
+ * encode followed by the first pooling, writing the coarse features and s; and
+ * unpool (skip = encoder output recomputed from r, lk) + message passing on
+ * level 0 + decode, writing z (and the (z, q) partials). */
+int tm_gnn_encode_pool(int64_t n, int64_t n_coarse, const int32_t* child_ptr, const int32_t* child, const float* w,
+                       const double* r, const float* d, const float* lk, const double* ss_part, int nparts,
+                       const float* we, const float* be, int n_linear, uint16_t* out, double* scale, const int* stop,
+                       void* stream);
+int tm_gnn_unpool_mp_decode(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values,
+                            const double* r, const float* d, const float* lk, const int32_t* agg, const float* w,
+                            const uint16_t* hc, const float* we, const float* be, const uint16_t* w_t,
+                            const float* bias, int n_linear, const float* wd, const float* bd, const double* scale,
+                            double* z, const double* q, double* zq_part, int* zq_nparts, const int* stop,
+                            void* stream);
+int tm_gnn_decode_backward(int64_t n, const uint16_t* h, const float* wd, const float* g_y, float* g_h,
+                           float* part, float* grad, void* stream);
+/* Energy loss of the normalised system: g_y = (A^ y - d r / s) / n and
+ * partials [y.A^y | y.r~ | e~.r~] (e~ = e / (d s), e nullable). */
+int tm_gnn_energy_loss(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values,
+                       const float* y, const double* r, const float* d, const double* scale, const double* e,
+                       float* g_y, double* part, int* nparts, void* stream);
+/* Level values of one matrix: d = diag^-1/2, values_normalized = D A D. */
+int tm_gnn_level_setup(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values, float* d,
+                       float* values_normalized, void* stream);
+/* Galerkin coarse values: coarse[s] = sum of fine[perm[k]], k in [seg[s], seg[s+1]). */
+int tm_gnn_level_coarsen(int64_t nnz_coarse, const int32_t* seg, const int32_t* perm, const float* fine_values,
+                         float* coarse_values, void* stream);
+/* w_i = d_coarse[agg_i] / d_fine[i]. */
+int tm_gnn_pool_weights(int64_t n_fine, const int32_t* agg, const float* d_fine, const float* d_coarse, float* w,
+                        void* stream);
+int tm_gnn_adam(int64_t n, float* param, const float* grad, float* m, float* v, float lr, float beta1, float beta2,
+                float eps, int64_t step, void* stream);
+
+/* Flexible CG (truncated to one direction) around an external preconditioner:
+ * tm_fcg_start, then per iteration the preconditioner writes z = M(r) and the
+ * (z, q) partials (tm_gnn_decode or tm_gnn_unpool_mp_decode with q and zq_part
+ * of the view) and tm_fcg_iterate finishes the iteration.  Everything is stream-ordered with
+ * device-side scalars, so iterations can be captured in a hipGraph. */
+typedef struct {
+  double* r;
+  double* z;
+  double* q;
+  double* zq_part;
+  double* ss_part;
+  double* scale;
+  double* relres;
+  int32_t* status; /* 0 running, 1 converged, 2 breakdown, 3 max iterations */
+  int32_t* iter;
+  int32_t nparts;
+  int32_t reserved;
+} tm_fcg_view;
+int64_t tm_fcg_work_size(int64_t n);
+int tm_fcg_layout(int64_t n, void* work, tm_fcg_view* view);
+int tm_fcg_start(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values, const double* b,
+                 const double* x, const float* d, void* work, double rtol, double atol, int64_t maxiter,
+                 void* stream);
+/* zq_nparts: how many (z, q) partials the preconditioner wrote. */
+int tm_fcg_iterate(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values, double* x,
+                   const float* d, void* work, int zq_nparts, void* stream);
 
 #ifdef __cplusplus
 }

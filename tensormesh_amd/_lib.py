@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libtensorme
 TM_OK, TM_ERR_ARG, TM_ERR_HIP, TM_ERR_LIMIT, TM_ERR_ALLOC, TM_ERR_NOTCONV = 0, -1, -2, -3, -4, -5
 FORM_LAPLACE, FORM_KAPPA_LAPLACE, FORM_MASS = 0, 1, 2
 LOAD_CONST, LOAD_NODAL = 0, 1
-F32, F64 = 0, 1
+F32, F64, BF16 = 0, 1, 2
 
 c_void_p = ctypes.c_void_p
 c_int = ctypes.c_int
@@ -44,6 +44,12 @@ class BatchedArgs(ctypes.Structure):
                 ("mesh_tile0", c_void_p), ("rtol", c_double), ("maxiter", c_i64),
                 ("work", c_void_p), ("mesh_status", c_void_p), ("mesh_iters", c_void_p),
                 ("mesh_relres", c_void_p)]
+
+
+class FcgView(ctypes.Structure):
+    _fields_ = [("r", c_void_p), ("z", c_void_p), ("q", c_void_p), ("zq_part", c_void_p),
+                ("ss_part", c_void_p), ("scale", c_void_p), ("relres", c_void_p), ("status", c_void_p),
+                ("iter", c_void_p), ("nparts", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class TMError(RuntimeError):
@@ -88,6 +94,29 @@ _SIGS = {
     "tm_batched_pcg_work_size": (c_i64, [c_i64, ctypes.c_int32]),
     "tm_batched_pcg": (c_int, [ctypes.POINTER(BatchedArgs), c_int, c_int, c_void_p]),
     "tm_batched_timeout_flag": (c_int, [ctypes.POINTER(BatchedArgs), c_void_p, c_void_p]),
+    "tm_gnn_mp_forward": (c_int, [c_i64] + [c_void_p] * 6 + [c_int] + [c_void_p] * 5),
+    "tm_gnn_part_floats": (c_i64, []),
+    "tm_gnn_mp_backward": (c_int, [c_i64] + [c_void_p] * 8 + [c_int] + [c_void_p] * 5),
+    "tm_gnn_pool": (c_int, [c_i64] + [c_void_p] * 5 + [c_int, c_void_p, c_void_p]),
+    "tm_gnn_unpool": (c_int, [c_i64] + [c_void_p] * 5 + [c_int, c_void_p, c_void_p]),
+    "tm_gnn_sumsq": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_int), c_void_p]),
+    "tm_gnn_encode": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int,
+                              c_void_p, c_void_p, c_void_p, c_void_p]),
+    "tm_gnn_encode_backward": (c_int, [c_i64] + [c_void_p] * 6 + [c_int] + [c_void_p] * 4),
+    "tm_gnn_decode": (c_int, [c_i64] + [c_void_p] * 12),
+    "tm_gnn_encode_pool": (c_int, [c_i64, c_i64] + [c_void_p] * 7 + [c_int, c_void_p, c_void_p, c_int] +
+                           [c_void_p] * 4),
+    "tm_gnn_unpool_mp_decode": (c_int, [c_i64] + [c_void_p] * 13 + [c_int] + [c_void_p] * 9),
+    "tm_gnn_decode_backward": (c_int, [c_i64] + [c_void_p] * 7),
+    "tm_gnn_energy_loss": (c_int, [c_i64] + [c_void_p] * 10 + [ctypes.POINTER(c_int), c_void_p]),
+    "tm_gnn_level_setup": (c_int, [c_i64] + [c_void_p] * 6),
+    "tm_gnn_level_coarsen": (c_int, [c_i64] + [c_void_p] * 5),
+    "tm_gnn_pool_weights": (c_int, [c_i64] + [c_void_p] * 5),
+    "tm_gnn_adam": (c_int, [c_i64] + [c_void_p] * 4 + [ctypes.c_float] * 4 + [c_i64, c_void_p]),
+    "tm_fcg_work_size": (c_i64, [c_i64]),
+    "tm_fcg_layout": (c_int, [c_i64, c_void_p, ctypes.POINTER(FcgView)]),
+    "tm_fcg_start": (c_int, [c_i64] + [c_void_p] * 7 + [c_double, c_double, c_i64, c_void_p]),
+    "tm_fcg_iterate": (c_int, [c_i64] + [c_void_p] * 6 + [c_int, c_void_p]),
     "tm_pcg_work_size": (c_i64, [c_i64]),
     "tm_pcg_jacobi": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                               c_double, c_double, c_i64, c_int, ctypes.POINTER(c_i64),

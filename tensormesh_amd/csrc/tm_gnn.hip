@@ -1,0 +1,1270 @@
+// Learned preconditioner for gfx950: a U-shaped edge message-passing network
+// over nested node aggregates, and the flexible CG that applies it.
+//
+// Features are node-major [n][64] BF16 rows (128 B per node).  One
+// message-passing layer at a level with normalised operator A^ (unit
+// diagonal, Galerkin-coarsened on coarse levels) is
+//     a = A^ h                      (edge messages weighted by matrix entries)
+//     u = [h | a] W + b             (W: 128 x 64, BF16 MFMA, FP32 accumulate)
+//     h' = h + act(u)               (act: identity on the first n_lin channels,
+//                                    ReLU on the rest)
+// and is one fused kernel: a 64-node tile gathers its messages into LDS, four
+// waves multiply the [64 x 128] tile by W with v_mfma_f32_16x16x32_bf16 and
+// write h' (and, for training, a and u).  The backward pass is one persistent
+// MFMA kernel (input gradients and per-workgroup weight-gradient partials)
+// plus a message-transpose kernel; every weight gradient is reduced in a
+// fixed order, so training steps are deterministic.
+//
+// MFMA operand layout (16x16x32 BF16, wave64): lane l holds A[l%16][8(l/16)..+7],
+// B[8(l/16)..+7][l%16]; the FP32 result holds D[4(l/16)+v][l%16], v = 0..3.
+#include <cmath>
+#include <cstdint>
+
+#include "tm_device.h"
+#include "tm_parts.h"
+
+namespace {
+
+using tmx::kWave;
+constexpr int F = 64;          // feature width
+constexpr int XP = 2 * F + 8;  // LDS pitch (BF16 elements) of a [h | a] row
+constexpr int GP = F + 8;      // LDS pitch of a 64-wide BF16 row
+constexpr int TILE = 64;       // node rows per tile (4 waves x 16 rows)
+constexpr int NT = 256;
+constexpr int kMaxParts = 1024;
+constexpr int NWP = 2 * F * F + F;  // weight-gradient partial of one layer: dW (128 x 64) + db (64)
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(uint16_t u) { return __builtin_bit_cast(float, (uint32_t)u << 16); }
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+__device__ __forceinline__ uint32_t pack2(float a, float b) {
+  return (uint32_t)f2bf(a) | ((uint32_t)f2bf(b) << 16);
+}
+__device__ __forceinline__ bf16x8 lds8(const uint16_t* p) { return __builtin_bit_cast(bf16x8, *(const uint4*)p); }
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ float act(float u, int j, int n_lin) { return j < n_lin ? u : fmaxf(u, 0.0f); }
+__device__ __forceinline__ float dact(float u, int j, int n_lin) { return (j < n_lin || u > 0.0f) ? 1.0f : 0.0f; }
+__device__ __forceinline__ bool stopped(const int* stop) { return stop != nullptr && *stop != 0; }
+
+// ---------------------------------------------------------------------------
+// Forward message-passing layer
+//
+// A workgroup owns 64-row tiles (a contiguous range of tiles per XCD-remapped
+// workgroup).  Per tile: the CSR entries of its rows are staged in LDS (up to
+// kEnt entries; larger tiles read them from global memory), each wave gathers
+// the messages of 16 rows with 4 lanes per row and 16 features per lane (32 B
+// of the neighbour's row per lane), then multiplies [h | a] by W on MFMA.
+//
+// Node features come from a source:
+//   Dense    - a materialised BF16 [n][64] tensor;
+//   Unpooled - computed on the fly as bf16(bf16(enc(r_j, lk_j)) + w_j hc[agg_j])
+//              (encoder output as skip plus the prolonged coarse features),
+//              i.e. exactly what the unfused encode + unpool would store.
+// With DECODE the epilogue also applies the decoder: y = bf16(h') . wd + bd,
+// z = s d y, and (optionally) the partial sums of (z, q).
+// ---------------------------------------------------------------------------
+constexpr int kEnt = 1536;
+
+struct Dense {
+  const uint16_t* h;
+  __device__ __forceinline__ void prepare(int) {}
+  __device__ __forceinline__ void load16(int64_t j, int f0, float (&o)[16]) const {
+    const uint4 a = *(const uint4*)&h[j * F + f0];
+    const uint4 b = *(const uint4*)&h[j * F + f0 + 8];
+    const uint32_t w8[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      o[2 * t] = bf2f((uint16_t)(w8[t] & 0xffffu));
+      o[2 * t + 1] = bf2f((uint16_t)(w8[t] >> 16));
+    }
+  }
+};
+
+struct Unpooled {
+  const double* r;
+  const float* d;
+  const float* lk;
+  const int32_t* agg;
+  const float* w;
+  const uint16_t* hc;
+  const float* we;
+  const float* be;
+  const double* scale;
+  int n_lin;
+  // per lane: encoder weights of its 16 features
+  float inv_s, w0[16], w1[16], b0[16];
+  int fl;
+  __device__ __forceinline__ void prepare(int f0) {
+    inv_s = (float)(1.0 / *scale);
+    fl = f0;
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      w0[t] = we[f0 + t];
+      w1[t] = we[F + f0 + t];
+      b0[t] = be[f0 + t];
+    }
+  }
+  __device__ __forceinline__ void load16(int64_t j, int f0, float (&o)[16]) const {
+    const float x0 = (float)((double)d[j] * r[j]) * inv_s, x1 = lk[j], wj = w[j];
+    Dense c{hc};
+    float hv[16];
+    c.load16(agg[j], f0, hv);
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      const float e = bf2f(f2bf(act(x0 * w0[t] + x1 * w1[t] + b0[t], fl + t, n_lin)));
+      o[t] = bf2f(f2bf(e + wj * hv[t]));
+    }
+  }
+};
+
+struct MpOut {
+  uint16_t* out;       // h' (non-decode)
+  uint16_t* s_agg;     // training saves (nullable)
+  uint16_t* s_u;
+  const float* wd;     // decoder (decode)
+  const float* bd;
+  const float* d;
+  const double* scale;
+  double* z;
+  const double* q;
+  double* zq_part;
+};
+
+template <class Src, bool SAVE, bool DECODE>
+__global__ __launch_bounds__(NT) void k_mp(int64_t n, const int32_t* __restrict__ rp,
+                                           const int32_t* __restrict__ ci, const float* __restrict__ av, Src src0,
+                                           const uint16_t* __restrict__ wt, const float* __restrict__ bias,
+                                           int n_lin, MpOut o, const int* stop) {
+  if (stopped(stop)) return;
+  __shared__ __align__(16) uint16_t X[TILE * XP];
+  __shared__ __align__(16) uint16_t W[F * XP];  // W^T: [64 outputs][128 inputs]
+  __shared__ int rps[TILE + 1];
+  __shared__ int ec[kEnt];
+  __shared__ float ev[kEnt];
+  const int tid = threadIdx.x, wv = tid / kWave, l = tid % kWave;
+  const int64_t tiles = (n + TILE - 1) / TILE;
+  const int64_t per = (tiles + gridDim.x - 1) / gridDim.x;
+  const int64_t t_begin = (int64_t)tmx::xcd_remap(blockIdx.x, gridDim.x) * per;
+  const int64_t t_end = t_begin + per < tiles ? t_begin + per : tiles;
+  for (int c = tid; c < F * 16; c += NT) {
+    const int j = c / 16, k8 = (c % 16) * 8;
+    *(uint4*)&W[j * XP + k8] = *(const uint4*)&wt[j * 2 * F + k8];
+  }
+  double zq = 0.0, s = 1.0;
+  if (DECODE) s = *o.scale;
+  const int ri = wv * 16 + l / 4, f0 = (l % 4) * 16;
+  Src src = src0;
+  src.prepare(f0);
+  for (int64_t t = t_begin; t < t_end; ++t) {
+    const int64_t r0 = t * TILE;
+    __syncthreads();  // previous tile's LDS reads are done
+    for (int c = tid; c <= TILE; c += NT) {
+      const int64_t row = r0 + c < n ? r0 + c : n;
+      rps[c] = rp[row];
+    }
+    __syncthreads();
+    const int e0 = rps[0], ecount = rps[TILE] - e0;
+    const bool staged = ecount <= kEnt;
+    if (staged)
+      for (int c = tid; c < ecount; c += NT) {
+        ec[c] = ci[e0 + c];
+        ev[c] = av[e0 + c];
+      }
+    __syncthreads();
+    // own features and messages of row ri (16 features from f0)
+    {
+      const int64_t row = r0 + ri;
+      float hv[16], a[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) a[u] = 0.0f;
+      if (row < n) {
+        src.load16(row, f0, hv);
+        const int kb = rps[ri] - e0, ke = rps[ri + 1] - e0;
+        int k = kb;
+        for (; k + 1 < ke; k += 2) {
+          int c0, c1;
+          float v0, v1;
+          if (staged) {
+            c0 = ec[k]; c1 = ec[k + 1]; v0 = ev[k]; v1 = ev[k + 1];
+          } else {
+            c0 = ci[e0 + k]; c1 = ci[e0 + k + 1]; v0 = av[e0 + k]; v1 = av[e0 + k + 1];
+          }
+          float h0[16], h1[16];
+          src.load16(c0, f0, h0);
+          src.load16(c1, f0, h1);
+#pragma unroll
+          for (int u = 0; u < 16; ++u) a[u] += v0 * h0[u] + v1 * h1[u];
+        }
+        if (k < ke) {
+          const int c0 = staged ? ec[k] : ci[e0 + k];
+          const float v0 = staged ? ev[k] : av[e0 + k];
+          float h0[16];
+          src.load16(c0, f0, h0);
+#pragma unroll
+          for (int u = 0; u < 16; ++u) a[u] += v0 * h0[u];
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) hv[u] = 0.0f;
+      }
+      uint32_t ph[8], pa[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        ph[u] = pack2(hv[2 * u], hv[2 * u + 1]);
+        pa[u] = pack2(a[2 * u], a[2 * u + 1]);
+      }
+      *(uint4*)&X[ri * XP + f0] = uint4{ph[0], ph[1], ph[2], ph[3]};
+      *(uint4*)&X[ri * XP + f0 + 8] = uint4{ph[4], ph[5], ph[6], ph[7]};
+      *(uint4*)&X[ri * XP + F + f0] = uint4{pa[0], pa[1], pa[2], pa[3]};
+      *(uint4*)&X[ri * XP + F + f0 + 8] = uint4{pa[4], pa[5], pa[6], pa[7]};
+      if (SAVE && row < n) {
+        *(uint4*)&o.s_agg[row * F + f0] = uint4{pa[0], pa[1], pa[2], pa[3]};
+        *(uint4*)&o.s_agg[row * F + f0 + 8] = uint4{pa[4], pa[5], pa[6], pa[7]};
+      }
+    }
+    __syncthreads();
+    f32x4 acc[4];
+#pragma unroll
+    for (int jt = 0; jt < 4; ++jt) acc[jt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    const int ar = wv * 16 + (l % 16), ak = 8 * (l / 16);
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      const bf16x8 a = lds8(&X[ar * XP + kk * 32 + ak]);
+#pragma unroll
+      for (int jt = 0; jt < 4; ++jt)
+        acc[jt] = mfma(a, lds8(&W[(jt * 16 + (l % 16)) * XP + kk * 32 + ak]), acc[jt]);
+    }
+    if (DECODE) {
+      float yp[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int jt = 0; jt < 4; ++jt) {
+        const int j = jt * 16 + (l % 16);
+        const float bj = bias[j], wdj = o.wd[j];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = wv * 16 + 4 * (l / 16) + v;
+          const float hn = bf2f(f2bf(bf2f(X[i * XP + j]) + act(acc[jt][v] + bj, j, n_lin)));
+          yp[v] += hn * wdj;
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int m = 1; m < 16; m <<= 1) yp[v] += __shfl_xor(yp[v], m, kWave);
+      if ((l % 16) == 0) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int64_t row = r0 + wv * 16 + 4 * (l / 16) + v;
+          if (row < n) {
+            const double zi = s * (double)o.d[row] * (double)(yp[v] + o.bd[0]);
+            o.z[row] = zi;
+            if (o.q) zq += zi * o.q[row];
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int jt = 0; jt < 4; ++jt) {
+        const int j = jt * 16 + (l % 16);
+        const float bj = bias[j];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = wv * 16 + 4 * (l / 16) + v;
+          const float u = acc[jt][v] + bj;
+          const float hn = bf2f(X[i * XP + j]) + act(u, j, n_lin);
+          X[i * XP + j] = f2bf(hn);
+          if (SAVE && r0 + i < n) o.s_u[(r0 + i) * F + j] = f2bf(u);
+        }
+      }
+      __syncthreads();
+      for (int c = l; c < 16 * 8; c += kWave) {
+        const int i = wv * 16 + c / 8, k8 = (c % 8) * 8;
+        if (r0 + i < n) *(uint4*)&o.out[(r0 + i) * F + k8] = *(const uint4*)&X[i * XP + k8];
+      }
+    }
+  }
+  if (DECODE && o.zq_part) {
+    double loc[1] = {zq};
+    const int which[1] = {0};
+    tmx::write_parts<NT, 1>(o.zq_part, kMaxParts, which, loc);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Backward message-passing layer
+//   g_u = g_out * act'(u)
+//   g_in = g_out + g_u W_h^T   (this kernel)  + A^ (g_u W_a^T)  (k_msg_add)
+//   dW += [h | a]^T g_u, db += sum g_u    (per-workgroup partials)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_mp_bwd(int64_t n, const float* __restrict__ g_out,
+                                               const uint16_t* __restrict__ h, const uint16_t* __restrict__ agg,
+                                               const uint16_t* __restrict__ u, const uint16_t* __restrict__ w,
+                                               int n_lin, float* __restrict__ g_in, float* __restrict__ g_msg,
+                                               float* __restrict__ part) {
+  __shared__ __align__(16) uint16_t Wl[2 * F * GP];  // W: [128 inputs][64 outputs]
+  __shared__ __align__(16) uint16_t Xs[TILE * XP];
+  __shared__ __align__(16) uint16_t Gb[TILE * GP];
+  __shared__ float Gf[TILE * F];
+  const int tid = threadIdx.x, wv = tid / kWave, l = tid % kWave;
+  for (int c = tid; c < 2 * F * 8; c += NT) {
+    const int k = c / 8, j8 = (c % 8) * 8;
+    *(uint4*)&Wl[k * GP + j8] = *(const uint4*)&w[k * F + j8];
+  }
+  f32x4 dw[2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) dw[a][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  float db = 0.0f;
+  const int64_t ntiles = (n + TILE - 1) / TILE;
+  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int64_t r0 = t * TILE;
+    __syncthreads();
+    for (int c = tid; c < TILE * 8; c += NT) {
+      const int i = c / 8, k8 = (c % 8) * 8;
+      const int64_t row = r0 + i;
+      uint4 vh = {0u, 0u, 0u, 0u}, va = {0u, 0u, 0u, 0u};
+      if (row < n) {
+        vh = *(const uint4*)&h[row * F + k8];
+        va = *(const uint4*)&agg[row * F + k8];
+      }
+      *(uint4*)&Xs[i * XP + k8] = vh;
+      *(uint4*)&Xs[i * XP + F + k8] = va;
+    }
+    for (int c = tid; c < TILE * F; c += NT) {
+      const int i = c / F, j = c % F;
+      const int64_t row = r0 + i;
+      float g = 0.0f;
+      if (row < n) g = g_out[row * F + j] * dact(bf2f(u[row * F + j]), j, n_lin);
+      Gf[i * F + j] = g;
+      Gb[i * GP + j] = f2bf(g);
+    }
+    __syncthreads();
+    // g_u (16 rows of this wave) x W^T -> [16 x 128]
+    {
+      f32x4 acc[8];
+#pragma unroll
+      for (int ct = 0; ct < 8; ++ct) acc[ct] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+      const int ar = wv * 16 + (l % 16), ak = 8 * (l / 16);
+#pragma unroll
+      for (int kk = 0; kk < 2; ++kk) {
+        const bf16x8 a = lds8(&Gb[ar * GP + kk * 32 + ak]);
+#pragma unroll
+        for (int ct = 0; ct < 8; ++ct) acc[ct] = mfma(a, lds8(&Wl[(ct * 16 + (l % 16)) * GP + kk * 32 + ak]), acc[ct]);
+      }
+#pragma unroll
+      for (int ct = 0; ct < 8; ++ct) {
+        const int c = ct * 16 + (l % 16);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int64_t row = r0 + wv * 16 + 4 * (l / 16) + v;
+          if (row < n) {
+            if (c < F)
+              g_in[row * F + c] = g_out[row * F + c] + acc[ct][v];
+            else
+              g_msg[row * F + (c - F)] = acc[ct][v];
+          }
+        }
+      }
+    }
+    // dW[k][j] += sum_i X[i][k] g_u[i][j]: this wave owns k in [32 wv, 32 wv + 32)
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk) {
+      const int i0 = kk * 32 + 8 * (l / 16);
+      bf16x8 bop[4];
+#pragma unroll
+      for (int jt = 0; jt < 4; ++jt)
+#pragma unroll
+        for (int v = 0; v < 8; ++v) bop[jt][v] = __builtin_bit_cast(__bf16, Gb[(i0 + v) * GP + jt * 16 + (l % 16)]);
+#pragma unroll
+      for (int kt = 0; kt < 2; ++kt) {
+        bf16x8 aop;
+        const int k = (2 * wv + kt) * 16 + (l % 16);
+#pragma unroll
+        for (int v = 0; v < 8; ++v) aop[v] = __builtin_bit_cast(__bf16, Xs[(i0 + v) * XP + k]);
+#pragma unroll
+        for (int jt = 0; jt < 4; ++jt) dw[kt][jt] = mfma(aop, bop[jt], dw[kt][jt]);
+      }
+    }
+    if (tid < F)
+      for (int i = 0; i < TILE; ++i) db += Gf[i * F + tid];
+  }
+  float* pp = part + (int64_t)blockIdx.x * NWP;
+#pragma unroll
+  for (int kt = 0; kt < 2; ++kt)
+#pragma unroll
+    for (int jt = 0; jt < 4; ++jt)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int k = (2 * wv + kt) * 16 + 4 * (l / 16) + v, j = jt * 16 + (l % 16);
+        pp[k * F + j] = dw[kt][jt][v];
+      }
+  if (tid < F) pp[2 * F * F + tid] = db;
+}
+
+// g_in[i] += sum_k A^_ik g_msg[col_k]   (A^ symmetric: the transpose of the
+// message aggregation is the aggregation itself).  One wave per row.
+__global__ __launch_bounds__(NT) void k_msg_add(int64_t n, const int32_t* __restrict__ rp,
+                                                const int32_t* __restrict__ ci, const float* __restrict__ av,
+                                                const float* __restrict__ g_msg, float* __restrict__ g_in) {
+  const int l = threadIdx.x % kWave;
+  for (int64_t row = blockIdx.x * (int64_t)(NT / kWave) + threadIdx.x / kWave; row < n;
+       row += (int64_t)gridDim.x * (NT / kWave)) {
+    float s = 0.0f;
+    const int b = rp[row], e = rp[row + 1];
+    int k = b;
+    for (; k + 3 < e; k += 4) {
+      int c[4];
+      float v[4], g[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        c[q] = ci[k + q];
+        v[q] = av[k + q];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) g[q] = g_msg[(int64_t)c[q] * F + l];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) s += v[q] * g[q];
+    }
+    for (; k < e; ++k) s += av[k] * g_msg[(int64_t)ci[k] * F + l];
+    g_in[row * F + l] += s;
+  }
+}
+
+// out[c] (+)= sum_b part[b][c] in block order.
+__global__ __launch_bounds__(NT) void k_reduce_parts(int nparts, int width, const float* __restrict__ part,
+                                                     float* __restrict__ out, int accumulate) {
+  for (int c = blockIdx.x * NT + threadIdx.x; c < width; c += gridDim.x * NT) {
+    float s = 0.0f;
+    for (int b = 0; b < nparts; ++b) s += part[(int64_t)b * width + c];
+    out[c] = accumulate ? out[c] + s : s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Aggregation levels: pool (restriction) and unpool (prolongation + skip)
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ float ld(const T* p);
+template <>
+__device__ __forceinline__ float ld<uint16_t>(const uint16_t* p) { return bf2f(*p); }
+template <>
+__device__ __forceinline__ float ld<float>(const float* p) { return *p; }
+template <typename T>
+__device__ __forceinline__ void st(T* p, float v);
+template <>
+__device__ __forceinline__ void st<uint16_t>(uint16_t* p, float v) { *p = f2bf(v); }
+template <>
+__device__ __forceinline__ void st<float>(float* p, float v) { *p = v; }
+
+template <typename T>
+struct Vec8;  // 8 features: one 16 B (BF16) or 32 B (FP32) chunk
+template <>
+struct Vec8<uint16_t> {
+  __device__ __forceinline__ static void load(const uint16_t* p, float (&o)[8]) {
+    const uint4 a = *(const uint4*)p;
+    const uint32_t w4[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      o[2 * t] = bf2f((uint16_t)(w4[t] & 0xffffu));
+      o[2 * t + 1] = bf2f((uint16_t)(w4[t] >> 16));
+    }
+  }
+  __device__ __forceinline__ static void store(uint16_t* p, const float (&v)[8]) {
+    *(uint4*)p = uint4{pack2(v[0], v[1]), pack2(v[2], v[3]), pack2(v[4], v[5]), pack2(v[6], v[7])};
+  }
+};
+template <>
+struct Vec8<float> {
+  __device__ __forceinline__ static void load(const float* p, float (&o)[8]) {
+    const float4 a = *(const float4*)p, b = *(const float4*)(p + 4);
+    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w; o[4] = b.x; o[5] = b.y; o[6] = b.z; o[7] = b.w;
+  }
+  __device__ __forceinline__ static void store(float* p, const float (&v)[8]) {
+    *(float4*)p = float4{v[0], v[1], v[2], v[3]};
+    *(float4*)(p + 4) = float4{v[4], v[5], v[6], v[7]};
+  }
+};
+
+// out[I][f] = sum over children i of I: w_i h[i][f].  8 lanes per coarse node.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_pool(int64_t nc, const int32_t* __restrict__ cp,
+                                             const int32_t* __restrict__ child, const float* __restrict__ w,
+                                             const T* __restrict__ h, T* __restrict__ out, const int* stop) {
+  if (stopped(stop)) return;
+  const int f0 = (threadIdx.x % 8) * 8;
+  for (int64_t I = blockIdx.x * (int64_t)(NT / 8) + threadIdx.x / 8; I < nc; I += (int64_t)gridDim.x * (NT / 8)) {
+    float s[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k = cp[I]; k < cp[I + 1]; ++k) {
+      const int i = child[k];
+      const float wi = w[i];
+      float v[8];
+      Vec8<T>::load(&h[(int64_t)i * F + f0], v);
+#pragma unroll
+      for (int t = 0; t < 8; ++t) s[t] += wi * v[t];
+    }
+    Vec8<T>::store(&out[I * F + f0], s);
+  }
+}
+
+// out[i][f] = skip[i][f] + w_i hc[agg_i][f]   (skip may be null; out may alias skip)
+template <typename T>
+__global__ __launch_bounds__(NT) void k_unpool(int64_t nf, const int32_t* __restrict__ agg,
+                                               const float* __restrict__ w, const T* __restrict__ hc,
+                                               const T* skip, T* out, const int* stop) {
+  if (stopped(stop)) return;
+  const int64_t total = nf * 8;
+  for (int64_t t = blockIdx.x * (int64_t)NT + threadIdx.x; t < total; t += (int64_t)gridDim.x * NT) {
+    const int64_t i = t / 8;
+    const int f0 = (int)(t % 8) * 8;
+    const float wi = w[i];
+    float v[8], sk[8];
+    Vec8<T>::load(&hc[(int64_t)agg[i] * F + f0], v);
+    if (skip) {
+      Vec8<T>::load(&skip[i * F + f0], sk);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = sk[u] + wi * v[u];
+    } else {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] *= wi;
+    }
+    Vec8<T>::store(&out[i * F + f0], v);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Encoder / decoder
+//   x_i = (d_i r_i / s, lk_i), s = sqrt(sum (d r)^2 / n)  (d = diag^-1/2)
+//   h0[i][f] = act(x_i0 We[0][f] + x_i1 We[1][f] + be[f])
+//   y_i = h[i] . wd + bd,   z_i = s d_i y_i
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double scale_from_parts(const double* ss_part, int nparts, int64_t n) {
+  const int which[1] = {0};
+  double t[1];
+  tmx::sum_parts<1>(ss_part, kMaxParts, which, nparts, t);
+  const double s = n > 0 ? sqrt(t[0] / (double)n) : 0.0;
+  return s > 0.0 ? s : 1.0;
+}
+
+__global__ __launch_bounds__(NT) void k_sumsq(int64_t n, const double* __restrict__ r, const float* __restrict__ d,
+                                              double* __restrict__ ss_part) {
+  double loc[1] = {0.0};
+  for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+    const double v = (double)d[i] * r[i];
+    loc[0] += v * v;
+  }
+  const int which[1] = {0};
+  tmx::write_parts<NT, 1>(ss_part, kMaxParts, which, loc);
+}
+
+__global__ __launch_bounds__(NT) void k_encode(int64_t n, const double* __restrict__ r, const float* __restrict__ d,
+                                               const float* __restrict__ lk, const double* __restrict__ ss_part,
+                                               int nparts, const float* __restrict__ we, const float* __restrict__ be,
+                                               int n_lin, uint16_t* __restrict__ h0, double* __restrict__ s_out,
+                                               const int* stop) {
+  if (stopped(stop)) return;
+  const double s = scale_from_parts(ss_part, nparts, n);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *s_out = s;
+  const float inv = (float)(1.0 / s);
+  const int q = threadIdx.x % 8, f0 = q * 8;
+  for (int64_t i = blockIdx.x * (int64_t)(NT / 8) + threadIdx.x / 8; i < n; i += (int64_t)gridDim.x * (NT / 8)) {
+    const float x0 = (float)((double)d[i] * r[i]) * inv, x1 = lk[i];
+    uint32_t pk[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int f = f0 + 2 * t;
+      const float u0 = x0 * we[f] + x1 * we[F + f] + be[f];
+      const float u1 = x0 * we[f + 1] + x1 * we[F + f + 1] + be[f + 1];
+      pk[t] = pack2(act(u0, f, n_lin), act(u1, f + 1, n_lin));
+    }
+    *(uint4*)&h0[i * F + f0] = uint4{pk[0], pk[1], pk[2], pk[3]};
+  }
+}
+
+// Fused encoder + first pooling (inference): out[I] = bf16(sum over children
+// i of w_i bf16(act([d_i r_i / s, lk_i] We + be))), the same values as
+// k_encode followed by k_pool.  8 lanes per coarse node, 8 features per lane.
+__global__ __launch_bounds__(NT) void k_encode_pool(int64_t n, int64_t nc, const int32_t* __restrict__ cp,
+                                                    const int32_t* __restrict__ child, const float* __restrict__ w,
+                                                    const double* __restrict__ r, const float* __restrict__ d,
+                                                    const float* __restrict__ lk, const double* __restrict__ ss_part,
+                                                    int nparts, const float* __restrict__ we,
+                                                    const float* __restrict__ be, int n_lin,
+                                                    uint16_t* __restrict__ out, double* __restrict__ s_out,
+                                                    const int* stop) {
+  if (stopped(stop)) return;
+  const double s = scale_from_parts(ss_part, nparts, n);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *s_out = s;
+  const float inv = (float)(1.0 / s);
+  const int f0 = (threadIdx.x % 8) * 8;
+  float w0[8], w1[8], b0[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    w0[t] = we[f0 + t];
+    w1[t] = we[F + f0 + t];
+    b0[t] = be[f0 + t];
+  }
+  for (int64_t I = blockIdx.x * (int64_t)(NT / 8) + threadIdx.x / 8; I < nc; I += (int64_t)gridDim.x * (NT / 8)) {
+    float acc[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k = cp[I]; k < cp[I + 1]; ++k) {
+      const int i = child[k];
+      const float x0 = (float)((double)d[i] * r[i]) * inv, x1 = lk[i], wi = w[i];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) acc[t] += wi * bf2f(f2bf(act(x0 * w0[t] + x1 * w1[t] + b0[t], f0 + t, n_lin)));
+    }
+    Vec8<uint16_t>::store(&out[I * F + f0], acc);
+  }
+}
+
+// dWe, dbe partials: [3][64] per workgroup
+__global__ __launch_bounds__(NT) void k_encode_bwd(int64_t n, const double* __restrict__ r,
+                                                   const float* __restrict__ d, const float* __restrict__ lk,
+                                                   const double* __restrict__ s_in, const float* __restrict__ we,
+                                                   const float* __restrict__ be, int n_lin,
+                                                   const float* __restrict__ g_h0, float* __restrict__ part) {
+  __shared__ float red[NT / kWave][3 * F];
+  const int f = threadIdx.x % F, wv = threadIdx.x / F;
+  const float inv = (float)(1.0 / *s_in);
+  float a0 = 0.0f, a1 = 0.0f, ab = 0.0f;
+  for (int64_t i = blockIdx.x * (int64_t)(NT / F) + wv; i < n; i += (int64_t)gridDim.x * (NT / F)) {
+    const float x0 = (float)((double)d[i] * r[i]) * inv, x1 = lk[i];
+    const float u = x0 * we[f] + x1 * we[F + f] + be[f];
+    const float g = g_h0[i * F + f] * dact(u, f, n_lin);
+    a0 += x0 * g;
+    a1 += x1 * g;
+    ab += g;
+  }
+  red[wv][f] = a0;
+  red[wv][F + f] = a1;
+  red[wv][2 * F + f] = ab;
+  __syncthreads();
+  if (threadIdx.x < 3 * F) {
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < NT / kWave; ++k) s += red[k][threadIdx.x];
+    part[(int64_t)blockIdx.x * 3 * F + threadIdx.x] = s;
+  }
+}
+
+// y (normalised output, optional), z = s d y (FP64), optional partial (z, q).
+__global__ __launch_bounds__(NT) void k_decode(int64_t n, const uint16_t* __restrict__ h,
+                                               const float* __restrict__ wd, const float* __restrict__ bd,
+                                               const float* __restrict__ d, const double* __restrict__ s_in,
+                                               float* __restrict__ y, double* __restrict__ z,
+                                               const double* __restrict__ q, double* __restrict__ zq_part,
+                                               const int* stop) {
+  if (stopped(stop)) return;
+  const double s = *s_in;
+  const int qd = threadIdx.x % 8, f0 = qd * 8;
+  float wl[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) wl[t] = wd[f0 + t];
+  const float b = bd[0];
+  double loc[1] = {0.0};
+  const int64_t stride = (int64_t)gridDim.x * (NT / 8);
+  for (int64_t i0 = blockIdx.x * (int64_t)(NT / 8); i0 < n; i0 += stride) {
+    const int64_t i = i0 + threadIdx.x / 8;
+    float acc = 0.0f;
+    if (i < n) {
+      const uint4 v = *(const uint4*)&h[i * F + f0];
+      const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        acc += bf2f((uint16_t)(w4[t] & 0xffffu)) * wl[2 * t];
+        acc += bf2f((uint16_t)(w4[t] >> 16)) * wl[2 * t + 1];
+      }
+    }
+#pragma unroll
+    for (int o = 4; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kWave);
+    if (qd == 0 && i < n) {
+      const float yi = acc + b;
+      if (y) y[i] = yi;
+      const double zi = s * (double)d[i] * (double)yi;
+      z[i] = zi;
+      if (q) loc[0] += zi * q[i];
+    }
+  }
+  if (zq_part) {
+    const int which[1] = {0};
+    tmx::write_parts<NT, 1>(zq_part, kMaxParts, which, loc);
+  }
+}
+
+// g_h[i][f] = g_y_i wd[f]; partials dwd[f] = sum h[i][f] g_y_i, dbd = sum g_y_i
+__global__ __launch_bounds__(NT) void k_decode_bwd(int64_t n, const uint16_t* __restrict__ h,
+                                                   const float* __restrict__ wd, const float* __restrict__ g_y,
+                                                   float* __restrict__ g_h, float* __restrict__ part) {
+  __shared__ float red[NT / kWave][F + 1];
+  const int f = threadIdx.x % F, wv = threadIdx.x / F;
+  const float wf = wd[f];
+  float a = 0.0f, ab = 0.0f;
+  for (int64_t i = blockIdx.x * (int64_t)(NT / F) + wv; i < n; i += (int64_t)gridDim.x * (NT / F)) {
+    const float g = g_y[i];
+    g_h[i * F + f] = g * wf;
+    a += bf2f(h[i * F + f]) * g;
+    ab += g;
+  }
+  red[wv][f] = a;
+  if (f == 0) red[wv][F] = ab;
+  __syncthreads();
+  if (threadIdx.x <= F) {
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < NT / kWave; ++k) s += red[k][threadIdx.x];
+    part[(int64_t)blockIdx.x * (F + 1) + threadIdx.x] = s;
+  }
+}
+
+// Training loss (energy functional of the normalised system A^ y = r~):
+//   L = (y.A^y / 2 - y.r~) / n,  g_y = (A^ y - r~) / n,  r~ = d r / s
+// Partials: [0] y.A^y, [1] y.r~, [2] e~.r~ with e~ = e / (d s) (e optional).
+__global__ __launch_bounds__(NT) void k_energy_loss(int64_t n, const int32_t* __restrict__ rp,
+                                                    const int32_t* __restrict__ ci, const float* __restrict__ av,
+                                                    const float* __restrict__ y, const double* __restrict__ r,
+                                                    const float* __restrict__ d, const double* __restrict__ s_in,
+                                                    const double* __restrict__ e, float* __restrict__ g_y,
+                                                    double* __restrict__ part) {
+  const double s = *s_in;
+  const double invn = n > 0 ? 1.0 / (double)n : 0.0;
+  double loc[3] = {0.0, 0.0, 0.0};
+  for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+    double ay = 0.0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) ay += (double)av[k] * (double)y[ci[k]];
+    const double rt = (double)d[i] * r[i] / s;
+    g_y[i] = (float)((ay - rt) * invn);
+    loc[0] += (double)y[i] * ay;
+    loc[1] += (double)y[i] * rt;
+    if (e) loc[2] += e[i] / ((double)d[i] * s) * rt;
+  }
+  const int which[3] = {0, 1, 2};
+  tmx::write_parts<NT, 3>(part, kMaxParts, which, loc);
+}
+
+// ---------------------------------------------------------------------------
+// Level set-up for one matrix
+// ---------------------------------------------------------------------------
+// d_i = A_ii^-1/2 (1 where the diagonal is missing or not positive)
+__global__ __launch_bounds__(NT) void k_level_diag(int64_t n, const int32_t* __restrict__ rp,
+                                                   const int32_t* __restrict__ ci, const float* __restrict__ v,
+                                                   float* __restrict__ d) {
+  for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+    double a = 0.0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k)
+      if (ci[k] == i) a += (double)v[k];
+    d[i] = a > 0.0 ? (float)(1.0 / sqrt(a)) : 1.0f;
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_level_normalize(int64_t n, const int32_t* __restrict__ rp,
+                                                        const int32_t* __restrict__ ci, const float* __restrict__ v,
+                                                        const float* __restrict__ d, float* __restrict__ vn) {
+  for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+    const float di = d[i];
+    for (int k = rp[i]; k < rp[i + 1]; ++k) vn[k] = v[k] * di * d[ci[k]];
+  }
+}
+
+// Galerkin coarse values: vc[s] = sum over fine entries mapped to coarse entry s
+__global__ __launch_bounds__(NT) void k_level_coarsen(int64_t nnz_c, const int32_t* __restrict__ seg,
+                                                      const int32_t* __restrict__ perm,
+                                                      const float* __restrict__ vf, float* __restrict__ vc) {
+  for (int64_t s = blockIdx.x * (int64_t)NT + threadIdx.x; s < nnz_c; s += (int64_t)gridDim.x * NT) {
+    double a = 0.0;
+    for (int k = seg[s]; k < seg[s + 1]; ++k) a += (double)vf[perm[k]];
+    vc[s] = (float)a;
+  }
+}
+
+// pooling weights w_i = d_c[agg_i] / d_i  (= sqrt(A_ii / Ac_II))
+__global__ __launch_bounds__(NT) void k_level_poolw(int64_t nf, const int32_t* __restrict__ agg,
+                                                    const float* __restrict__ df, const float* __restrict__ dc,
+                                                    float* __restrict__ w) {
+  for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < nf; i += (int64_t)gridDim.x * NT)
+    w[i] = dc[agg[i]] / df[i];
+}
+
+// Adam (bias-corrected), FP32 master weights
+__global__ __launch_bounds__(NT) void k_adam(int64_t n, float* __restrict__ p, const float* __restrict__ g,
+                                             float* __restrict__ m, float* __restrict__ v, float lr, float b1,
+                                             float b2, float eps, float c1, float c2) {
+  for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Flexible CG, truncated to one previous direction (FCG(1)):
+//   z = M(r); beta = (z, q_old) / (p_old, q_old); p = z - beta p_old
+//   q = A p; alpha = (p, r) / (p, q); x += alpha p; r -= alpha q
+// ---------------------------------------------------------------------------
+struct FcgScalars {
+  double bb, tol2, pq, relres, s;
+};
+struct FcgFlags {
+  int status;  // 0 running, 1 converged, 2 breakdown, 3 max iterations
+  int iter;
+  long long maxiter;
+};
+constexpr int Q_PQ = 0, Q_PR = 1, Q_RR = 2, Q_BB = 3, Q_ZQ = 4, Q_SS = 5;
+constexpr int kFcgSlots = 6;
+
+struct FcgWork {
+  double *r, *p, *q, *z, *part;
+  FcgScalars* s;
+  FcgFlags* f;
+};
+
+FcgWork fcg_carve(void* work, int64_t n) {
+  FcgWork w;
+  double* b = (double*)work;
+  w.r = b;
+  w.p = b + n;
+  w.q = b + 2 * n;
+  w.z = b + 3 * n;
+  w.part = b + 4 * n;
+  w.s = (FcgScalars*)(w.part + kFcgSlots * kMaxParts);
+  w.f = (FcgFlags*)(w.s + 1);
+  return w;
+}
+
+// r = b - q (q = A x0), p = q = 0; partials (r,r), (b,b), (d r)^2
+__global__ __launch_bounds__(NT) void k_fcg_init(int64_t n, const double* __restrict__ b, const float* __restrict__ d,
+                                                 FcgWork w) {
+  double loc[3] = {0.0, 0.0, 0.0};
+  for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+    const double ri = b[i] - w.q[i];
+    w.r[i] = ri;
+    w.p[i] = 0.0;
+    w.q[i] = 0.0;
+    loc[0] += ri * ri;
+    loc[1] += b[i] * b[i];
+    const double dr = (double)d[i] * ri;
+    loc[2] += dr * dr;
+  }
+  const int which[3] = {Q_RR, Q_BB, Q_SS};
+  tmx::write_parts<NT, 3>(w.part, kMaxParts, which, loc);
+}
+
+__global__ __launch_bounds__(NT) void k_fcg_ctl0(FcgWork w, int nparts, double rtol2, double atol2,
+                                                 long long maxiter) {
+  const int which[2] = {Q_RR, Q_BB};
+  double t[2];
+  tmx::sum_parts<2>(w.part, kMaxParts, which, nparts, t);
+  if (threadIdx.x == 0) {
+    w.s->bb = t[1];
+    w.s->tol2 = fmax(rtol2 * t[1], atol2);
+    w.s->pq = 1.0;
+    w.s->relres = t[1] > 0.0 ? sqrt(t[0] / t[1]) : 0.0;
+    w.f->iter = 0;
+    w.f->maxiter = maxiter;
+    w.f->status = (t[0] <= w.s->tol2) ? 1 : 0;
+  }
+}
+
+// beta = (z, q_old) / (p_old, q_old), p = z - beta p  (zq_nparts partials of (z, q_old))
+__global__ __launch_bounds__(NT) void k_fcg_dir(int64_t n, FcgWork w, int zq_nparts) {
+  if (w.f->status != 0) return;
+  const int which[1] = {Q_ZQ};
+  double t[1];
+  tmx::sum_parts<1>(w.part, kMaxParts, which, zq_nparts, t);
+  const double beta = t[0] / w.s->pq;
+  for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
+    w.p[i] = w.z[i] - beta * w.p[i];
+}
+
+// q = A p; partials (p, q), (p, r).  One wave per 64/S rows, S lanes per row.
+template <int S>
+__global__ __launch_bounds__(NT) void k_fcg_spmv(int64_t n, const int32_t* __restrict__ rp,
+                                                 const int32_t* __restrict__ ci, const float* __restrict__ v,
+                                                 FcgWork w) {
+  if (w.f->status != 0) return;
+  const int sl = threadIdx.x % S;
+  double loc[2] = {0.0, 0.0};
+  for (int64_t row = blockIdx.x * (int64_t)(NT / S) + threadIdx.x / S; row < n;
+       row += (int64_t)gridDim.x * (NT / S)) {
+    double o = 0.0;
+    const int b = rp[row], e = rp[row + 1];
+    for (int k = b + sl; k < e; k += S) o += (double)v[k] * w.p[ci[k]];
+#pragma unroll
+    for (int dd = S / 2; dd > 0; dd >>= 1) o += __shfl_xor(o, dd, kWave);
+    if (sl == 0) {
+      w.q[row] = o;
+      loc[0] += o * w.p[row];
+      loc[1] += w.p[row] * w.r[row];
+    }
+  }
+  const int which[2] = {Q_PQ, Q_PR};
+  tmx::write_parts<NT, 2>(w.part, kMaxParts, which, loc);
+}
+
+// alpha = (p, r) / (p, q); x += alpha p; r -= alpha q; partials (r, r), (d r)^2
+__global__ __launch_bounds__(NT) void k_fcg_update(int64_t n, double* __restrict__ x, const float* __restrict__ d,
+                                                   FcgWork w, int nparts) {
+  if (w.f->status != 0) return;
+  const int which_in[2] = {Q_PQ, Q_PR};
+  double t[2];
+  tmx::sum_parts<2>(w.part, kMaxParts, which_in, nparts, t);
+  const bool ok = t[0] > 0.0 && __builtin_isfinite(t[0]) && __builtin_isfinite(t[1]);
+  const double alpha = ok ? t[1] / t[0] : 0.0;
+  double loc[2] = {0.0, 0.0};
+  if (ok) {
+    for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+      x[i] += alpha * w.p[i];
+      const double ri = w.r[i] - alpha * w.q[i];
+      w.r[i] = ri;
+      loc[0] += ri * ri;
+      const double dr = (double)d[i] * ri;
+      loc[1] += dr * dr;
+    }
+  }
+  const int which[2] = {Q_RR, Q_SS};
+  tmx::write_parts<NT, 2>(w.part, kMaxParts, which, loc);
+}
+
+// Single workgroup: pq for the next beta, iteration count, convergence test.
+__global__ __launch_bounds__(NT) void k_fcg_check(FcgWork w, int nparts) {
+  if (w.f->status != 0) return;
+  const int which[2] = {Q_PQ, Q_RR};
+  double t[2];
+  tmx::sum_parts<2>(w.part, kMaxParts, which, nparts, t);
+  if (threadIdx.x == 0) {
+    const int it = w.f->iter + 1;
+    w.f->iter = it;
+    if (!(t[0] > 0.0) || !__builtin_isfinite(t[0]) || !__builtin_isfinite(t[1])) {
+      w.f->status = 2;
+      return;
+    }
+    w.s->pq = t[0];
+    w.s->relres = w.s->bb > 0.0 ? sqrt(t[1] / w.s->bb) : 0.0;
+    if (t[1] <= w.s->tol2)
+      w.f->status = 1;
+    else if (it >= w.f->maxiter)
+      w.f->status = 3;
+  }
+}
+
+int grid_of(int64_t work, int per_block, int cap) {
+  int64_t g = (work + per_block - 1) / per_block;
+  return (int)(g < 1 ? 1 : (g > cap ? cap : g));
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" int tm_gnn_mp_forward(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values,
+                                 const uint16_t* h, const uint16_t* w_t, const float* bias, int n_linear,
+                                 uint16_t* out, uint16_t* save_msg, uint16_t* save_pre, const int* stop,
+                                 void* stream) {
+  TM_REQUIRE(n >= 0 && row_ptr && col_idx && values && h && w_t && bias && out, TM_ERR_ARG, "null argument");
+  TM_REQUIRE((save_msg == nullptr) == (save_pre == nullptr), TM_ERR_ARG, "save_msg and save_pre go together");
+  TM_REQUIRE(out != h, TM_ERR_ARG, "tm_gnn_mp_forward: out must not alias h");
+  if (n == 0) return TM_OK;
+  const int64_t tiles = (n + TILE - 1) / TILE;
+  TM_REQUIRE(tiles <= INT32_MAX, TM_ERR_LIMIT, "too many rows");
+  hipStream_t st = (hipStream_t)stream;
+  MpOut o{};
+  o.out = out;
+  o.s_agg = save_msg;
+  o.s_u = save_pre;
+  if (save_msg)
+    hipLaunchKernelGGL((k_mp<Dense, true, false>), dim3((unsigned)tiles), dim3(NT), 0, st, n, row_ptr, col_idx,
+                       values, Dense{h}, w_t, bias, n_linear, o, stop);
+  else
+    hipLaunchKernelGGL((k_mp<Dense, false, false>), dim3((unsigned)tiles), dim3(NT), 0, st, n, row_ptr, col_idx,
+                       values, Dense{h}, w_t, bias, n_linear, o, stop);
+  TM_HIP(hipGetLastError());
+  return TM_OK;
+}
+
+extern "C" int tm_gnn_encode_pool(int64_t n, int64_t n_coarse, const int32_t* child_ptr, const int32_t* child,
+                                  const float* w, const double* r, const float* d, const float* lk,
+                                  const double* ss_part, int nparts, const float* we, const float* be, int n_linear,
+                                  uint16_t* out, double* scale, const int* stop, void* stream) {
+  TM_REQUIRE(n >= 0 && n_coarse >= 0 && child_ptr && child && w && r && d && lk && ss_part && we && be && out &&
+                 scale,
+             TM_ERR_ARG, "null argument");
+  TM_REQUIRE(nparts >= 1 && nparts <= kMaxParts, TM_ERR_ARG, "encode_pool: bad partial count %d", nparts);
+  const int g = grid_of(n_coarse, NT / 8, 8192);
+  hipLaunchKernelGGL(k_encode_pool, dim3(g), dim3(NT), 0, (hipStream_t)stream, n, n_coarse, child_ptr, child, w, r,
+                     d, lk, ss_part, nparts, we, be, n_linear, out, scale, stop);
+  TM_HIP(hipGetLastError());
+  return TM_OK;
+}
+
+extern "C" int tm_gnn_unpool_mp_decode(int64_t n, const int32_t* row_ptr, const int32_t* col_idx,
+                                       const float* values, const double* r, const float* d, const float* lk,
+                                       const int32_t* agg, const float* w, const uint16_t* hc, const float* we,
+                                       const float* be, const uint16_t* w_t, const float* bias, int n_linear,
+                                       const float* wd, const float* bd, const double* scale, double* z,
+                                       const double* q, double* zq_part, int* zq_nparts, const int* stop,
+                                       void* stream) {
+  TM_REQUIRE(n >= 0 && row_ptr && col_idx && values && r && d && lk && agg && w && hc && we && be && w_t && bias &&
+                 wd && bd && scale && z,
+             TM_ERR_ARG, "null argument");
+  TM_REQUIRE((q == nullptr) == (zq_part == nullptr), TM_ERR_ARG, "q and zq_part go together");
+  if (n == 0) return TM_OK;
+  const int64_t tiles = (n + TILE - 1) / TILE;
+  const int g = grid_of(tiles, 1, kMaxParts);
+  if (zq_nparts) *zq_nparts = g;
+  MpOut o{};
+  o.wd = wd;
+  o.bd = bd;
+  o.d = d;
+  o.scale = scale;
+  o.z = z;
+  o.q = q;
+  o.zq_part = zq_part;
+  Unpooled src{};
+  src.r = r;
+  src.d = d;
+  src.lk = lk;
+  src.agg = agg;
+  src.w = w;
+  src.hc = hc;
+  src.we = we;
+  src.be = be;
+  src.scale = scale;
+  src.n_lin = n_linear;
+  hipLaunchKernelGGL((k_mp<Unpooled, false, true>), dim3(g), dim3(NT), 0, (hipStream_t)stream, n, row_ptr, col_idx,
+                     values, src, w_t, bias, n_linear, o, stop);
+  TM_HIP(hipGetLastError());
+  return TM_OK;
+}
+
+extern "C" int64_t tm_gnn_part_floats(void) { return (int64_t)kMaxParts * NWP; }
+
+extern "C" int tm_gnn_mp_backward(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values,
+                                  const float* g_out, const uint16_t* h, const uint16_t* msg, const uint16_t* pre,
+                                  const uint16_t* w, int n_linear, float* g_in, float* g_msg, float* part,
+                                  float* grad_w, void* stream) {
+  TM_REQUIRE(n >= 0 && row_ptr && col_idx && values && g_out && h && msg && pre && w && g_in && g_msg && part &&
+                 grad_w,
+             TM_ERR_ARG, "null argument");
+  TM_REQUIRE(g_in != g_out, TM_ERR_ARG, "tm_gnn_mp_backward: g_in must not alias g_out");
+  hipStream_t st = (hipStream_t)stream;
+  const int G = grid_of((n + TILE - 1) / TILE, 1, 512);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_mp_bwd, dim3(G), dim3(NT), 0, st, n, g_out, h, msg, pre, w, n_linear, g_in, g_msg, part);
+    hipLaunchKernelGGL(k_msg_add, dim3(grid_of(n, NT / kWave, 8192)), dim3(NT), 0, st, n, row_ptr, col_idx, values,
+                       g_msg, g_in);
+    hipLaunchKernelGGL(k_reduce_parts, dim3(grid_of(NWP, NT, 64)), dim3(NT), 0, st, G, NWP, part, grad_w, 1);
+  }
+  TM_HIP(hipGetLastError());
+  return TM_OK;
+}
+
+extern "C" int tm_gnn_pool(int64_t n_coarse, const int32_t* child_ptr, const int32_t* child, const float* w,
+                           const void* h, void* out, int dtype, const int* stop, void* stream) {
+  TM_REQUIRE(n_coarse >= 0 && child_ptr && child && w && h && out, TM_ERR_ARG, "null argument");
+  TM_REQUIRE(dtype == TM_BF16 || dtype == TM_F32, TM_ERR_ARG, "pool: dtype must be bf16 or f32");
+  if (n_coarse == 0) return TM_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const int g = grid_of(n_coarse, NT / 8, 8192);
+  if (dtype == TM_BF16)
+    hipLaunchKernelGGL(k_pool<uint16_t>, dim3(g), dim3(NT), 0, st, n_coarse, child_ptr, child, w,
+                       (const uint16_t*)h, (uint16_t*)out, stop);
+  else
+    hipLaunchKernelGGL(k_pool<float>, dim3(g), dim3(NT), 0, st, n_coarse, child_ptr, child, w, (const float*)h,
+                       (float*)out, stop);
+  TM_HIP(hipGetLastError());
+  return TM_OK;
+}
+
+extern "C" int tm_gnn_unpool(int64_t n_fine, const int32_t* agg, const float* w, const void* hc, const void* skip,
+                             void* out, int dtype, const int* stop, void* stream) {
+  TM_REQUIRE(n_fine >= 0 && agg && w && hc && out, TM_ERR_ARG, "null argument");
+  TM_REQUIRE(dtype == TM_BF16 || dtype == TM_F32, TM_ERR_ARG, "unpool: dtype must be bf16 or f32");
+  if (n_fine == 0) return TM_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const int g = grid_of(n_fine * 8, NT, 16384);
+  if (dtype == TM_BF16)
+    hipLaunchKernelGGL(k_unpool<uint16_t>, dim3(g), dim3(NT), 0, st, n_fine, agg, w, (const uint16_t*)hc,
+                       (const uint16_t*)skip, (uint16_t*)out, stop);
+  else
+    hipLaunchKernelGGL(k_unpool<float>, dim3(g), dim3(NT), 0, st, n_fine, agg, w, (const float*)hc,
+                       (const float*)skip, (float*)out, stop);
+  TM_HIP(hipGetLastError());
+  return TM_OK;
+}
+
+extern "C" int tm_gnn_sumsq(int64_t n, const double* r, const float* d, double* ss_part, int* nparts,
+                            void* stream) {
+  TM_REQUIRE(n >= 0 && r && d && ss_part && nparts, TM_ERR_ARG, "null argument");
+  const int g = grid_of(n, NT * 4, kMaxParts);
+  *nparts = g;
+  hipLaunchKernelGGL(k_sumsq, dim3(g), dim3(NT), 0, (hipStream_t)stream, n, r, d, ss_part);
+  TM_HIP(hipGetLastError());
+  return TM_OK;
+}
+
+extern "C" int tm_gnn_encode(int64_t n, const double* r, const float* d, const float* lk, const double* ss_part,
+                             int nparts, const float* we, const float* be, int n_linear, uint16_t* h0,
+                             double* scale, const int* stop, void* stream) {
+  TM_REQUIRE(n >= 0 && r && d && lk && ss_part && we && be && h0 && scale, TM_ERR_ARG, "null argument");
+  TM_REQUIRE(nparts >= 1 && nparts <= kMaxParts, TM_ERR_ARG, "encode: bad partial count %d", nparts);
+  const int g = grid_of(n, NT / 8, 4096);
+  hipLaunchKernelGGL(k_encode, dim3(g), dim3(NT), 0, (hipStream_t)stream, n, r, d, lk, ss_part, nparts, we, be,
+                     n_linear, h0, scale, stop);
+  TM_HIP(hipGetLastError());
+  return TM_OK;
+}
+
+extern "C" int tm_gnn_encode_backward(int64_t n, const double* r, const float* d, const float* lk,
+                                      const double* scale, const float* we, const float* be, int n_linear,
+                                      const float* g_h0, float* part, float* grad, void* stream) {
+  TM_REQUIRE(n >= 0 && r && d && lk && scale && we && be && g_h0 && part && grad, TM_ERR_ARG, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  const int G = grid_of(n, NT / F * 16, kMaxParts);
+  hipLaunchKernelGGL(k_encode_bwd, dim3(G), dim3(NT), 0, st, n, r, d, lk, scale, we, be, n_linear, g_h0, part);
+  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(NT), 0, st, G, 3 * F, part, grad, 1);
+  TM_HIP(hipGetLastError());
+  return TM_OK;
+}
+
+extern "C" int tm_gnn_decode(int64_t n, const uint16_t* h, const float* wd, const float* bd, const float* d,
+                             const double* scale, float* y, double* z, const double* q, double* zq_part,
+                             int* zq_nparts, const int* stop, void* stream) {
+  TM_REQUIRE(n >= 0 && h && wd && bd && d && scale && z, TM_ERR_ARG, "null argument");
+  TM_REQUIRE((q == nullptr) == (zq_part == nullptr), TM_ERR_ARG, "decode: q and zq_part go together");
+  // with (z, q) partials the grid is the flexible-CG grid, whose kernels sum them
+  const int g = zq_part ? grid_of(n, NT * 2, kMaxParts) : grid_of(n, NT / 8, kMaxParts);
+  if (zq_nparts) *zq_nparts = g;
+  hipLaunchKernelGGL(k_decode, dim3(g), dim3(NT), 0, (hipStream_t)stream, n, h, wd, bd, d, scale, y, z, q, zq_part,
+                     stop);
+  TM_HIP(hipGetLastError());
+  return TM_OK;
+}
+
+extern "C" int tm_gnn_decode_backward(int64_t n, const uint16_t* h, const float* wd, const float* g_y, float* g_h,
+                                      float* part, float* grad, void* stream) {
+  TM_REQUIRE(n >= 0 && h && wd && g_y && g_h && part && grad, TM_ERR_ARG, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  const int G = grid_of(n, NT / F * 16, kMaxParts);
+  hipLaunchKernelGGL(k_decode_bwd, dim3(G), dim3(NT), 0, st, n, h, wd, g_y, g_h, part);
+  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(NT), 0, st, G, F + 1, part, grad, 1);
+  TM_HIP(hipGetLastError());
+  return TM_OK;
+}
+
+extern "C" int tm_gnn_energy_loss(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values,
+                                  const float* y, const double* r, const float* d, const double* scale,
+                                  const double* e, float* g_y, double* part, int* nparts, void* stream) {
+  TM_REQUIRE(n >= 0 && row_ptr && col_idx && values && y && r && d && scale && g_y && part && nparts, TM_ERR_ARG,
+             "null argument");
+  const int g = grid_of(n, NT * 4, kMaxParts);
+  *nparts = g;
+  hipLaunchKernelGGL(k_energy_loss, dim3(g), dim3(NT), 0, (hipStream_t)stream, n, row_ptr, col_idx, values, y, r, d,
+                     scale, e, g_y, part);
+  TM_HIP(hipGetLastError());
+  return TM_OK;
+}
+
+extern "C" int tm_gnn_level_setup(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values,
+                                  float* d, float* values_normalized, void* stream) {
+  TM_REQUIRE(n >= 0 && row_ptr && col_idx && values && d && values_normalized, TM_ERR_ARG, "null argument");
+  if (n == 0) return TM_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const int g = grid_of(n, NT, 8192);
+  hipLaunchKernelGGL(k_level_diag, dim3(g), dim3(NT), 0, st, n, row_ptr, col_idx, values, d);
+  hipLaunchKernelGGL(k_level_normalize, dim3(g), dim3(NT), 0, st, n, row_ptr, col_idx, values, d, values_normalized);
+  TM_HIP(hipGetLastError());
+  return TM_OK;
+}
+
+extern "C" int tm_gnn_level_coarsen(int64_t nnz_coarse, const int32_t* seg, const int32_t* perm,
+                                    const float* fine_values, float* coarse_values, void* stream) {
+  TM_REQUIRE(nnz_coarse >= 0 && seg && perm && fine_values && coarse_values, TM_ERR_ARG, "null argument");
+  if (nnz_coarse == 0) return TM_OK;
+  hipLaunchKernelGGL(k_level_coarsen, dim3(grid_of(nnz_coarse, NT, 16384)), dim3(NT), 0, (hipStream_t)stream,
+                     nnz_coarse, seg, perm, fine_values, coarse_values);
+  TM_HIP(hipGetLastError());
+  return TM_OK;
+}
+
+extern "C" int tm_gnn_pool_weights(int64_t n_fine, const int32_t* agg, const float* d_fine, const float* d_coarse,
+                                   float* w, void* stream) {
+  TM_REQUIRE(n_fine >= 0 && agg && d_fine && d_coarse && w, TM_ERR_ARG, "null argument");
+  if (n_fine == 0) return TM_OK;
+  hipLaunchKernelGGL(k_level_poolw, dim3(grid_of(n_fine, NT, 8192)), dim3(NT), 0, (hipStream_t)stream, n_fine, agg,
+                     d_fine, d_coarse, w);
+  TM_HIP(hipGetLastError());
+  return TM_OK;
+}
+
+extern "C" int tm_gnn_adam(int64_t n, float* param, const float* grad, float* m, float* v, float lr, float beta1,
+                           float beta2, float eps, int64_t step, void* stream) {
+  TM_REQUIRE(n >= 0 && param && grad && m && v && step >= 1, TM_ERR_ARG, "bad adam arguments");
+  if (n == 0) return TM_OK;
+  const float c1 = 1.0f - powf(beta1, (float)step), c2 = 1.0f - powf(beta2, (float)step);
+  hipLaunchKernelGGL(k_adam, dim3(grid_of(n, NT, 1024)), dim3(NT), 0, (hipStream_t)stream, n, param, grad, m, v, lr,
+                     beta1, beta2, eps, c1, c2);
+  TM_HIP(hipGetLastError());
+  return TM_OK;
+}
+
+// ---- flexible CG ----------------------------------------------------------
+extern "C" int64_t tm_fcg_work_size(int64_t n) {
+  return (int64_t)sizeof(double) * (4 * n + kFcgSlots * kMaxParts) + (int64_t)sizeof(FcgScalars) +
+         (int64_t)sizeof(FcgFlags) + 64;
+}
+
+extern "C" int tm_fcg_layout(int64_t n, void* work, tm_fcg_view* view) {
+  TM_REQUIRE(n >= 0 && work && view, TM_ERR_ARG, "null argument");
+  FcgWork w = fcg_carve(work, n);
+  view->r = w.r;
+  view->z = w.z;
+  view->q = w.q;
+  view->zq_part = w.part + Q_ZQ * kMaxParts;
+  view->ss_part = w.part + Q_SS * kMaxParts;
+  view->scale = &w.s->s;
+  view->relres = &w.s->relres;
+  view->status = &w.f->status;
+  view->iter = &w.f->iter;
+  view->nparts = grid_of(n, NT * 2, kMaxParts);
+  return TM_OK;
+}
+
+extern "C" int tm_fcg_start(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values,
+                            const double* b, const double* x, const float* d, void* work, double rtol, double atol,
+                            int64_t maxiter, void* stream) {
+  TM_REQUIRE(n > 0 && row_ptr && col_idx && values && b && x && d && work, TM_ERR_ARG, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  FcgWork w = fcg_carve(work, n);
+  const int G = grid_of(n, NT * 2, kMaxParts);
+  // q = A x0 (p holds x0 for the product)
+  TM_HIP(hipMemcpyAsync(w.p, x, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  TM_HIP(hipMemsetAsync(w.f, 0, sizeof(FcgFlags), st));
+  hipLaunchKernelGGL(k_fcg_spmv<8>, dim3(G), dim3(NT), 0, st, n, row_ptr, col_idx, values, w);
+  hipLaunchKernelGGL(k_fcg_init, dim3(G), dim3(NT), 0, st, n, b, d, w);
+  hipLaunchKernelGGL(k_fcg_ctl0, dim3(1), dim3(NT), 0, st, w, G, rtol * rtol, atol * atol, (long long)maxiter);
+  TM_HIP(hipGetLastError());
+  return TM_OK;
+}
+
+// One FCG iteration after the preconditioner wrote z and the (z, q_old)
+// partials into slot 4 of the work partials.
+extern "C" int tm_fcg_iterate(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values,
+                              double* x, const float* d, void* work, int zq_nparts, void* stream) {
+  TM_REQUIRE(n > 0 && row_ptr && col_idx && values && x && d && work, TM_ERR_ARG, "null argument");
+  TM_REQUIRE(zq_nparts >= 1 && zq_nparts <= kMaxParts, TM_ERR_ARG, "fcg: bad (z, q) partial count %d", zq_nparts);
+  hipStream_t st = (hipStream_t)stream;
+  FcgWork w = fcg_carve(work, n);
+  const int G = grid_of(n, NT * 2, kMaxParts);
+  hipLaunchKernelGGL(k_fcg_dir, dim3(G), dim3(NT), 0, st, n, w, zq_nparts);
+  hipLaunchKernelGGL(k_fcg_spmv<8>, dim3(G), dim3(NT), 0, st, n, row_ptr, col_idx, values, w);
+  hipLaunchKernelGGL(k_fcg_update, dim3(G), dim3(NT), 0, st, n, x, d, w, G);
+  hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(NT), 0, st, w, G);
+  TM_HIP(hipGetLastError());
+  return TM_OK;
+}

@@ -6,15 +6,17 @@
 //   K2  x += alpha p, r -= alpha q, z = D^-1 r, (r, z) and (r, r)
 //                                        -> beta, convergence test
 //   K3  p = D^-1 r + beta p
-// Each reduction is finished inside its own kernel by the last workgroup to
-// arrive (fixed-order sum of the per-workgroup partials, so the result does
-// not depend on arrival order).  Scalars live in device memory; kernels return
-// immediately once the device-side status flag is set, so the host only reads
-// the flag every few iterations.
+// Each kernel writes one partial sum per workgroup; the next kernel, which
+// needs the scalar, sums the partials in a fixed order (so the result does not
+// depend on scheduling and no atomics or grid-wide fences are needed).
+// Scalars live in device memory; kernels return immediately once the
+// device-side status flag is set, so the host only reads the flag every few
+// iterations.
 #include <cstdlib>
 #include <vector>
 
 #include "tm_device.h"
+#include "tm_parts.h"
 
 namespace {
 
@@ -139,29 +141,12 @@ __global__ __launch_bounds__(kThreads) void k_spmv(int64_t nrows, const int32_t*
 // every workgroup of a kernel obtains bit-identical values.
 template <int NS>
 __device__ __forceinline__ void sum_parts(const double* part, const int (&which)[NS], int n, double (&out)[NS]) {
-  __shared__ double sh[NS];
-  if (threadIdx.x < kWave) {
-#pragma unroll
-    for (int i = 0; i < NS; ++i) {
-      double s = 0.0;
-      for (int k = threadIdx.x; k < n; k += kWave) s += part[which[i] * kMaxBlocks + k];
-      s = tmx::wave_sum(s);
-      if (threadIdx.x == 0) sh[i] = s;
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < NS; ++i) out[i] = sh[i];
+  tmx::sum_parts<NS>(part, kMaxBlocks, which, n, out);
 }
 
 template <int NS>
 __device__ __forceinline__ void write_parts(double* part, const int (&which)[NS], double (&loc)[NS]) {
-  __shared__ double red[kThreads / kWave];
-#pragma unroll
-  for (int i = 0; i < NS; ++i) {
-    const double b = tmx::block_sum<kThreads>(loc[i], red);
-    if (threadIdx.x == 0) part[which[i] * kMaxBlocks + blockIdx.x] = b;
-  }
+  tmx::write_parts<kThreads, NS>(part, kMaxBlocks, which, loc);
 }
 
 template <typename VT, int BS>

@@ -76,3 +76,33 @@ def lognormal_kappa_points(points: np.ndarray, length: float, sigma: float = 1.0
             wgt = wgt * (f[:, ax] if bit else 1.0 - f[:, ax])
         out += wgt * g[tuple(idx)]
     return np.exp(out)
+
+
+def grf_lattice_device(shape, spacing: float, length: float, sigma: float, seed: int,
+                       device, pad_factor: float = 4.0):
+    """:func:`grf_lattice` on the GPU (torch FFT), white noise from a seeded
+    torch generator on ``device``.  Returns a float64 tensor of ``shape``."""
+    import torch
+    shape = tuple(int(s) for s in shape)
+    pad = int(math.ceil(pad_factor * length / spacing))
+    big = tuple(s + pad for s in shape)
+    r2 = torch.zeros(big, dtype=torch.float64, device=device)
+    for ax, m in enumerate(big):
+        k = torch.arange(m, device=device, dtype=torch.float64)
+        d = torch.minimum(k, m - k) * spacing
+        sh = [1] * len(big)
+        sh[ax] = -1
+        r2 = r2 + (d ** 2).reshape(sh)
+    lam = torch.fft.rfftn(sigma ** 2 * torch.exp(-r2 / (2.0 * length ** 2))).real.clamp_(min=0.0)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed))
+    w = torch.randn(big, generator=gen, dtype=torch.float64, device=device)
+    g = torch.fft.irfftn(torch.sqrt(lam) * torch.fft.rfftn(w), s=big)
+    return g[tuple(slice(0, s) for s in shape)].contiguous()
+
+
+def lognormal_kappa_cube_device(n: int, length: float, sigma: float, seed: int, device):
+    """Nodal ``kappa = exp(g)`` (float32) on the ``(n+1)^3`` Kuhn lattice, x fastest."""
+    import torch
+    g = grf_lattice_device((n + 1,) * 3, 1.0 / n, length, sigma, seed, device)
+    return torch.exp(g).to(torch.float32).reshape(-1)
