@@ -255,32 +255,39 @@ int tm_gnn_unpool(int64_t n_fine, const int32_t* agg, const float* w, const void
                   int dtype, const int* stop, void* stream);
 /* Per-workgroup partials of sum (d_i r_i)^2 (for the input scale). */
 int tm_gnn_sumsq(int64_t n, const double* r, const float* d, double* ss_part, int* nparts, void* stream);
-/* h0 = act([d r / s, lk] We + be), s = rms(d r) from the partials; writes s. */
+/* Linear encoder h0 = [d r / s, lk] We + be, s = rms(d r) from the partials;
+ * writes s.  We is [2][64], be [64]. */
 int tm_gnn_encode(int64_t n, const double* r, const float* d, const float* lk, const double* ss_part, int nparts,
-                  const float* we, const float* be, int n_linear, uint16_t* h0, double* scale, const int* stop,
-                  void* stream);
+                  const float* we, const float* be, uint16_t* h0, double* scale, const int* stop, void* stream);
+/* grad[2*64 + 64] += (dWe | dbe) for dL/dh0 = g_h0. */
 int tm_gnn_encode_backward(int64_t n, const double* r, const float* d, const float* lk, const double* scale,
-                           const float* we, const float* be, int n_linear, const float* g_h0, float* part,
-                           float* grad, void* stream);
-/* y = h wd + bd (nullable output), z = s d y; with q, partials of (z, q)
- * (their count goes to *zq_nparts when non-null). */
+                           const float* g_h0, float* part, float* grad, void* stream);
+/* y = h wd + bd (nullable output), z = s d y. */
 int tm_gnn_decode(int64_t n, const uint16_t* h, const float* wd, const float* bd, const float* d,
-                  const double* scale, float* y, double* z, const double* q, double* zq_part, int* zq_nparts,
-                  const int* stop, void* stream);
-/* Inference fusions of the finest level (same values as the unfused kernels):
- * encode followed by the first pooling, writing the coarse features and s; and
- * unpool (skip = encoder output recomputed from r, lk) + message passing on
- * level 0 + decode, writing z (and the (z, q) partials). */
+                  const double* scale, float* y, double* z, const int* stop, void* stream);
+/* Inference fusions of the finest level (the unfused kernels compute the same
+ * values up to BF16 rounding of intermediate level-0 features):
+ *  - tm_gnn_encode_pool: encoder + first pooling, writing the coarse
+ *    features, s and xs[i] = (d_i r_i / s, lk_i) (float2 per node);
+ *  - tm_gnn_final: unpool (skip = encoder output) + message passing on level 0
+ *    + decode, using xs, the row sums of A^ and B = A^ P^ (b_* arrays, see
+ *    tm_gnn_level_prolong); writes z. */
 int tm_gnn_encode_pool(int64_t n, int64_t n_coarse, const int32_t* child_ptr, const int32_t* child, const float* w,
                        const double* r, const float* d, const float* lk, const double* ss_part, int nparts,
-                       const float* we, const float* be, int n_linear, uint16_t* out, double* scale, const int* stop,
+                       const float* we, const float* be, uint16_t* out, float* xs, double* scale, const int* stop,
                        void* stream);
-int tm_gnn_unpool_mp_decode(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values,
-                            const double* r, const float* d, const float* lk, const int32_t* agg, const float* w,
-                            const uint16_t* hc, const float* we, const float* be, const uint16_t* w_t,
-                            const float* bias, int n_linear, const float* wd, const float* bd, const double* scale,
-                            double* z, const double* q, double* zq_part, int* zq_nparts, const int* stop,
-                            void* stream);
+int tm_gnn_final(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values, const float* xs,
+                 const float* row_sums, const int32_t* agg, const float* w, const uint16_t* hc,
+                 const int32_t* b_row_ptr, const int32_t* b_col, const float* b_values, const float* we,
+                 const float* be, const uint16_t* w_t, const float* bias, int n_linear, const float* wd,
+                 const float* bd, const float* d, const double* scale, double* z, const int* stop,
+                 void* stream);
+/* Finest-level prolongation B = A^ P^: b_values[s] = sum of A^_k w_col(k) over
+ * the entries k in perm[seg[s] .. seg[s+1]).  Row sums of a CSR. */
+int tm_gnn_level_prolong(int64_t nnz_b, const int32_t* seg, const int32_t* perm, const int32_t* col_idx,
+                         const float* values_normalized, const float* w, float* b_values, void* stream);
+int tm_gnn_row_sums(int64_t n, const int32_t* row_ptr, const float* values, float* out, void* stream);
+/* g_h[i] = g_y[i] wd; grad[64 + 1] += (dwd | dbd). */
 int tm_gnn_decode_backward(int64_t n, const uint16_t* h, const float* wd, const float* g_y, float* g_h,
                            float* part, float* grad, void* stream);
 /* Energy loss of the normalised system: g_y = (A^ y - d r / s) / n and
@@ -301,15 +308,15 @@ int tm_gnn_adam(int64_t n, float* param, const float* grad, float* m, float* v, 
                 float eps, int64_t step, void* stream);
 
 /* Flexible CG (truncated to one direction) around an external preconditioner:
- * tm_fcg_start, then per iteration the preconditioner writes z = M(r) and the
- * (z, q) partials (tm_gnn_decode or tm_gnn_unpool_mp_decode with q and zq_part
- * of the view) and tm_fcg_iterate finishes the iteration.  Everything is stream-ordered with
- * device-side scalars, so iterations can be captured in a hipGraph. */
+ * tm_fcg_start, then per iteration the preconditioner writes z = M(r) into the
+ * view and tm_fcg_iterate finishes the iteration (one SpMV with its dot
+ * products, one update, one single-workgroup convergence check).  Everything
+ * is stream-ordered with device-side scalars, so iterations can be captured
+ * in a hipGraph.  ss_part holds the nparts partials of sum (d r)^2 that the
+ * encoder turns into its input scale. */
 typedef struct {
   double* r;
   double* z;
-  double* q;
-  double* zq_part;
   double* ss_part;
   double* scale;
   double* relres;
@@ -323,9 +330,8 @@ int tm_fcg_layout(int64_t n, void* work, tm_fcg_view* view);
 int tm_fcg_start(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values, const double* b,
                  const double* x, const float* d, void* work, double rtol, double atol, int64_t maxiter,
                  void* stream);
-/* zq_nparts: how many (z, q) partials the preconditioner wrote. */
 int tm_fcg_iterate(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values, double* x,
-                   const float* d, void* work, int zq_nparts, void* stream);
+                   const float* d, void* work, void* stream);
 
 #ifdef __cplusplus
 }

@@ -11,7 +11,7 @@ definition the HIP kernels are checked against:
   unpool ``h = skip + w (P h_c)``;
 * message passing ``h' = h + act([h | A^ h] W + b)``, act = identity on the
   first ``n_lin`` channels and ReLU on the rest;
-* encoder ``act([d r / s, lk] We + be)`` with ``s = rms(d r)``, decoder
+* linear encoder ``[d r / s, lk] We + be`` with ``s = rms(d r)``, decoder
   ``y = h wd + bd``, output ``z = s d y``;
 * loss ``L = (y^T A^ y / 2 - y^T r~) / n`` with ``r~ = d r / s``.
 
@@ -121,7 +121,7 @@ def forward(levels, params, schedule, n_lin, r, lk, bf16=False):
     if bf16:  # the kernel evaluates the encoder in float32
         x = x.astype(np.float32).astype(np.float64)
     pre0 = x @ we + be
-    bufs = {0: rnd(_act(pre0, n_lin))}
+    bufs = {0: rnd(pre0)}
     ops, final = _program(schedule)
     tape = {"x": x, "pre0": pre0, "ops": ops, "final": final, "bufs": bufs, "msg": {}, "pre": {}}
     for i, (kind, lev, a, b, extra) in enumerate(ops):
@@ -184,7 +184,7 @@ def backward(levels, params, schedule, n_lin, r, lk, tape, s, bf16=False):
         else:
             acc(extra, g)
             acc(a, L["P"].T @ (L["w"][:, None] * g))
-    g0 = grads.pop(0) * _dact(tape["pre0"], n_lin)
+    g0 = grads.pop(0)
     x = tape["x"]
     out = [(x.T @ g0).ravel(), g0.sum(0)]
     for (gW, gb) in g_layers:

@@ -47,9 +47,9 @@ class BatchedArgs(ctypes.Structure):
 
 
 class FcgView(ctypes.Structure):
-    _fields_ = [("r", c_void_p), ("z", c_void_p), ("q", c_void_p), ("zq_part", c_void_p),
-                ("ss_part", c_void_p), ("scale", c_void_p), ("relres", c_void_p), ("status", c_void_p),
-                ("iter", c_void_p), ("nparts", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+    _fields_ = [("r", c_void_p), ("z", c_void_p), ("ss_part", c_void_p), ("scale", c_void_p),
+                ("relres", c_void_p), ("status", c_void_p), ("iter", c_void_p), ("nparts", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 class TMError(RuntimeError):
@@ -100,13 +100,13 @@ _SIGS = {
     "tm_gnn_pool": (c_int, [c_i64] + [c_void_p] * 5 + [c_int, c_void_p, c_void_p]),
     "tm_gnn_unpool": (c_int, [c_i64] + [c_void_p] * 5 + [c_int, c_void_p, c_void_p]),
     "tm_gnn_sumsq": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_int), c_void_p]),
-    "tm_gnn_encode": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int,
-                              c_void_p, c_void_p, c_void_p, c_void_p]),
-    "tm_gnn_encode_backward": (c_int, [c_i64] + [c_void_p] * 6 + [c_int] + [c_void_p] * 4),
-    "tm_gnn_decode": (c_int, [c_i64] + [c_void_p] * 12),
-    "tm_gnn_encode_pool": (c_int, [c_i64, c_i64] + [c_void_p] * 7 + [c_int, c_void_p, c_void_p, c_int] +
-                           [c_void_p] * 4),
-    "tm_gnn_unpool_mp_decode": (c_int, [c_i64] + [c_void_p] * 13 + [c_int] + [c_void_p] * 9),
+    "tm_gnn_encode": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_void_p, c_int] + [c_void_p] * 6),
+    "tm_gnn_encode_backward": (c_int, [c_i64] + [c_void_p] * 8),
+    "tm_gnn_decode": (c_int, [c_i64] + [c_void_p] * 9),
+    "tm_gnn_encode_pool": (c_int, [c_i64, c_i64] + [c_void_p] * 7 + [c_int] + [c_void_p] * 7),
+    "tm_gnn_final": (c_int, [c_i64] + [c_void_p] * 15 + [c_int] + [c_void_p] * 7),
+    "tm_gnn_level_prolong": (c_int, [c_i64] + [c_void_p] * 7),
+    "tm_gnn_row_sums": (c_int, [c_i64] + [c_void_p] * 4),
     "tm_gnn_decode_backward": (c_int, [c_i64] + [c_void_p] * 7),
     "tm_gnn_energy_loss": (c_int, [c_i64] + [c_void_p] * 10 + [ctypes.POINTER(c_int), c_void_p]),
     "tm_gnn_level_setup": (c_int, [c_i64] + [c_void_p] * 6),
@@ -116,7 +116,7 @@ _SIGS = {
     "tm_fcg_work_size": (c_i64, [c_i64]),
     "tm_fcg_layout": (c_int, [c_i64, c_void_p, ctypes.POINTER(FcgView)]),
     "tm_fcg_start": (c_int, [c_i64] + [c_void_p] * 7 + [c_double, c_double, c_i64, c_void_p]),
-    "tm_fcg_iterate": (c_int, [c_i64] + [c_void_p] * 6 + [c_int, c_void_p]),
+    "tm_fcg_iterate": (c_int, [c_i64] + [c_void_p] * 7),
     "tm_pcg_work_size": (c_i64, [c_i64]),
     "tm_pcg_jacobi": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                               c_double, c_double, c_i64, c_int, ctypes.POINTER(c_i64),

@@ -22,6 +22,7 @@
 
 #include "tm_device.h"
 #include "tm_parts.h"
+#include "tm_spmv.h"
 
 namespace {
 
@@ -31,6 +32,7 @@ constexpr int XP = 2 * F + 8;  // LDS pitch (BF16 elements) of a [h | a] row
 constexpr int GP = F + 8;      // LDS pitch of a 64-wide BF16 row
 constexpr int TILE = 64;       // node rows per tile (4 waves x 16 rows)
 constexpr int NT = 256;
+constexpr int NTF = 512;  // flexible-CG vector kernels
 constexpr int kMaxParts = 1024;
 constexpr int NWP = 2 * F * F + F;  // weight-gradient partial of one layer: dW (128 x 64) + db (64)
 
@@ -53,92 +55,61 @@ __device__ __forceinline__ bool stopped(const int* stop) { return stop != nullpt
 // ---------------------------------------------------------------------------
 // Forward message-passing layer
 //
-// A workgroup owns 64-row tiles (a contiguous range of tiles per XCD-remapped
-// workgroup).  Per tile: the CSR entries of its rows are staged in LDS (up to
-// kEnt entries; larger tiles read them from global memory), each wave gathers
-// the messages of 16 rows with 4 lanes per row and 16 features per lane (32 B
-// of the neighbour's row per lane), then multiplies [h | a] by W on MFMA.
-//
-// Node features come from a source:
-//   Dense    - a materialised BF16 [n][64] tensor;
-//   Unpooled - computed on the fly as bf16(bf16(enc(r_j, lk_j)) + w_j hc[agg_j])
-//              (encoder output as skip plus the prolonged coarse features),
-//              i.e. exactly what the unfused encode + unpool would store.
-// With DECODE the epilogue also applies the decoder: y = bf16(h') . wd + bd,
-// z = s d y, and (optionally) the partial sums of (z, q).
+// A workgroup owns one 64-row tile.  The CSR entries of its rows are staged in
+// LDS (up to kEnt entries; larger tiles read them from global memory), each
+// wave gathers the messages of 16 rows with 4 lanes per row and 16 features
+// per lane (32 B of the neighbour's row per lane), then the tile [h | a] is
+// multiplied by W on MFMA.
 // ---------------------------------------------------------------------------
 constexpr int kEnt = 1536;
 
-struct Dense {
-  const uint16_t* h;
-  __device__ __forceinline__ void prepare(int) {}
-  __device__ __forceinline__ void load16(int64_t j, int f0, float (&o)[16]) const {
-    const uint4 a = *(const uint4*)&h[j * F + f0];
-    const uint4 b = *(const uint4*)&h[j * F + f0 + 8];
-    const uint32_t w8[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+__device__ __forceinline__ void load16(const uint16_t* __restrict__ h, int64_t j, int f0, float (&o)[16]) {
+  const uint4 a = *(const uint4*)&h[j * F + f0];
+  const uint4 b = *(const uint4*)&h[j * F + f0 + 8];
+  const uint32_t w8[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      o[2 * t] = bf2f((uint16_t)(w8[t] & 0xffffu));
-      o[2 * t + 1] = bf2f((uint16_t)(w8[t] >> 16));
-    }
+  for (int t = 0; t < 8; ++t) {
+    o[2 * t] = bf2f((uint16_t)(w8[t] & 0xffffu));
+    o[2 * t + 1] = bf2f((uint16_t)(w8[t] >> 16));
   }
-};
+}
 
-struct Unpooled {
-  const double* r;
-  const float* d;
-  const float* lk;
-  const int32_t* agg;
-  const float* w;
-  const uint16_t* hc;
-  const float* we;
-  const float* be;
-  const double* scale;
-  int n_lin;
-  // per lane: encoder weights of its 16 features
-  float inv_s, w0[16], w1[16], b0[16];
-  int fl;
-  __device__ __forceinline__ void prepare(int f0) {
-    inv_s = (float)(1.0 / *scale);
-    fl = f0;
+// 16 features of a row as BF16 into LDS (32 B)
+__device__ __forceinline__ void store16(uint16_t* p, const float (&v)[16]) {
+  uint32_t pk[8];
 #pragma unroll
-    for (int t = 0; t < 16; ++t) {
-      w0[t] = we[f0 + t];
-      w1[t] = we[F + f0 + t];
-      b0[t] = be[f0 + t];
-    }
+  for (int u = 0; u < 8; ++u) pk[u] = pack2(v[2 * u], v[2 * u + 1]);
+  *(uint4*)p = uint4{pk[0], pk[1], pk[2], pk[3]};
+  *(uint4*)(p + 8) = uint4{pk[4], pk[5], pk[6], pk[7]};
+}
+
+__device__ __forceinline__ void load_wt(uint16_t* W, const uint16_t* __restrict__ wt) {
+  for (int c = threadIdx.x; c < F * 16; c += NT) {
+    const int j = c / 16, k8 = (c % 16) * 8;
+    *(uint4*)&W[j * XP + k8] = *(const uint4*)&wt[j * 2 * F + k8];
   }
-  __device__ __forceinline__ void load16(int64_t j, int f0, float (&o)[16]) const {
-    const float x0 = (float)((double)d[j] * r[j]) * inv_s, x1 = lk[j], wj = w[j];
-    Dense c{hc};
-    float hv[16];
-    c.load16(agg[j], f0, hv);
+}
+
+// acc[jt] = X[16 rows of this wave][0:128] x W^T[jt*16 .. +16][0:128]^T
+__device__ __forceinline__ void tile_mfma(const uint16_t* X, const uint16_t* W, int wv, int l, f32x4 (&acc)[4]) {
 #pragma unroll
-    for (int t = 0; t < 16; ++t) {
-      const float e = bf2f(f2bf(act(x0 * w0[t] + x1 * w1[t] + b0[t], fl + t, n_lin)));
-      o[t] = bf2f(f2bf(e + wj * hv[t]));
-    }
+  for (int jt = 0; jt < 4; ++jt) acc[jt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  const int ar = wv * 16 + (l % 16), ak = 8 * (l / 16);
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk) {
+    const bf16x8 a = lds8(&X[ar * XP + kk * 32 + ak]);
+#pragma unroll
+    for (int jt = 0; jt < 4; ++jt) acc[jt] = mfma(a, lds8(&W[(jt * 16 + (l % 16)) * XP + kk * 32 + ak]), acc[jt]);
   }
-};
+}
 
-struct MpOut {
-  uint16_t* out;       // h' (non-decode)
-  uint16_t* s_agg;     // training saves (nullable)
-  uint16_t* s_u;
-  const float* wd;     // decoder (decode)
-  const float* bd;
-  const float* d;
-  const double* scale;
-  double* z;
-  const double* q;
-  double* zq_part;
-};
-
-template <class Src, bool SAVE, bool DECODE>
+template <bool SAVE>
 __global__ __launch_bounds__(NT) void k_mp(int64_t n, const int32_t* __restrict__ rp,
-                                           const int32_t* __restrict__ ci, const float* __restrict__ av, Src src0,
-                                           const uint16_t* __restrict__ wt, const float* __restrict__ bias,
-                                           int n_lin, MpOut o, const int* stop) {
+                                           const int32_t* __restrict__ ci, const float* __restrict__ av,
+                                           const uint16_t* __restrict__ h, const uint16_t* __restrict__ wt,
+                                           const float* __restrict__ bias, int n_lin, uint16_t* __restrict__ out,
+                                           uint16_t* __restrict__ s_agg, uint16_t* __restrict__ s_u,
+                                           const int* stop) {
   if (stopped(stop)) return;
   __shared__ __align__(16) uint16_t X[TILE * XP];
   __shared__ __align__(16) uint16_t W[F * XP];  // W^T: [64 outputs][128 inputs]
@@ -146,151 +117,219 @@ __global__ __launch_bounds__(NT) void k_mp(int64_t n, const int32_t* __restrict_
   __shared__ int ec[kEnt];
   __shared__ float ev[kEnt];
   const int tid = threadIdx.x, wv = tid / kWave, l = tid % kWave;
-  const int64_t tiles = (n + TILE - 1) / TILE;
-  const int64_t per = (tiles + gridDim.x - 1) / gridDim.x;
-  const int64_t t_begin = (int64_t)tmx::xcd_remap(blockIdx.x, gridDim.x) * per;
-  const int64_t t_end = t_begin + per < tiles ? t_begin + per : tiles;
-  for (int c = tid; c < F * 16; c += NT) {
-    const int j = c / 16, k8 = (c % 16) * 8;
-    *(uint4*)&W[j * XP + k8] = *(const uint4*)&wt[j * 2 * F + k8];
-  }
-  double zq = 0.0, s = 1.0;
-  if (DECODE) s = *o.scale;
+  const int64_t r0 = (int64_t)tmx::xcd_remap(blockIdx.x, gridDim.x) * TILE;
+  load_wt(W, wt);
+  for (int c = tid; c <= TILE; c += NT) rps[c] = rp[r0 + c < n ? r0 + c : n];
+  __syncthreads();
+  const int e0 = rps[0], ecount = rps[TILE] - e0;
+  const bool staged = ecount <= kEnt;
+  if (staged)
+    for (int c = tid; c < ecount; c += NT) {
+      ec[c] = ci[e0 + c];
+      ev[c] = av[e0 + c];
+    }
+  __syncthreads();
   const int ri = wv * 16 + l / 4, f0 = (l % 4) * 16;
-  Src src = src0;
-  src.prepare(f0);
-  for (int64_t t = t_begin; t < t_end; ++t) {
-    const int64_t r0 = t * TILE;
-    __syncthreads();  // previous tile's LDS reads are done
-    for (int c = tid; c <= TILE; c += NT) {
-      const int64_t row = r0 + c < n ? r0 + c : n;
-      rps[c] = rp[row];
+  {
+    const int64_t row = r0 + ri;
+    float hv[16], a[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      a[u] = 0.0f;
+      hv[u] = 0.0f;
     }
-    __syncthreads();
-    const int e0 = rps[0], ecount = rps[TILE] - e0;
-    const bool staged = ecount <= kEnt;
-    if (staged)
-      for (int c = tid; c < ecount; c += NT) {
-        ec[c] = ci[e0 + c];
-        ev[c] = av[e0 + c];
+    if (row < n) {
+      load16(h, row, f0, hv);
+      const int kb = rps[ri] - e0, ke = rps[ri + 1] - e0;
+      int k = kb;
+      for (; k + 1 < ke; k += 2) {
+        const int c0 = staged ? ec[k] : ci[e0 + k], c1 = staged ? ec[k + 1] : ci[e0 + k + 1];
+        const float v0 = staged ? ev[k] : av[e0 + k], v1 = staged ? ev[k + 1] : av[e0 + k + 1];
+        float h0[16], h1[16];
+        load16(h, c0, f0, h0);
+        load16(h, c1, f0, h1);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) a[u] += v0 * h0[u] + v1 * h1[u];
       }
-    __syncthreads();
-    // own features and messages of row ri (16 features from f0)
-    {
-      const int64_t row = r0 + ri;
-      float hv[16], a[16];
+      if (k < ke) {
+        const int c0 = staged ? ec[k] : ci[e0 + k];
+        const float v0 = staged ? ev[k] : av[e0 + k];
+        float h0[16];
+        load16(h, c0, f0, h0);
 #pragma unroll
-      for (int u = 0; u < 16; ++u) a[u] = 0.0f;
-      if (row < n) {
-        src.load16(row, f0, hv);
-        const int kb = rps[ri] - e0, ke = rps[ri + 1] - e0;
-        int k = kb;
-        for (; k + 1 < ke; k += 2) {
-          int c0, c1;
-          float v0, v1;
-          if (staged) {
-            c0 = ec[k]; c1 = ec[k + 1]; v0 = ev[k]; v1 = ev[k + 1];
-          } else {
-            c0 = ci[e0 + k]; c1 = ci[e0 + k + 1]; v0 = av[e0 + k]; v1 = av[e0 + k + 1];
-          }
-          float h0[16], h1[16];
-          src.load16(c0, f0, h0);
-          src.load16(c1, f0, h1);
-#pragma unroll
-          for (int u = 0; u < 16; ++u) a[u] += v0 * h0[u] + v1 * h1[u];
-        }
-        if (k < ke) {
-          const int c0 = staged ? ec[k] : ci[e0 + k];
-          const float v0 = staged ? ev[k] : av[e0 + k];
-          float h0[16];
-          src.load16(c0, f0, h0);
-#pragma unroll
-          for (int u = 0; u < 16; ++u) a[u] += v0 * h0[u];
-        }
-      } else {
-#pragma unroll
-        for (int u = 0; u < 16; ++u) hv[u] = 0.0f;
-      }
-      uint32_t ph[8], pa[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        ph[u] = pack2(hv[2 * u], hv[2 * u + 1]);
-        pa[u] = pack2(a[2 * u], a[2 * u + 1]);
-      }
-      *(uint4*)&X[ri * XP + f0] = uint4{ph[0], ph[1], ph[2], ph[3]};
-      *(uint4*)&X[ri * XP + f0 + 8] = uint4{ph[4], ph[5], ph[6], ph[7]};
-      *(uint4*)&X[ri * XP + F + f0] = uint4{pa[0], pa[1], pa[2], pa[3]};
-      *(uint4*)&X[ri * XP + F + f0 + 8] = uint4{pa[4], pa[5], pa[6], pa[7]};
-      if (SAVE && row < n) {
-        *(uint4*)&o.s_agg[row * F + f0] = uint4{pa[0], pa[1], pa[2], pa[3]};
-        *(uint4*)&o.s_agg[row * F + f0 + 8] = uint4{pa[4], pa[5], pa[6], pa[7]};
+        for (int u = 0; u < 16; ++u) a[u] += v0 * h0[u];
       }
     }
-    __syncthreads();
-    f32x4 acc[4];
-#pragma unroll
-    for (int jt = 0; jt < 4; ++jt) acc[jt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    const int ar = wv * 16 + (l % 16), ak = 8 * (l / 16);
-#pragma unroll
-    for (int kk = 0; kk < 4; ++kk) {
-      const bf16x8 a = lds8(&X[ar * XP + kk * 32 + ak]);
-#pragma unroll
-      for (int jt = 0; jt < 4; ++jt)
-        acc[jt] = mfma(a, lds8(&W[(jt * 16 + (l % 16)) * XP + kk * 32 + ak]), acc[jt]);
-    }
-    if (DECODE) {
-      float yp[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-      for (int jt = 0; jt < 4; ++jt) {
-        const int j = jt * 16 + (l % 16);
-        const float bj = bias[j], wdj = o.wd[j];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int i = wv * 16 + 4 * (l / 16) + v;
-          const float hn = bf2f(f2bf(bf2f(X[i * XP + j]) + act(acc[jt][v] + bj, j, n_lin)));
-          yp[v] += hn * wdj;
-        }
-      }
-#pragma unroll
-      for (int v = 0; v < 4; ++v)
-#pragma unroll
-        for (int m = 1; m < 16; m <<= 1) yp[v] += __shfl_xor(yp[v], m, kWave);
-      if ((l % 16) == 0) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int64_t row = r0 + wv * 16 + 4 * (l / 16) + v;
-          if (row < n) {
-            const double zi = s * (double)o.d[row] * (double)(yp[v] + o.bd[0]);
-            o.z[row] = zi;
-            if (o.q) zq += zi * o.q[row];
-          }
-        }
-      }
-    } else {
-#pragma unroll
-      for (int jt = 0; jt < 4; ++jt) {
-        const int j = jt * 16 + (l % 16);
-        const float bj = bias[j];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int i = wv * 16 + 4 * (l / 16) + v;
-          const float u = acc[jt][v] + bj;
-          const float hn = bf2f(X[i * XP + j]) + act(u, j, n_lin);
-          X[i * XP + j] = f2bf(hn);
-          if (SAVE && r0 + i < n) o.s_u[(r0 + i) * F + j] = f2bf(u);
-        }
-      }
-      __syncthreads();
-      for (int c = l; c < 16 * 8; c += kWave) {
-        const int i = wv * 16 + c / 8, k8 = (c % 8) * 8;
-        if (r0 + i < n) *(uint4*)&o.out[(r0 + i) * F + k8] = *(const uint4*)&X[i * XP + k8];
-      }
+    store16(&X[ri * XP + f0], hv);
+    store16(&X[ri * XP + F + f0], a);
+    if (SAVE && row < n) {
+      *(uint4*)&s_agg[row * F + f0] = *(const uint4*)&X[ri * XP + F + f0];
+      *(uint4*)&s_agg[row * F + f0 + 8] = *(const uint4*)&X[ri * XP + F + f0 + 8];
     }
   }
-  if (DECODE && o.zq_part) {
-    double loc[1] = {zq};
-    const int which[1] = {0};
-    tmx::write_parts<NT, 1>(o.zq_part, kMaxParts, which, loc);
+  __syncthreads();
+  f32x4 acc[4];
+  tile_mfma(X, W, wv, l, acc);
+#pragma unroll
+  for (int jt = 0; jt < 4; ++jt) {
+    const int j = jt * 16 + (l % 16);
+    const float bj = bias[j];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = wv * 16 + 4 * (l / 16) + v;
+      const float u = acc[jt][v] + bj;
+      const float hn = bf2f(X[i * XP + j]) + act(u, j, n_lin);
+      X[i * XP + j] = f2bf(hn);
+      if (SAVE && r0 + i < n) s_u[(r0 + i) * F + j] = f2bf(u);
+    }
+  }
+  __syncthreads();
+  for (int c = l; c < 16 * 8; c += kWave) {
+    const int i = wv * 16 + c / 8, k8 = (c % 8) * 8;
+    if (r0 + i < n) *(uint4*)&out[(r0 + i) * F + k8] = *(const uint4*)&X[i * XP + k8];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Finest level at inference, fused: unpool + message passing + decode.
+//
+// With the linear encoder e_j = x_j0 We0 + x_j1 We1 + be, the unpooled level-0
+// features are h_j = e_j + w_j hc[agg_j] and their messages
+//     a_i = sum_j A^_ij h_j
+//         = (A^ x0)_i We0 + (A^ x1)_i We1 + (A^ 1)_i be + sum_J B_iJ hc[J]
+// with B = A^ P^ (B_iJ = sum over neighbours j of i in aggregate J of
+// A^_ij w_j, formed once per matrix).  So a row needs two scalar products
+// over its CSR row, its few B entries (coarse feature rows, shared by the
+// rows of one aggregate) and its own coarse row - no 64-wide level-0 tensor.
+// One 64-row tile per workgroup; the gathers of a row are issued together.
+// ---------------------------------------------------------------------------
+struct FinalArgs {
+  const int32_t* rp;
+  const int32_t* ci;
+  const float* av;
+  const float2* xs;   // (d r / s, lk) per node, written by k_encode_pool
+  const float* a1;    // row sums of A^
+  const int32_t* agg;
+  const float* w;
+  const uint16_t* hc;
+  const int32_t* brp;
+  const int32_t* bcol;
+  const float* bv;
+  const float* we;
+  const float* be;
+  const uint16_t* wt;
+  const float* bias;
+  int n_lin;
+  const float* wd;
+  const float* bd;
+  const float* d;
+  const double* scale;
+  double* z;
+};
+
+__global__ __launch_bounds__(NT) void k_final(int64_t n, FinalArgs A, const int* stop) {
+  if (stopped(stop)) return;
+  __shared__ __align__(16) uint16_t X[TILE * XP];
+  __shared__ __align__(16) uint16_t W[F * XP];
+  const int tid = threadIdx.x, wv = tid / kWave, l = tid % kWave;
+  const int64_t r0 = (int64_t)tmx::xcd_remap(blockIdx.x, gridDim.x) * TILE;
+  load_wt(W, A.wt);
+  const int ri = wv * 16 + l / 4, qd = l % 4, f0 = qd * 16;
+  const int64_t row = r0 + ri;
+  float hv[16], a[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    hv[u] = 0.0f;
+    a[u] = 0.0f;
+  }
+  // scalar products (A^ x0)_i, (A^ x1)_i: each of the 4 lanes of the row takes
+  // every 4th entry, column indices first so the xs gathers overlap
+  float ax0 = 0.0f, ax1 = 0.0f;
+  if (row < n) {
+    const int kb = A.rp[row], ke = A.rp[row + 1];
+    int c[4];
+    float v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = kb + qd + 4 * u;
+      c[u] = k < ke ? A.ci[k] : -1;
+      v[u] = k < ke ? A.av[k] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (c[u] >= 0) {
+        const float2 xj = A.xs[c[u]];
+        ax0 += v[u] * xj.x;
+        ax1 += v[u] * xj.y;
+      }
+    for (int k = kb + qd + 16; k < ke; k += 4) {
+      const float2 xj = A.xs[A.ci[k]];
+      ax0 += A.av[k] * xj.x;
+      ax1 += A.av[k] * xj.y;
+    }
+  }
+  ax0 += __shfl_xor(ax0, 1, kWave);
+  ax0 += __shfl_xor(ax0, 2, kWave);
+  ax1 += __shfl_xor(ax1, 1, kWave);
+  ax1 += __shfl_xor(ax1, 2, kWave);
+  if (row < n) {
+    const float2 xi = A.xs[row];
+    const float wi = A.w[row], ai = A.a1[row];
+    const int bb = A.brp[row], be_ = A.brp[row + 1];
+    float hc[16];
+    load16(A.hc, A.agg[row], f0, hc);
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const float e = xi.x * A.we[f0 + u] + xi.y * A.we[F + f0 + u] + A.be[f0 + u];
+      hv[u] = e + wi * hc[u];
+      a[u] = ax0 * A.we[f0 + u] + ax1 * A.we[F + f0 + u] + ai * A.be[f0 + u];
+    }
+    int k = bb;
+    for (; k + 1 < be_; k += 2) {
+      const int j0 = A.bcol[k], j1 = A.bcol[k + 1];
+      const float b0 = A.bv[k], b1 = A.bv[k + 1];
+      float g0[16], g1[16];
+      load16(A.hc, j0, f0, g0);
+      load16(A.hc, j1, f0, g1);
+#pragma unroll
+      for (int u = 0; u < 16; ++u) a[u] += b0 * g0[u] + b1 * g1[u];
+    }
+    if (k < be_) {
+      const float b0 = A.bv[k];
+      float g0[16];
+      load16(A.hc, A.bcol[k], f0, g0);
+#pragma unroll
+      for (int u = 0; u < 16; ++u) a[u] += b0 * g0[u];
+    }
+  }
+  store16(&X[ri * XP + f0], hv);
+  store16(&X[ri * XP + F + f0], a);
+  __syncthreads();
+  f32x4 acc[4];
+  tile_mfma(X, W, wv, l, acc);
+  float yp[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int jt = 0; jt < 4; ++jt) {
+    const int j = jt * 16 + (l % 16);
+    const float bj = A.bias[j], wdj = A.wd[j];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = wv * 16 + 4 * (l / 16) + v;
+      const float hn = bf2f(f2bf(bf2f(X[i * XP + j]) + act(acc[jt][v] + bj, j, A.n_lin)));
+      yp[v] += hn * wdj;
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int m = 1; m < 16; m <<= 1) yp[v] += __shfl_xor(yp[v], m, kWave);
+  if ((l % 16) == 0) {
+    const double s = *A.scale;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t rr = r0 + wv * 16 + 4 * (l / 16) + v;
+      if (rr < n) A.z[rr] = s * (double)A.d[rr] * (double)(yp[v] + A.bd[0]);
+    }
   }
 }
 
@@ -564,7 +603,7 @@ __global__ __launch_bounds__(NT) void k_sumsq(int64_t n, const double* __restric
 __global__ __launch_bounds__(NT) void k_encode(int64_t n, const double* __restrict__ r, const float* __restrict__ d,
                                                const float* __restrict__ lk, const double* __restrict__ ss_part,
                                                int nparts, const float* __restrict__ we, const float* __restrict__ be,
-                                               int n_lin, uint16_t* __restrict__ h0, double* __restrict__ s_out,
+                                               uint16_t* __restrict__ h0, double* __restrict__ s_out,
                                                const int* stop) {
   if (stopped(stop)) return;
   const double s = scale_from_parts(ss_part, nparts, n);
@@ -573,67 +612,69 @@ __global__ __launch_bounds__(NT) void k_encode(int64_t n, const double* __restri
   const int q = threadIdx.x % 8, f0 = q * 8;
   for (int64_t i = blockIdx.x * (int64_t)(NT / 8) + threadIdx.x / 8; i < n; i += (int64_t)gridDim.x * (NT / 8)) {
     const float x0 = (float)((double)d[i] * r[i]) * inv, x1 = lk[i];
-    uint32_t pk[4];
+    float v[8];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const int f = f0 + 2 * t;
-      const float u0 = x0 * we[f] + x1 * we[F + f] + be[f];
-      const float u1 = x0 * we[f + 1] + x1 * we[F + f + 1] + be[f + 1];
-      pk[t] = pack2(act(u0, f, n_lin), act(u1, f + 1, n_lin));
-    }
-    *(uint4*)&h0[i * F + f0] = uint4{pk[0], pk[1], pk[2], pk[3]};
+    for (int t = 0; t < 8; ++t) v[t] = x0 * we[f0 + t] + x1 * we[F + f0 + t] + be[f0 + t];
+    Vec8<uint16_t>::store(&h0[i * F + f0], v);
   }
 }
 
-// Fused encoder + first pooling (inference): out[I] = bf16(sum over children
-// i of w_i bf16(act([d_i r_i / s, lk_i] We + be))), the same values as
-// k_encode followed by k_pool.  8 lanes per coarse node, 8 features per lane.
+// Fused encoder + first pooling (inference).  The encoder is linear, so
+//   out[I] = (sum w_i x_i0) We0 + (sum w_i x_i1) We1 + (sum w_i) be
+// over the children i of I; also writes x_i = (d_i r_i / s, lk_i) for the
+// finest-level kernel.  8 lanes per coarse node.
 __global__ __launch_bounds__(NT) void k_encode_pool(int64_t n, int64_t nc, const int32_t* __restrict__ cp,
                                                     const int32_t* __restrict__ child, const float* __restrict__ w,
                                                     const double* __restrict__ r, const float* __restrict__ d,
                                                     const float* __restrict__ lk, const double* __restrict__ ss_part,
                                                     int nparts, const float* __restrict__ we,
-                                                    const float* __restrict__ be, int n_lin,
-                                                    uint16_t* __restrict__ out, double* __restrict__ s_out,
+                                                    const float* __restrict__ be, uint16_t* __restrict__ out,
+                                                    float2* __restrict__ xs, double* __restrict__ s_out,
                                                     const int* stop) {
   if (stopped(stop)) return;
   const double s = scale_from_parts(ss_part, nparts, n);
   if (blockIdx.x == 0 && threadIdx.x == 0) *s_out = s;
   const float inv = (float)(1.0 / s);
-  const int f0 = (threadIdx.x % 8) * 8;
-  float w0[8], w1[8], b0[8];
+  const int q = threadIdx.x % 8, f0 = q * 8;
+  for (int64_t I0 = blockIdx.x * (int64_t)(NT / 8); I0 < nc; I0 += (int64_t)gridDim.x * (NT / 8)) {
+    const int64_t I = I0 + threadIdx.x / 8;
+    float s0 = 0.0f, s1 = 0.0f, sw = 0.0f;
+    if (I < nc)
+      for (int k = cp[I] + q; k < cp[I + 1]; k += 8) {
+        const int i = child[k];
+        const float x0 = (float)((double)d[i] * r[i]) * inv, x1 = lk[i], wi = w[i];
+        xs[i] = float2{x0, x1};
+        s0 += wi * x0;
+        s1 += wi * x1;
+        sw += wi;
+      }
 #pragma unroll
-  for (int t = 0; t < 8; ++t) {
-    w0[t] = we[f0 + t];
-    w1[t] = we[F + f0 + t];
-    b0[t] = be[f0 + t];
-  }
-  for (int64_t I = blockIdx.x * (int64_t)(NT / 8) + threadIdx.x / 8; I < nc; I += (int64_t)gridDim.x * (NT / 8)) {
-    float acc[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    for (int k = cp[I]; k < cp[I + 1]; ++k) {
-      const int i = child[k];
-      const float x0 = (float)((double)d[i] * r[i]) * inv, x1 = lk[i], wi = w[i];
-#pragma unroll
-      for (int t = 0; t < 8; ++t) acc[t] += wi * bf2f(f2bf(act(x0 * w0[t] + x1 * w1[t] + b0[t], f0 + t, n_lin)));
+    for (int m = 1; m < 8; m <<= 1) {
+      s0 += __shfl_xor(s0, m, kWave);
+      s1 += __shfl_xor(s1, m, kWave);
+      sw += __shfl_xor(sw, m, kWave);
     }
-    Vec8<uint16_t>::store(&out[I * F + f0], acc);
+    if (I < nc) {
+      float v[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) v[t] = s0 * we[f0 + t] + s1 * we[F + f0 + t] + sw * be[f0 + t];
+      Vec8<uint16_t>::store(&out[I * F + f0], v);
+    }
   }
 }
 
-// dWe, dbe partials: [3][64] per workgroup
+// dWe, dbe partials: [3][64] per workgroup (linear encoder)
 __global__ __launch_bounds__(NT) void k_encode_bwd(int64_t n, const double* __restrict__ r,
                                                    const float* __restrict__ d, const float* __restrict__ lk,
-                                                   const double* __restrict__ s_in, const float* __restrict__ we,
-                                                   const float* __restrict__ be, int n_lin,
-                                                   const float* __restrict__ g_h0, float* __restrict__ part) {
+                                                   const double* __restrict__ s_in, const float* __restrict__ g_h0,
+                                                   float* __restrict__ part) {
   __shared__ float red[NT / kWave][3 * F];
   const int f = threadIdx.x % F, wv = threadIdx.x / F;
   const float inv = (float)(1.0 / *s_in);
   float a0 = 0.0f, a1 = 0.0f, ab = 0.0f;
   for (int64_t i = blockIdx.x * (int64_t)(NT / F) + wv; i < n; i += (int64_t)gridDim.x * (NT / F)) {
     const float x0 = (float)((double)d[i] * r[i]) * inv, x1 = lk[i];
-    const float u = x0 * we[f] + x1 * we[F + f] + be[f];
-    const float g = g_h0[i * F + f] * dact(u, f, n_lin);
+    const float g = g_h0[i * F + f];
     a0 += x0 * g;
     a1 += x1 * g;
     ab += g;
@@ -643,20 +684,18 @@ __global__ __launch_bounds__(NT) void k_encode_bwd(int64_t n, const double* __re
   red[wv][2 * F + f] = ab;
   __syncthreads();
   if (threadIdx.x < 3 * F) {
-    float s = 0.0f;
+    float sum = 0.0f;
 #pragma unroll
-    for (int k = 0; k < NT / kWave; ++k) s += red[k][threadIdx.x];
-    part[(int64_t)blockIdx.x * 3 * F + threadIdx.x] = s;
+    for (int k = 0; k < NT / kWave; ++k) sum += red[k][threadIdx.x];
+    part[(int64_t)blockIdx.x * 3 * F + threadIdx.x] = sum;
   }
 }
 
-// y (normalised output, optional), z = s d y (FP64), optional partial (z, q).
+// y (normalised output, optional), z = s d y (FP64).  8 lanes per row.
 __global__ __launch_bounds__(NT) void k_decode(int64_t n, const uint16_t* __restrict__ h,
                                                const float* __restrict__ wd, const float* __restrict__ bd,
                                                const float* __restrict__ d, const double* __restrict__ s_in,
-                                               float* __restrict__ y, double* __restrict__ z,
-                                               const double* __restrict__ q, double* __restrict__ zq_part,
-                                               const int* stop) {
+                                               float* __restrict__ y, double* __restrict__ z, const int* stop) {
   if (stopped(stop)) return;
   const double s = *s_in;
   const int qd = threadIdx.x % 8, f0 = qd * 8;
@@ -664,33 +703,22 @@ __global__ __launch_bounds__(NT) void k_decode(int64_t n, const uint16_t* __rest
 #pragma unroll
   for (int t = 0; t < 8; ++t) wl[t] = wd[f0 + t];
   const float b = bd[0];
-  double loc[1] = {0.0};
-  const int64_t stride = (int64_t)gridDim.x * (NT / 8);
-  for (int64_t i0 = blockIdx.x * (int64_t)(NT / 8); i0 < n; i0 += stride) {
+  for (int64_t i0 = blockIdx.x * (int64_t)(NT / 8); i0 < n; i0 += (int64_t)gridDim.x * (NT / 8)) {
     const int64_t i = i0 + threadIdx.x / 8;
     float acc = 0.0f;
     if (i < n) {
-      const uint4 v = *(const uint4*)&h[i * F + f0];
-      const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+      float v[8];
+      Vec8<uint16_t>::load(&h[i * F + f0], v);
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        acc += bf2f((uint16_t)(w4[t] & 0xffffu)) * wl[2 * t];
-        acc += bf2f((uint16_t)(w4[t] >> 16)) * wl[2 * t + 1];
-      }
+      for (int t = 0; t < 8; ++t) acc += v[t] * wl[t];
     }
 #pragma unroll
     for (int o = 4; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kWave);
     if (qd == 0 && i < n) {
       const float yi = acc + b;
       if (y) y[i] = yi;
-      const double zi = s * (double)d[i] * (double)yi;
-      z[i] = zi;
-      if (q) loc[0] += zi * q[i];
+      z[i] = s * (double)d[i] * (double)yi;
     }
-  }
-  if (zq_part) {
-    const int which[1] = {0};
-    tmx::write_parts<NT, 1>(zq_part, kMaxParts, which, loc);
   }
 }
 
@@ -787,6 +815,31 @@ __global__ __launch_bounds__(NT) void k_level_poolw(int64_t nf, const int32_t* _
     w[i] = dc[agg[i]] / df[i];
 }
 
+// B = A^ P^ on the finest level: bv[s] = sum over the fine entries k mapped to
+// B entry s of A^_k w_col(k)   (fixed order through perm / seg)
+__global__ __launch_bounds__(NT) void k_level_prolong(int64_t nnz_b, const int32_t* __restrict__ seg,
+                                                      const int32_t* __restrict__ perm,
+                                                      const int32_t* __restrict__ ci, const float* __restrict__ vn,
+                                                      const float* __restrict__ w, float* __restrict__ bv) {
+  for (int64_t s = blockIdx.x * (int64_t)NT + threadIdx.x; s < nnz_b; s += (int64_t)gridDim.x * NT) {
+    double a = 0.0;
+    for (int k = seg[s]; k < seg[s + 1]; ++k) {
+      const int e = perm[k];
+      a += (double)vn[e] * (double)w[ci[e]];
+    }
+    bv[s] = (float)a;
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_row_sums(int64_t n, const int32_t* __restrict__ rp,
+                                                 const float* __restrict__ v, float* __restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+    double a = 0.0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) a += (double)v[k];
+    out[i] = (float)a;
+  }
+}
+
 // Adam (bias-corrected), FP32 master weights
 __global__ __launch_bounds__(NT) void k_adam(int64_t n, float* __restrict__ p, const float* __restrict__ g,
                                              float* __restrict__ m, float* __restrict__ v, float lr, float b1,
@@ -802,23 +855,28 @@ __global__ __launch_bounds__(NT) void k_adam(int64_t n, float* __restrict__ p, c
 }
 
 // ---------------------------------------------------------------------------
-// Flexible CG, truncated to one previous direction (FCG(1)):
-//   z = M(r); beta = (z, q_old) / (p_old, q_old); p = z - beta p_old
-//   q = A p; alpha = (p, r) / (p, q); x += alpha p; r -= alpha q
+// Flexible CG, truncated to one previous direction (FCG(1), Notay 2000):
+//   z = M(r); beta = (z, q_o) / (p_o, q_o); p = z - beta p_o; q = A p
+//   alpha = (p, r) / (p, q); x += alpha p; r -= alpha q
+// evaluated with one reduction phase per iteration: w = A z and the dots
+// (z,q_o), (z,r), (z,w), (p_o,r) come out of one SpMV kernel, then
+//   q = w - beta q_o,  (p,r) = (z,r) - beta (p_o,r),
+//   (p,q) = (z,w) - 2 beta (z,q_o) + beta^2 (p_o,q_o)     (A symmetric)
+// and one update kernel applies everything.
 // ---------------------------------------------------------------------------
 struct FcgScalars {
-  double bb, tol2, pq, relres, s;
+  double bb, tol2, pq, pq_new, relres, s;
 };
 struct FcgFlags {
   int status;  // 0 running, 1 converged, 2 breakdown, 3 max iterations
   int iter;
   long long maxiter;
 };
-constexpr int Q_PQ = 0, Q_PR = 1, Q_RR = 2, Q_BB = 3, Q_ZQ = 4, Q_SS = 5;
-constexpr int kFcgSlots = 6;
+constexpr int Q_ZQ = 0, Q_ZR = 1, Q_ZW = 2, Q_PR = 3, Q_RR = 4, Q_BB = 5, Q_SS = 6, Q_PQ = 7;
+constexpr int kFcgSlots = 8;
 
 struct FcgWork {
-  double *r, *p, *q, *z, *part;
+  double *r, *p, *q, *z, *w, *part;
   FcgScalars* s;
   FcgFlags* f;
 };
@@ -830,17 +888,60 @@ FcgWork fcg_carve(void* work, int64_t n) {
   w.p = b + n;
   w.q = b + 2 * n;
   w.z = b + 3 * n;
-  w.part = b + 4 * n;
+  w.w = b + 4 * n;
+  w.part = b + 5 * n;
   w.s = (FcgScalars*)(w.part + kFcgSlots * kMaxParts);
   w.f = (FcgFlags*)(w.s + 1);
   return w;
 }
 
+// START: q = A x.  Otherwise w = A z with partials (z,q), (z,r), (z,w), (p,r).
+template <int S, bool START>
+__global__ __launch_bounds__(NTF) void k_fcg_spmv(int64_t n, const int32_t* __restrict__ rp,
+                                                  const int32_t* __restrict__ ci, const float* __restrict__ v,
+                                                  const double* __restrict__ x, FcgWork w) {
+  if (!START && w.f->status != 0) return;
+  constexpr int R = 2;
+  constexpr int64_t stride = NTF / S;
+  const int sl = threadIdx.x % S;
+  const tmx::RowChunk rc = tmx::row_chunk(n);
+  const double* src = START ? x : w.z;
+  double* dst = START ? w.q : w.w;
+  double loc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t row = rc.r0 + threadIdx.x / S; row < rc.r1; row += stride * R) {
+    double o[R][1];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      o[r][0] = 0.0;
+      if (row + r * stride < rc.r1) tmx::row_accum<float, S, 1>(rp, ci, v, src, row + r * stride, sl, o[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      tmx::sub_reduce<S, 1>(o[r]);
+      const int64_t rr = row + r * stride;
+      if (sl == 0 && rr < rc.r1) {
+        dst[rr] = o[r][0];
+        if (!START) {
+          const double zi = w.z[rr], ri = w.r[rr];
+          loc[0] += zi * w.q[rr];
+          loc[1] += zi * ri;
+          loc[2] += zi * o[r][0];
+          loc[3] += w.p[rr] * ri;
+        }
+      }
+    }
+  }
+  if (!START) {
+    const int which[4] = {Q_ZQ, Q_ZR, Q_ZW, Q_PR};
+    tmx::write_parts<NTF, 4>(w.part, kMaxParts, which, loc);
+  }
+}
+
 // r = b - q (q = A x0), p = q = 0; partials (r,r), (b,b), (d r)^2
-__global__ __launch_bounds__(NT) void k_fcg_init(int64_t n, const double* __restrict__ b, const float* __restrict__ d,
-                                                 FcgWork w) {
+__global__ __launch_bounds__(NTF) void k_fcg_init(int64_t n, const double* __restrict__ b, const float* __restrict__ d,
+                                                  FcgWork w) {
   double loc[3] = {0.0, 0.0, 0.0};
-  for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+  for (int64_t i = blockIdx.x * (int64_t)NTF + threadIdx.x; i < n; i += (int64_t)gridDim.x * NTF) {
     const double ri = b[i] - w.q[i];
     w.r[i] = ri;
     w.p[i] = 0.0;
@@ -851,7 +952,7 @@ __global__ __launch_bounds__(NT) void k_fcg_init(int64_t n, const double* __rest
     loc[2] += dr * dr;
   }
   const int which[3] = {Q_RR, Q_BB, Q_SS};
-  tmx::write_parts<NT, 3>(w.part, kMaxParts, which, loc);
+  tmx::write_parts<NTF, 3>(w.part, kMaxParts, which, loc);
 }
 
 __global__ __launch_bounds__(NT) void k_fcg_ctl0(FcgWork w, int nparts, double rtol2, double atol2,
@@ -863,6 +964,7 @@ __global__ __launch_bounds__(NT) void k_fcg_ctl0(FcgWork w, int nparts, double r
     w.s->bb = t[1];
     w.s->tol2 = fmax(rtol2 * t[1], atol2);
     w.s->pq = 1.0;
+    w.s->pq_new = 1.0;
     w.s->relres = t[1] > 0.0 ? sqrt(t[0] / t[1]) : 0.0;
     w.f->iter = 0;
     w.f->maxiter = maxiter;
@@ -870,82 +972,72 @@ __global__ __launch_bounds__(NT) void k_fcg_ctl0(FcgWork w, int nparts, double r
   }
 }
 
-// beta = (z, q_old) / (p_old, q_old), p = z - beta p  (zq_nparts partials of (z, q_old))
-__global__ __launch_bounds__(NT) void k_fcg_dir(int64_t n, FcgWork w, int zq_nparts) {
+// beta, alpha from the partials; p = z - beta p, q = w - beta q, x += alpha p,
+// r -= alpha q; partials (r,r), (d r)^2; (p,q) goes to pq_new via k_fcg_check.
+__global__ __launch_bounds__(NTF) void k_fcg_update(int64_t n, double* __restrict__ x, const float* __restrict__ d,
+                                                    FcgWork w, int nparts) {
   if (w.f->status != 0) return;
-  const int which[1] = {Q_ZQ};
-  double t[1];
-  tmx::sum_parts<1>(w.part, kMaxParts, which, zq_nparts, t);
-  const double beta = t[0] / w.s->pq;
-  for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
-    w.p[i] = w.z[i] - beta * w.p[i];
-}
-
-// q = A p; partials (p, q), (p, r).  One wave per 64/S rows, S lanes per row.
-template <int S>
-__global__ __launch_bounds__(NT) void k_fcg_spmv(int64_t n, const int32_t* __restrict__ rp,
-                                                 const int32_t* __restrict__ ci, const float* __restrict__ v,
-                                                 FcgWork w) {
-  if (w.f->status != 0) return;
-  const int sl = threadIdx.x % S;
-  double loc[2] = {0.0, 0.0};
-  for (int64_t row = blockIdx.x * (int64_t)(NT / S) + threadIdx.x / S; row < n;
-       row += (int64_t)gridDim.x * (NT / S)) {
-    double o = 0.0;
-    const int b = rp[row], e = rp[row + 1];
-    for (int k = b + sl; k < e; k += S) o += (double)v[k] * w.p[ci[k]];
-#pragma unroll
-    for (int dd = S / 2; dd > 0; dd >>= 1) o += __shfl_xor(o, dd, kWave);
-    if (sl == 0) {
-      w.q[row] = o;
-      loc[0] += o * w.p[row];
-      loc[1] += w.p[row] * w.r[row];
-    }
-  }
-  const int which[2] = {Q_PQ, Q_PR};
-  tmx::write_parts<NT, 2>(w.part, kMaxParts, which, loc);
-}
-
-// alpha = (p, r) / (p, q); x += alpha p; r -= alpha q; partials (r, r), (d r)^2
-__global__ __launch_bounds__(NT) void k_fcg_update(int64_t n, double* __restrict__ x, const float* __restrict__ d,
-                                                   FcgWork w, int nparts) {
-  if (w.f->status != 0) return;
-  const int which_in[2] = {Q_PQ, Q_PR};
-  double t[2];
-  tmx::sum_parts<2>(w.part, kMaxParts, which_in, nparts, t);
-  const bool ok = t[0] > 0.0 && __builtin_isfinite(t[0]) && __builtin_isfinite(t[1]);
-  const double alpha = ok ? t[1] / t[0] : 0.0;
-  double loc[2] = {0.0, 0.0};
+  const int which_in[4] = {Q_ZQ, Q_ZR, Q_ZW, Q_PR};
+  double t[4];
+  tmx::sum_parts<4>(w.part, kMaxParts, which_in, nparts, t);
+  const double pq_o = w.s->pq;
+  const double beta = t[0] / pq_o;
+  const double pr = t[1] - beta * t[3];
+  const double pq = t[2] - 2.0 * beta * t[0] + beta * beta * pq_o;
+  const bool ok = pq > 0.0 && __builtin_isfinite(pq) && __builtin_isfinite(pr);
+  const double alpha = ok ? pr / pq : 0.0;
+  double loc[3] = {0.0, 0.0, 0.0};
   if (ok) {
-    for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
-      x[i] += alpha * w.p[i];
-      const double ri = w.r[i] - alpha * w.q[i];
+    for (int64_t i = blockIdx.x * (int64_t)NTF + threadIdx.x; i < n; i += (int64_t)gridDim.x * NTF) {
+      const double pi = w.z[i] - beta * w.p[i];
+      const double qi = w.w[i] - beta * w.q[i];
+      w.p[i] = pi;
+      w.q[i] = qi;
+      x[i] += alpha * pi;
+      const double ri = w.r[i] - alpha * qi;
       w.r[i] = ri;
       loc[0] += ri * ri;
       const double dr = (double)d[i] * ri;
       loc[1] += dr * dr;
     }
   }
-  const int which[2] = {Q_RR, Q_SS};
-  tmx::write_parts<NT, 2>(w.part, kMaxParts, which, loc);
+  loc[2] = (blockIdx.x == 0 && threadIdx.x == 0) ? (ok ? pq : -1.0) : 0.0;
+  const int which[3] = {Q_RR, Q_SS, Q_PQ};
+  tmx::write_parts<NTF, 3>(w.part, kMaxParts, which, loc);
 }
 
-// Single workgroup: pq for the next beta, iteration count, convergence test.
+// Single workgroup: (p,q) for the next beta, iteration count, convergence.
+// All four waves read the partials.
 __global__ __launch_bounds__(NT) void k_fcg_check(FcgWork w, int nparts) {
   if (w.f->status != 0) return;
-  const int which[2] = {Q_PQ, Q_RR};
-  double t[2];
-  tmx::sum_parts<2>(w.part, kMaxParts, which, nparts, t);
+  __shared__ double red[NT / kWave][2];
+  double a = 0.0, b = 0.0;
+  for (int k = threadIdx.x; k < nparts; k += NT) {
+    a += w.part[Q_RR * kMaxParts + k];
+    b += w.part[Q_PQ * kMaxParts + k];
+  }
+  a = tmx::wave_sum(a);
+  b = tmx::wave_sum(b);
+  if (threadIdx.x % kWave == 0) {
+    red[threadIdx.x / kWave][0] = a;
+    red[threadIdx.x / kWave][1] = b;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
+    double rr = 0.0, pq = 0.0;
+    for (int k = 0; k < NT / kWave; ++k) {
+      rr += red[k][0];
+      pq += red[k][1];
+    }
     const int it = w.f->iter + 1;
     w.f->iter = it;
-    if (!(t[0] > 0.0) || !__builtin_isfinite(t[0]) || !__builtin_isfinite(t[1])) {
+    if (!(pq > 0.0) || !__builtin_isfinite(pq) || !__builtin_isfinite(rr)) {
       w.f->status = 2;
       return;
     }
-    w.s->pq = t[0];
-    w.s->relres = w.s->bb > 0.0 ? sqrt(t[1] / w.s->bb) : 0.0;
-    if (t[1] <= w.s->tol2)
+    w.s->pq = pq;
+    w.s->relres = w.s->bb > 0.0 ? sqrt(rr / w.s->bb) : 0.0;
+    if (rr <= w.s->tol2)
       w.f->status = 1;
     else if (it >= w.f->maxiter)
       w.f->status = 3;
@@ -956,6 +1048,9 @@ int grid_of(int64_t work, int per_block, int cap) {
   int64_t g = (work + per_block - 1) / per_block;
   return (int)(g < 1 ? 1 : (g > cap ? cap : g));
 }
+
+// one grid for every flexible-CG vector kernel: their partial slots hold G entries
+int fcg_grid(int64_t n) { return grid_of(n, NTF, kMaxParts); }
 
 }  // namespace
 
@@ -973,71 +1068,65 @@ extern "C" int tm_gnn_mp_forward(int64_t n, const int32_t* row_ptr, const int32_
   const int64_t tiles = (n + TILE - 1) / TILE;
   TM_REQUIRE(tiles <= INT32_MAX, TM_ERR_LIMIT, "too many rows");
   hipStream_t st = (hipStream_t)stream;
-  MpOut o{};
-  o.out = out;
-  o.s_agg = save_msg;
-  o.s_u = save_pre;
   if (save_msg)
-    hipLaunchKernelGGL((k_mp<Dense, true, false>), dim3((unsigned)tiles), dim3(NT), 0, st, n, row_ptr, col_idx,
-                       values, Dense{h}, w_t, bias, n_linear, o, stop);
+    hipLaunchKernelGGL(k_mp<true>, dim3((unsigned)tiles), dim3(NT), 0, st, n, row_ptr, col_idx, values, h, w_t,
+                       bias, n_linear, out, save_msg, save_pre, stop);
   else
-    hipLaunchKernelGGL((k_mp<Dense, false, false>), dim3((unsigned)tiles), dim3(NT), 0, st, n, row_ptr, col_idx,
-                       values, Dense{h}, w_t, bias, n_linear, o, stop);
+    hipLaunchKernelGGL(k_mp<false>, dim3((unsigned)tiles), dim3(NT), 0, st, n, row_ptr, col_idx, values, h, w_t,
+                       bias, n_linear, out, save_msg, save_pre, stop);
   TM_HIP(hipGetLastError());
   return TM_OK;
 }
 
 extern "C" int tm_gnn_encode_pool(int64_t n, int64_t n_coarse, const int32_t* child_ptr, const int32_t* child,
                                   const float* w, const double* r, const float* d, const float* lk,
-                                  const double* ss_part, int nparts, const float* we, const float* be, int n_linear,
-                                  uint16_t* out, double* scale, const int* stop, void* stream) {
+                                  const double* ss_part, int nparts, const float* we, const float* be, uint16_t* out,
+                                  float* xs, double* scale, const int* stop, void* stream) {
   TM_REQUIRE(n >= 0 && n_coarse >= 0 && child_ptr && child && w && r && d && lk && ss_part && we && be && out &&
-                 scale,
+                 xs && scale,
              TM_ERR_ARG, "null argument");
   TM_REQUIRE(nparts >= 1 && nparts <= kMaxParts, TM_ERR_ARG, "encode_pool: bad partial count %d", nparts);
   const int g = grid_of(n_coarse, NT / 8, 8192);
   hipLaunchKernelGGL(k_encode_pool, dim3(g), dim3(NT), 0, (hipStream_t)stream, n, n_coarse, child_ptr, child, w, r,
-                     d, lk, ss_part, nparts, we, be, n_linear, out, scale, stop);
+                     d, lk, ss_part, nparts, we, be, out, (float2*)xs, scale, stop);
   TM_HIP(hipGetLastError());
   return TM_OK;
 }
 
-extern "C" int tm_gnn_unpool_mp_decode(int64_t n, const int32_t* row_ptr, const int32_t* col_idx,
-                                       const float* values, const double* r, const float* d, const float* lk,
-                                       const int32_t* agg, const float* w, const uint16_t* hc, const float* we,
-                                       const float* be, const uint16_t* w_t, const float* bias, int n_linear,
-                                       const float* wd, const float* bd, const double* scale, double* z,
-                                       const double* q, double* zq_part, int* zq_nparts, const int* stop,
-                                       void* stream) {
-  TM_REQUIRE(n >= 0 && row_ptr && col_idx && values && r && d && lk && agg && w && hc && we && be && w_t && bias &&
-                 wd && bd && scale && z,
+extern "C" int tm_gnn_final(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values,
+                            const float* xs, const float* row_sums, const int32_t* agg, const float* w,
+                            const uint16_t* hc, const int32_t* b_row_ptr, const int32_t* b_col, const float* b_values,
+                            const float* we, const float* be, const uint16_t* w_t, const float* bias, int n_linear,
+                            const float* wd, const float* bd, const float* d, const double* scale, double* z,
+                            const int* stop, void* stream) {
+  TM_REQUIRE(n >= 0 && row_ptr && col_idx && values && xs && row_sums && agg && w && hc && b_row_ptr && b_col &&
+                 b_values && we && be && w_t && bias && wd && bd && d && scale && z,
              TM_ERR_ARG, "null argument");
-  TM_REQUIRE((q == nullptr) == (zq_part == nullptr), TM_ERR_ARG, "q and zq_part go together");
   if (n == 0) return TM_OK;
   const int64_t tiles = (n + TILE - 1) / TILE;
-  const int g = grid_of(tiles, 1, kMaxParts);
-  if (zq_nparts) *zq_nparts = g;
-  MpOut o{};
-  o.wd = wd;
-  o.bd = bd;
-  o.d = d;
-  o.scale = scale;
-  o.z = z;
-  o.q = q;
-  o.zq_part = zq_part;
-  Unpooled src{};
-  src.r = r;
-  src.d = d;
-  src.lk = lk;
-  src.agg = agg;
-  src.w = w;
-  src.hc = hc;
-  src.we = we;
-  src.be = be;
-  src.scale = scale;
-  src.n_lin = n_linear;
-  hipLaunchKernelGGL((k_mp<Unpooled, false, true>), dim3(g), dim3(NT), 0, (hipStream_t)stream, n, row_ptr, col_idx,
-                     values, src, w_t, bias, n_linear, o, stop);
+  TM_REQUIRE(tiles <= INT32_MAX, TM_ERR_LIMIT, "too many rows");
+  FinalArgs A{row_ptr, col_idx, values, (const float2*)xs, row_sums, agg, w, hc, b_row_ptr, b_col, b_values, we, be,
+              w_t, bias, n_linear, wd, bd, d, scale, z};
+  hipLaunchKernelGGL(k_final, dim3((unsigned)tiles), dim3(NT), 0, (hipStream_t)stream, n, A, stop);
+  TM_HIP(hipGetLastError());
+  return TM_OK;
+}
+
+extern "C" int tm_gnn_level_prolong(int64_t nnz_b, const int32_t* seg, const int32_t* perm, const int32_t* col_idx,
+                                    const float* values_normalized, const float* w, float* b_values, void* stream) {
+  TM_REQUIRE(nnz_b >= 0 && seg && perm && col_idx && values_normalized && w && b_values, TM_ERR_ARG, "null argument");
+  if (nnz_b == 0) return TM_OK;
+  hipLaunchKernelGGL(k_level_prolong, dim3(grid_of(nnz_b, NT, 16384)), dim3(NT), 0, (hipStream_t)stream, nnz_b, seg,
+                     perm, col_idx, values_normalized, w, b_values);
+  TM_HIP(hipGetLastError());
+  return TM_OK;
+}
+
+extern "C" int tm_gnn_row_sums(int64_t n, const int32_t* row_ptr, const float* values, float* out, void* stream) {
+  TM_REQUIRE(n >= 0 && row_ptr && values && out, TM_ERR_ARG, "null argument");
+  if (n == 0) return TM_OK;
+  hipLaunchKernelGGL(k_row_sums, dim3(grid_of(n, NT, 8192)), dim3(NT), 0, (hipStream_t)stream, n, row_ptr, values,
+                     out);
   TM_HIP(hipGetLastError());
   return TM_OK;
 }
@@ -1109,39 +1198,34 @@ extern "C" int tm_gnn_sumsq(int64_t n, const double* r, const float* d, double* 
 }
 
 extern "C" int tm_gnn_encode(int64_t n, const double* r, const float* d, const float* lk, const double* ss_part,
-                             int nparts, const float* we, const float* be, int n_linear, uint16_t* h0,
-                             double* scale, const int* stop, void* stream) {
+                             int nparts, const float* we, const float* be, uint16_t* h0, double* scale,
+                             const int* stop, void* stream) {
   TM_REQUIRE(n >= 0 && r && d && lk && ss_part && we && be && h0 && scale, TM_ERR_ARG, "null argument");
   TM_REQUIRE(nparts >= 1 && nparts <= kMaxParts, TM_ERR_ARG, "encode: bad partial count %d", nparts);
   const int g = grid_of(n, NT / 8, 4096);
   hipLaunchKernelGGL(k_encode, dim3(g), dim3(NT), 0, (hipStream_t)stream, n, r, d, lk, ss_part, nparts, we, be,
-                     n_linear, h0, scale, stop);
+                     h0, scale, stop);
   TM_HIP(hipGetLastError());
   return TM_OK;
 }
 
 extern "C" int tm_gnn_encode_backward(int64_t n, const double* r, const float* d, const float* lk,
-                                      const double* scale, const float* we, const float* be, int n_linear,
-                                      const float* g_h0, float* part, float* grad, void* stream) {
-  TM_REQUIRE(n >= 0 && r && d && lk && scale && we && be && g_h0 && part && grad, TM_ERR_ARG, "null argument");
+                                      const double* scale, const float* g_h0, float* part, float* grad,
+                                      void* stream) {
+  TM_REQUIRE(n >= 0 && r && d && lk && scale && g_h0 && part && grad, TM_ERR_ARG, "null argument");
   hipStream_t st = (hipStream_t)stream;
   const int G = grid_of(n, NT / F * 16, kMaxParts);
-  hipLaunchKernelGGL(k_encode_bwd, dim3(G), dim3(NT), 0, st, n, r, d, lk, scale, we, be, n_linear, g_h0, part);
+  hipLaunchKernelGGL(k_encode_bwd, dim3(G), dim3(NT), 0, st, n, r, d, lk, scale, g_h0, part);
   hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(NT), 0, st, G, 3 * F, part, grad, 1);
   TM_HIP(hipGetLastError());
   return TM_OK;
 }
 
 extern "C" int tm_gnn_decode(int64_t n, const uint16_t* h, const float* wd, const float* bd, const float* d,
-                             const double* scale, float* y, double* z, const double* q, double* zq_part,
-                             int* zq_nparts, const int* stop, void* stream) {
+                             const double* scale, float* y, double* z, const int* stop, void* stream) {
   TM_REQUIRE(n >= 0 && h && wd && bd && d && scale && z, TM_ERR_ARG, "null argument");
-  TM_REQUIRE((q == nullptr) == (zq_part == nullptr), TM_ERR_ARG, "decode: q and zq_part go together");
-  // with (z, q) partials the grid is the flexible-CG grid, whose kernels sum them
-  const int g = zq_part ? grid_of(n, NT * 2, kMaxParts) : grid_of(n, NT / 8, kMaxParts);
-  if (zq_nparts) *zq_nparts = g;
-  hipLaunchKernelGGL(k_decode, dim3(g), dim3(NT), 0, (hipStream_t)stream, n, h, wd, bd, d, scale, y, z, q, zq_part,
-                     stop);
+  const int g = grid_of(n, NT / 8, 8192);
+  hipLaunchKernelGGL(k_decode, dim3(g), dim3(NT), 0, (hipStream_t)stream, n, h, wd, bd, d, scale, y, z, stop);
   TM_HIP(hipGetLastError());
   return TM_OK;
 }
@@ -1215,7 +1299,7 @@ extern "C" int tm_gnn_adam(int64_t n, float* param, const float* grad, float* m,
 
 // ---- flexible CG ----------------------------------------------------------
 extern "C" int64_t tm_fcg_work_size(int64_t n) {
-  return (int64_t)sizeof(double) * (4 * n + kFcgSlots * kMaxParts) + (int64_t)sizeof(FcgScalars) +
+  return (int64_t)sizeof(double) * (5 * n + kFcgSlots * kMaxParts) + (int64_t)sizeof(FcgScalars) +
          (int64_t)sizeof(FcgFlags) + 64;
 }
 
@@ -1224,14 +1308,13 @@ extern "C" int tm_fcg_layout(int64_t n, void* work, tm_fcg_view* view) {
   FcgWork w = fcg_carve(work, n);
   view->r = w.r;
   view->z = w.z;
-  view->q = w.q;
-  view->zq_part = w.part + Q_ZQ * kMaxParts;
   view->ss_part = w.part + Q_SS * kMaxParts;
   view->scale = &w.s->s;
   view->relres = &w.s->relres;
   view->status = &w.f->status;
   view->iter = &w.f->iter;
-  view->nparts = grid_of(n, NT * 2, kMaxParts);
+  view->nparts = fcg_grid(n);
+  view->reserved = 0;
   return TM_OK;
 }
 
@@ -1241,29 +1324,24 @@ extern "C" int tm_fcg_start(int64_t n, const int32_t* row_ptr, const int32_t* co
   TM_REQUIRE(n > 0 && row_ptr && col_idx && values && b && x && d && work, TM_ERR_ARG, "null argument");
   hipStream_t st = (hipStream_t)stream;
   FcgWork w = fcg_carve(work, n);
-  const int G = grid_of(n, NT * 2, kMaxParts);
-  // q = A x0 (p holds x0 for the product)
-  TM_HIP(hipMemcpyAsync(w.p, x, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  const int G = fcg_grid(n);
   TM_HIP(hipMemsetAsync(w.f, 0, sizeof(FcgFlags), st));
-  hipLaunchKernelGGL(k_fcg_spmv<8>, dim3(G), dim3(NT), 0, st, n, row_ptr, col_idx, values, w);
-  hipLaunchKernelGGL(k_fcg_init, dim3(G), dim3(NT), 0, st, n, b, d, w);
+  hipLaunchKernelGGL((k_fcg_spmv<8, true>), dim3(G), dim3(NTF), 0, st, n, row_ptr, col_idx, values, x, w);
+  hipLaunchKernelGGL(k_fcg_init, dim3(G), dim3(NTF), 0, st, n, b, d, w);
   hipLaunchKernelGGL(k_fcg_ctl0, dim3(1), dim3(NT), 0, st, w, G, rtol * rtol, atol * atol, (long long)maxiter);
   TM_HIP(hipGetLastError());
   return TM_OK;
 }
 
-// One FCG iteration after the preconditioner wrote z and the (z, q_old)
-// partials into slot 4 of the work partials.
+// One FCG iteration after the preconditioner wrote z = M(r) into the view.
 extern "C" int tm_fcg_iterate(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values,
-                              double* x, const float* d, void* work, int zq_nparts, void* stream) {
+                              double* x, const float* d, void* work, void* stream) {
   TM_REQUIRE(n > 0 && row_ptr && col_idx && values && x && d && work, TM_ERR_ARG, "null argument");
-  TM_REQUIRE(zq_nparts >= 1 && zq_nparts <= kMaxParts, TM_ERR_ARG, "fcg: bad (z, q) partial count %d", zq_nparts);
   hipStream_t st = (hipStream_t)stream;
   FcgWork w = fcg_carve(work, n);
-  const int G = grid_of(n, NT * 2, kMaxParts);
-  hipLaunchKernelGGL(k_fcg_dir, dim3(G), dim3(NT), 0, st, n, w, zq_nparts);
-  hipLaunchKernelGGL(k_fcg_spmv<8>, dim3(G), dim3(NT), 0, st, n, row_ptr, col_idx, values, w);
-  hipLaunchKernelGGL(k_fcg_update, dim3(G), dim3(NT), 0, st, n, x, d, w, G);
+  const int G = fcg_grid(n);
+  hipLaunchKernelGGL((k_fcg_spmv<8, false>), dim3(G), dim3(NTF), 0, st, n, row_ptr, col_idx, values, x, w);
+  hipLaunchKernelGGL(k_fcg_update, dim3(G), dim3(NTF), 0, st, n, x, d, w, G);
   hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(NT), 0, st, w, G);
   TM_HIP(hipGetLastError());
   return TM_OK;

@@ -17,10 +17,15 @@
 
 #include "tm_device.h"
 #include "tm_parts.h"
+#include "tm_spmv.h"
 
 namespace {
 
 using tmx::kWave;
+using tmx::row_accum;
+using tmx::row_chunk;
+using tmx::RowChunk;
+using tmx::sub_reduce;
 constexpr int kThreads = 512;
 constexpr int kMaxBlocks = 1024;
 
@@ -59,56 +64,6 @@ PcgWork carve_work(void* work, int64_t n) {
 }
 
 constexpr int P_PQ = 0, P_RZ = 1, P_RR = 2, P_BB = 3;
-
-// out[i] += sum over entries of one block row, 4-deep unrolled so a lane keeps
-// several independent (col, value, x) load chains in flight.
-template <typename VT, int S, int BS>
-__device__ __forceinline__ void row_accum(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
-                                          const VT* __restrict__ v, const double* __restrict__ x, int64_t row,
-                                          int sl, double (&out)[BS]) {
-  const int b = rp[row], e = rp[row + 1];
-  int k = b + sl;
-  for (; k + 3 * S < e; k += 4 * S) {
-    int64_t c[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) c[u] = ci[k + u * S];
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int i = 0; i < BS; ++i)
-#pragma unroll
-        for (int j = 0; j < BS; ++j)
-          out[i] += (double)v[(int64_t)(k + u * S) * BS * BS + i * BS + j] * x[c[u] * BS + j];
-  }
-  for (; k < e; k += S) {
-    const int64_t c = ci[k];
-#pragma unroll
-    for (int i = 0; i < BS; ++i)
-#pragma unroll
-      for (int j = 0; j < BS; ++j) out[i] += (double)v[(int64_t)k * BS * BS + i * BS + j] * x[c * BS + j];
-  }
-}
-
-// Rows of logical workgroup L: the contiguous range [L*chunk, (L+1)*chunk),
-// with L = XCD-remapped block id, so each XCD sweeps one contiguous slab of
-// rows and the x entries of neighbouring rows stay in its L2.
-struct RowChunk {
-  int64_t r0, r1;
-};
-__device__ __forceinline__ RowChunk row_chunk(int64_t nrows) {
-  const int L = tmx::xcd_remap(blockIdx.x, gridDim.x);
-  const int64_t chunk = (nrows + gridDim.x - 1) / gridDim.x;
-  const int64_t r0 = (int64_t)L * chunk;
-  return RowChunk{r0, r0 + chunk < nrows ? r0 + chunk : nrows};
-}
-
-template <int S, int BS>
-__device__ __forceinline__ void sub_reduce(double (&o)[BS]) {
-#pragma unroll
-  for (int i = 0; i < BS; ++i)
-#pragma unroll
-    for (int d = S / 2; d > 0; d >>= 1) o[i] += __shfl_xor(o[i], d, kWave);
-}
 
 // y = A x; each S-lane group handles R rows per trip (rows row, row + stride, ...).
 template <typename VT, int S, int BS, int R>

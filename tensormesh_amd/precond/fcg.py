@@ -2,7 +2,7 @@
 
 ``LearnedFCG`` owns persistent device buffers (matrix values, right-hand
 side, iterate, flexible-CG work) so that ``chunk`` iterations - the network's
-~20 kernels plus the four flexible-CG kernels each - are captured once in a
+~14 kernels plus the three flexible-CG kernels each - are captured once in a
 hipGraph and replayed; the host reads the device status after every replay.
 
 Fallback rules (the converged residual is the judge):
@@ -68,10 +68,10 @@ class LearnedFCG:
         L = _lib.lib()
         v = self.view
         st = _lib.stream_of(self.dev)
-        self.prog.forward(v.r, self.lk, ss_part=v.ss_part, nparts=v.nparts, z=v.z, scale=v.scale, q=v.q,
-                          zq_part=v.zq_part, stop=v.status, fused=self.fused)
+        self.prog.forward(v.r, self.lk, ss_part=v.ss_part, nparts=v.nparts, z=v.z, scale=v.scale, stop=v.status,
+                          fused=self.fused)
         check(L.tm_fcg_iterate(self.n, ptr(self.csr.row_ptr), ptr(self.csr.col_idx), ptr(self.values),
-                               ptr(self.x), ptr(self.hier.levels[0].d), ptr(self.work), self.prog.zq_nparts, st))
+                               ptr(self.x), ptr(self.hier.levels[0].d), ptr(self.work), st))
 
     def _capture(self):
         s = torch.cuda.Stream(self.dev)

@@ -36,6 +36,11 @@ class HostLevel:
     child: Optional[np.ndarray] = None      # int32
     perm: Optional[np.ndarray] = None       # int32 [nnz_prev]: previous-level entries by coarse slot
     seg: Optional[np.ndarray] = None        # int32 [nnz+1]
+    # level 0 only: pattern of B = A P (rows of this level, columns on level 1)
+    b_row_ptr: Optional[np.ndarray] = None
+    b_col: Optional[np.ndarray] = None
+    b_perm: Optional[np.ndarray] = None     # int32 [nnz]: entries of this level by B slot
+    b_seg: Optional[np.ndarray] = None
 
 
 def _csr_rows(row_ptr: np.ndarray) -> np.ndarray:
@@ -92,6 +97,15 @@ def build_host_levels(coords: np.ndarray, row_ptr: np.ndarray, col: np.ndarray, 
         crp = np.zeros(nc + 1, dtype=np.int64)
         np.cumsum(np.bincount(crow, minlength=nc), out=crp[1:])
         prev.agg = agg.astype(np.int32)
+        if lev == 1:
+            bkey = rows * nc + agg[prev.col.astype(np.int64)]
+            bu, binv = np.unique(bkey, return_inverse=True)
+            bseg = np.zeros(len(bu) + 1, dtype=np.int64)
+            np.cumsum(np.bincount(binv, minlength=len(bu)), out=bseg[1:])
+            brp = np.zeros(prev.n + 1, dtype=np.int64)
+            np.cumsum(np.bincount(bu // nc, minlength=prev.n), out=brp[1:])
+            prev.b_row_ptr, prev.b_col = brp.astype(np.int32), (bu % nc).astype(np.int32)
+            prev.b_perm, prev.b_seg = np.argsort(binv, kind="stable").astype(np.int32), bseg.astype(np.int32)
         levels.append(HostLevel(nc, crp.astype(np.int32), ccol.astype(np.int32), child_ptr=child_ptr.astype(np.int32),
                                 child=order.astype(np.int32), perm=perm.astype(np.int32), seg=seg.astype(np.int32)))
         member = node_of0
@@ -115,6 +129,12 @@ class DeviceLevel:
     d: Optional[torch.Tensor] = None        # diag^-1/2
     vn: Optional[torch.Tensor] = None       # normalised values D A D
     w: Optional[torch.Tensor] = None        # pooling weight of each node towards its parent
+    b_row_ptr: Optional[torch.Tensor] = None  # level 0: B = A^ P^ (CSR onto level 1)
+    b_col: Optional[torch.Tensor] = None
+    b_perm: Optional[torch.Tensor] = None
+    b_seg: Optional[torch.Tensor] = None
+    b_values: Optional[torch.Tensor] = None
+    row_sums: Optional[torch.Tensor] = None   # level 0: A^ 1
 
 
 class Hierarchy:
@@ -124,7 +144,8 @@ class Hierarchy:
         self.device = torch.device(device)
         t = lambda a: None if a is None else torch.from_numpy(np.ascontiguousarray(a)).to(self.device)
         self.levels = [DeviceLevel(h.n, int(h.col.size), t(h.row_ptr), t(h.col), t(h.agg), t(h.child_ptr),
-                                   t(h.child), t(h.perm), t(h.seg)) for h in host_levels]
+                                   t(h.child), t(h.perm), t(h.seg), b_row_ptr=t(h.b_row_ptr), b_col=t(h.b_col),
+                                   b_perm=t(h.b_perm), b_seg=t(h.b_seg)) for h in host_levels]
 
     @classmethod
     def from_csr(cls, coords: np.ndarray, row_ptr: np.ndarray, col: np.ndarray, n_levels: int,
@@ -164,3 +185,11 @@ class Hierarchy:
             if lev.w is None:
                 lev.w = torch.empty(lev.n, dtype=torch.float32, device=self.device)
             check(L.tm_gnn_pool_weights(lev.n, ptr(lev.agg), ptr(lev.d), ptr(nxt.d), ptr(lev.w), st))
+        l0 = self.levels[0]
+        if l0.b_row_ptr is not None:
+            if l0.b_values is None:
+                l0.b_values = torch.empty(l0.b_col.numel(), dtype=torch.float32, device=self.device)
+                l0.row_sums = torch.empty(l0.n, dtype=torch.float32, device=self.device)
+            check(L.tm_gnn_level_prolong(l0.b_col.numel(), ptr(l0.b_seg), ptr(l0.b_perm), ptr(l0.col), ptr(l0.vn),
+                                         ptr(l0.w), ptr(l0.b_values), st))
+            check(L.tm_gnn_row_sums(l0.n, ptr(l0.row_ptr), ptr(l0.vn), ptr(l0.row_sums), st))

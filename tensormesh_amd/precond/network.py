@@ -5,7 +5,7 @@ its RMS ``s``) and a log-coefficient feature ``lk``; output the correction
 ``z = s d y``.  Layers (``schedule`` lists the level of each of the six
 message-passing layers, default ``(1, 2, 3, 2, 1, 0)``):
 
-* encoder ``h = act([d r / s, lk] We + be)`` at level 0;
+* linear encoder ``h = [d r / s, lk] We + be`` at level 0;
 * before a layer on a coarser level the features are pooled level by level
   (``h_c[I] = sum_i w_i h[i]``, ``w_i = sqrt(A_ii / Ac_II)``), and the
   features left behind are kept as that level's skip;
@@ -17,10 +17,12 @@ message-passing layers, default ``(1, 2, 3, 2, 1, 0)``):
 
 When the schedule starts below level 0 and ends on it, inference fuses the
 finest level: encode + first pooling in one kernel, and unpool (with the
-encoder output recomputed as the skip) + the level-0 layer + decode in
-another, so no 64-wide level-0 feature tensor is written.  Training runs the
-unfused kernels (which save what the backward pass needs); both compute the
-same values.
+encoder output as the skip) + the level-0 layer + decode in another.  With
+the linear encoder the level-0 messages reduce to two scalar products over
+the CSR row plus ``B = A^ P^`` applied to coarse features, so no 64-wide
+level-0 tensor is read or written.  Training runs the unfused kernels (which
+save what the backward pass needs); the two agree up to BF16 rounding of the
+level-0 features.
 
 Every kernel is a HIP kernel of ``libtensormesh_amd.so`` (forward, backward,
 loss and Adam); torch only holds the buffers.  Weights live as an FP32 master
@@ -189,13 +191,14 @@ class Program:
         self.fusable = (len(ops) >= 3 and ops[0][0] == "pool" and ops[0][1] == 0 and ops[0][2] == self.enc_out
                         and ops[-2][0] == "unpool" and ops[-2][1] == 0 and ops[-2][4] == self.enc_out
                         and ops[-1][0] == "mp" and ops[-1][1] == 0 and ops[-1][2] == ops[-2][3]
-                        and ops[-1][3] == self.final and uses_enc == 2)
-        self.zq_nparts = 0
+                        and ops[-1][3] == self.final and uses_enc == 2
+                        and getattr(hier.levels[0], "b_row_ptr", None) is not None)
         self.saves = {}
         self.y = torch.zeros(self.n0, dtype=torch.float32, device=dev)
         self.z = torch.zeros(self.n0, dtype=torch.float64, device=dev)
         self.scale = torch.zeros(1, dtype=torch.float64, device=dev)
         self.ss_part = torch.zeros(1024, dtype=torch.float64, device=dev)
+        self.xs = torch.zeros((self.n0, 2), dtype=torch.float32, device=dev) if self.fusable else None
 
     def alloc_training(self):
         dev = self.hier.device
@@ -205,11 +208,11 @@ class Program:
                 self.saves[i] = (torch.zeros((n, F), dtype=torch.bfloat16, device=dev),
                                  torch.zeros((n, F), dtype=torch.bfloat16, device=dev))
 
-    def forward(self, r, lk: torch.Tensor, *, ss_part=None, nparts: int = 0, z=None, scale=None, q=None,
-                zq_part=None, stop=None, train: bool = False, want_y: bool = False, fused: bool = True):
+    def forward(self, r, lk: torch.Tensor, *, ss_part=None, nparts: int = 0, z=None, scale=None, stop=None,
+                train: bool = False, want_y: bool = False, fused: bool = True):
         """z = M(r).  Without ``ss_part`` the RMS partials are computed here.
-        ``r``, ``z``, ``q``, ``scale``, ``ss_part``, ``zq_part`` and ``stop`` may be
-        tensors or raw device addresses (the flexible-CG work buffer)."""
+        ``r``, ``z``, ``scale``, ``ss_part`` and ``stop`` may be tensors or raw
+        device addresses (the flexible-CG work buffer)."""
         L = _lib.lib()
         net, hier = self.net, self.hier
         st = _lib.stream_of(hier.device)
@@ -224,31 +227,27 @@ class Program:
         scale_p = ptr(self.scale) if scale is None else _addr(scale)
         we, be = net.enc()
         zt = self.z if z is None else z
-        npc = ctypes.c_int(0)
         if fused and self.fusable and not train and not want_y:
             first, unp, last = self.ops[0], self.ops[-2], self.ops[-1]
             lv1 = hier.levels[1]
             check(L.tm_gnn_encode_pool(self.n0, lv1.n, ptr(lv1.child_ptr), ptr(lv1.child), ptr(lv0.w), _addr(r),
-                                       ptr(lv0.d), ptr(lk), ss_ptr, nparts, ptr(we), ptr(be), net.n_linear,
-                                       ptr(self.bufs[first[3]]), scale_p, stop_p, st))
+                                       ptr(lv0.d), ptr(lk), ss_ptr, nparts, ptr(we), ptr(be),
+                                       ptr(self.bufs[first[3]]), ptr(self.xs), scale_p, stop_p, st))
             self._run_ops(self.ops[1:-2], 1, train, stop_p, st)
             _, bias = net.layer(last[4])
             wd, bd = net.dec()
-            check(L.tm_gnn_unpool_mp_decode(self.n0, ptr(lv0.row_ptr), ptr(lv0.col), ptr(lv0.vn), _addr(r),
-                                            ptr(lv0.d), ptr(lk), ptr(lv0.agg), ptr(lv0.w), ptr(self.bufs[unp[2]]),
-                                            ptr(we), ptr(be), ptr(net.wt[last[4]]), ptr(bias), net.n_linear,
-                                            ptr(wd), ptr(bd), scale_p, _addr(zt), _addr(q), _addr(zq_part),
-                                            ctypes.byref(npc), stop_p, st))
-            self.zq_nparts = npc.value
+            check(L.tm_gnn_final(self.n0, ptr(lv0.row_ptr), ptr(lv0.col), ptr(lv0.vn), ptr(self.xs),
+                                 ptr(lv0.row_sums), ptr(lv0.agg), ptr(lv0.w), ptr(self.bufs[unp[2]]),
+                                 ptr(lv0.b_row_ptr), ptr(lv0.b_col), ptr(lv0.b_values), ptr(we), ptr(be),
+                                 ptr(net.wt[last[4]]), ptr(bias), net.n_linear, ptr(wd), ptr(bd), ptr(lv0.d), scale_p,
+                                 _addr(zt), stop_p, st))
             return zt
         check(L.tm_gnn_encode(self.n0, _addr(r), ptr(lv0.d), ptr(lk), ss_ptr, nparts, ptr(we), ptr(be),
-                              net.n_linear, ptr(self.bufs[self.enc_out]), scale_p, stop_p, st))
+                              ptr(self.bufs[self.enc_out]), scale_p, stop_p, st))
         self._run_ops(self.ops, 0, train, stop_p, st)
         wd, bd = net.dec()
         check(L.tm_gnn_decode(self.n0, ptr(self.bufs[self.final]), ptr(wd), ptr(bd), ptr(lv0.d), scale_p,
-                              ptr(self.y) if (train or want_y) else _addr(None), _addr(zt), _addr(q),
-                              _addr(zq_part), ctypes.byref(npc), stop_p, st))
-        self.zq_nparts = npc.value
+                              ptr(self.y) if (train or want_y) else _addr(None), _addr(zt), stop_p, st))
         return zt
 
     def _run_ops(self, ops, first_index, train, stop_p, st):
@@ -333,8 +332,8 @@ class Program:
                 self._accum(grads, extra, gout)
         genc = grads.pop(self.enc_out)
         we, be = net.enc()
-        check(L.tm_gnn_encode_backward(self.n0, ptr(r), ptr(lv0.d), ptr(lk), ptr(self.scale), ptr(we), ptr(be),
-                                       net.n_linear, ptr(genc), ptr(net._part), ptr(gview(0)), st))
+        check(L.tm_gnn_encode_backward(self.n0, ptr(r), ptr(lv0.d), ptr(lk), ptr(self.scale), ptr(genc),
+                                       ptr(net._part), ptr(gview(0)), st))
         return sums
 
     @staticmethod

@@ -109,7 +109,9 @@ def test_network_forward(cuda):
 
 @pytest.mark.parametrize("n", [8, 13])
 def test_fused_inference_matches_unfused(cuda, n):
-    """Fused finest level (encode+pool, unpool+layer+decode) = unfused kernels."""
+    """Fused finest level (encode+pool, unpool+layer+decode with B = A^ P^) =
+    unfused kernels, up to the BF16 rounding of the level-0 features that only
+    the unfused path stores."""
     print(f"case precond fused-vs-unfused n={n}", flush=True)
     Ai, H, hier, levels, values, csr, lk = _problem(n, 10 + n, cuda)
     net = _perturbed(cuda, seed=n)
@@ -119,7 +121,7 @@ def test_fused_inference_matches_unfused(cuda, n):
     lkt = torch.tensor(lk, device=cuda)
     zf = prog.forward(r, lkt, fused=True).clone()
     zu = prog.forward(r, lkt, fused=False).clone()
-    assert torch.linalg.vector_norm(zf - zu) <= 1e-3 * torch.linalg.vector_norm(zu)
+    assert torch.linalg.vector_norm(zf - zu) <= 1e-2 * torch.linalg.vector_norm(zu)
     z_ref = og.forward(levels, _params64(net), net.schedule, 32, r.cpu().numpy(), lk.astype(np.float64), bf16=True)[0]
     assert np.linalg.norm(zf.cpu().numpy() - z_ref) <= 2e-2 * np.linalg.norm(z_ref)
 
