@@ -282,6 +282,11 @@ int tm_gnn_final(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, cons
                  const float* be, const uint16_t* w_t, const float* bias, int n_linear, const float* wd,
                  const float* bd, const float* d, const double* scale, double* z, const int* stop,
                  void* stream);
+/* Finest level without a layer (inference): z = s d ((e + w hc[agg]) wd + bd)
+ * with e the encoder output from xs. */
+int tm_gnn_unpool_decode(int64_t n, const float* xs, const int32_t* agg, const float* w, const uint16_t* hc,
+                         const float* we, const float* be, const float* wd, const float* bd, const float* d,
+                         const double* scale, double* z, const int* stop, void* stream);
 /* Finest-level prolongation B = A^ P^: b_values[s] = sum of A^_k w_col(k) over
  * the entries k in perm[seg[s] .. seg[s+1]).  Row sums of a CSR. */
 int tm_gnn_level_prolong(int64_t nnz_b, const int32_t* seg, const int32_t* perm, const int32_t* col_idx,
@@ -304,10 +309,13 @@ int tm_gnn_level_coarsen(int64_t nnz_coarse, const int32_t* seg, const int32_t* 
 /* w_i = d_coarse[agg_i] / d_fine[i]. */
 int tm_gnn_pool_weights(int64_t n_fine, const int32_t* agg, const float* d_fine, const float* d_coarse, float* w,
                         void* stream);
-int tm_gnn_adam(int64_t n, float* param, const float* grad, float* m, float* v, float lr, float beta1, float beta2,
-                float eps, int64_t step, void* stream);
+/* Adam step on the FP32 master vector; entries whose mask is 0 stay fixed
+ * (mask nullable). */
+int tm_gnn_adam(int64_t n, float* param, const float* grad, float* m, float* v, const float* mask, float lr,
+                float beta1, float beta2, float eps, int64_t step, void* stream);
 
-/* Flexible CG (truncated to one direction) around an external preconditioner:
+/* Flexible CG truncated to the last m directions (m = 1, 2, 4, 8) around an
+ * external preconditioner:
  * tm_fcg_start, then per iteration the preconditioner writes z = M(r) into the
  * view and tm_fcg_iterate finishes the iteration (one SpMV with its dot
  * products, one update, one single-workgroup convergence check).  Everything
@@ -325,13 +333,13 @@ typedef struct {
   int32_t nparts;
   int32_t reserved;
 } tm_fcg_view;
-int64_t tm_fcg_work_size(int64_t n);
-int tm_fcg_layout(int64_t n, void* work, tm_fcg_view* view);
-int tm_fcg_start(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values, const double* b,
-                 const double* x, const float* d, void* work, double rtol, double atol, int64_t maxiter,
-                 void* stream);
-int tm_fcg_iterate(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values, double* x,
-                   const float* d, void* work, void* stream);
+int64_t tm_fcg_work_size(int64_t n, int m);
+int tm_fcg_layout(int64_t n, int m, void* work, tm_fcg_view* view);
+int tm_fcg_start(int64_t n, int m, const int32_t* row_ptr, const int32_t* col_idx, const float* values,
+                 const double* b, const double* x, const float* d, void* work, double rtol, double atol,
+                 int64_t maxiter, void* stream);
+int tm_fcg_iterate(int64_t n, int m, const int32_t* row_ptr, const int32_t* col_idx, const float* values,
+                   double* x, const float* d, void* work, void* stream);
 
 #ifdef __cplusplus
 }

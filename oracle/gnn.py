@@ -194,26 +194,27 @@ def backward(levels, params, schedule, n_lin, r, lk, tape, s, bf16=False):
 
 
 def fcg(A: sp.csr_matrix, b: np.ndarray, precond: Callable[[np.ndarray], np.ndarray], rtol: float = 1e-8,
-        maxiter: int = 1000):
-    """Flexible CG truncated to one direction (Notay 2000, FCG(1)), float64.
+        maxiter: int = 1000, m: int = 1):
+    """Flexible CG truncated to the last ``m`` directions (Notay 2000, FCG(m)),
+    float64, with the explicit orthogonalisation p = z - sum_j (z.Ap_j)/(p_j.Ap_j) p_j.
     Returns (x, iterations, relres history)."""
     x = np.zeros_like(b)
     r = b.copy()
     bb = np.sqrt(b @ b)
-    p = np.zeros_like(b)
-    q = np.zeros_like(b)
-    pq = 1.0
+    dirs = []          # (p_j, q_j = A p_j, p_j.q_j), newest last
     hist = [np.sqrt(r @ r) / bb]
     it = 0
     while hist[-1] > rtol and it < maxiter:
         z = precond(r)
-        beta = (z @ q) / pq
-        p = z - beta * p
+        p = z.copy()
+        for pj, qj, pqj in dirs:
+            p -= (z @ qj) / pqj * pj
         q = A @ p
         pq = p @ q
         alpha = (p @ r) / pq
         x = x + alpha * p
         r = r - alpha * q
+        dirs = (dirs + [(p, q, pq)])[-m:]
         it += 1
         hist.append(np.sqrt(r @ r) / bb)
     return x, it, hist

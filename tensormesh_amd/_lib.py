@@ -106,17 +106,18 @@ _SIGS = {
     "tm_gnn_encode_pool": (c_int, [c_i64, c_i64] + [c_void_p] * 7 + [c_int] + [c_void_p] * 7),
     "tm_gnn_final": (c_int, [c_i64] + [c_void_p] * 15 + [c_int] + [c_void_p] * 7),
     "tm_gnn_level_prolong": (c_int, [c_i64] + [c_void_p] * 7),
+    "tm_gnn_unpool_decode": (c_int, [c_i64] + [c_void_p] * 13),
     "tm_gnn_row_sums": (c_int, [c_i64] + [c_void_p] * 4),
     "tm_gnn_decode_backward": (c_int, [c_i64] + [c_void_p] * 7),
     "tm_gnn_energy_loss": (c_int, [c_i64] + [c_void_p] * 10 + [ctypes.POINTER(c_int), c_void_p]),
     "tm_gnn_level_setup": (c_int, [c_i64] + [c_void_p] * 6),
     "tm_gnn_level_coarsen": (c_int, [c_i64] + [c_void_p] * 5),
     "tm_gnn_pool_weights": (c_int, [c_i64] + [c_void_p] * 5),
-    "tm_gnn_adam": (c_int, [c_i64] + [c_void_p] * 4 + [ctypes.c_float] * 4 + [c_i64, c_void_p]),
-    "tm_fcg_work_size": (c_i64, [c_i64]),
-    "tm_fcg_layout": (c_int, [c_i64, c_void_p, ctypes.POINTER(FcgView)]),
-    "tm_fcg_start": (c_int, [c_i64] + [c_void_p] * 7 + [c_double, c_double, c_i64, c_void_p]),
-    "tm_fcg_iterate": (c_int, [c_i64] + [c_void_p] * 7),
+    "tm_gnn_adam": (c_int, [c_i64] + [c_void_p] * 5 + [ctypes.c_float] * 4 + [c_i64, c_void_p]),
+    "tm_fcg_work_size": (c_i64, [c_i64, c_int]),
+    "tm_fcg_layout": (c_int, [c_i64, c_int, c_void_p, ctypes.POINTER(FcgView)]),
+    "tm_fcg_start": (c_int, [c_i64, c_int] + [c_void_p] * 7 + [c_double, c_double, c_i64, c_void_p]),
+    "tm_fcg_iterate": (c_int, [c_i64, c_int] + [c_void_p] * 7),
     "tm_pcg_work_size": (c_i64, [c_i64]),
     "tm_pcg_jacobi": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                               c_double, c_double, c_i64, c_int, ctypes.POINTER(c_i64),

@@ -722,6 +722,48 @@ __global__ __launch_bounds__(NT) void k_decode(int64_t n, const uint16_t* __rest
   }
 }
 
+// Finest level at inference when no layer runs there: unpool with the
+// encoder output as skip, then decode:
+//   y_i = (e_i + w_i hc[agg_i]) . wd + bd,  e_i = x_i0 We0 + x_i1 We1 + be,
+//   z_i = s d_i y_i.   8 lanes per node, xs from k_encode_pool.
+__global__ __launch_bounds__(NT) void k_unpool_decode(int64_t n, const float2* __restrict__ xs,
+                                                      const int32_t* __restrict__ agg, const float* __restrict__ w,
+                                                      const uint16_t* __restrict__ hc, const float* __restrict__ we,
+                                                      const float* __restrict__ be, const float* __restrict__ wd,
+                                                      const float* __restrict__ bd, const float* __restrict__ d,
+                                                      const double* __restrict__ s_in, double* __restrict__ z,
+                                                      const int* stop) {
+  if (stopped(stop)) return;
+  const double s = *s_in;
+  const int q = threadIdx.x % 8, f0 = q * 8;
+  float c0 = 0.0f, c1 = 0.0f, cb = 0.0f, wl[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    wl[t] = wd[f0 + t];
+    c0 += we[f0 + t] * wl[t];
+    c1 += we[F + f0 + t] * wl[t];
+    cb += be[f0 + t] * wl[t];
+  }
+  const float b = bd[0];
+  for (int64_t i0 = blockIdx.x * (int64_t)(NT / 8); i0 < n; i0 += (int64_t)gridDim.x * (NT / 8)) {
+    const int64_t i = i0 + threadIdx.x / 8;
+    float acc = 0.0f;
+    float2 xi = float2{0.0f, 0.0f};
+    if (i < n) {
+      xi = xs[i];
+      float v[8];
+      Vec8<uint16_t>::load(&hc[(int64_t)agg[i] * F + f0], v);
+      float dot = 0.0f;
+#pragma unroll
+      for (int t = 0; t < 8; ++t) dot += v[t] * wl[t];
+      acc = xi.x * c0 + xi.y * c1 + cb + w[i] * dot;
+    }
+#pragma unroll
+    for (int o = 4; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kWave);
+    if (q == 0 && i < n) z[i] = s * (double)d[i] * (double)(acc + b);
+  }
+}
+
 // g_h[i][f] = g_y_i wd[f]; partials dwd[f] = sum h[i][f] g_y_i, dbd = sum g_y_i
 __global__ __launch_bounds__(NT) void k_decode_bwd(int64_t n, const uint16_t* __restrict__ h,
                                                    const float* __restrict__ wd, const float* __restrict__ g_y,
@@ -840,11 +882,13 @@ __global__ __launch_bounds__(NT) void k_row_sums(int64_t n, const int32_t* __res
   }
 }
 
-// Adam (bias-corrected), FP32 master weights
+// Adam (bias-corrected), FP32 master weights; entries with mask[i] == 0 stay fixed
 __global__ __launch_bounds__(NT) void k_adam(int64_t n, float* __restrict__ p, const float* __restrict__ g,
-                                             float* __restrict__ m, float* __restrict__ v, float lr, float b1,
-                                             float b2, float eps, float c1, float c2) {
+                                             float* __restrict__ m, float* __restrict__ v,
+                                             const float* __restrict__ mask, float lr, float b1, float b2, float eps,
+                                             float c1, float c2) {
   for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+    if (mask && mask[i] == 0.0f) continue;
     const float gi = g[i];
     const float mi = b1 * m[i] + (1.0f - b1) * gi;
     const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
@@ -855,48 +899,58 @@ __global__ __launch_bounds__(NT) void k_adam(int64_t n, float* __restrict__ p, c
 }
 
 // ---------------------------------------------------------------------------
-// Flexible CG, truncated to one previous direction (FCG(1), Notay 2000):
-//   z = M(r); beta = (z, q_o) / (p_o, q_o); p = z - beta p_o; q = A p
-//   alpha = (p, r) / (p, q); x += alpha p; r -= alpha q
-// evaluated with one reduction phase per iteration: w = A z and the dots
-// (z,q_o), (z,r), (z,w), (p_o,r) come out of one SpMV kernel, then
-//   q = w - beta q_o,  (p,r) = (z,r) - beta (p_o,r),
-//   (p,q) = (z,w) - 2 beta (z,q_o) + beta^2 (p_o,q_o)     (A symmetric)
-// and one update kernel applies everything.
+// Flexible CG truncated to the last m directions (FCG(m), Notay 2000):
+//   z = M(r); p = z - sum_j beta_j p_j, beta_j = (z, q_j) / (p_j, q_j)
+//   q = A p; alpha = (p, r) / (p, q); x += alpha p; r -= alpha q
+// over a ring of m stored (p_j, q_j = A p_j), evaluated with one reduction
+// phase per iteration: w = A z and the dots (z,q_j), (p_j,r), (z,r), (z,w)
+// come out of one SpMV kernel, then (the stored directions being mutually
+// A-orthogonal)
+//   q = w - sum beta_j q_j,  (p,r) = (z,r) - sum beta_j (p_j,r),
+//   (p,q) = (z,w) - 2 sum beta_j (z,q_j) + sum beta_j^2 (p_j,q_j)
+// and one update kernel applies everything.  Unfilled slots hold zeros with
+// (p_j,q_j) = 1, so their beta_j is 0.
 // ---------------------------------------------------------------------------
+constexpr int kFcgMaxM = 8;
 struct FcgScalars {
-  double bb, tol2, pq, pq_new, relres, s;
+  double bb, tol2, relres, s, pq_new;
+  double pq[kFcgMaxM];
 };
 struct FcgFlags {
   int status;  // 0 running, 1 converged, 2 breakdown, 3 max iterations
   int iter;
   long long maxiter;
 };
-constexpr int Q_ZQ = 0, Q_ZR = 1, Q_ZW = 2, Q_PR = 3, Q_RR = 4, Q_BB = 5, Q_SS = 6, Q_PQ = 7;
-constexpr int kFcgSlots = 8;
+// partial slots: (z,q_j) [m] | (p_j,r) [m] | zr | zw | rr | bb | ss | pq
+constexpr int S_ZQ = 0, S_PR = kFcgMaxM, S_ZR = 2 * kFcgMaxM, S_ZW = S_ZR + 1, S_RR = S_ZR + 2, S_BB = S_ZR + 3,
+              S_SS = S_ZR + 4, S_PQ = S_ZR + 5;
+constexpr int kFcgSlots = S_PQ + 1;
 
 struct FcgWork {
-  double *r, *p, *q, *z, *w, *part;
+  double *r, *z, *w, *P, *Q, *part;  // P, Q: [m][n]
   FcgScalars* s;
   FcgFlags* f;
+  int m;
 };
 
-FcgWork fcg_carve(void* work, int64_t n) {
+FcgWork fcg_carve(void* work, int64_t n, int m) {
   FcgWork w;
   double* b = (double*)work;
   w.r = b;
-  w.p = b + n;
-  w.q = b + 2 * n;
-  w.z = b + 3 * n;
-  w.w = b + 4 * n;
-  w.part = b + 5 * n;
+  w.z = b + n;
+  w.w = b + 2 * n;
+  w.P = b + 3 * n;
+  w.Q = w.P + (int64_t)m * n;
+  w.part = w.Q + (int64_t)m * n;
   w.s = (FcgScalars*)(w.part + kFcgSlots * kMaxParts);
   w.f = (FcgFlags*)(w.s + 1);
+  w.m = m;
   return w;
 }
 
-// START: q = A x.  Otherwise w = A z with partials (z,q), (z,r), (z,w), (p,r).
-template <int S, bool START>
+// START: w = A x (the residual is formed by k_fcg_init).  Otherwise w = A z
+// with partials (z,q_j), (p_j,r), (z,r), (z,w).
+template <int S, int M, bool START>
 __global__ __launch_bounds__(NTF) void k_fcg_spmv(int64_t n, const int32_t* __restrict__ rp,
                                                   const int32_t* __restrict__ ci, const float* __restrict__ v,
                                                   const double* __restrict__ x, FcgWork w) {
@@ -906,8 +960,9 @@ __global__ __launch_bounds__(NTF) void k_fcg_spmv(int64_t n, const int32_t* __re
   const int sl = threadIdx.x % S;
   const tmx::RowChunk rc = tmx::row_chunk(n);
   const double* src = START ? x : w.z;
-  double* dst = START ? w.q : w.w;
-  double loc[4] = {0.0, 0.0, 0.0, 0.0};
+  double loc[2 * M + 2];
+#pragma unroll
+  for (int k = 0; k < 2 * M + 2; ++k) loc[k] = 0.0;
   for (int64_t row = rc.r0 + threadIdx.x / S; row < rc.r1; row += stride * R) {
     double o[R][1];
 #pragma unroll
@@ -918,52 +973,66 @@ __global__ __launch_bounds__(NTF) void k_fcg_spmv(int64_t n, const int32_t* __re
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       tmx::sub_reduce<S, 1>(o[r]);
-      const int64_t rr = row + r * stride;
-      if (sl == 0 && rr < rc.r1) {
-        dst[rr] = o[r][0];
+      const int64_t i = row + r * stride;
+      if (sl == 0 && i < rc.r1) {
+        w.w[i] = o[r][0];
         if (!START) {
-          const double zi = w.z[rr], ri = w.r[rr];
-          loc[0] += zi * w.q[rr];
-          loc[1] += zi * ri;
-          loc[2] += zi * o[r][0];
-          loc[3] += w.p[rr] * ri;
+          const double zi = w.z[i], ri = w.r[i];
+#pragma unroll
+          for (int j = 0; j < M; ++j) {
+            loc[j] += zi * w.Q[(int64_t)j * n + i];
+            loc[M + j] += w.P[(int64_t)j * n + i] * ri;
+          }
+          loc[2 * M] += zi * ri;
+          loc[2 * M + 1] += zi * o[r][0];
         }
       }
     }
   }
   if (!START) {
-    const int which[4] = {Q_ZQ, Q_ZR, Q_ZW, Q_PR};
-    tmx::write_parts<NTF, 4>(w.part, kMaxParts, which, loc);
+    int which[2 * M + 2];
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      which[j] = S_ZQ + j;
+      which[M + j] = S_PR + j;
+    }
+    which[2 * M] = S_ZR;
+    which[2 * M + 1] = S_ZW;
+    tmx::write_parts<NTF, 2 * M + 2>(w.part, kMaxParts, which, loc);
   }
 }
 
-// r = b - q (q = A x0), p = q = 0; partials (r,r), (b,b), (d r)^2
+// r = b - w (w = A x0), P = Q = 0; partials (r,r), (b,b), (d r)^2
+template <int M>
 __global__ __launch_bounds__(NTF) void k_fcg_init(int64_t n, const double* __restrict__ b, const float* __restrict__ d,
                                                   FcgWork w) {
   double loc[3] = {0.0, 0.0, 0.0};
   for (int64_t i = blockIdx.x * (int64_t)NTF + threadIdx.x; i < n; i += (int64_t)gridDim.x * NTF) {
-    const double ri = b[i] - w.q[i];
+    const double ri = b[i] - w.w[i];
     w.r[i] = ri;
-    w.p[i] = 0.0;
-    w.q[i] = 0.0;
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      w.P[(int64_t)j * n + i] = 0.0;
+      w.Q[(int64_t)j * n + i] = 0.0;
+    }
     loc[0] += ri * ri;
     loc[1] += b[i] * b[i];
     const double dr = (double)d[i] * ri;
     loc[2] += dr * dr;
   }
-  const int which[3] = {Q_RR, Q_BB, Q_SS};
+  const int which[3] = {S_RR, S_BB, S_SS};
   tmx::write_parts<NTF, 3>(w.part, kMaxParts, which, loc);
 }
 
 __global__ __launch_bounds__(NT) void k_fcg_ctl0(FcgWork w, int nparts, double rtol2, double atol2,
                                                  long long maxiter) {
-  const int which[2] = {Q_RR, Q_BB};
+  const int which[2] = {S_RR, S_BB};
   double t[2];
   tmx::sum_parts<2>(w.part, kMaxParts, which, nparts, t);
   if (threadIdx.x == 0) {
     w.s->bb = t[1];
     w.s->tol2 = fmax(rtol2 * t[1], atol2);
-    w.s->pq = 1.0;
+    for (int j = 0; j < kFcgMaxM; ++j) w.s->pq[j] = 1.0;
     w.s->pq_new = 1.0;
     w.s->relres = t[1] > 0.0 ? sqrt(t[0] / t[1]) : 0.0;
     w.f->iter = 0;
@@ -972,27 +1041,45 @@ __global__ __launch_bounds__(NT) void k_fcg_ctl0(FcgWork w, int nparts, double r
   }
 }
 
-// beta, alpha from the partials; p = z - beta p, q = w - beta q, x += alpha p,
-// r -= alpha q; partials (r,r), (d r)^2; (p,q) goes to pq_new via k_fcg_check.
+// betas and alpha from the partials; p, q into ring slot iter % m; x, r;
+// partials (r,r), (d r)^2 and (p,q) (workgroup 0 only)
+template <int M>
 __global__ __launch_bounds__(NTF) void k_fcg_update(int64_t n, double* __restrict__ x, const float* __restrict__ d,
                                                     FcgWork w, int nparts) {
   if (w.f->status != 0) return;
-  const int which_in[4] = {Q_ZQ, Q_ZR, Q_ZW, Q_PR};
-  double t[4];
-  tmx::sum_parts<4>(w.part, kMaxParts, which_in, nparts, t);
-  const double pq_o = w.s->pq;
-  const double beta = t[0] / pq_o;
-  const double pr = t[1] - beta * t[3];
-  const double pq = t[2] - 2.0 * beta * t[0] + beta * beta * pq_o;
+  __shared__ double sh[2 * M + 2];
+  if (threadIdx.x < kWave) {
+    for (int k = 0; k < 2 * M + 2; ++k) {
+      const int slot = k < M ? S_ZQ + k : (k < 2 * M ? S_PR + (k - M) : (k == 2 * M ? S_ZR : S_ZW));
+      double a = 0.0;
+      for (int b = threadIdx.x; b < nparts; b += kWave) a += w.part[slot * kMaxParts + b];
+      a = tmx::wave_sum(a);
+      if (threadIdx.x == 0) sh[k] = a;
+    }
+  }
+  __syncthreads();
+  double beta[M];
+  double pr = sh[2 * M], pq = sh[2 * M + 1];
+#pragma unroll
+  for (int j = 0; j < M; ++j) {
+    beta[j] = sh[j] / w.s->pq[j];
+    pr -= beta[j] * sh[M + j];
+    pq += beta[j] * (beta[j] * w.s->pq[j] - 2.0 * sh[j]);
+  }
   const bool ok = pq > 0.0 && __builtin_isfinite(pq) && __builtin_isfinite(pr);
   const double alpha = ok ? pr / pq : 0.0;
+  const int slot = w.f->iter % M;
   double loc[3] = {0.0, 0.0, 0.0};
   if (ok) {
     for (int64_t i = blockIdx.x * (int64_t)NTF + threadIdx.x; i < n; i += (int64_t)gridDim.x * NTF) {
-      const double pi = w.z[i] - beta * w.p[i];
-      const double qi = w.w[i] - beta * w.q[i];
-      w.p[i] = pi;
-      w.q[i] = qi;
+      double pi = w.z[i], qi = w.w[i];
+#pragma unroll
+      for (int j = 0; j < M; ++j) {
+        pi -= beta[j] * w.P[(int64_t)j * n + i];
+        qi -= beta[j] * w.Q[(int64_t)j * n + i];
+      }
+      w.P[(int64_t)slot * n + i] = pi;
+      w.Q[(int64_t)slot * n + i] = qi;
       x[i] += alpha * pi;
       const double ri = w.r[i] - alpha * qi;
       w.r[i] = ri;
@@ -1002,19 +1089,18 @@ __global__ __launch_bounds__(NTF) void k_fcg_update(int64_t n, double* __restric
     }
   }
   loc[2] = (blockIdx.x == 0 && threadIdx.x == 0) ? (ok ? pq : -1.0) : 0.0;
-  const int which[3] = {Q_RR, Q_SS, Q_PQ};
+  const int which[3] = {S_RR, S_SS, S_PQ};
   tmx::write_parts<NTF, 3>(w.part, kMaxParts, which, loc);
 }
 
-// Single workgroup: (p,q) for the next beta, iteration count, convergence.
-// All four waves read the partials.
+// Single workgroup: (p,q) of the new ring slot, iteration count, convergence.
 __global__ __launch_bounds__(NT) void k_fcg_check(FcgWork w, int nparts) {
   if (w.f->status != 0) return;
   __shared__ double red[NT / kWave][2];
   double a = 0.0, b = 0.0;
   for (int k = threadIdx.x; k < nparts; k += NT) {
-    a += w.part[Q_RR * kMaxParts + k];
-    b += w.part[Q_PQ * kMaxParts + k];
+    a += w.part[S_RR * kMaxParts + k];
+    b += w.part[S_PQ * kMaxParts + k];
   }
   a = tmx::wave_sum(a);
   b = tmx::wave_sum(b);
@@ -1029,17 +1115,17 @@ __global__ __launch_bounds__(NT) void k_fcg_check(FcgWork w, int nparts) {
       rr += red[k][0];
       pq += red[k][1];
     }
-    const int it = w.f->iter + 1;
-    w.f->iter = it;
+    const int it = w.f->iter;
+    w.f->iter = it + 1;
     if (!(pq > 0.0) || !__builtin_isfinite(pq) || !__builtin_isfinite(rr)) {
       w.f->status = 2;
       return;
     }
-    w.s->pq = pq;
+    w.s->pq[it % w.m] = pq;
     w.s->relres = w.s->bb > 0.0 ? sqrt(rr / w.s->bb) : 0.0;
     if (rr <= w.s->tol2)
       w.f->status = 1;
-    else if (it >= w.f->maxiter)
+    else if (it + 1 >= w.f->maxiter)
       w.f->status = 3;
   }
 }
@@ -1108,6 +1194,19 @@ extern "C" int tm_gnn_final(int64_t n, const int32_t* row_ptr, const int32_t* co
   FinalArgs A{row_ptr, col_idx, values, (const float2*)xs, row_sums, agg, w, hc, b_row_ptr, b_col, b_values, we, be,
               w_t, bias, n_linear, wd, bd, d, scale, z};
   hipLaunchKernelGGL(k_final, dim3((unsigned)tiles), dim3(NT), 0, (hipStream_t)stream, n, A, stop);
+  TM_HIP(hipGetLastError());
+  return TM_OK;
+}
+
+extern "C" int tm_gnn_unpool_decode(int64_t n, const float* xs, const int32_t* agg, const float* w,
+                                    const uint16_t* hc, const float* we, const float* be, const float* wd,
+                                    const float* bd, const float* d, const double* scale, double* z, const int* stop,
+                                    void* stream) {
+  TM_REQUIRE(n >= 0 && xs && agg && w && hc && we && be && wd && bd && d && scale && z, TM_ERR_ARG,
+             "null argument");
+  if (n == 0) return TM_OK;
+  hipLaunchKernelGGL(k_unpool_decode, dim3(grid_of(n, NT / 8, 8192)), dim3(NT), 0, (hipStream_t)stream, n,
+                     (const float2*)xs, agg, w, hc, we, be, wd, bd, d, scale, z, stop);
   TM_HIP(hipGetLastError());
   return TM_OK;
 }
@@ -1286,29 +1385,31 @@ extern "C" int tm_gnn_pool_weights(int64_t n_fine, const int32_t* agg, const flo
   return TM_OK;
 }
 
-extern "C" int tm_gnn_adam(int64_t n, float* param, const float* grad, float* m, float* v, float lr, float beta1,
-                           float beta2, float eps, int64_t step, void* stream) {
+extern "C" int tm_gnn_adam(int64_t n, float* param, const float* grad, float* m, float* v, const float* mask,
+                           float lr, float beta1, float beta2, float eps, int64_t step, void* stream) {
   TM_REQUIRE(n >= 0 && param && grad && m && v && step >= 1, TM_ERR_ARG, "bad adam arguments");
   if (n == 0) return TM_OK;
   const float c1 = 1.0f - powf(beta1, (float)step), c2 = 1.0f - powf(beta2, (float)step);
-  hipLaunchKernelGGL(k_adam, dim3(grid_of(n, NT, 1024)), dim3(NT), 0, (hipStream_t)stream, n, param, grad, m, v, lr,
-                     beta1, beta2, eps, c1, c2);
+  hipLaunchKernelGGL(k_adam, dim3(grid_of(n, NT, 1024)), dim3(NT), 0, (hipStream_t)stream, n, param, grad, m, v, mask,
+                     lr, beta1, beta2, eps, c1, c2);
   TM_HIP(hipGetLastError());
   return TM_OK;
 }
 
 // ---- flexible CG ----------------------------------------------------------
-extern "C" int64_t tm_fcg_work_size(int64_t n) {
-  return (int64_t)sizeof(double) * (5 * n + kFcgSlots * kMaxParts) + (int64_t)sizeof(FcgScalars) +
-         (int64_t)sizeof(FcgFlags) + 64;
+extern "C" int64_t tm_fcg_work_size(int64_t n, int m) {
+  if (!(m == 1 || m == 2 || m == 4 || m == 8)) return -1;
+  return (int64_t)sizeof(double) * ((3 + 2 * (int64_t)m) * n + kFcgSlots * kMaxParts) +
+         (int64_t)sizeof(FcgScalars) + (int64_t)sizeof(FcgFlags) + 64;
 }
 
-extern "C" int tm_fcg_layout(int64_t n, void* work, tm_fcg_view* view) {
+extern "C" int tm_fcg_layout(int64_t n, int m, void* work, tm_fcg_view* view) {
   TM_REQUIRE(n >= 0 && work && view, TM_ERR_ARG, "null argument");
-  FcgWork w = fcg_carve(work, n);
+  TM_REQUIRE(m == 1 || m == 2 || m == 4 || m == 8, TM_ERR_ARG, "fcg: m must be 1, 2, 4 or 8 (got %d)", m);
+  FcgWork w = fcg_carve(work, n, m);
   view->r = w.r;
   view->z = w.z;
-  view->ss_part = w.part + Q_SS * kMaxParts;
+  view->ss_part = w.part + S_SS * kMaxParts;
   view->scale = &w.s->s;
   view->relres = &w.s->relres;
   view->status = &w.f->status;
@@ -1318,31 +1419,57 @@ extern "C" int tm_fcg_layout(int64_t n, void* work, tm_fcg_view* view) {
   return TM_OK;
 }
 
-extern "C" int tm_fcg_start(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values,
-                            const double* b, const double* x, const float* d, void* work, double rtol, double atol,
-                            int64_t maxiter, void* stream) {
-  TM_REQUIRE(n > 0 && row_ptr && col_idx && values && b && x && d && work, TM_ERR_ARG, "null argument");
-  hipStream_t st = (hipStream_t)stream;
-  FcgWork w = fcg_carve(work, n);
+template <int M>
+int fcg_start_m(int64_t n, const int32_t* rp, const int32_t* ci, const float* v, const double* b, const double* x,
+                const float* d, FcgWork w, double rtol, double atol, int64_t maxiter, hipStream_t st) {
   const int G = fcg_grid(n);
   TM_HIP(hipMemsetAsync(w.f, 0, sizeof(FcgFlags), st));
-  hipLaunchKernelGGL((k_fcg_spmv<8, true>), dim3(G), dim3(NTF), 0, st, n, row_ptr, col_idx, values, x, w);
-  hipLaunchKernelGGL(k_fcg_init, dim3(G), dim3(NTF), 0, st, n, b, d, w);
+  hipLaunchKernelGGL((k_fcg_spmv<8, M, true>), dim3(G), dim3(NTF), 0, st, n, rp, ci, v, x, w);
+  hipLaunchKernelGGL(k_fcg_init<M>, dim3(G), dim3(NTF), 0, st, n, b, d, w);
   hipLaunchKernelGGL(k_fcg_ctl0, dim3(1), dim3(NT), 0, st, w, G, rtol * rtol, atol * atol, (long long)maxiter);
   TM_HIP(hipGetLastError());
   return TM_OK;
 }
 
-// One FCG iteration after the preconditioner wrote z = M(r) into the view.
-extern "C" int tm_fcg_iterate(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values,
-                              double* x, const float* d, void* work, void* stream) {
-  TM_REQUIRE(n > 0 && row_ptr && col_idx && values && x && d && work, TM_ERR_ARG, "null argument");
+extern "C" int tm_fcg_start(int64_t n, int m, const int32_t* row_ptr, const int32_t* col_idx, const float* values,
+                            const double* b, const double* x, const float* d, void* work, double rtol, double atol,
+                            int64_t maxiter, void* stream) {
+  TM_REQUIRE(n > 0 && row_ptr && col_idx && values && b && x && d && work, TM_ERR_ARG, "null argument");
   hipStream_t st = (hipStream_t)stream;
-  FcgWork w = fcg_carve(work, n);
+  FcgWork w = fcg_carve(work, n, m);
+  switch (m) {
+    case 1: return fcg_start_m<1>(n, row_ptr, col_idx, values, b, x, d, w, rtol, atol, maxiter, st);
+    case 2: return fcg_start_m<2>(n, row_ptr, col_idx, values, b, x, d, w, rtol, atol, maxiter, st);
+    case 4: return fcg_start_m<4>(n, row_ptr, col_idx, values, b, x, d, w, rtol, atol, maxiter, st);
+    case 8: return fcg_start_m<8>(n, row_ptr, col_idx, values, b, x, d, w, rtol, atol, maxiter, st);
+  }
+  tmx::set_error("fcg: m must be 1, 2, 4 or 8 (got %d)", m);
+  return TM_ERR_ARG;
+}
+
+template <int M>
+int fcg_iterate_m(int64_t n, const int32_t* rp, const int32_t* ci, const float* v, double* x, const float* d,
+                  FcgWork w, hipStream_t st) {
   const int G = fcg_grid(n);
-  hipLaunchKernelGGL((k_fcg_spmv<8, false>), dim3(G), dim3(NTF), 0, st, n, row_ptr, col_idx, values, x, w);
-  hipLaunchKernelGGL(k_fcg_update, dim3(G), dim3(NTF), 0, st, n, x, d, w, G);
+  hipLaunchKernelGGL((k_fcg_spmv<8, M, false>), dim3(G), dim3(NTF), 0, st, n, rp, ci, v, x, w);
+  hipLaunchKernelGGL(k_fcg_update<M>, dim3(G), dim3(NTF), 0, st, n, x, d, w, G);
   hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(NT), 0, st, w, G);
   TM_HIP(hipGetLastError());
   return TM_OK;
+}
+
+// One FCG iteration after the preconditioner wrote z = M(r) into the view.
+extern "C" int tm_fcg_iterate(int64_t n, int m, const int32_t* row_ptr, const int32_t* col_idx, const float* values,
+                              double* x, const float* d, void* work, void* stream) {
+  TM_REQUIRE(n > 0 && row_ptr && col_idx && values && x && d && work, TM_ERR_ARG, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  FcgWork w = fcg_carve(work, n, m);
+  switch (m) {
+    case 1: return fcg_iterate_m<1>(n, row_ptr, col_idx, values, x, d, w, st);
+    case 2: return fcg_iterate_m<2>(n, row_ptr, col_idx, values, x, d, w, st);
+    case 4: return fcg_iterate_m<4>(n, row_ptr, col_idx, values, x, d, w, st);
+    case 8: return fcg_iterate_m<8>(n, row_ptr, col_idx, values, x, d, w, st);
+  }
+  tmx::set_error("fcg: m must be 1, 2, 4 or 8 (got %d)", m);
+  return TM_ERR_ARG;
 }

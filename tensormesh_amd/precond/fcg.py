@@ -33,17 +33,20 @@ from .network import GNNPreconditioner
 
 class LearnedFCG:
     def __init__(self, net: GNNPreconditioner, hier: Hierarchy, csr: CSR, chunk: int = 8,
-                 use_graph: bool = True, fused: bool = True):
+                 use_graph: bool = True, fused: bool = True, m: int = 4):
         self.net, self.hier, self.csr = net, hier, csr
         self.dev = hier.device
         self.n = csr.n_rows
         assert hier.levels[0].n == self.n, "hierarchy level 0 must be the matrix graph"
         self.prog = net.plan(hier)
         L = _lib.lib()
-        ws = int(L.tm_fcg_work_size(self.n))
+        self.m = int(m)
+        ws = int(L.tm_fcg_work_size(self.n, self.m))
+        if ws < 0:
+            raise ValueError(f"flexible CG keeps 1, 2, 4 or 8 directions, got m={m}")
         self.work = torch.zeros(ws // 8 + 8, dtype=torch.float64, device=self.dev)
         self.view = _lib.FcgView()
-        check(L.tm_fcg_layout(self.n, ptr(self.work), ctypes.byref(self.view)))
+        check(L.tm_fcg_layout(self.n, self.m, ptr(self.work), ctypes.byref(self.view)))
         base = self.work.data_ptr()
         off = lambda a: (a - base)
         self._status = self.work.view(torch.int32)[off(self.view.status) // 4: off(self.view.status) // 4 + 2]
@@ -70,7 +73,7 @@ class LearnedFCG:
         st = _lib.stream_of(self.dev)
         self.prog.forward(v.r, self.lk, ss_part=v.ss_part, nparts=v.nparts, z=v.z, scale=v.scale, stop=v.status,
                           fused=self.fused)
-        check(L.tm_fcg_iterate(self.n, ptr(self.csr.row_ptr), ptr(self.csr.col_idx), ptr(self.values),
+        check(L.tm_fcg_iterate(self.n, self.m, ptr(self.csr.row_ptr), ptr(self.csr.col_idx), ptr(self.values),
                                ptr(self.x), ptr(self.hier.levels[0].d), ptr(self.work), st))
 
     def _capture(self):
@@ -98,8 +101,8 @@ class LearnedFCG:
             self.x.zero_()
         else:
             self.x.copy_(x0.detach().to(torch.float64))
-        check(L.tm_fcg_start(self.n, ptr(self.csr.row_ptr), ptr(self.csr.col_idx), ptr(self.values), ptr(self.b),
-                             ptr(self.x), ptr(self.hier.levels[0].d), ptr(self.work), float(rtol), 0.0,
+        check(L.tm_fcg_start(self.n, self.m, ptr(self.csr.row_ptr), ptr(self.csr.col_idx), ptr(self.values),
+                             ptr(self.b), ptr(self.x), ptr(self.hier.levels[0].d), ptr(self.work), float(rtol), 0.0,
                              int(maxiter), st))
         if self.use_graph and self.graph is None:
             self._capture()

@@ -55,16 +55,24 @@ def _addr(x) -> ctypes.c_void_p:
 class GNNPreconditioner:
     """Weights and kernels of the network (no torch autograd involved)."""
 
-    def __init__(self, schedule: Sequence[int] = (1, 2, 3, 2, 1, 0), n_linear: int = 32,
-                 device: torch.device = torch.device("cuda:0"), seed: int = 0):
+    def __init__(self, schedule: Sequence[int] = (1, 2, 3, 3, 2, 1), n_linear: int = 64,
+                 device: torch.device = torch.device("cuda:0"), seed: int = 0, homogeneous: bool = True,
+                 symmetric: bool = True):
         self.schedule = tuple(int(s) for s in schedule)
         self.n_linear = int(n_linear)
+        self.symmetric = bool(symmetric)
+        self.homogeneous = bool(homogeneous) or self.symmetric
+        if self.symmetric:
+            L = len(self.schedule)
+            assert L % 2 == 0 and self.schedule == self.schedule[::-1] and self.n_linear == F, \
+                "a symmetric network needs a palindromic schedule of even length and identity activations"
         self.device = torch.device(device)
         nl = len(self.schedule)
         self.sizes = [3 * F] + [2 * F * F + F] * nl + [F + 1]
         self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
         self.n_params = int(self.offsets[-1])
         self.params = torch.zeros(self.n_params, dtype=torch.float32, device=self.device)
+        self.mask = self._mask()
         self.init_weights(seed)
         self._adam = None
         self._step = 0
@@ -85,6 +93,37 @@ class GNNPreconditioner:
     def dec(self):
         v = self._view(1 + len(self.schedule))
         return v[:F], v[F:]
+
+    @staticmethod
+    def block_t(W: torch.Tensor) -> torch.Tensor:
+        """Weights of the adjoint layer: [W_h; W_n] -> [W_h^T; W_n^T]
+        (the layer h -> h + h W_h + (A^ h) W_n with A^ symmetric)."""
+        return torch.cat([W[:F].t(), W[F:].t()], 0)
+
+    def mirror(self, t: int) -> int:
+        """Free layer whose adjoint layer ``t`` is (``t`` itself if it is free)."""
+        L = len(self.schedule)
+        return L - 1 - t if (self.symmetric and t >= L // 2) else t
+
+    def _mask(self) -> torch.Tensor:
+        """1 for trainable entries.  A homogeneous network keeps every bias and
+        the log-coefficient input weights at zero: then M(a r) = a M(r), and with
+        identity activations M is linear in r for a fixed matrix (the
+        coefficient still enters through the normalised matrix entries), which
+        is what flexible CG converges best with."""
+        m = torch.ones(self.n_params, dtype=torch.float32)
+        if self.homogeneous:
+            o = self.offsets
+            m[int(o[0]) + F:int(o[1])] = 0.0                      # We1 (log kappa) and be
+            for t in range(len(self.schedule)):
+                m[int(o[1 + t]) + 2 * F * F:int(o[2 + t])] = 0.0   # layer biases
+            m[int(o[-1]) - 1] = 0.0                                # decoder bias
+        if self.symmetric:
+            L = len(self.schedule)
+            for t in range(L // 2, L):                             # adjoint layers: derived
+                m[int(o[1 + t]):int(o[2 + t])] = 0.0
+            m[int(o[-2]):int(o[-1])] = 0.0                         # decoder = encoder^T
+        return m.to(self.device)
 
     def init_weights(self, seed: int = 0, noise: float = 0.05):
         """Additive multilevel start.  Channel 0 carries the scaled residual
@@ -108,11 +147,27 @@ class GNNPreconditioner:
             p[a:a + 2 * F * F] = W.ravel()
         a = int(o[1 + len(self.schedule)])
         p[a] = 1.0
-        self.params.copy_(torch.from_numpy(p))
+        self.params.copy_(torch.from_numpy(p).to(self.device) * self.mask)
         self.refresh()
 
     def refresh(self):
-        """BF16 copies for the kernels (W^T for the forward, W for the backward)."""
+        """Derived weights (adjoint layers, tied decoder) and the BF16 copies for
+        the kernels (W^T for the forward, W for the backward).
+
+        A symmetric network applies M = sum_l R_l^T R_l with R_0 = E (encoder)
+        and R_l = L_l P_{l-1}^T R_{l-1}: the second half of the schedule runs
+        the adjoints of the first half's layers, pooling and unpooling are
+        adjoint, and the decoder is the encoder's transpose, so M is symmetric
+        positive definite whatever the weights."""
+        if self.symmetric:
+            with torch.no_grad():
+                for t in range(len(self.schedule) // 2, len(self.schedule)):
+                    W, b = self.layer(t)
+                    W.copy_(self.block_t(self.layer(self.mirror(t))[0]))
+                    b.zero_()
+                wd, bd = self.dec()
+                wd.copy_(self.enc()[0][:F])
+                bd.zero_()
         self.wt, self.w = [], []
         for t in range(len(self.schedule)):
             W, _ = self.layer(t)
@@ -120,9 +175,21 @@ class GNNPreconditioner:
             self.w.append(wb.contiguous())
             self.wt.append(wb.t().contiguous())
 
+    def fold(self, grad: torch.Tensor):
+        """Move the gradient of derived weights onto the weights they derive from."""
+        if not self.symmetric:
+            return
+        o = self.offsets
+        view = lambda k: grad[int(o[k]):int(o[k + 1])]
+        for t in range(len(self.schedule) // 2, len(self.schedule)):
+            gt = view(1 + t)[:2 * F * F].view(2 * F, F)
+            view(1 + self.mirror(t))[:2 * F * F].view(2 * F, F).add_(self.block_t(gt))
+        view(0)[:F].add_(view(1 + len(self.schedule))[:F])
+
     def state_dict(self) -> Dict[str, np.ndarray]:
         return {"params": self.params.detach().cpu().numpy(), "schedule": np.array(self.schedule),
-                "n_linear": np.array(self.n_linear)}
+                "n_linear": np.array(self.n_linear), "homogeneous": np.array(int(self.homogeneous)),
+                "symmetric": np.array(int(self.symmetric))}
 
     def save(self, path: str):
         np.savez(path, **self.state_dict())
@@ -130,7 +197,9 @@ class GNNPreconditioner:
     @classmethod
     def load(cls, path: str, device: torch.device = torch.device("cuda:0")) -> "GNNPreconditioner":
         z = np.load(path, allow_pickle=False)
-        net = cls(tuple(int(s) for s in z["schedule"]), int(z["n_linear"]), device)
+        hom = bool(int(z["homogeneous"])) if "homogeneous" in z.files else False
+        sym = bool(int(z["symmetric"])) if "symmetric" in z.files else False
+        net = cls(tuple(int(s) for s in z["schedule"]), int(z["n_linear"]), device, homogeneous=hom, symmetric=sym)
         net.params.copy_(torch.from_numpy(z["params"]).to(torch.float32))
         net.refresh()
         return net
@@ -193,12 +262,17 @@ class Program:
                         and ops[-1][0] == "mp" and ops[-1][1] == 0 and ops[-1][2] == ops[-2][3]
                         and ops[-1][3] == self.final and uses_enc == 2
                         and getattr(hier.levels[0], "b_row_ptr", None) is not None)
+        # finest level with no layer: encode+pool ... unpool+decode
+        self.fusable_plain = (len(ops) >= 2 and ops[0][0] == "pool" and ops[0][1] == 0
+                              and ops[0][2] == self.enc_out and ops[-1][0] == "unpool" and ops[-1][1] == 0
+                              and ops[-1][4] == self.enc_out and ops[-1][3] == self.final and uses_enc == 2)
         self.saves = {}
         self.y = torch.zeros(self.n0, dtype=torch.float32, device=dev)
         self.z = torch.zeros(self.n0, dtype=torch.float64, device=dev)
         self.scale = torch.zeros(1, dtype=torch.float64, device=dev)
         self.ss_part = torch.zeros(1024, dtype=torch.float64, device=dev)
-        self.xs = torch.zeros((self.n0, 2), dtype=torch.float32, device=dev) if self.fusable else None
+        self.xs = (torch.zeros((self.n0, 2), dtype=torch.float32, device=dev)
+                   if (self.fusable or self.fusable_plain) else None)
 
     def alloc_training(self):
         dev = self.hier.device
@@ -227,6 +301,18 @@ class Program:
         scale_p = ptr(self.scale) if scale is None else _addr(scale)
         we, be = net.enc()
         zt = self.z if z is None else z
+        if fused and self.fusable_plain and not train and not want_y:
+            first, last = self.ops[0], self.ops[-1]
+            lv1 = hier.levels[1]
+            check(L.tm_gnn_encode_pool(self.n0, lv1.n, ptr(lv1.child_ptr), ptr(lv1.child), ptr(lv0.w), _addr(r),
+                                       ptr(lv0.d), ptr(lk), ss_ptr, nparts, ptr(we), ptr(be),
+                                       ptr(self.bufs[first[3]]), ptr(self.xs), scale_p, stop_p, st))
+            self._run_ops(self.ops[1:-1], 1, train, stop_p, st)
+            wd, bd = net.dec()
+            check(L.tm_gnn_unpool_decode(self.n0, ptr(self.xs), ptr(lv0.agg), ptr(lv0.w), ptr(self.bufs[last[2]]),
+                                         ptr(we), ptr(be), ptr(wd), ptr(bd), ptr(lv0.d), scale_p, _addr(zt), stop_p,
+                                         st))
+            return zt
         if fused and self.fusable and not train and not want_y:
             first, unp, last = self.ops[0], self.ops[-2], self.ops[-1]
             lv1 = hier.levels[1]
@@ -356,7 +442,8 @@ class Adam:
     def step(self, grad: torch.Tensor, lr: Optional[float] = None):
         self.step_count += 1
         L = _lib.lib()
+        self.net.fold(grad)
         check(L.tm_gnn_adam(self.net.n_params, ptr(self.net.params), ptr(grad), ptr(self.m), ptr(self.v),
-                            float(self.lr if lr is None else lr), self.b1, self.b2, self.eps, self.step_count,
-                            _lib.stream_of(self.net.device)))
+                            ptr(self.net.mask), float(self.lr if lr is None else lr), self.b1, self.b2, self.eps,
+                            self.step_count, _lib.stream_of(self.net.device)))
         self.net.refresh()

@@ -74,11 +74,16 @@ def make_samples(prob: PoissonCube, seeds: Sequence[int], snapshots: Sequence[in
 
 
 def train(prob: PoissonCube, net: GNNPreconditioner, data: Dict, steps: int, lr: float = 2e-3,
-          seed: int = 0, log_every: int = 100, time_limit: float = 1e30) -> List[float]:
+          seed: int = 0, log_every: int = 100, time_limit: float = 1e30, noise_residuals: int = 1) -> List[float]:
+    """``noise_residuals`` extra residuals per step are ``r = A e`` for white
+    noise ``e`` (their exact error is ``e``), so the loss covers the rough end
+    of the spectrum that plain-solver residuals leave out."""
     prog = net.plan(prob.hier)
     opt = Adam(net, lr=lr)
     grad = torch.zeros_like(net.params)
     rng = np.random.default_rng(seed)
+    gen = torch.Generator(device=prob.device)
+    gen.manual_seed(seed)
     hist, window = [], []
     n = prob.n_inner
     d0 = prob.hier.levels[0]
@@ -91,9 +96,12 @@ def train(prob: PoissonCube, net: GNNPreconditioner, data: Dict, steps: int, lr:
         lk = prob.log_kappa_feature(kappa)
         grad.zero_()
         sums = []
-        for k in range(data["r"][j].shape[0]):
-            r = data["r"][j][k].to(torch.float64)
-            e = data["e"][j][k].to(torch.float64)
+        pairs = [(data["r"][j][k].to(torch.float64), data["e"][j][k].to(torch.float64))
+                 for k in range(data["r"][j].shape[0])]
+        for _ in range(noise_residuals):
+            e = torch.randn(n, generator=gen, dtype=torch.float64, device=prob.device)
+            pairs.append((_spmv_raw(prob.csr, values, e), e))
+        for r, e in pairs:
             # normaliser e~.r~ = e.r / s^2, s^2 = mean (d r)^2; scale g_y by 2 n / (e~.r~)
             s2 = torch.mean((d0.d.to(torch.float64) * r) ** 2)
             er = torch.dot(e, r) / s2
@@ -115,9 +123,9 @@ def train(prob: PoissonCube, net: GNNPreconditioner, data: Dict, steps: int, lr:
 
 
 def evaluate(prob: PoissonCube, net: GNNPreconditioner, seeds: Sequence[int], chunk: int = 8,
-             rtol: float = 1e-8) -> Dict:
-    """Time to ``rtol`` of Jacobi-PCG and of the learned FCG per held-out seed."""
-    solver = LearnedFCG(net, prob.hier, prob.csr, chunk=chunk)
+             rtol: float = 1e-8, m: int = 4) -> Dict:
+    """Time to ``rtol`` of Jacobi-PCG and of the learned FCG(m) per held-out seed."""
+    solver = LearnedFCG(net, prob.hier, prob.csr, chunk=chunk, m=m)
     dev = prob.device
     rows = []
     for sd in seeds:
@@ -149,10 +157,16 @@ def main(argv=None):
     ap.add_argument("--test", type=int, default=200, help="held-out seeds (from 5000)")
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--lr", type=float, default=2e-3)
-    ap.add_argument("--schedule", type=str, default="0,1,2,3,1,0")
-    ap.add_argument("--n-linear", type=int, default=32)
+    ap.add_argument("--schedule", type=str, default="1,2,3,3,2,1")
+    ap.add_argument("--nonsymmetric", action="store_true",
+                    help="independent up-path layers and decoder (the operator is then not SPD)")
+    ap.add_argument("--n-linear", type=int, default=64)
+    ap.add_argument("--general", action="store_true",
+                    help="train biases and the log-coefficient input too (not homogeneous in r)")
+    ap.add_argument("--noise-residuals", type=int, default=1)
     ap.add_argument("--levels", type=int, default=0, help="hierarchy levels (default: what the schedule needs)")
     ap.add_argument("--chunk", type=int, default=8)
+    ap.add_argument("--fcg-m", type=str, default="4", help="directions kept by flexible CG (comma list to compare)")
     ap.add_argument("--init", type=str, default="")
     ap.add_argument("--out", type=str, default="")
     ap.add_argument("--report", type=str, default="")
@@ -165,21 +179,25 @@ def main(argv=None):
     log(f"{args.n}^3: {prob.n_inner} unknowns, levels {[l.n for l in prob.hier.levels]} "
         f"({time.perf_counter() - t0:.1f}s)")
     net = (GNNPreconditioner.load(args.init, dev) if args.init
-           else GNNPreconditioner(schedule, args.n_linear, dev))
-    report = {"n": args.n, "schedule": list(net.schedule), "n_linear": net.n_linear}
+           else GNNPreconditioner(schedule, args.n_linear, dev, homogeneous=not args.general,
+                                  symmetric=not (args.nonsymmetric or args.general)))
+    report = {"n": args.n, "schedule": list(net.schedule), "n_linear": net.n_linear,
+              "homogeneous": net.homogeneous, "symmetric": net.symmetric}
     if args.train > 0 and args.steps > 0:
         data = make_samples(prob, list(range(1000, 1000 + args.train)))
         report["train_median_jacobi_iters"] = float(np.median(data["jacobi_iters"]))
-        report["loss_history"] = train(prob, net, data, args.steps, lr=args.lr, time_limit=args.time_limit)
+        report["loss_history"] = train(prob, net, data, args.steps, lr=args.lr, time_limit=args.time_limit,
+                                       noise_residuals=args.noise_residuals)
         del data
         if args.out:
             net.save(args.out)
             log(f"saved {args.out}")
     if args.test > 0:
-        ev = evaluate(prob, net, list(range(5000, 5000 + args.test)), chunk=args.chunk)
-        report["eval"] = ev
-        log(f"held-out: median speedup {ev['median_speedup']:.3f}, Jacobi its {ev['median_jacobi_iters']}, "
-            f"FCG its {ev['median_fcg_iters']}, fallbacks {ev['fallbacks']}")
+        for m in [int(v) for v in args.fcg_m.split(",")]:
+            ev = evaluate(prob, net, list(range(5000, 5000 + args.test)), chunk=args.chunk, m=m)
+            report[f"eval_m{m}"] = ev
+            log(f"held-out FCG({m}): median speedup {ev['median_speedup']:.3f}, Jacobi its "
+                f"{ev['median_jacobi_iters']}, FCG its {ev['median_fcg_iters']}, fallbacks {ev['fallbacks']}")
     if args.report:
         with open(args.report, "w") as f:
             json.dump(report, f, indent=1)

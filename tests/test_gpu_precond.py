@@ -40,8 +40,10 @@ def _problem(n, seed, cuda, sigma=1.0):
     return Ai, H, hier, levels, values, csr, lk
 
 
-def _perturbed(cuda, seed=7, scale=0.05):
-    net = GNNPreconditioner(device=cuda)
+def _perturbed(cuda, seed=7, scale=0.05, schedule=(1, 2, 3, 3, 2, 1)):
+    """General (non-homogeneous, non-symmetric) network: ReLU channels, biases
+    and the log-coefficient input all active."""
+    net = GNNPreconditioner(schedule, device=cuda, n_linear=32, homogeneous=False, symmetric=False)
     g = torch.Generator(device="cpu").manual_seed(seed)
     net.params.add_((torch.randn(net.n_params, generator=g) * scale).to(cuda))
     net.refresh()
@@ -103,26 +105,26 @@ def test_network_forward(cuda):
     prog = net.plan(hier)
     r = np.random.default_rng(4).standard_normal(Ai.shape[0])
     z = prog.forward(torch.tensor(r, device=cuda), torch.tensor(lk, device=cuda)).cpu().numpy()
-    z_ref = og.forward(levels, _params64(net), net.schedule, 32, r, lk.astype(np.float64), bf16=True)[0]
+    z_ref = og.forward(levels, _params64(net), net.schedule, net.n_linear, r, lk.astype(np.float64), bf16=True)[0]
     assert np.linalg.norm(z - z_ref) <= 2e-2 * np.linalg.norm(z_ref)
 
 
-@pytest.mark.parametrize("n", [8, 13])
-def test_fused_inference_matches_unfused(cuda, n):
+@pytest.mark.parametrize("n,schedule", [(8, (1, 2, 3, 2, 1, 0)), (13, (1, 2, 3, 2, 1, 0)), (9, (1, 2, 3, 3, 2, 1))])
+def test_fused_inference_matches_unfused(cuda, n, schedule):
     """Fused finest level (encode+pool, unpool+layer+decode with B = A^ P^) =
     unfused kernels, up to the BF16 rounding of the level-0 features that only
     the unfused path stores."""
     print(f"case precond fused-vs-unfused n={n}", flush=True)
     Ai, H, hier, levels, values, csr, lk = _problem(n, 10 + n, cuda)
-    net = _perturbed(cuda, seed=n)
+    net = _perturbed(cuda, seed=n, schedule=schedule)
     prog = net.plan(hier)
-    assert prog.fusable
+    assert prog.fusable or prog.fusable_plain
     r = torch.tensor(np.random.default_rng(n).standard_normal(Ai.shape[0]), device=cuda)
     lkt = torch.tensor(lk, device=cuda)
     zf = prog.forward(r, lkt, fused=True).clone()
     zu = prog.forward(r, lkt, fused=False).clone()
     assert torch.linalg.vector_norm(zf - zu) <= 1e-2 * torch.linalg.vector_norm(zu)
-    z_ref = og.forward(levels, _params64(net), net.schedule, 32, r.cpu().numpy(), lk.astype(np.float64), bf16=True)[0]
+    z_ref = og.forward(levels, _params64(net), net.schedule, net.n_linear, r.cpu().numpy(), lk.astype(np.float64), bf16=True)[0]
     assert np.linalg.norm(zf.cpu().numpy() - z_ref) <= 2e-2 * np.linalg.norm(z_ref)
 
 
@@ -137,8 +139,8 @@ def test_network_gradient(cuda):
     grad = torch.zeros_like(net.params)
     prog.loss_and_grad(torch.tensor(r, device=cuda), torch.tensor(lk, device=cuda), grad)
     p = _params64(net)
-    z, y, s, tape = og.forward(levels, p, net.schedule, 32, r, lk.astype(np.float64), bf16=True)
-    g_ref = og.backward(levels, p, net.schedule, 32, r, lk.astype(np.float64), tape, s, bf16=True)
+    z, y, s, tape = og.forward(levels, p, net.schedule, net.n_linear, r, lk.astype(np.float64), bf16=True)
+    g_ref = og.backward(levels, p, net.schedule, net.n_linear, r, lk.astype(np.float64), tape, s, bf16=True)
     g = grad.double().cpu().numpy()
     for k in range(len(net.offsets) - 1):
         a, b = int(net.offsets[k]), int(net.offsets[k + 1])
@@ -165,9 +167,28 @@ def test_fcg_untrained_additive_multilevel(cuda, graph):
     _, itj, _, _ = pcg_raw(csr, values, b, rtol=1e-8)
     assert info["fcg_iterations"] < itj, (info["fcg_iterations"], itj)
     p = _params64(net)
-    pre = lambda r: og.forward(levels, p, net.schedule, 32, r, lk.astype(np.float64), bf16=True)[0]
-    _, it_ref, _ = og.fcg(Ai, np.ones(len(xr)), pre, rtol=1e-8)
+    pre = lambda r: og.forward(levels, p, net.schedule, net.n_linear, r, lk.astype(np.float64), bf16=True)[0]
+    _, it_ref, _ = og.fcg(Ai, np.ones(len(xr)), pre, rtol=1e-8, m=solver.m)
     assert abs(info["fcg_iterations"] - it_ref) <= max(2, 0.1 * it_ref), (info["fcg_iterations"], it_ref)
+
+
+@pytest.mark.parametrize("m", [1, 2, 8])
+def test_fcg_ring_sizes_vs_oracle(cuda, m):
+    """FCG(m) kernels (direction ring, single reduction phase) against the
+    oracle's explicitly orthogonalising FCG(m) with the same network."""
+    print(f"case precond fcg m={m} n=12", flush=True)
+    Ai, H, hier, levels, values, csr, lk = _problem(12, 20 + m, cuda)
+    net = _perturbed(cuda, seed=m, scale=0.02)
+    solver = LearnedFCG(net, hier, csr, chunk=2, use_graph=False, m=m)
+    solver.set_system(values, torch.tensor(lk, device=cuda))
+    b = torch.ones(Ai.shape[0], dtype=torch.float64, device=cuda)
+    x, info = solver.solve(b, rtol=1e-8, maxiter=400)
+    assert info["converged"], info
+    p = _params64(net)
+    pre = lambda r: og.forward(levels, p, net.schedule, net.n_linear, r, lk.astype(np.float64), bf16=True)[0]
+    _, it_ref, _ = og.fcg(Ai, np.ones(Ai.shape[0]), pre, rtol=1e-8, m=m)
+    assert abs(info["fcg_iterations"] + info["fallback_iterations"] - it_ref) <= max(3, 0.15 * it_ref), \
+        (info, it_ref)
 
 
 def test_fcg_graph_replay_matches_eager(cuda):

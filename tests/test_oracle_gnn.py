@@ -71,7 +71,8 @@ def test_oracle_backward_matches_finite_differences():
     coords, Ai = _poisson(6, seed=4)
     H = build_host_levels(coords, Ai.indptr, Ai.indices, 4)
     levels = og.galerkin_levels(Ai, [h.agg for h in H[:-1]])
-    net = GNNPreconditioner(device=torch.device("cpu"))
+    net = GNNPreconditioner((0, 1, 2, 3, 1, 0), n_linear=32, device=torch.device("cpu"), homogeneous=False,
+                            symmetric=False)
     rng = np.random.default_rng(1)
     p = net.params.numpy().astype(np.float64) + rng.standard_normal(net.n_params) * 0.05
     r = rng.standard_normal(Ai.shape[0])
@@ -96,18 +97,72 @@ def test_untrained_network_is_additive_multilevel():
     H = build_host_levels(coords, Ai.indptr, Ai.indices, 4)
     levels = og.galerkin_levels(Ai, [h.agg for h in H[:-1]])
     net = GNNPreconditioner(device=torch.device("cpu"))
+    net.init_weights(0, noise=0.0)
     r = np.random.default_rng(0).standard_normal(Ai.shape[0])
-    z, y, s, _ = og.forward(levels, net.params.numpy(), net.schedule, 32, r, np.zeros_like(r))
+    z, y, s, _ = og.forward(levels, net.params.numpy(), net.schedule, net.n_linear, r, np.zeros_like(r))
     x0 = levels[0]["d"] * r / s
     Ph = [sp.diags(levels[k]["w"]) @ levels[k]["P"] for k in range(3)]
     I = lambda m: sp.eye(m.shape[1])
     M = sp.eye(len(r)) + Ph[0] @ (I(Ph[0]) + Ph[1] @ (I(Ph[1]) + Ph[2] @ Ph[2].T) @ Ph[1].T) @ Ph[0].T
-    assert np.allclose(y, M @ x0, rtol=1e-12, atol=1e-12)
+    # every channel carries the same multilevel sum; the decoder weighs them by we . wd
+    scale = float(net.enc()[0][:64].double() @ net.dec()[0].double())
+    assert np.allclose(y, scale * (M @ x0), rtol=1e-10, atol=1e-12)
     # symmetric positive definite operator r -> z
-    Z = np.stack([og.forward(levels, net.params.numpy(), net.schedule, 32, e, np.zeros_like(r))[0]
-                  / og.forward(levels, net.params.numpy(), net.schedule, 32, e, np.zeros_like(r))[2]
+    Z = np.stack([og.forward(levels, net.params.numpy(), net.schedule, net.n_linear, e, np.zeros_like(r))[0]
+                  / og.forward(levels, net.params.numpy(), net.schedule, net.n_linear, e, np.zeros_like(r))[2]
                   for e in np.eye(len(r))[:40]])
     assert np.allclose(Z[:, :40], Z[:, :40].T, atol=1e-12)
+
+
+def _operator(levels, net, n):
+    """Dense matrix of r -> z (the network is homogeneous and linear in r)."""
+    cols = []
+    for e in np.eye(n):
+        z, _, s, _ = og.forward(levels, net.params.double().numpy(), net.schedule, net.n_linear, e, np.zeros(n))
+        cols.append(z)
+    return np.stack(cols, 1)
+
+
+def test_symmetric_network_is_spd_for_any_weights():
+    """Tied adjoint layers + tied decoder: M = sum R_l^T R_l is symmetric
+    positive definite for arbitrary (random) weights, and the gradient folding
+    keeps the tie under training."""
+    coords, Ai = _poisson(6, seed=6)
+    H = build_host_levels(coords, Ai.indptr, Ai.indices, 4)
+    levels = og.galerkin_levels(Ai, [h.agg for h in H[:-1]])
+    net = GNNPreconditioner(device=torch.device("cpu"))
+    g = torch.Generator().manual_seed(3)
+    net.params.add_(torch.randn(net.n_params, generator=g) * 0.3 * net.mask)
+    net.refresh()
+    M = _operator(levels, net, Ai.shape[0])
+    assert np.abs(M - M.T).max() <= 1e-10 * np.abs(M).max()
+    assert np.linalg.eigvalsh(0.5 * (M + M.T)).min() > 0
+    # a non-symmetric network with the same schedule is not symmetric
+    ns = GNNPreconditioner(device=torch.device("cpu"), symmetric=False)
+    ns.params.add_(torch.randn(ns.n_params, generator=g) * 0.3 * ns.mask)
+    ns.refresh()
+    Mn = _operator(levels, ns, Ai.shape[0])
+    assert np.abs(Mn - Mn.T).max() > 1e-3 * np.abs(Mn).max()
+    # folding: the oracle gradient of the tied parameters equals the sum over their uses
+    r = np.random.default_rng(2).standard_normal(Ai.shape[0])
+    p = net.params.double().numpy()
+    z, y, s, tape = og.forward(levels, p, net.schedule, net.n_linear, r, np.zeros_like(r))
+    gfull = torch.tensor(og.backward(levels, p, net.schedule, net.n_linear, r, np.zeros_like(r), tape, s))
+    net.fold(gfull)
+    i = int(net.offsets[1]) + 5 * 64 + 7          # a free weight of layer 0
+    h = 1e-6
+    loss = []
+    for sgn in (1, -1):
+        q = net.params.double().clone()
+        q[i] += sgn * h
+        t = GNNPreconditioner(device=torch.device("cpu"))
+        t.params.copy_(q.float())
+        t.params.data = q                                        # keep float64 for the difference
+        t.refresh()
+        pp = t.params.numpy()
+        loss.append(og.loss(levels, og.forward(levels, pp, t.schedule, t.n_linear, r, np.zeros_like(r))[1], r, s))
+    fd = (loss[0] - loss[1]) / (2 * h)
+    assert abs(float(gfull[i]) - fd) <= 1e-5 * max(abs(fd), 1e-3), (float(gfull[i]), fd)
 
 
 def test_fcg_reduces_to_pcg_and_direct():
