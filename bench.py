@@ -63,6 +63,74 @@ def cpu_baseline(n_cpu: int, repeats: int = 5):
             "seconds_per_assembly": round(t, 4), "cpu": model, "sockets": sockets}
 
 
+def learned_leg(n: int, seeds, log_fn):
+    """Learned-preconditioner FCG vs Jacobi-PCG to 1e-8 on held-out log-normal
+    Poisson problems (n^3 Kuhn grid, GRF length 0.05-0.2, sigma 0.5-2.3)."""
+    import numpy as np
+    import torch
+    from tensormesh_amd.precond import GNNPreconditioner, LearnedFCG
+    from tensormesh_amd.precond.problem import PoissonCube, field_params
+    from tensormesh_amd.precond.train import plain_solve
+    dev = torch.device("cuda:0")
+    wpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tensormesh_amd", "precond", "weights",
+                         f"poisson{n}_sym.npz")
+    if not os.path.exists(wpath):
+        return {"skipped": f"no trained weights at {os.path.relpath(wpath)}"}
+    t0 = time.perf_counter()
+    prob = PoissonCube(n, dev, n_levels=4)
+    net = GNNPreconditioner.load(wpath, dev)
+    solver = LearnedFCG(net, prob.hier, prob.csr, chunk=8, m=1)
+    log_fn(f"learned: {n}^3 set-up {time.perf_counter() - t0:.1f}s, levels {[l.n for l in prob.hier.levels]}")
+    # warm-up (kernel loading, graph capture) on a training-range seed
+    kappa = prob.kappa(999)
+    values, b = prob.system(kappa)
+    solver.set_system(values, prob.log_kappa_feature(kappa))
+    solver.solve(b, rtol=1e-8)
+    plain_solve(prob, values, maxiter=50)
+    rows = []
+    for sd in seeds:
+        kappa = prob.kappa(sd)
+        values, b = prob.system(kappa)
+        torch.cuda.synchronize(dev)
+        t1 = time.perf_counter()
+        _, itj, _, okj = plain_solve(prob, values)
+        torch.cuda.synchronize(dev)
+        tj = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        solver.set_system(values, prob.log_kappa_feature(kappa))
+        _, info = solver.solve(b, rtol=1e-8)
+        tl = time.perf_counter() - t1
+        ell, sig = field_params(sd)
+        rows.append({"seed": sd, "length": round(ell, 3), "sigma": round(sig, 3), "jacobi_iters": itj,
+                     "jacobi_s": round(tj, 5), "fcg_iters": info["fcg_iterations"],
+                     "fallback_iters": info["fallback_iterations"], "fcg_s": round(tl, 5),
+                     "relres": info["relres"], "speedup": round(tj / tl, 3)})
+    # one network application, HIP events on the launch stream
+    st = torch.cuda.current_stream(dev)
+    r = torch.randn(prob.n_inner, dtype=torch.float64, device=dev)
+    lk = prob.log_kappa_feature(kappa)
+    prog = solver.prog
+    prog.forward(r, lk)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(st)
+    for _ in range(20):
+        prog.forward(r, lk)
+    ev1.record(st)
+    torch.cuda.synchronize(dev)
+    t_app = ev0.elapsed_time(ev1) / 1e3 / 20
+    flops = sum(2 * prob.hier.levels[lev].n * 128 * 64 for lev in net.schedule)
+    sp = np.array([r_["speedup"] for r_ in rows])
+    return {"workload": f"{n}^3 Kuhn log-normal Poisson, held-out seeds {seeds[0]}-{seeds[-1]}",
+            "network": {"schedule": list(net.schedule), "symmetric": net.symmetric, "width": 64,
+                        "apply_ms": round(t_app * 1e3, 4), "gemm_gflop_per_apply": round(flops / 1e9, 3),
+                        "mfma_tflops": round(flops / t_app / 1e12, 2),
+                        "mfma_util_vs_2500_tflops_bf16": round(flops / t_app / 2.5e15, 5)},
+            "median_speedup_vs_jacobi": float(np.median(sp)),
+            "median_jacobi_iters": float(np.median([r_["jacobi_iters"] for r_ in rows])),
+            "median_fcg_iters": float(np.median([r_["fcg_iters"] for r_ in rows])),
+            "fallbacks": int(sum(r_["fallback_iters"] > 0 for r_ in rows)), "rows": rows}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +140,9 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=48)
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--learned-n", type=int, default=96)
+    ap.add_argument("--learned-seeds", type=int, default=8)
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="MiB of a plain device copy run before the timed region (HBM counter calibration)")
     args = ap.parse_args()
@@ -189,6 +260,14 @@ def main():
                         "pcg_iter_GBps": round(b_iter / (t_solve / max(it, 1)) / 1e9, 1),
                         "spmv_us": round(t_spmv * 1e6, 2), "spmv_GBps": round(b_spmv / t_spmv / 1e9, 1)}
         log(f"solve: {it} its, {t_solve:.3f}s, true relres {true_rel:.2e}, spmv {t_spmv * 1e6:.1f} us")
+
+    if not args.no_learned:
+        seeds = list(range(5000, 5000 + args.learned_seeds))
+        out["learned_fpcg"] = learned_leg(args.learned_n, seeds, log)
+        lf = out["learned_fpcg"]
+        if "median_speedup_vs_jacobi" in lf:
+            log(f"learned FCG: median speedup {lf['median_speedup_vs_jacobi']:.2f}x over Jacobi-PCG, "
+                f"iterations {lf['median_fcg_iters']} vs {lf['median_jacobi_iters']}")
 
     if not args.no_cpu:
         log(f"cpu baseline on {args.cpu_n}^3 ...")

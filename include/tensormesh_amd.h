@@ -268,7 +268,8 @@ int tm_gnn_decode(int64_t n, const uint16_t* h, const float* wd, const float* bd
 /* Inference fusions of the finest level (the unfused kernels compute the same
  * values up to BF16 rounding of intermediate level-0 features):
  *  - tm_gnn_encode_pool: encoder + first pooling, writing the coarse
- *    features, s and xs[i] = (d_i r_i / s, lk_i) (float2 per node);
+ *    features, s and xs[i] = (d_i r_i / s, lk_i) (float2 per node; lk may be
+ *    NULL for a network without the log-coefficient input);
  *  - tm_gnn_final: unpool (skip = encoder output) + message passing on level 0
  *    + decode, using xs, the row sums of A^ and B = A^ P^ (b_* arrays, see
  *    tm_gnn_level_prolong); writes z. */
@@ -283,10 +284,12 @@ int tm_gnn_final(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, cons
                  const float* bd, const float* d, const double* scale, double* z, const int* stop,
                  void* stream);
 /* Finest level without a layer (inference): z = s d ((e + w hc[agg]) wd + bd)
- * with e the encoder output from xs. */
-int tm_gnn_unpool_decode(int64_t n, const float* xs, const int32_t* agg, const float* w, const uint16_t* hc,
-                         const float* we, const float* be, const float* wd, const float* bd, const float* d,
-                         const double* scale, double* z, const int* stop, void* stream);
+ * with e the encoder output from xs; coarse_dot [n_coarse] is scratch for
+ * hc . wd. */
+int tm_gnn_unpool_decode(int64_t n, int64_t n_coarse, const float* xs, const int32_t* agg, const float* w,
+                         const uint16_t* hc, const float* we, const float* be, const float* wd, const float* bd,
+                         const float* d, const double* scale, float* coarse_dot, double* z, const int* stop,
+                         void* stream);
 /* Finest-level prolongation B = A^ P^: b_values[s] = sum of A^_k w_col(k) over
  * the entries k in perm[seg[s] .. seg[s+1]).  Row sums of a CSR. */
 int tm_gnn_level_prolong(int64_t nnz_b, const int32_t* seg, const int32_t* perm, const int32_t* col_idx,

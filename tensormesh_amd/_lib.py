@@ -106,7 +106,7 @@ _SIGS = {
     "tm_gnn_encode_pool": (c_int, [c_i64, c_i64] + [c_void_p] * 7 + [c_int] + [c_void_p] * 7),
     "tm_gnn_final": (c_int, [c_i64] + [c_void_p] * 15 + [c_int] + [c_void_p] * 7),
     "tm_gnn_level_prolong": (c_int, [c_i64] + [c_void_p] * 7),
-    "tm_gnn_unpool_decode": (c_int, [c_i64] + [c_void_p] * 13),
+    "tm_gnn_unpool_decode": (c_int, [c_i64, c_i64] + [c_void_p] * 14),
     "tm_gnn_row_sums": (c_int, [c_i64] + [c_void_p] * 4),
     "tm_gnn_decode_backward": (c_int, [c_i64] + [c_void_p] * 7),
     "tm_gnn_energy_loss": (c_int, [c_i64] + [c_void_p] * 10 + [ctypes.POINTER(c_int), c_void_p]),

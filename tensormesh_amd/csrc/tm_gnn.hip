@@ -141,23 +141,23 @@ __global__ __launch_bounds__(NT) void k_mp(int64_t n, const int32_t* __restrict_
     if (row < n) {
       load16(h, row, f0, hv);
       const int kb = rps[ri] - e0, ke = rps[ri + 1] - e0;
-      int k = kb;
-      for (; k + 1 < ke; k += 2) {
-        const int c0 = staged ? ec[k] : ci[e0 + k], c1 = staged ? ec[k + 1] : ci[e0 + k + 1];
-        const float v0 = staged ? ev[k] : av[e0 + k], v1 = staged ? ev[k + 1] : av[e0 + k + 1];
-        float h0[16], h1[16];
-        load16(h, c0, f0, h0);
-        load16(h, c1, f0, h1);
+      // four neighbour rows in flight per lane (the gathers are latency-bound)
+      for (int k = kb; k < ke; k += 4) {
+        int c[4];
+        float v[4];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) a[u] += v0 * h0[u] + v1 * h1[u];
-      }
-      if (k < ke) {
-        const int c0 = staged ? ec[k] : ci[e0 + k];
-        const float v0 = staged ? ev[k] : av[e0 + k];
-        float h0[16];
-        load16(h, c0, f0, h0);
+        for (int q = 0; q < 4; ++q) {
+          const bool in = k + q < ke;
+          c[q] = in ? (staged ? ec[k + q] : ci[e0 + k + q]) : (int)row;
+          v[q] = in ? (staged ? ev[k + q] : av[e0 + k + q]) : 0.0f;
+        }
+        float hq[4][16];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) a[u] += v0 * h0[u];
+        for (int q = 0; q < 4; ++q) load16(h, c[q], f0, hq[q]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int u = 0; u < 16; ++u) a[u] += v[q] * hq[q][u];
       }
     }
     store16(&X[ri * XP + f0], hv);
@@ -642,7 +642,7 @@ __global__ __launch_bounds__(NT) void k_encode_pool(int64_t n, int64_t nc, const
     if (I < nc)
       for (int k = cp[I] + q; k < cp[I + 1]; k += 8) {
         const int i = child[k];
-        const float x0 = (float)((double)d[i] * r[i]) * inv, x1 = lk[i], wi = w[i];
+        const float x0 = (float)((double)d[i] * r[i]) * inv, x1 = lk ? lk[i] : 0.0f, wi = w[i];
         xs[i] = float2{x0, x1};
         s0 += wi * x0;
         s1 += wi * x1;
@@ -725,42 +725,63 @@ __global__ __launch_bounds__(NT) void k_decode(int64_t n, const uint16_t* __rest
 // Finest level at inference when no layer runs there: unpool with the
 // encoder output as skip, then decode:
 //   y_i = (e_i + w_i hc[agg_i]) . wd + bd,  e_i = x_i0 We0 + x_i1 We1 + be,
-//   z_i = s d_i y_i.   8 lanes per node, xs from k_encode_pool.
+//   z_i = s d_i y_i.
+// The decoder is linear, so it is applied to the coarse rows first
+// (u[J] = hc[J] . wd, one dot per coarse node) and the finest level only
+// gathers one scalar per node.
+__global__ __launch_bounds__(NT) void k_row_dot(int64_t nc, const uint16_t* __restrict__ hc,
+                                                const float* __restrict__ wd, float* __restrict__ u,
+                                                const int* stop) {
+  if (stopped(stop)) return;
+  const int q = threadIdx.x % 8, f0 = q * 8;
+  float wl[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) wl[t] = wd[f0 + t];
+  for (int64_t J0 = blockIdx.x * (int64_t)(NT / 8); J0 < nc; J0 += (int64_t)gridDim.x * (NT / 8)) {
+    const int64_t J = J0 + threadIdx.x / 8;
+    float acc = 0.0f;
+    if (J < nc) {
+      float v[8];
+      Vec8<uint16_t>::load(&hc[J * F + f0], v);
+#pragma unroll
+      for (int t = 0; t < 8; ++t) acc += v[t] * wl[t];
+    }
+#pragma unroll
+    for (int o = 4; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kWave);
+    if (q == 0 && J < nc) u[J] = acc;
+  }
+}
+
 __global__ __launch_bounds__(NT) void k_unpool_decode(int64_t n, const float2* __restrict__ xs,
                                                       const int32_t* __restrict__ agg, const float* __restrict__ w,
-                                                      const uint16_t* __restrict__ hc, const float* __restrict__ we,
+                                                      const float* __restrict__ u, const float* __restrict__ we,
                                                       const float* __restrict__ be, const float* __restrict__ wd,
                                                       const float* __restrict__ bd, const float* __restrict__ d,
                                                       const double* __restrict__ s_in, double* __restrict__ z,
                                                       const int* stop) {
   if (stopped(stop)) return;
-  const double s = *s_in;
-  const int q = threadIdx.x % 8, f0 = q * 8;
-  float c0 = 0.0f, c1 = 0.0f, cb = 0.0f, wl[8];
+  __shared__ float cs[3];
+  if (threadIdx.x < kWave) {
+    const int f = threadIdx.x;
+    float c0 = we[f] * wd[f], c1 = we[F + f] * wd[f], cb = be[f] * wd[f];
 #pragma unroll
-  for (int t = 0; t < 8; ++t) {
-    wl[t] = wd[f0 + t];
-    c0 += we[f0 + t] * wl[t];
-    c1 += we[F + f0 + t] * wl[t];
-    cb += be[f0 + t] * wl[t];
-  }
-  const float b = bd[0];
-  for (int64_t i0 = blockIdx.x * (int64_t)(NT / 8); i0 < n; i0 += (int64_t)gridDim.x * (NT / 8)) {
-    const int64_t i = i0 + threadIdx.x / 8;
-    float acc = 0.0f;
-    float2 xi = float2{0.0f, 0.0f};
-    if (i < n) {
-      xi = xs[i];
-      float v[8];
-      Vec8<uint16_t>::load(&hc[(int64_t)agg[i] * F + f0], v);
-      float dot = 0.0f;
-#pragma unroll
-      for (int t = 0; t < 8; ++t) dot += v[t] * wl[t];
-      acc = xi.x * c0 + xi.y * c1 + cb + w[i] * dot;
+    for (int o = 32; o > 0; o >>= 1) {
+      c0 += __shfl_xor(c0, o, kWave);
+      c1 += __shfl_xor(c1, o, kWave);
+      cb += __shfl_xor(cb, o, kWave);
     }
-#pragma unroll
-    for (int o = 4; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kWave);
-    if (q == 0 && i < n) z[i] = s * (double)d[i] * (double)(acc + b);
+    if (f == 0) {
+      cs[0] = c0;
+      cs[1] = c1;
+      cs[2] = cb + bd[0];
+    }
+  }
+  __syncthreads();
+  const double s = *s_in;
+  const float c0 = cs[0], c1 = cs[1], cb = cs[2];
+  for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+    const float2 xi = xs[i];
+    z[i] = s * (double)d[i] * (double)(xi.x * c0 + xi.y * c1 + cb + w[i] * u[agg[i]]);
   }
 }
 
@@ -904,8 +925,8 @@ __global__ __launch_bounds__(NT) void k_adam(int64_t n, float* __restrict__ p, c
 //   q = A p; alpha = (p, r) / (p, q); x += alpha p; r -= alpha q
 // over a ring of m stored (p_j, q_j = A p_j), evaluated with one reduction
 // phase per iteration: w = A z and the dots (z,q_j), (p_j,r), (z,r), (z,w)
-// come out of one SpMV kernel, then (the stored directions being mutually
-// A-orthogonal)
+// come out of one SpMV kernel, a single workgroup turns their partial sums
+// into the betas and alpha (the stored directions being mutually A-orthogonal)
 //   q = w - sum beta_j q_j,  (p,r) = (z,r) - sum beta_j (p_j,r),
 //   (p,q) = (z,w) - 2 sum beta_j (z,q_j) + sum beta_j^2 (p_j,q_j)
 // and one update kernel applies everything.  Unfilled slots hold zeros with
@@ -913,18 +934,19 @@ __global__ __launch_bounds__(NT) void k_adam(int64_t n, float* __restrict__ p, c
 // ---------------------------------------------------------------------------
 constexpr int kFcgMaxM = 8;
 struct FcgScalars {
-  double bb, tol2, relres, s, pq_new;
+  double bb, tol2, relres, s, pq_new, alpha;
   double pq[kFcgMaxM];
+  double beta[kFcgMaxM];
 };
 struct FcgFlags {
   int status;  // 0 running, 1 converged, 2 breakdown, 3 max iterations
   int iter;
   long long maxiter;
 };
-// partial slots: (z,q_j) [m] | (p_j,r) [m] | zr | zw | rr | bb | ss | pq
+// partial slots: (z,q_j) [m] | (p_j,r) [m] | zr | zw | rr | bb | ss
 constexpr int S_ZQ = 0, S_PR = kFcgMaxM, S_ZR = 2 * kFcgMaxM, S_ZW = S_ZR + 1, S_RR = S_ZR + 2, S_BB = S_ZR + 3,
-              S_SS = S_ZR + 4, S_PQ = S_ZR + 5;
-constexpr int kFcgSlots = S_PQ + 1;
+              S_SS = S_ZR + 4;
+constexpr int kFcgSlots = S_SS + 1;
 
 struct FcgWork {
   double *r, *z, *w, *P, *Q, *part;  // P, Q: [m][n]
@@ -1041,35 +1063,47 @@ __global__ __launch_bounds__(NT) void k_fcg_ctl0(FcgWork w, int nparts, double r
   }
 }
 
-// betas and alpha from the partials; p, q into ring slot iter % m; x, r;
-// partials (r,r), (d r)^2 and (p,q) (workgroup 0 only)
+// Single workgroup: betas and alpha from the SpMV partials (one wave per
+// group of slots), written to the scalar block for the update kernel.
 template <int M>
-__global__ __launch_bounds__(NTF) void k_fcg_update(int64_t n, double* __restrict__ x, const float* __restrict__ d,
-                                                    FcgWork w, int nparts) {
+__global__ __launch_bounds__(NT) void k_fcg_scalars(FcgWork w, int nparts) {
   if (w.f->status != 0) return;
   __shared__ double sh[2 * M + 2];
-  if (threadIdx.x < kWave) {
-    for (int k = 0; k < 2 * M + 2; ++k) {
-      const int slot = k < M ? S_ZQ + k : (k < 2 * M ? S_PR + (k - M) : (k == 2 * M ? S_ZR : S_ZW));
-      double a = 0.0;
-      for (int b = threadIdx.x; b < nparts; b += kWave) a += w.part[slot * kMaxParts + b];
-      a = tmx::wave_sum(a);
-      if (threadIdx.x == 0) sh[k] = a;
-    }
+  const int wv = threadIdx.x / kWave, l = threadIdx.x % kWave;
+  for (int k = wv; k < 2 * M + 2; k += NT / kWave) {
+    const int slot = k < M ? S_ZQ + k : (k < 2 * M ? S_PR + (k - M) : (k == 2 * M ? S_ZR : S_ZW));
+    double a = 0.0;
+    for (int b = l; b < nparts; b += kWave) a += w.part[slot * kMaxParts + b];
+    a = tmx::wave_sum(a);
+    if (l == 0) sh[k] = a;
   }
   __syncthreads();
-  double beta[M];
-  double pr = sh[2 * M], pq = sh[2 * M + 1];
-#pragma unroll
-  for (int j = 0; j < M; ++j) {
-    beta[j] = sh[j] / w.s->pq[j];
-    pr -= beta[j] * sh[M + j];
-    pq += beta[j] * (beta[j] * w.s->pq[j] - 2.0 * sh[j]);
+  if (threadIdx.x == 0) {
+    double pr = sh[2 * M], pq = sh[2 * M + 1];
+    for (int j = 0; j < M; ++j) {
+      const double beta = sh[j] / w.s->pq[j];
+      w.s->beta[j] = beta;
+      pr -= beta * sh[M + j];
+      pq += beta * (beta * w.s->pq[j] - 2.0 * sh[j]);
+    }
+    const bool ok = pq > 0.0 && __builtin_isfinite(pq) && __builtin_isfinite(pr);
+    w.s->alpha = ok ? pr / pq : 0.0;
+    w.s->pq_new = ok ? pq : -1.0;
   }
-  const bool ok = pq > 0.0 && __builtin_isfinite(pq) && __builtin_isfinite(pr);
-  const double alpha = ok ? pr / pq : 0.0;
+}
+
+// p, q into ring slot iter % m; x += alpha p, r -= alpha q; partials (r,r), (d r)^2
+template <int M>
+__global__ __launch_bounds__(NTF) void k_fcg_update(int64_t n, double* __restrict__ x, const float* __restrict__ d,
+                                                    FcgWork w) {
+  if (w.f->status != 0) return;
+  const double alpha = w.s->alpha;
+  const bool ok = w.s->pq_new > 0.0;
+  double beta[M];
+#pragma unroll
+  for (int j = 0; j < M; ++j) beta[j] = w.s->beta[j];
   const int slot = w.f->iter % M;
-  double loc[3] = {0.0, 0.0, 0.0};
+  double loc[2] = {0.0, 0.0};
   if (ok) {
     for (int64_t i = blockIdx.x * (int64_t)NTF + threadIdx.x; i < n; i += (int64_t)gridDim.x * NTF) {
       double pi = w.z[i], qi = w.w[i];
@@ -1088,33 +1122,23 @@ __global__ __launch_bounds__(NTF) void k_fcg_update(int64_t n, double* __restric
       loc[1] += dr * dr;
     }
   }
-  loc[2] = (blockIdx.x == 0 && threadIdx.x == 0) ? (ok ? pq : -1.0) : 0.0;
-  const int which[3] = {S_RR, S_SS, S_PQ};
-  tmx::write_parts<NTF, 3>(w.part, kMaxParts, which, loc);
+  const int which[2] = {S_RR, S_SS};
+  tmx::write_parts<NTF, 2>(w.part, kMaxParts, which, loc);
 }
 
 // Single workgroup: (p,q) of the new ring slot, iteration count, convergence.
 __global__ __launch_bounds__(NT) void k_fcg_check(FcgWork w, int nparts) {
   if (w.f->status != 0) return;
-  __shared__ double red[NT / kWave][2];
-  double a = 0.0, b = 0.0;
-  for (int k = threadIdx.x; k < nparts; k += NT) {
-    a += w.part[S_RR * kMaxParts + k];
-    b += w.part[S_PQ * kMaxParts + k];
-  }
+  __shared__ double red[NT / kWave];
+  double a = 0.0;
+  for (int k = threadIdx.x; k < nparts; k += NT) a += w.part[S_RR * kMaxParts + k];
   a = tmx::wave_sum(a);
-  b = tmx::wave_sum(b);
-  if (threadIdx.x % kWave == 0) {
-    red[threadIdx.x / kWave][0] = a;
-    red[threadIdx.x / kWave][1] = b;
-  }
+  if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = a;
   __syncthreads();
   if (threadIdx.x == 0) {
-    double rr = 0.0, pq = 0.0;
-    for (int k = 0; k < NT / kWave; ++k) {
-      rr += red[k][0];
-      pq += red[k][1];
-    }
+    double rr = 0.0;
+    for (int k = 0; k < NT / kWave; ++k) rr += red[k];
+    const double pq = w.s->pq_new;
     const int it = w.f->iter;
     w.f->iter = it + 1;
     if (!(pq > 0.0) || !__builtin_isfinite(pq) || !__builtin_isfinite(rr)) {
@@ -1168,8 +1192,8 @@ extern "C" int tm_gnn_encode_pool(int64_t n, int64_t n_coarse, const int32_t* ch
                                   const float* w, const double* r, const float* d, const float* lk,
                                   const double* ss_part, int nparts, const float* we, const float* be, uint16_t* out,
                                   float* xs, double* scale, const int* stop, void* stream) {
-  TM_REQUIRE(n >= 0 && n_coarse >= 0 && child_ptr && child && w && r && d && lk && ss_part && we && be && out &&
-                 xs && scale,
+  TM_REQUIRE(n >= 0 && n_coarse >= 0 && child_ptr && child && w && r && d && ss_part && we && be && out && xs &&
+                 scale,
              TM_ERR_ARG, "null argument");
   TM_REQUIRE(nparts >= 1 && nparts <= kMaxParts, TM_ERR_ARG, "encode_pool: bad partial count %d", nparts);
   const int g = grid_of(n_coarse, NT / 8, 8192);
@@ -1198,15 +1222,20 @@ extern "C" int tm_gnn_final(int64_t n, const int32_t* row_ptr, const int32_t* co
   return TM_OK;
 }
 
-extern "C" int tm_gnn_unpool_decode(int64_t n, const float* xs, const int32_t* agg, const float* w,
-                                    const uint16_t* hc, const float* we, const float* be, const float* wd,
-                                    const float* bd, const float* d, const double* scale, double* z, const int* stop,
-                                    void* stream) {
-  TM_REQUIRE(n >= 0 && xs && agg && w && hc && we && be && wd && bd && d && scale && z, TM_ERR_ARG,
-             "null argument");
+extern "C" int tm_gnn_unpool_decode(int64_t n, int64_t n_coarse, const float* xs, const int32_t* agg,
+                                    const float* w, const uint16_t* hc, const float* we, const float* be,
+                                    const float* wd, const float* bd, const float* d, const double* scale,
+                                    float* coarse_dot, double* z, const int* stop, void* stream) {
+  TM_REQUIRE(n >= 0 && n_coarse >= 0 && xs && agg && w && hc && we && be && wd && bd && d && scale && coarse_dot &&
+                 z,
+             TM_ERR_ARG, "null argument");
   if (n == 0) return TM_OK;
-  hipLaunchKernelGGL(k_unpool_decode, dim3(grid_of(n, NT / 8, 8192)), dim3(NT), 0, (hipStream_t)stream, n,
-                     (const float2*)xs, agg, w, hc, we, be, wd, bd, d, scale, z, stop);
+  hipStream_t st = (hipStream_t)stream;
+  if (n_coarse > 0)
+    hipLaunchKernelGGL(k_row_dot, dim3(grid_of(n_coarse, NT / 8, 8192)), dim3(NT), 0, st, n_coarse, hc, wd,
+                       coarse_dot, stop);
+  hipLaunchKernelGGL(k_unpool_decode, dim3(grid_of(n, NT, 8192)), dim3(NT), 0, st, n, (const float2*)xs, agg, w,
+                     coarse_dot, we, be, wd, bd, d, scale, z, stop);
   TM_HIP(hipGetLastError());
   return TM_OK;
 }
@@ -1452,7 +1481,8 @@ int fcg_iterate_m(int64_t n, const int32_t* rp, const int32_t* ci, const float* 
                   FcgWork w, hipStream_t st) {
   const int G = fcg_grid(n);
   hipLaunchKernelGGL((k_fcg_spmv<8, M, false>), dim3(G), dim3(NTF), 0, st, n, rp, ci, v, x, w);
-  hipLaunchKernelGGL(k_fcg_update<M>, dim3(G), dim3(NTF), 0, st, n, x, d, w, G);
+  hipLaunchKernelGGL(k_fcg_scalars<M>, dim3(1), dim3(NT), 0, st, w, G);
+  hipLaunchKernelGGL(k_fcg_update<M>, dim3(G), dim3(NTF), 0, st, n, x, d, w);
   hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(NT), 0, st, w, G);
   TM_HIP(hipGetLastError());
   return TM_OK;

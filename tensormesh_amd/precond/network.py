@@ -273,6 +273,8 @@ class Program:
         self.ss_part = torch.zeros(1024, dtype=torch.float64, device=dev)
         self.xs = (torch.zeros((self.n0, 2), dtype=torch.float32, device=dev)
                    if (self.fusable or self.fusable_plain) else None)
+        self.coarse_dot = (torch.zeros(hier.levels[1].n, dtype=torch.float32, device=dev)
+                           if self.fusable_plain else None)
 
     def alloc_training(self):
         dev = self.hier.device
@@ -305,13 +307,14 @@ class Program:
             first, last = self.ops[0], self.ops[-1]
             lv1 = hier.levels[1]
             check(L.tm_gnn_encode_pool(self.n0, lv1.n, ptr(lv1.child_ptr), ptr(lv1.child), ptr(lv0.w), _addr(r),
-                                       ptr(lv0.d), ptr(lk), ss_ptr, nparts, ptr(we), ptr(be),
-                                       ptr(self.bufs[first[3]]), ptr(self.xs), scale_p, stop_p, st))
+                                       ptr(lv0.d), _addr(None) if net.homogeneous else ptr(lk), ss_ptr, nparts,
+                                       ptr(we), ptr(be), ptr(self.bufs[first[3]]), ptr(self.xs), scale_p, stop_p,
+                                       st))
             self._run_ops(self.ops[1:-1], 1, train, stop_p, st)
             wd, bd = net.dec()
-            check(L.tm_gnn_unpool_decode(self.n0, ptr(self.xs), ptr(lv0.agg), ptr(lv0.w), ptr(self.bufs[last[2]]),
-                                         ptr(we), ptr(be), ptr(wd), ptr(bd), ptr(lv0.d), scale_p, _addr(zt), stop_p,
-                                         st))
+            check(L.tm_gnn_unpool_decode(self.n0, hier.levels[1].n, ptr(self.xs), ptr(lv0.agg), ptr(lv0.w),
+                                         ptr(self.bufs[last[2]]), ptr(we), ptr(be), ptr(wd), ptr(bd), ptr(lv0.d),
+                                         scale_p, ptr(self.coarse_dot), _addr(zt), stop_p, st))
             return zt
         if fused and self.fusable and not train and not want_y:
             first, unp, last = self.ops[0], self.ops[-2], self.ops[-1]
