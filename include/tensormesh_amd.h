@@ -269,7 +269,8 @@ int tm_gnn_decode(int64_t n, const uint16_t* h, const float* wd, const float* bd
  * values up to BF16 rounding of intermediate level-0 features):
  *  - tm_gnn_encode_pool: encoder + first pooling, writing the coarse
  *    features, s and xs[i] = (d_i r_i / s, lk_i) (float2 per node; lk may be
- *    NULL for a network without the log-coefficient input);
+ *    NULL for a network without the log-coefficient input; with nparts = 0, s
+ *    is read from *scale instead of reduced from ss_part);
  *  - tm_gnn_final: unpool (skip = encoder output) + message passing on level 0
  *    + decode, using xs, the row sums of A^ and B = A^ P^ (b_* arrays, see
  *    tm_gnn_level_prolong); writes z. */
@@ -303,6 +304,11 @@ int tm_gnn_decode_backward(int64_t n, const uint16_t* h, const float* wd, const 
 int tm_gnn_energy_loss(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values,
                        const float* y, const double* r, const float* d, const double* scale, const double* e,
                        float* g_y, double* part, int* nparts, void* stream);
+/* Scale-invariant loss gradient, in place on the g_y of tm_gnn_energy_loss:
+ * L = 1 - (y.r~)^2 / ((y.A^y)(e~.r~)) (the error left after the optimal step
+ * along y), from that call's partial sums. */
+int tm_gnn_cosine_grad(int64_t n, const double* r, const float* d, const double* scale, const double* part,
+                       int nparts, float* g_y, void* stream);
 /* Level values of one matrix: d = diag^-1/2, values_normalized = D A D. */
 int tm_gnn_level_setup(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values, float* d,
                        float* values_normalized, void* stream);
@@ -323,8 +329,9 @@ int tm_gnn_adam(int64_t n, float* param, const float* grad, float* m, float* v, 
  * view and tm_fcg_iterate finishes the iteration (one SpMV with its dot
  * products, one update, one single-workgroup convergence check).  Everything
  * is stream-ordered with device-side scalars, so iterations can be captured
- * in a hipGraph.  ss_part holds the nparts partials of sum (d r)^2 that the
- * encoder turns into its input scale. */
+ * in a hipGraph.  ss_part holds the nparts partials of sum (d r)^2, and the
+ * start / check kernels store s = rms(d r) at *scale, so an encoder may read
+ * it directly (tm_gnn_encode_pool with nparts = 0). */
 typedef struct {
   double* r;
   double* z;

@@ -93,11 +93,17 @@ def _dact(u, n_lin):
     return g
 
 
-def _program(schedule):
-    """(kind, level, in, out, extra) ops mirroring the U-shaped schedule."""
+def _program(schedule, bottom=None):
+    """(kind, level, in, out, extra) ops mirroring the U-shaped schedule; between
+    the down and the up half the features are pooled down to ``bottom``."""
     ops, cur, h, nb = [], 0, 0, 1
     skips = {}
+    bottom = max(schedule) if bottom is None else bottom
     for t, lev in enumerate(schedule):
+        if t == len(schedule) // 2:
+            while cur < max(lev, bottom):
+                skips[cur] = h
+                ops.append(("pool", cur, h, nb, None)); h, cur, nb = nb, cur + 1, nb + 1
         while cur < lev:
             skips[cur] = h
             ops.append(("pool", cur, h, nb, None)); h, cur, nb = nb, cur + 1, nb + 1
@@ -109,7 +115,7 @@ def _program(schedule):
     return ops, h
 
 
-def forward(levels, params, schedule, n_lin, r, lk, bf16=False):
+def forward(levels, params, schedule, n_lin, r, lk, bf16=False, bottom=None):
     """Returns (z, y, s, tape)."""
     rnd = bf16_round if bf16 else (lambda a: np.asarray(a, dtype=np.float64))
     we, be, layers, wd, bd = unpack(params, len(schedule))
@@ -122,7 +128,7 @@ def forward(levels, params, schedule, n_lin, r, lk, bf16=False):
         x = x.astype(np.float32).astype(np.float64)
     pre0 = x @ we + be
     bufs = {0: rnd(pre0)}
-    ops, final = _program(schedule)
+    ops, final = _program(schedule, bottom)
     tape = {"x": x, "pre0": pre0, "ops": ops, "final": final, "bufs": bufs, "msg": {}, "pre": {}}
     for i, (kind, lev, a, b, extra) in enumerate(ops):
         L = levels[lev]
@@ -150,14 +156,32 @@ def loss(levels, y, r, s):
     return (0.5 * y @ (An @ y) - y @ rt) / len(y)
 
 
-def backward(levels, params, schedule, n_lin, r, lk, tape, s, bf16=False):
-    """Gradient of ``loss`` w.r.t. the flat parameter vector (float64)."""
+def cosine_loss(levels, y, r, s, e):
+    """A^-norm error left after the optimal step along y:
+    1 - (y.r~)^2 / ((y.A^y)(e~.r~)), with r~ = d r / s and e~ = e / (d s)."""
+    d0, An = levels[0]["d"], levels[0]["An"]
+    rt, et = d0 * r / s, e / (d0 * s)
+    return 1.0 - (y @ rt) ** 2 / ((y @ (An @ y)) * (et @ rt))
+
+
+def cosine_grad_y(levels, y, r, s, e):
+    d0, An = levels[0]["d"], levels[0]["An"]
+    rt, et = d0 * r / s, e / (d0 * s)
+    ay = An @ y
+    yay, yr, er = y @ ay, y @ rt, et @ rt
+    return -2.0 * yr / (yay * er) * (rt - yr / yay * ay)
+
+
+def backward(levels, params, schedule, n_lin, r, lk, tape, s, bf16=False, gy=None):
+    """Gradient of ``loss`` (or, with ``gy``, of any loss whose gradient with
+    respect to the output y is ``gy``) w.r.t. the flat parameter vector."""
     rnd = bf16_round if bf16 else (lambda a: np.asarray(a, dtype=np.float64))
     we, be, layers, wd, bd = unpack(params, len(schedule))
     d0, An = levels[0]["d"], levels[0]["An"]
     n = len(r)
     y = tape["y"]
-    gy = (An @ y - d0 * r / s) / n
+    if gy is None:
+        gy = (An @ y - d0 * r / s) / n
     bufs, ops = tape["bufs"], tape["ops"]
     g_layers = [None] * len(schedule)
     grads = {tape["final"]: np.outer(gy, wd)}

@@ -111,6 +111,7 @@ _SIGS = {
     "tm_gnn_decode_backward": (c_int, [c_i64] + [c_void_p] * 7),
     "tm_gnn_energy_loss": (c_int, [c_i64] + [c_void_p] * 10 + [ctypes.POINTER(c_int), c_void_p]),
     "tm_gnn_level_setup": (c_int, [c_i64] + [c_void_p] * 6),
+    "tm_gnn_cosine_grad": (c_int, [c_i64] + [c_void_p] * 4 + [c_int, c_void_p, c_void_p]),
     "tm_gnn_level_coarsen": (c_int, [c_i64] + [c_void_p] * 5),
     "tm_gnn_pool_weights": (c_int, [c_i64] + [c_void_p] * 5),
     "tm_gnn_adam": (c_int, [c_i64] + [c_void_p] * 5 + [ctypes.c_float] * 4 + [c_i64, c_void_p]),

@@ -61,7 +61,7 @@ __device__ __forceinline__ bool stopped(const int* stop) { return stop != nullpt
 // per lane (32 B of the neighbour's row per lane), then the tile [h | a] is
 // multiplied by W on MFMA.
 // ---------------------------------------------------------------------------
-constexpr int kEnt = 1536;
+constexpr int kEnt = 1024;
 
 __device__ __forceinline__ void load16(const uint16_t* __restrict__ h, int64_t j, int f0, float (&o)[16]) {
   const uint4 a = *(const uint4*)&h[j * F + f0];
@@ -90,6 +90,25 @@ __device__ __forceinline__ void load_wt(uint16_t* W, const uint16_t* __restrict_
   }
 }
 
+// acc[jt] = X[16 rows of this wave][0:128] x W^T[jt*16 .. +16][0:128]^T with the
+// B operands read straight from the global W^T ([64][128], 16 KB: every
+// workgroup reads the same rows, so they come from the L1/L2 caches and the
+// LDS stays free for more resident workgroups)
+__device__ __forceinline__ void tile_mfma_gw(const uint16_t* X, const uint16_t* __restrict__ wt, int wv, int l,
+                                             f32x4 (&acc)[4]) {
+#pragma unroll
+  for (int jt = 0; jt < 4; ++jt) acc[jt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  const int ar = wv * 16 + (l % 16), ak = 8 * (l / 16);
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk) {
+    const bf16x8 a = lds8(&X[ar * XP + kk * 32 + ak]);
+#pragma unroll
+    for (int jt = 0; jt < 4; ++jt)
+      acc[jt] = mfma(a, __builtin_bit_cast(bf16x8, *(const uint4*)&wt[(jt * 16 + (l % 16)) * 2 * F + kk * 32 + ak]),
+                     acc[jt]);
+  }
+}
+
 // acc[jt] = X[16 rows of this wave][0:128] x W^T[jt*16 .. +16][0:128]^T
 __device__ __forceinline__ void tile_mfma(const uint16_t* X, const uint16_t* W, int wv, int l, f32x4 (&acc)[4]) {
 #pragma unroll
@@ -112,13 +131,11 @@ __global__ __launch_bounds__(NT) void k_mp(int64_t n, const int32_t* __restrict_
                                            const int* stop) {
   if (stopped(stop)) return;
   __shared__ __align__(16) uint16_t X[TILE * XP];
-  __shared__ __align__(16) uint16_t W[F * XP];  // W^T: [64 outputs][128 inputs]
   __shared__ int rps[TILE + 1];
   __shared__ int ec[kEnt];
   __shared__ float ev[kEnt];
   const int tid = threadIdx.x, wv = tid / kWave, l = tid % kWave;
   const int64_t r0 = (int64_t)tmx::xcd_remap(blockIdx.x, gridDim.x) * TILE;
-  load_wt(W, wt);
   for (int c = tid; c <= TILE; c += NT) rps[c] = rp[r0 + c < n ? r0 + c : n];
   __syncthreads();
   const int e0 = rps[0], ecount = rps[TILE] - e0;
@@ -169,7 +186,7 @@ __global__ __launch_bounds__(NT) void k_mp(int64_t n, const int32_t* __restrict_
   }
   __syncthreads();
   f32x4 acc[4];
-  tile_mfma(X, W, wv, l, acc);
+  tile_mfma_gw(X, wt, wv, l, acc);
 #pragma unroll
   for (int jt = 0; jt < 4; ++jt) {
     const int j = jt * 16 + (l % 16);
@@ -632,8 +649,9 @@ __global__ __launch_bounds__(NT) void k_encode_pool(int64_t n, int64_t nc, const
                                                     float2* __restrict__ xs, double* __restrict__ s_out,
                                                     const int* stop) {
   if (stopped(stop)) return;
-  const double s = scale_from_parts(ss_part, nparts, n);
-  if (blockIdx.x == 0 && threadIdx.x == 0) *s_out = s;
+  // nparts == 0: the caller already stored s (the flexible-CG check kernel does)
+  const double s = nparts > 0 ? scale_from_parts(ss_part, nparts, n) : *s_out;
+  if (nparts > 0 && blockIdx.x == 0 && threadIdx.x == 0) *s_out = s;
   const float inv = (float)(1.0 / s);
   const int q = threadIdx.x % 8, f0 = q * 8;
   for (int64_t I0 = blockIdx.x * (int64_t)(NT / 8); I0 < nc; I0 += (int64_t)gridDim.x * (NT / 8)) {
@@ -833,6 +851,29 @@ __global__ __launch_bounds__(NT) void k_energy_loss(int64_t n, const int32_t* __
   }
   const int which[3] = {0, 1, 2};
   tmx::write_parts<NT, 3>(part, kMaxParts, which, loc);
+}
+
+// Scale-invariant loss: the A^-norm error left after the optimal step along y,
+//   L = 1 - (y.r~)^2 / ((y.A^y)(e~.r~)),
+//   dL/dy = c ((1 - t) r~ - t n g),  t = y.r~ / y.A^y,  c = -2 y.r~ / (y.A^y e~.r~),
+// applied in place to g = (A^y - r~) / n written by k_energy_loss, whose
+// partial sums [y.A^y | y.r~ | e~.r~] give the scalars.
+__global__ __launch_bounds__(NT) void k_cosine_grad(int64_t n, const double* __restrict__ r,
+                                                    const float* __restrict__ d, const double* __restrict__ s_in,
+                                                    const double* __restrict__ part, int nparts,
+                                                    float* __restrict__ g_y) {
+  const int which[3] = {0, 1, 2};
+  double t3[3];
+  tmx::sum_parts<3>(part, kMaxParts, which, nparts, t3);
+  const double yay = t3[0], yr = t3[1], er = t3[2];
+  const bool ok = yay > 0.0 && er > 0.0;
+  const double t = ok ? yr / yay : 0.0;
+  const double c = ok ? -2.0 * yr / (yay * er) : 0.0;
+  const double s = *s_in;
+  for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+    const double rt = (double)d[i] * r[i] / s;
+    g_y[i] = (float)(c * ((1.0 - t) * rt - t * (double)n * (double)g_y[i]));
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1046,12 +1087,14 @@ __global__ __launch_bounds__(NTF) void k_fcg_init(int64_t n, const double* __res
   tmx::write_parts<NTF, 3>(w.part, kMaxParts, which, loc);
 }
 
-__global__ __launch_bounds__(NT) void k_fcg_ctl0(FcgWork w, int nparts, double rtol2, double atol2,
+__global__ __launch_bounds__(NT) void k_fcg_ctl0(FcgWork w, int64_t n, int nparts, double rtol2, double atol2,
                                                  long long maxiter) {
-  const int which[2] = {S_RR, S_BB};
-  double t[2];
-  tmx::sum_parts<2>(w.part, kMaxParts, which, nparts, t);
+  const int which[3] = {S_RR, S_BB, S_SS};
+  double t[3];
+  tmx::sum_parts<3>(w.part, kMaxParts, which, nparts, t);
   if (threadIdx.x == 0) {
+    const double sc = n > 0 ? sqrt(t[2] / (double)n) : 0.0;
+    w.s->s = sc > 0.0 ? sc : 1.0;
     w.s->bb = t[1];
     w.s->tol2 = fmax(rtol2 * t[1], atol2);
     for (int j = 0; j < kFcgMaxM; ++j) w.s->pq[j] = 1.0;
@@ -1126,18 +1169,31 @@ __global__ __launch_bounds__(NTF) void k_fcg_update(int64_t n, double* __restric
   tmx::write_parts<NTF, 2>(w.part, kMaxParts, which, loc);
 }
 
-// Single workgroup: (p,q) of the new ring slot, iteration count, convergence.
-__global__ __launch_bounds__(NT) void k_fcg_check(FcgWork w, int nparts) {
+// Single workgroup: (p,q) of the new ring slot, iteration count, convergence,
+// and the input scale s = rms(d r) of the next preconditioner application.
+__global__ __launch_bounds__(NT) void k_fcg_check(FcgWork w, int64_t n, int nparts) {
   if (w.f->status != 0) return;
-  __shared__ double red[NT / kWave];
-  double a = 0.0;
-  for (int k = threadIdx.x; k < nparts; k += NT) a += w.part[S_RR * kMaxParts + k];
+  __shared__ double red[NT / kWave][2];
+  double a = 0.0, c = 0.0;
+  for (int k = threadIdx.x; k < nparts; k += NT) {
+    a += w.part[S_RR * kMaxParts + k];
+    c += w.part[S_SS * kMaxParts + k];
+  }
   a = tmx::wave_sum(a);
-  if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = a;
+  c = tmx::wave_sum(c);
+  if (threadIdx.x % kWave == 0) {
+    red[threadIdx.x / kWave][0] = a;
+    red[threadIdx.x / kWave][1] = c;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double rr = 0.0;
-    for (int k = 0; k < NT / kWave; ++k) rr += red[k];
+    double rr = 0.0, ss = 0.0;
+    for (int k = 0; k < NT / kWave; ++k) {
+      rr += red[k][0];
+      ss += red[k][1];
+    }
+    const double sc = n > 0 ? sqrt(ss / (double)n) : 0.0;
+    w.s->s = sc > 0.0 ? sc : 1.0;  // input scale of the next preconditioner application
     const double pq = w.s->pq_new;
     const int it = w.f->iter;
     w.f->iter = it + 1;
@@ -1195,7 +1251,7 @@ extern "C" int tm_gnn_encode_pool(int64_t n, int64_t n_coarse, const int32_t* ch
   TM_REQUIRE(n >= 0 && n_coarse >= 0 && child_ptr && child && w && r && d && ss_part && we && be && out && xs &&
                  scale,
              TM_ERR_ARG, "null argument");
-  TM_REQUIRE(nparts >= 1 && nparts <= kMaxParts, TM_ERR_ARG, "encode_pool: bad partial count %d", nparts);
+  TM_REQUIRE(nparts >= 0 && nparts <= kMaxParts, TM_ERR_ARG, "encode_pool: bad partial count %d", nparts);
   const int g = grid_of(n_coarse, NT / 8, 8192);
   hipLaunchKernelGGL(k_encode_pool, dim3(g), dim3(NT), 0, (hipStream_t)stream, n, n_coarse, child_ptr, child, w, r,
                      d, lk, ss_part, nparts, we, be, out, (float2*)xs, scale, stop);
@@ -1382,6 +1438,16 @@ extern "C" int tm_gnn_energy_loss(int64_t n, const int32_t* row_ptr, const int32
   return TM_OK;
 }
 
+extern "C" int tm_gnn_cosine_grad(int64_t n, const double* r, const float* d, const double* scale,
+                                  const double* part, int nparts, float* g_y, void* stream) {
+  TM_REQUIRE(n >= 0 && r && d && scale && part && g_y, TM_ERR_ARG, "null argument");
+  TM_REQUIRE(nparts >= 1 && nparts <= kMaxParts, TM_ERR_ARG, "cosine_grad: bad partial count %d", nparts);
+  hipLaunchKernelGGL(k_cosine_grad, dim3(grid_of(n, NT * 4, 1024)), dim3(NT), 0, (hipStream_t)stream, n, r, d,
+                     scale, part, nparts, g_y);
+  TM_HIP(hipGetLastError());
+  return TM_OK;
+}
+
 extern "C" int tm_gnn_level_setup(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values,
                                   float* d, float* values_normalized, void* stream) {
   TM_REQUIRE(n >= 0 && row_ptr && col_idx && values && d && values_normalized, TM_ERR_ARG, "null argument");
@@ -1455,7 +1521,7 @@ int fcg_start_m(int64_t n, const int32_t* rp, const int32_t* ci, const float* v,
   TM_HIP(hipMemsetAsync(w.f, 0, sizeof(FcgFlags), st));
   hipLaunchKernelGGL((k_fcg_spmv<8, M, true>), dim3(G), dim3(NTF), 0, st, n, rp, ci, v, x, w);
   hipLaunchKernelGGL(k_fcg_init<M>, dim3(G), dim3(NTF), 0, st, n, b, d, w);
-  hipLaunchKernelGGL(k_fcg_ctl0, dim3(1), dim3(NT), 0, st, w, G, rtol * rtol, atol * atol, (long long)maxiter);
+  hipLaunchKernelGGL(k_fcg_ctl0, dim3(1), dim3(NT), 0, st, w, n, G, rtol * rtol, atol * atol, (long long)maxiter);
   TM_HIP(hipGetLastError());
   return TM_OK;
 }
@@ -1483,7 +1549,7 @@ int fcg_iterate_m(int64_t n, const int32_t* rp, const int32_t* ci, const float* 
   hipLaunchKernelGGL((k_fcg_spmv<8, M, false>), dim3(G), dim3(NTF), 0, st, n, rp, ci, v, x, w);
   hipLaunchKernelGGL(k_fcg_scalars<M>, dim3(1), dim3(NT), 0, st, w, G);
   hipLaunchKernelGGL(k_fcg_update<M>, dim3(G), dim3(NTF), 0, st, n, x, d, w);
-  hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(NT), 0, st, w, G);
+  hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(NT), 0, st, w, n, G);
   TM_HIP(hipGetLastError());
   return TM_OK;
 }

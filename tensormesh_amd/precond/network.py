@@ -57,8 +57,12 @@ class GNNPreconditioner:
 
     def __init__(self, schedule: Sequence[int] = (1, 2, 3, 3, 2, 1), n_linear: int = 64,
                  device: torch.device = torch.device("cuda:0"), seed: int = 0, homogeneous: bool = True,
-                 symmetric: bool = True):
+                 symmetric: bool = True, bottom: Optional[int] = None):
         self.schedule = tuple(int(s) for s in schedule)
+        # levels below the deepest layer that still contribute a pooled term
+        # (between the down and the up path, at layer index ``turn``)
+        self.bottom = max(self.schedule) if bottom is None else int(bottom)
+        self.turn = len(self.schedule) // 2
         self.n_linear = int(n_linear)
         self.symmetric = bool(symmetric)
         self.homogeneous = bool(homogeneous) or self.symmetric
@@ -189,7 +193,7 @@ class GNNPreconditioner:
     def state_dict(self) -> Dict[str, np.ndarray]:
         return {"params": self.params.detach().cpu().numpy(), "schedule": np.array(self.schedule),
                 "n_linear": np.array(self.n_linear), "homogeneous": np.array(int(self.homogeneous)),
-                "symmetric": np.array(int(self.symmetric))}
+                "symmetric": np.array(int(self.symmetric)), "bottom": np.array(self.bottom)}
 
     def save(self, path: str):
         np.savez(path, **self.state_dict())
@@ -199,14 +203,16 @@ class GNNPreconditioner:
         z = np.load(path, allow_pickle=False)
         hom = bool(int(z["homogeneous"])) if "homogeneous" in z.files else False
         sym = bool(int(z["symmetric"])) if "symmetric" in z.files else False
-        net = cls(tuple(int(s) for s in z["schedule"]), int(z["n_linear"]), device, homogeneous=hom, symmetric=sym)
+        bottom = int(z["bottom"]) if "bottom" in z.files else None
+        net = cls(tuple(int(s) for s in z["schedule"]), int(z["n_linear"]), device, homogeneous=hom, symmetric=sym,
+                  bottom=bottom)
         net.params.copy_(torch.from_numpy(z["params"]).to(torch.float32))
         net.refresh()
         return net
 
     @property
     def n_levels(self) -> int:
-        return max(self.schedule) + 1
+        return max(max(self.schedule), self.bottom) + 1
 
     # ---- forward -----------------------------------------------------------
     def plan(self, hier: Hierarchy) -> "Program":
@@ -238,6 +244,13 @@ class Program:
         self.enc_out = h
         skips: Dict[int, int] = {}
         for t, lev in enumerate(net.schedule):
+            if t == net.turn:
+                lev_down = max(lev, net.bottom)  # pooled-only levels below the last layer
+                while cur < lev_down:
+                    skips[cur] = h
+                    o = new(cur + 1)
+                    self.ops.append(("pool", cur, h, o, None))
+                    h, cur = o, cur + 1
             while cur < lev:
                 skips[cur] = h
                 o = new(cur + 1)
@@ -360,11 +373,12 @@ class Program:
 
     # ---- training ------------------------------------------------------------
     def loss_and_grad(self, r: torch.Tensor, lk: torch.Tensor, grad: torch.Tensor,
-                      e: Optional[torch.Tensor] = None, gy_scale=None) -> torch.Tensor:
-        """Forward, energy loss and backward for one residual; accumulates the
+                      e: Optional[torch.Tensor] = None, gy_scale=None, loss: str = "energy") -> torch.Tensor:
+        """Forward, loss and backward for one residual; accumulates the
         parameter gradient of ``gy_scale * L`` into ``grad`` (``gy_scale`` a
-        float or a device scalar; default 1).  Returns the device sums
-        [y.A^y, y.r~, e~.r~] as a 3-vector."""
+        float or a device scalar; default 1).  ``loss``: ``"energy"``
+        ``(y.A^y/2 - y.r~)/n`` or ``"cosine"`` ``1 - (y.r~)^2/((y.A^y)(e~.r~))``
+        (needs ``e``).  Returns the device sums [y.A^y, y.r~, e~.r~]."""
         L = _lib.lib()
         net, hier = self.net, self.hier
         st = _lib.stream_of(hier.device)
@@ -377,6 +391,12 @@ class Program:
         check(L.tm_gnn_energy_loss(self.n0, ptr(lv0.row_ptr), ptr(lv0.col), ptr(lv0.vn), ptr(self.y), ptr(r),
                                    ptr(lv0.d), ptr(self.scale), ptr(e), ptr(gy), ptr(part), ctypes.byref(npc), st))
         sums = part.view(3, 1024)[:, :npc.value].sum(1)
+        if loss == "cosine":
+            assert e is not None, "the cosine loss needs the error"
+            check(L.tm_gnn_cosine_grad(self.n0, ptr(r), ptr(lv0.d), ptr(self.scale), ptr(part), npc.value, ptr(gy),
+                                       st))
+        else:
+            assert loss == "energy", f"unknown loss {loss}"
         if gy_scale is not None:
             gy.mul_(gy_scale)
         if net._part is None:

@@ -74,10 +74,15 @@ def make_samples(prob: PoissonCube, seeds: Sequence[int], snapshots: Sequence[in
 
 
 def train(prob: PoissonCube, net: GNNPreconditioner, data: Dict, steps: int, lr: float = 2e-3,
-          seed: int = 0, log_every: int = 100, time_limit: float = 1e30, noise_residuals: int = 1) -> List[float]:
+          seed: int = 0, log_every: int = 100, time_limit: float = 1e30, noise_residuals: int = 1,
+          loss: str = "cosine") -> List[float]:
     """``noise_residuals`` extra residuals per step are ``r = A e`` for white
     noise ``e`` (their exact error is ``e``), so the loss covers the rough end
-    of the spectrum that plain-solver residuals leave out."""
+    of the spectrum that plain-solver residuals leave out.  ``loss``:
+    ``"cosine"`` (default) is the A-norm error left after the optimal step
+    along the correction, ``1 - (y.r~)^2 / ((y.A^y)(e~.r~))`` - invariant to the
+    scale of M, as flexible CG is; ``"energy"`` is the relative energy error
+    of the correction itself."""
     prog = net.plan(prob.hier)
     opt = Adam(net, lr=lr)
     grad = torch.zeros_like(net.params)
@@ -105,17 +110,23 @@ def train(prob: PoissonCube, net: GNNPreconditioner, data: Dict, steps: int, lr:
             # normaliser e~.r~ = e.r / s^2, s^2 = mean (d r)^2; scale g_y by 2 n / (e~.r~)
             s2 = torch.mean((d0.d.to(torch.float64) * r) ** 2)
             er = torch.dot(e, r) / s2
-            sm = prog.loss_and_grad(r, lk, grad, e, gy_scale=(2.0 * n) / er)
+            if loss == "cosine":
+                sm = prog.loss_and_grad(r, lk, grad, e, loss="cosine")
+            else:
+                sm = prog.loss_and_grad(r, lk, grad, e, gy_scale=(2.0 * n) / er)
             sums.append(sm)
         frac = step / max(1, steps - 1)
         opt.step(grad, lr=lr * 0.5 * (1.0 + math.cos(math.pi * frac)))
         S = torch.stack(sums)
-        window.append((S[:, 0] - 2.0 * S[:, 1] + S[:, 2]) / S[:, 2])
+        if loss == "cosine":
+            window.append(1.0 - S[:, 1] ** 2 / (S[:, 0] * S[:, 2]))
+        else:
+            window.append((S[:, 0] - 2.0 * S[:, 1] + S[:, 2]) / S[:, 2])
         if (step + 1) % log_every == 0 or step == steps - 1:
             E = torch.cat(window).mean().item()
             hist.append(E)
             window = []
-            log(f"step {step + 1}/{steps}: relative energy error {E:.4f} ({time.perf_counter() - t0:.0f}s)")
+            log(f"step {step + 1}/{steps}: {loss} loss {E:.4f} ({time.perf_counter() - t0:.0f}s)")
             if time.perf_counter() - t0 > time_limit:
                 log("time limit reached")
                 break
@@ -161,9 +172,11 @@ def main(argv=None):
     ap.add_argument("--nonsymmetric", action="store_true",
                     help="independent up-path layers and decoder (the operator is then not SPD)")
     ap.add_argument("--n-linear", type=int, default=64)
+    ap.add_argument("--bottom", type=int, default=-1, help="deepest pooled level (default: deepest layer level)")
     ap.add_argument("--general", action="store_true",
                     help="train biases and the log-coefficient input too (not homogeneous in r)")
     ap.add_argument("--noise-residuals", type=int, default=1)
+    ap.add_argument("--loss", choices=["cosine", "energy"], default="cosine")
     ap.add_argument("--levels", type=int, default=0, help="hierarchy levels (default: what the schedule needs)")
     ap.add_argument("--chunk", type=int, default=8)
     ap.add_argument("--fcg-m", type=str, default="4", help="directions kept by flexible CG (comma list to compare)")
@@ -175,19 +188,21 @@ def main(argv=None):
     dev = torch.device("cuda:0")
     schedule = tuple(int(s) for s in args.schedule.split(","))
     t0 = time.perf_counter()
-    prob = PoissonCube(args.n, dev, n_levels=max(args.levels, max(schedule) + 1))
+    prob = PoissonCube(args.n, dev, n_levels=max(args.levels, max(schedule) + 1, args.bottom + 1))
     log(f"{args.n}^3: {prob.n_inner} unknowns, levels {[l.n for l in prob.hier.levels]} "
         f"({time.perf_counter() - t0:.1f}s)")
     net = (GNNPreconditioner.load(args.init, dev) if args.init
            else GNNPreconditioner(schedule, args.n_linear, dev, homogeneous=not args.general,
-                                  symmetric=not (args.nonsymmetric or args.general)))
+                                  symmetric=not (args.nonsymmetric or args.general),
+                                  bottom=None if args.bottom < 0 else args.bottom))
     report = {"n": args.n, "schedule": list(net.schedule), "n_linear": net.n_linear,
-              "homogeneous": net.homogeneous, "symmetric": net.symmetric}
+              "homogeneous": net.homogeneous, "symmetric": net.symmetric, "bottom": net.bottom}
     if args.train > 0 and args.steps > 0:
         data = make_samples(prob, list(range(1000, 1000 + args.train)))
         report["train_median_jacobi_iters"] = float(np.median(data["jacobi_iters"]))
+        report["loss"] = args.loss
         report["loss_history"] = train(prob, net, data, args.steps, lr=args.lr, time_limit=args.time_limit,
-                                       noise_residuals=args.noise_residuals)
+                                       noise_residuals=args.noise_residuals, loss=args.loss)
         del data
         if args.out:
             net.save(args.out)

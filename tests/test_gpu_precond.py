@@ -149,6 +149,28 @@ def test_network_gradient(cuda):
     assert np.linalg.norm(g - g_ref) <= 2e-2 * np.linalg.norm(g_ref)
 
 
+def test_cosine_loss_gradient(cuda):
+    """Scale-invariant loss: device gradient vs the oracle backward."""
+    print("case precond cosine gradient n=8", flush=True)
+    Ai, H, hier, levels, values, csr, lk = _problem(8, 11, cuda)
+    net = GNNPreconditioner(device=cuda)
+    net.params.add_(torch.randn(net.n_params, generator=torch.Generator().manual_seed(5)).to(cuda) * 0.05 * net.mask)
+    net.refresh()
+    prog = net.plan(hier)
+    e = np.random.default_rng(6).standard_normal(Ai.shape[0])
+    r = Ai @ e
+    zl = np.zeros_like(r)
+    grad = torch.zeros_like(net.params)
+    prog.loss_and_grad(torch.tensor(r, device=cuda), torch.zeros(len(r), dtype=torch.float32, device=cuda), grad,
+                       torch.tensor(e, device=cuda), loss="cosine")
+    p = _params64(net)
+    z, y, s, tape = og.forward(levels, p, net.schedule, net.n_linear, r, zl, bf16=True)
+    g_ref = og.backward(levels, p, net.schedule, net.n_linear, r, zl, tape, s, bf16=True,
+                        gy=og.cosine_grad_y(levels, y, r, s, e))
+    g = grad.double().cpu().numpy()
+    assert np.linalg.norm(g - g_ref) <= 3e-2 * np.linalg.norm(g_ref)
+
+
 @pytest.mark.parametrize("graph", [True, False])
 def test_fcg_untrained_additive_multilevel(cuda, graph):
     """The untrained network is an SPD additive multilevel preconditioner: FCG

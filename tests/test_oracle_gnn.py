@@ -175,3 +175,30 @@ def test_fcg_reduces_to_pcg_and_direct():
     Ad = np.linalg.inv(Ai.toarray())
     x1, it1, _ = og.fcg(Ai, b, lambda r: Ad @ r, rtol=1e-10)
     assert it1 == 1
+
+
+def test_cosine_loss_gradient_matches_finite_differences():
+    coords, Ai = _poisson(6, seed=8)
+    H = build_host_levels(coords, Ai.indptr, Ai.indices, 4)
+    levels = og.galerkin_levels(Ai, [h.agg for h in H[:-1]])
+    net = GNNPreconditioner(device=torch.device("cpu"))
+    rng = np.random.default_rng(4)
+    p = net.params.numpy().astype(np.float64) + rng.standard_normal(net.n_params) * 0.05 * net.mask.numpy()
+    e = rng.standard_normal(Ai.shape[0])
+    r = Ai @ e
+    lk = np.zeros_like(r)
+    z, y, s, tape = og.forward(levels, p, net.schedule, net.n_linear, r, lk)
+    g = og.backward(levels, p, net.schedule, net.n_linear, r, lk, tape, s,
+                    gy=og.cosine_grad_y(levels, y, r, s, e))
+    L0 = og.cosine_loss(levels, y, r, s, e)
+    assert 0.0 <= L0 <= 1.0
+    h = 1e-6
+    gmax = np.abs(g).max()
+    for i in np.concatenate([rng.choice(np.nonzero(net.mask.numpy())[0], 12, replace=False), [0, 5]]):
+        vals = []
+        for sgn in (1, -1):
+            q = p.copy()
+            q[i] += sgn * h
+            vals.append(og.cosine_loss(levels, og.forward(levels, q, net.schedule, net.n_linear, r, lk)[1], r, s, e))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        assert abs(g[i] - fd) <= 1e-5 * abs(fd) + 1e-7 * gmax, (i, g[i], fd)
