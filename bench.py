@@ -151,8 +151,11 @@ def batched_leg(n_mesh: int, log_fn):
     bp.kappa.copy_(torch.tensor(np.concatenate([m[3] for m in ms]), dtype=torch.float32, device=dev))
     t_setup = time.perf_counter() - t0
     log_fn(f"batched: {n_mesh} meshes, {bp.n_elements} elements, {bp.n_inner} unknowns ({t_setup:.1f}s set-up)")
-    cfg = autotune_batched(bp, use_cache=False)
-    g = bp.capture()
+    space = {"rows_per_tile": (1024, 2048), "lanes": (4, 8), "blocks_per_cu": (1, 2)}
+    log_fn("batched: autotuning " + ", ".join(f"{k} {v}" for k, v in space.items()))
+    cfg = autotune_batched(bp, space=space, use_cache=False, maxiter=10000, verbose=True)
+    log_fn(f"batched: tuned {[(k, cfg[k]) for k in ('rows_per_tile', 'lanes', 'blocks')]}")
+    g = bp.capture(maxiter=10000)
     st = torch.cuda.current_stream(dev)
     g.replay()
     torch.cuda.synchronize(dev)
@@ -166,6 +169,7 @@ def batched_leg(n_mesh: int, log_fn):
         times.append(ev0.elapsed_time(ev1) / 1e3)
     _, status, iters, rel = bp.results()
     t_graph = sorted(times)[len(times) // 2]
+    log_fn(f"batched: graph {t_graph * 1e3:.1f} ms, converged {int((status == 1).sum())}/{len(status)}")
     # the same condensed systems, one device PCG call each
     rp = bp.rp_in.cpu().numpy()
     torch.cuda.synchronize(dev)

@@ -253,8 +253,10 @@ int tm_gnn_pool(int64_t n_coarse, const int32_t* child_ptr, const int32_t* child
 /* out[i] = skip[i] + w_i hc[agg_i] (skip nullable; out may alias skip). */
 int tm_gnn_unpool(int64_t n_fine, const int32_t* agg, const float* w, const void* hc, const void* skip, void* out,
                   int dtype, const int* stop, void* stream);
-/* Per-workgroup partials of sum (d_i r_i)^2 (for the input scale). */
+/* Per-workgroup partials of sum (d_i r_i)^2 (for the input scale), and the
+ * scale s = sqrt(sum / n) from them. */
 int tm_gnn_sumsq(int64_t n, const double* r, const float* d, double* ss_part, int* nparts, void* stream);
+int tm_gnn_scale(int64_t n, const double* ss_part, int nparts, double* scale, void* stream);
 /* Linear encoder h0 = [d r / s, lk] We + be, s = rms(d r) from the partials;
  * writes s.  We is [2][64], be [64]. */
 int tm_gnn_encode(int64_t n, const double* r, const float* d, const float* lk, const double* ss_part, int nparts,
@@ -291,6 +293,22 @@ int tm_gnn_unpool_decode(int64_t n, int64_t n_coarse, const float* xs, const int
                          const uint16_t* hc, const float* we, const float* be, const float* wd, const float* bd,
                          const float* d, const double* scale, float* coarse_dot, double* z, const int* stop,
                          void* stream);
+/* Symmetric network with a layer pair on level 0 (inference).  The level-0
+ * features of the rank-one encoder output stay in the span of x = d r / s and
+ * A^x, so the finest level reduces to scalar CSR products:
+ *  - tm_gnn_sym0_down writes xs = (x, A^x) from dr = d r [n] (FP32; with
+ *    dr_ready = 0 it is formed from r first, otherwise it is the dr of the
+ *    flexible-CG view); tm_gnn_encode_pool with r = NULL,
+ *    nparts = 0, we = [a; b], be = 0 then pools x (x) a + (A^x) (x) b;
+ *  - tm_gnn_sym0_up takes the level-1 up-path features hc, the vectors a, b and
+ *    c3 = (a.a, a.b, b.b), and writes z; ua, ub [n_coarse] and t [n] are scratch. */
+int tm_gnn_sym0_down(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values_normalized,
+                     const double* r, const float* d, float* dr, int dr_ready, const double* scale, float* xs,
+                     const int* stop, void* stream);
+int tm_gnn_sym0_up(int64_t n, int64_t n_coarse, const int32_t* row_ptr, const int32_t* col_idx,
+                   const float* values_normalized, const float* xs, const int32_t* agg, const float* w,
+                   const uint16_t* hc, const float* va, const float* vb, const float* c3, const float* d,
+                   const double* scale, float* ua, float* ub, float* t, double* z, const int* stop, void* stream);
 /* Finest-level prolongation B = A^ P^: b_values[s] = sum of A^_k w_col(k) over
  * the entries k in perm[seg[s] .. seg[s+1]).  Row sums of a CSR. */
 int tm_gnn_level_prolong(int64_t nnz_b, const int32_t* seg, const int32_t* perm, const int32_t* col_idx,
@@ -342,6 +360,7 @@ typedef struct {
   int32_t* iter;
   int32_t nparts;
   int32_t reserved;
+  float* dr; /* d r in FP32, refreshed with r by the start and update kernels */
 } tm_fcg_view;
 int64_t tm_fcg_work_size(int64_t n, int m);
 int tm_fcg_layout(int64_t n, int m, void* work, tm_fcg_view* view);

@@ -49,7 +49,7 @@ class BatchedArgs(ctypes.Structure):
 class FcgView(ctypes.Structure):
     _fields_ = [("r", c_void_p), ("z", c_void_p), ("ss_part", c_void_p), ("scale", c_void_p),
                 ("relres", c_void_p), ("status", c_void_p), ("iter", c_void_p), ("nparts", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("reserved", ctypes.c_int32), ("dr", c_void_p)]
 
 
 class TMError(RuntimeError):
@@ -100,6 +100,7 @@ _SIGS = {
     "tm_gnn_pool": (c_int, [c_i64] + [c_void_p] * 5 + [c_int, c_void_p, c_void_p]),
     "tm_gnn_unpool": (c_int, [c_i64] + [c_void_p] * 5 + [c_int, c_void_p, c_void_p]),
     "tm_gnn_sumsq": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_int), c_void_p]),
+    "tm_gnn_scale": (c_int, [c_i64, c_void_p, c_int, c_void_p, c_void_p]),
     "tm_gnn_encode": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_void_p, c_int] + [c_void_p] * 6),
     "tm_gnn_encode_backward": (c_int, [c_i64] + [c_void_p] * 8),
     "tm_gnn_decode": (c_int, [c_i64] + [c_void_p] * 9),
@@ -107,6 +108,8 @@ _SIGS = {
     "tm_gnn_final": (c_int, [c_i64] + [c_void_p] * 15 + [c_int] + [c_void_p] * 7),
     "tm_gnn_level_prolong": (c_int, [c_i64] + [c_void_p] * 7),
     "tm_gnn_unpool_decode": (c_int, [c_i64, c_i64] + [c_void_p] * 14),
+    "tm_gnn_sym0_down": (c_int, [c_i64] + [c_void_p] * 6 + [c_int] + [c_void_p] * 4),
+    "tm_gnn_sym0_up": (c_int, [c_i64, c_i64] + [c_void_p] * 18),
     "tm_gnn_row_sums": (c_int, [c_i64] + [c_void_p] * 4),
     "tm_gnn_decode_backward": (c_int, [c_i64] + [c_void_p] * 7),
     "tm_gnn_energy_loss": (c_int, [c_i64] + [c_void_p] * 10 + [ctypes.POINTER(c_int), c_void_p]),

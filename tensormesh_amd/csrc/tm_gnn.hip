@@ -606,6 +606,13 @@ __device__ __forceinline__ double scale_from_parts(const double* ss_part, int np
   return s > 0.0 ? s : 1.0;
 }
 
+// s = sqrt(sum of the partials / n) into *s_out (single workgroup)
+__global__ __launch_bounds__(NT) void k_scale(int64_t n, const double* __restrict__ ss_part, int nparts,
+                                              double* __restrict__ s_out) {
+  const double s = scale_from_parts(ss_part, nparts, n);
+  if (threadIdx.x == 0) *s_out = s;
+}
+
 __global__ __launch_bounds__(NT) void k_sumsq(int64_t n, const double* __restrict__ r, const float* __restrict__ d,
                                               double* __restrict__ ss_part) {
   double loc[1] = {0.0};
@@ -660,8 +667,17 @@ __global__ __launch_bounds__(NT) void k_encode_pool(int64_t n, int64_t nc, const
     if (I < nc)
       for (int k = cp[I] + q; k < cp[I + 1]; k += 8) {
         const int i = child[k];
-        const float x0 = (float)((double)d[i] * r[i]) * inv, x1 = lk ? lk[i] : 0.0f, wi = w[i];
-        xs[i] = float2{x0, x1};
+        float x0, x1;
+        if (r) {
+          x0 = (float)((double)d[i] * r[i]) * inv;
+          x1 = lk ? lk[i] : 0.0f;
+          xs[i] = float2{x0, x1};
+        } else {  // inputs already in xs (k_sym0_down)
+          const float2 v = xs[i];
+          x0 = v.x;
+          x1 = v.y;
+        }
+        const float wi = w[i];
         s0 += wi * x0;
         s1 += wi * x1;
         sw += wi;
@@ -800,6 +816,137 @@ __global__ __launch_bounds__(NT) void k_unpool_decode(int64_t n, const float2* _
   for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
     const float2 xi = xs[i];
     z[i] = s * (double)d[i] * (double)(xi.x * c0 + xi.y * c1 + cb + w[i] * u[agg[i]]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Symmetric network with a layer pair on level 0, at inference.
+// The encoder output x (x) we is rank one, so the level-0 layer
+// L0(h) = h + h W_h + (A^ h) W_n maps it to  x (x) a + (A^x) (x) b  with
+// a = we + we W_h, b = we W_n, and the decoder of the adjoint layer reads
+//   y_i = u_i . a + (A^ (u . b))_i,   u = x (x) a + (A^x) (x) b + w hc[agg]
+// (hc: level-1 features of the up path).  So the whole finest level is two
+// scalar CSR products and a few per-node scalars:
+//   k_sym0_down : xs_i = (x_i, (A^x)_i),  x = d r / s
+//   (k_encode_pool with xs as input pools x (x) a + (A^x) (x) b)
+//   k_row_dot2  : ua[J] = hc[J] . a,  ub[J] = hc[J] . b
+//   k_sym0_up   : t_j = x_j ab + (A^x)_j bb + w_j ub[agg_j];
+//                 y_i = x_i aa + (A^x)_i ab + w_i ua[agg_i] + (A^ t)_i,  z = s d y
+// where aa = a.a, ab = a.b, bb = b.b.
+// ---------------------------------------------------------------------------
+// (A^ v)_i for float v on 4-lane groups with the unrolled row loop
+template <int S>
+__device__ __forceinline__ float row_dot_f(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                           const float* __restrict__ av, const float* __restrict__ v, int64_t row,
+                                           int sl) {
+  const int b = rp[row], e = rp[row + 1];
+  float acc = 0.0f;
+  int k = b + sl;
+  for (; k + 3 * S < e; k += 4 * S) {
+    int c[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) c[u] = ci[k + u * S];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc += av[k + u * S] * v[c[u]];
+  }
+  for (; k < e; k += S) acc += av[k] * v[ci[k]];
+#pragma unroll
+  for (int o = S / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kWave);
+  return acc;
+}
+
+// xs_i = (x_i, (A^x)_i), x = d r / s from dr = d r (FP32, written by the
+// flexible-CG update) or, without it, from r and d
+__global__ __launch_bounds__(NT) void k_sym0_dr(int64_t n, const double* __restrict__ r, const float* __restrict__ d,
+                                                float* __restrict__ dr, const int* stop) {
+  if (stopped(stop)) return;
+  for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
+    dr[i] = (float)((double)d[i] * r[i]);
+}
+
+// xs_i = (x_i, (A^x)_i), x = d r / s, from dr = d r (FP32)
+template <int S>
+__global__ __launch_bounds__(NT) void k_sym0_down(int64_t n, const int32_t* __restrict__ rp,
+                                                  const int32_t* __restrict__ ci, const float* __restrict__ av,
+                                                  const float* __restrict__ dr, const double* __restrict__ s_in,
+                                                  float2* __restrict__ xs, const int* stop) {
+  if (stopped(stop)) return;
+  const float inv = (float)(1.0 / *s_in);
+  const int sl = threadIdx.x % S;
+  const tmx::RowChunk rc = tmx::row_chunk(n);
+  for (int64_t i = rc.r0 + threadIdx.x / S; i < rc.r1; i += NT / S) {
+    const float acc = row_dot_f<S>(rp, ci, av, dr, i, sl);
+    if (sl == 0) xs[i] = float2{dr[i] * inv, acc * inv};
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_row_dot2(int64_t nc, const uint16_t* __restrict__ hc,
+                                                 const float* __restrict__ va, const float* __restrict__ vb,
+                                                 float* __restrict__ ua, float* __restrict__ ub, const int* stop) {
+  if (stopped(stop)) return;
+  const int q = threadIdx.x % 8, f0 = q * 8;
+  float la[8], lb[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    la[t] = va[f0 + t];
+    lb[t] = vb[f0 + t];
+  }
+  for (int64_t J0 = blockIdx.x * (int64_t)(NT / 8); J0 < nc; J0 += (int64_t)gridDim.x * (NT / 8)) {
+    const int64_t J = J0 + threadIdx.x / 8;
+    float sa = 0.0f, sb = 0.0f;
+    if (J < nc) {
+      float v[8];
+      Vec8<uint16_t>::load(&hc[J * F + f0], v);
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        sa += v[t] * la[t];
+        sb += v[t] * lb[t];
+      }
+    }
+#pragma unroll
+    for (int o = 4; o > 0; o >>= 1) {
+      sa += __shfl_xor(sa, o, kWave);
+      sb += __shfl_xor(sb, o, kWave);
+    }
+    if (q == 0 && J < nc) {
+      ua[J] = sa;
+      ub[J] = sb;
+    }
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_sym0_t(int64_t n, const float2* __restrict__ xs,
+                                               const int32_t* __restrict__ agg, const float* __restrict__ w,
+                                               const float* __restrict__ ub, const float* __restrict__ c3,
+                                               float* __restrict__ t, const int* stop) {
+  if (stopped(stop)) return;
+  const float ab = c3[1], bb = c3[2];
+  for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+    const float2 x = xs[i];
+    t[i] = x.x * ab + x.y * bb + w[i] * ub[agg[i]];
+  }
+}
+
+template <int S>
+__global__ __launch_bounds__(NT) void k_sym0_up(int64_t n, const int32_t* __restrict__ rp,
+                                                const int32_t* __restrict__ ci, const float* __restrict__ av,
+                                                const float2* __restrict__ xs, const int32_t* __restrict__ agg,
+                                                const float* __restrict__ w, const float* __restrict__ ua,
+                                                const float* __restrict__ t, const float* __restrict__ c3,
+                                                const float* __restrict__ d, const double* __restrict__ s_in,
+                                                double* __restrict__ z, const int* stop) {
+  if (stopped(stop)) return;
+  const float aa = c3[0], ab = c3[1];
+  const double s = *s_in;
+  const int sl = threadIdx.x % S;
+  const tmx::RowChunk rc = tmx::row_chunk(n);
+  for (int64_t i = rc.r0 + threadIdx.x / S; i < rc.r1; i += NT / S) {
+    const float acc = row_dot_f<S>(rp, ci, av, t, i, sl);
+    if (sl == 0) {
+      const float2 x = xs[i];
+      const float y = x.x * aa + x.y * ab + w[i] * ua[agg[i]] + acc;
+      z[i] = s * (double)d[i] * (double)y;
+    }
   }
 }
 
@@ -991,6 +1138,7 @@ constexpr int kFcgSlots = S_SS + 1;
 
 struct FcgWork {
   double *r, *z, *w, *P, *Q, *part;  // P, Q: [m][n]
+  float* dr;                         // d r in FP32 (input of a finest-level CSR product)
   FcgScalars* s;
   FcgFlags* f;
   int m;
@@ -1007,6 +1155,7 @@ FcgWork fcg_carve(void* work, int64_t n, int m) {
   w.part = w.Q + (int64_t)m * n;
   w.s = (FcgScalars*)(w.part + kFcgSlots * kMaxParts);
   w.f = (FcgFlags*)(w.s + 1);
+  w.dr = (float*)(w.f + 1);
   w.m = m;
   return w;
 }
@@ -1044,7 +1193,8 @@ __global__ __launch_bounds__(NTF) void k_fcg_spmv(int64_t n, const int32_t* __re
 #pragma unroll
           for (int j = 0; j < M; ++j) {
             loc[j] += zi * w.Q[(int64_t)j * n + i];
-            loc[M + j] += w.P[(int64_t)j * n + i] * ri;
+            // with one direction (p_old, r) = 0: alpha made the new residual orthogonal to it
+            if (M > 1) loc[M + j] += w.P[(int64_t)j * n + i] * ri;
           }
           loc[2 * M] += zi * ri;
           loc[2 * M + 1] += zi * o[r][0];
@@ -1081,6 +1231,7 @@ __global__ __launch_bounds__(NTF) void k_fcg_init(int64_t n, const double* __res
     loc[0] += ri * ri;
     loc[1] += b[i] * b[i];
     const double dr = (double)d[i] * ri;
+    w.dr[i] = (float)dr;
     loc[2] += dr * dr;
   }
   const int which[3] = {S_RR, S_BB, S_SS};
@@ -1162,6 +1313,7 @@ __global__ __launch_bounds__(NTF) void k_fcg_update(int64_t n, double* __restric
       w.r[i] = ri;
       loc[0] += ri * ri;
       const double dr = (double)d[i] * ri;
+      w.dr[i] = (float)dr;
       loc[1] += dr * dr;
     }
   }
@@ -1248,9 +1400,10 @@ extern "C" int tm_gnn_encode_pool(int64_t n, int64_t n_coarse, const int32_t* ch
                                   const float* w, const double* r, const float* d, const float* lk,
                                   const double* ss_part, int nparts, const float* we, const float* be, uint16_t* out,
                                   float* xs, double* scale, const int* stop, void* stream) {
-  TM_REQUIRE(n >= 0 && n_coarse >= 0 && child_ptr && child && w && r && d && ss_part && we && be && out && xs &&
-                 scale,
+  TM_REQUIRE(n >= 0 && n_coarse >= 0 && child_ptr && child && w && d && we && be && out && xs && scale,
              TM_ERR_ARG, "null argument");
+  TM_REQUIRE(r || nparts == 0, TM_ERR_ARG, "encode_pool: without r the scale must already be stored (nparts 0)");
+  TM_REQUIRE(nparts == 0 || ss_part, TM_ERR_ARG, "encode_pool: partials missing");
   TM_REQUIRE(nparts >= 0 && nparts <= kMaxParts, TM_ERR_ARG, "encode_pool: bad partial count %d", nparts);
   const int g = grid_of(n_coarse, NT / 8, 8192);
   hipLaunchKernelGGL(k_encode_pool, dim3(g), dim3(NT), 0, (hipStream_t)stream, n, n_coarse, child_ptr, child, w, r,
@@ -1292,6 +1445,42 @@ extern "C" int tm_gnn_unpool_decode(int64_t n, int64_t n_coarse, const float* xs
                        coarse_dot, stop);
   hipLaunchKernelGGL(k_unpool_decode, dim3(grid_of(n, NT, 8192)), dim3(NT), 0, st, n, (const float2*)xs, agg, w,
                      coarse_dot, we, be, wd, bd, d, scale, z, stop);
+  TM_HIP(hipGetLastError());
+  return TM_OK;
+}
+
+extern "C" int tm_gnn_sym0_down(int64_t n, const int32_t* row_ptr, const int32_t* col_idx,
+                                const float* values_normalized, const double* r, const float* d, float* dr,
+                                int dr_ready, const double* scale, float* xs, const int* stop, void* stream) {
+  TM_REQUIRE(n >= 0 && row_ptr && col_idx && values_normalized && d && dr && scale && xs, TM_ERR_ARG,
+             "null argument");
+  TM_REQUIRE(dr_ready || r, TM_ERR_ARG, "sym0_down: r needed when dr is not ready");
+  if (n == 0) return TM_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (!dr_ready)
+    hipLaunchKernelGGL(k_sym0_dr, dim3(grid_of(n, NT, 8192)), dim3(NT), 0, st, n, r, d, dr, stop);
+  hipLaunchKernelGGL(k_sym0_down<4>, dim3(grid_of(n, NT / 4 * 4, 4096)), dim3(NT), 0, st, n, row_ptr, col_idx,
+                     values_normalized, dr, scale, (float2*)xs, stop);
+  TM_HIP(hipGetLastError());
+  return TM_OK;
+}
+
+extern "C" int tm_gnn_sym0_up(int64_t n, int64_t n_coarse, const int32_t* row_ptr, const int32_t* col_idx,
+                              const float* values_normalized, const float* xs, const int32_t* agg, const float* w,
+                              const uint16_t* hc, const float* va, const float* vb, const float* c3,
+                              const float* d, const double* scale, float* ua, float* ub, float* t, double* z,
+                              const int* stop, void* stream) {
+  TM_REQUIRE(n >= 0 && n_coarse >= 0 && row_ptr && col_idx && values_normalized && xs && agg && w && hc && va &&
+                 vb && c3 && d && scale && ua && ub && t && z,
+             TM_ERR_ARG, "null argument");
+  if (n == 0) return TM_OK;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_row_dot2, dim3(grid_of(n_coarse, NT / 8, 8192)), dim3(NT), 0, st, n_coarse, hc, va, vb, ua,
+                     ub, stop);
+  hipLaunchKernelGGL(k_sym0_t, dim3(grid_of(n, NT, 8192)), dim3(NT), 0, st, n, (const float2*)xs, agg, w, ub, c3, t,
+                     stop);
+  hipLaunchKernelGGL(k_sym0_up<4>, dim3(grid_of(n, NT / 4 * 4, 4096)), dim3(NT), 0, st, n, row_ptr, col_idx,
+                     values_normalized, (const float2*)xs, agg, w, ua, t, c3, d, scale, z, stop);
   TM_HIP(hipGetLastError());
   return TM_OK;
 }
@@ -1377,6 +1566,13 @@ extern "C" int tm_gnn_sumsq(int64_t n, const double* r, const float* d, double* 
   const int g = grid_of(n, NT * 4, kMaxParts);
   *nparts = g;
   hipLaunchKernelGGL(k_sumsq, dim3(g), dim3(NT), 0, (hipStream_t)stream, n, r, d, ss_part);
+  TM_HIP(hipGetLastError());
+  return TM_OK;
+}
+
+extern "C" int tm_gnn_scale(int64_t n, const double* ss_part, int nparts, double* scale, void* stream) {
+  TM_REQUIRE(n >= 0 && ss_part && scale && nparts >= 1 && nparts <= kMaxParts, TM_ERR_ARG, "bad scale arguments");
+  hipLaunchKernelGGL(k_scale, dim3(1), dim3(NT), 0, (hipStream_t)stream, n, ss_part, nparts, scale);
   TM_HIP(hipGetLastError());
   return TM_OK;
 }
@@ -1495,7 +1691,7 @@ extern "C" int tm_gnn_adam(int64_t n, float* param, const float* grad, float* m,
 extern "C" int64_t tm_fcg_work_size(int64_t n, int m) {
   if (!(m == 1 || m == 2 || m == 4 || m == 8)) return -1;
   return (int64_t)sizeof(double) * ((3 + 2 * (int64_t)m) * n + kFcgSlots * kMaxParts) +
-         (int64_t)sizeof(FcgScalars) + (int64_t)sizeof(FcgFlags) + 64;
+         (int64_t)sizeof(FcgScalars) + (int64_t)sizeof(FcgFlags) + (int64_t)sizeof(float) * n + 64;
 }
 
 extern "C" int tm_fcg_layout(int64_t n, int m, void* work, tm_fcg_view* view) {
@@ -1511,6 +1707,7 @@ extern "C" int tm_fcg_layout(int64_t n, int m, void* work, tm_fcg_view* view) {
   view->iter = &w.f->iter;
   view->nparts = fcg_grid(n);
   view->reserved = 0;
+  view->dr = w.dr;
   return TM_OK;
 }
 

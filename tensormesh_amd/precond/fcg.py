@@ -72,9 +72,9 @@ class LearnedFCG:
         v = self.view
         st = _lib.stream_of(self.dev)
         # the flexible-CG kernels keep s = rms(d r) at v.scale; the fused encoder reads it (nparts 0)
-        fused_scale = self.fused and self.prog.fusable_plain
+        fused_scale = self.fused and (self.prog.fusable_plain or self.prog.fusable_sym0)
         self.prog.forward(v.r, self.lk, ss_part=v.ss_part, nparts=0 if fused_scale else v.nparts, z=v.z,
-                          scale=v.scale, stop=v.status, fused=self.fused)
+                          scale=v.scale, stop=v.status, fused=self.fused, dr=v.dr)
         check(L.tm_fcg_iterate(self.n, self.m, ptr(self.csr.row_ptr), ptr(self.csr.col_idx), ptr(self.values),
                                ptr(self.x), ptr(self.hier.levels[0].d), ptr(self.work), st))
 

@@ -178,6 +178,17 @@ class GNNPreconditioner:
             wb = W.to(torch.bfloat16)
             self.w.append(wb.contiguous())
             self.wt.append(wb.t().contiguous())
+        # level-0 layer of a symmetric network on the rank-one encoder output:
+        # L0(x (x) we) = x (x) a + (A^x) (x) b
+        self.sym0 = None
+        if self.symmetric and self.schedule[0] == 0:
+            we = self.enc()[0][:F].double()
+            W0 = self.w[0].double()
+            a = we + we @ W0[:F]
+            b = we @ W0[F:]
+            self.sym0 = {"ab": torch.cat([a, b]).float().contiguous(),
+                         "zero": torch.zeros(F, dtype=torch.float32, device=self.device),
+                         "c3": torch.stack([a @ a, a @ b, b @ b]).float().contiguous()}
 
     def fold(self, grad: torch.Tensor):
         """Move the gradient of derived weights onto the weights they derive from."""
@@ -284,8 +295,20 @@ class Program:
         self.z = torch.zeros(self.n0, dtype=torch.float64, device=dev)
         self.scale = torch.zeros(1, dtype=torch.float64, device=dev)
         self.ss_part = torch.zeros(1024, dtype=torch.float64, device=dev)
+        # symmetric network with its first and last layer on level 0 (rank-one encoder)
+        self.fusable_sym0 = (net.symmetric and net.homogeneous and len(ops) >= 4 and ops[0][0] == "mp"
+                             and ops[0][1] == 0 and ops[0][2] == self.enc_out and ops[1][0] == "pool"
+                             and ops[1][1] == 0 and ops[1][2] == ops[0][3] and ops[-2][0] == "unpool"
+                             and ops[-2][1] == 0 and ops[-2][4] == ops[0][3] and ops[-1][0] == "mp"
+                             and ops[-1][1] == 0 and ops[-1][3] == self.final)
         self.xs = (torch.zeros((self.n0, 2), dtype=torch.float32, device=dev)
-                   if (self.fusable or self.fusable_plain) else None)
+                   if (self.fusable or self.fusable_plain or self.fusable_sym0) else None)
+        if self.fusable_sym0:
+            n1 = hier.levels[1].n
+            self.sym_ua = torch.zeros(n1, dtype=torch.float32, device=dev)
+            self.sym_ub = torch.zeros(n1, dtype=torch.float32, device=dev)
+            self.sym_t = torch.zeros(self.n0, dtype=torch.float32, device=dev)
+            self.sym_dr = torch.zeros(self.n0, dtype=torch.float32, device=dev)
         self.coarse_dot = (torch.zeros(hier.levels[1].n, dtype=torch.float32, device=dev)
                            if self.fusable_plain else None)
 
@@ -298,7 +321,7 @@ class Program:
                                  torch.zeros((n, F), dtype=torch.bfloat16, device=dev))
 
     def forward(self, r, lk: torch.Tensor, *, ss_part=None, nparts: int = 0, z=None, scale=None, stop=None,
-                train: bool = False, want_y: bool = False, fused: bool = True):
+                train: bool = False, want_y: bool = False, fused: bool = True, dr=None):
         """z = M(r).  Without ``ss_part`` the RMS partials are computed here.
         ``r``, ``z``, ``scale``, ``ss_part`` and ``stop`` may be tensors or raw
         device addresses (the flexible-CG work buffer)."""
@@ -316,6 +339,23 @@ class Program:
         scale_p = ptr(self.scale) if scale is None else _addr(scale)
         we, be = net.enc()
         zt = self.z if z is None else z
+        if fused and self.fusable_sym0 and not train and not want_y:
+            if nparts > 0:   # scale from the partials (the flexible-CG kernels store it themselves)
+                check(L.tm_gnn_scale(self.n0, ss_ptr, nparts, scale_p, st))
+            lv1 = hier.levels[1]
+            s0 = net.sym0
+            check(L.tm_gnn_sym0_down(self.n0, ptr(lv0.row_ptr), ptr(lv0.col), ptr(lv0.vn), _addr(r), ptr(lv0.d),
+                                     ptr(self.sym_dr) if dr is None else _addr(dr), 0 if dr is None else 1,
+                                     scale_p, ptr(self.xs), stop_p, st))
+            check(L.tm_gnn_encode_pool(self.n0, lv1.n, ptr(lv1.child_ptr), ptr(lv1.child), ptr(lv0.w), _addr(None),
+                                       ptr(lv0.d), _addr(None), _addr(None), 0, ptr(s0["ab"]), ptr(s0["zero"]),
+                                       ptr(self.bufs[self.ops[1][3]]), ptr(self.xs), scale_p, stop_p, st))
+            self._run_ops(self.ops[2:-2], 2, train, stop_p, st)
+            check(L.tm_gnn_sym0_up(self.n0, lv1.n, ptr(lv0.row_ptr), ptr(lv0.col), ptr(lv0.vn), ptr(self.xs),
+                                   ptr(lv0.agg), ptr(lv0.w), ptr(self.bufs[self.ops[-2][2]]), ptr(s0["ab"]),
+                                   ptr(s0["ab"][F:]), ptr(s0["c3"]), ptr(lv0.d), scale_p, ptr(self.sym_ua),
+                                   ptr(self.sym_ub), ptr(self.sym_t), _addr(zt), stop_p, st))
+            return zt
         if fused and self.fusable_plain and not train and not want_y:
             first, last = self.ops[0], self.ops[-1]
             lv1 = hier.levels[1]

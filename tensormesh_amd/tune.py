@@ -17,6 +17,7 @@ from __future__ import annotations
 import itertools
 import json
 import os
+import sys
 import time
 from typing import Dict, Iterable, Optional
 
@@ -98,7 +99,8 @@ def autotune_batched(bp, space: Optional[Dict[str, Iterable]] = None, rtol: floa
         t = time_config(bp, rtol, maxiter)
         trials.append(dict(rows_per_tile=rpt, lanes=lanes, blocks=bpc * ncu, plan_rows=prow, seconds=t))
         if verbose:
-            print(f"[autotune] tile {rpt} lanes {lanes} blocks/CU {bpc} plan {prow}: {t * 1e3:.2f} ms", flush=True)
+            print(f"[autotune] tile {rpt} lanes {lanes} blocks/CU {bpc} plan {prow}: {t * 1e3:.2f} ms",
+                  file=sys.stderr, flush=True)
     best = min(trials, key=lambda d: d["seconds"])
     if best["plan_rows"] is not None:
         bp.plan = bp.topology.device_plan(bp.device, "scalar", best["plan_rows"])

@@ -128,6 +128,28 @@ def test_fused_inference_matches_unfused(cuda, n, schedule):
     assert np.linalg.norm(zf.cpu().numpy() - z_ref) <= 2e-2 * np.linalg.norm(z_ref)
 
 
+@pytest.mark.parametrize("n", [9, 12])
+def test_fused_symmetric_level0_matches_unfused(cuda, n):
+    """Symmetric network with a level-0 layer pair: the rank-two fused finest
+    level (two scalar CSR products) = the unfused 64-wide level-0 layers, and
+    both match the oracle."""
+    print(f"case precond fused sym0 n={n}", flush=True)
+    Ai, H, hier, levels, values, csr, lk = _problem(n, 30 + n, cuda)
+    net = GNNPreconditioner((0, 1, 2, 2, 1, 0), device=cuda, bottom=3)
+    net.params.add_(torch.randn(net.n_params, generator=torch.Generator().manual_seed(n)).to(cuda) * 0.1 * net.mask)
+    net.refresh()
+    prog = net.plan(hier)
+    assert prog.fusable_sym0
+    r = torch.tensor(np.random.default_rng(n).standard_normal(Ai.shape[0]), device=cuda)
+    lkt = torch.tensor(lk, device=cuda)
+    zf = prog.forward(r, lkt, fused=True).clone()
+    zu = prog.forward(r, lkt, fused=False).clone()
+    assert torch.linalg.vector_norm(zf - zu) <= 1e-2 * torch.linalg.vector_norm(zu)
+    z_ref = og.forward(levels, _params64(net), net.schedule, net.n_linear, r.cpu().numpy(), lk.astype(np.float64),
+                       bf16=True, bottom=net.bottom)[0]
+    assert np.linalg.norm(zf.cpu().numpy() - z_ref) <= 2e-2 * np.linalg.norm(z_ref)
+
+
 def test_network_gradient(cuda):
     """Parameter gradient of the energy loss: hand-written backward kernels vs
     the oracle backward (itself pinned by finite differences)."""
