@@ -79,9 +79,13 @@ int tm_plan_build(const double* points, int dim, const int64_t* conn, int64_t n_
  * [4] max LDS bytes of one block, [5] max halo, [6] max records, [7] max acc */
 int tm_plan_info(const tm_plan* plan, int64_t* info);
 /* blk[n_blocks*8] = {halo_off, n_halo, n_own, rec_off, n_rec, run_off, n_run, n_acc};
- * halo[n_halo_total] global node ids (owned rows first, each part ascending);
- * meta[n_halo_total] = acc_offset | diag_rank << 16 (owned entries only);
- * recs[n_records*4] packed element records; rec_elem[n_records] element ids;
+ * halo[n_halo_total] global node id of each LDS slot (slots [0, n_own) hold
+ * the block's owned nodes, the others its halo; slots are ordered to spread
+ * LDS bank accesses);
+ * meta[n_halo_total] = acc_offset | diag_rank << 16 | row_length << 24 for
+ * owned slots, 0 for the others;
+ * recs[n_records*4] packed element records (an element's vertices may be
+ * listed in any order); rec_elem[n_records] element ids;
  * runs[n_runs*4] = {value_start, acc_start, length, first_row}. */
 int tm_plan_export(const tm_plan* plan, int32_t* blk, int32_t* halo, uint32_t* meta,
                    uint32_t* recs, int32_t* rec_elem, int32_t* runs);
@@ -104,7 +108,11 @@ typedef struct {
 } tm_plan_dev;
 
 /* Reference-element moments of the quadrature rule: wsum = sum_q w_q,
- * W[c] = sum_q w_q phi_c(q), M[a][b] = sum_q w_q phi_a(q) phi_b(q). */
+ * W[c] = sum_q w_q phi_c(q), M[a][b] = sum_q w_q phi_a(q) phi_b(q).
+ * Plan records may list an element's vertices in any order, so the moments
+ * must be invariant under vertex relabelling (all W[c] equal, all M[a][a]
+ * equal, all M[a][b], a != b, equal - true of every symmetric rule); the
+ * assembly calls return TM_ERR_ARG otherwise. */
 typedef struct {
   double wsum;
   double W[4];

@@ -25,7 +25,7 @@ from ..element import ELEMENT_DIM, ELEMENT_NB, quadrature_moments
 # Plan kinds: (accumulator bytes per CSR entry, LDS bytes per halo node,
 # LDS bytes per owned row).  "scalar" serves the matrix, load and adjoint
 # kernels of scalar problems.
-PLAN_KINDS = {"scalar": (8, 32, 16), "vector3": (72, 56, 32)}
+PLAN_KINDS = {"scalar": (8, 36, 8), "vector3": (72, 56, 32)}
 
 
 def _default_plan_params(kind: str, dim: int) -> Tuple[int, int]:

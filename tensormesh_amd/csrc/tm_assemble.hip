@@ -6,7 +6,7 @@
 //      into LDS and zeroes an FP64 accumulator holding the owned rows' CSR
 //      entries,
 //   2. walks the block's element records (one element per lane, 16-byte
-//      records carrying 10-bit halo-local ids and the 6-bit in-row rank of
+//      records carrying 10-bit LDS node slots and the 6-bit in-row rank of
 //      every (owned row, column) pair), computes the element geometry in FP64
 //      (cofactor vectors c_b = |J| grad phi_b and det J) and adds the owned
 //      rows of the element matrix into LDS with ds_add_f64,
@@ -36,16 +36,23 @@ struct QuadArg {
 
 __host__ __device__ constexpr size_t a16(size_t x) { return (x + 15) & ~size_t(15); }
 
-// LDS regions of one block, identical in every kernel:
+// LDS regions of one block, identical in every kernel (nh node slots; slots
+// [0, nown) hold the owned nodes):
 //   acc   [nacc]   owned-row CSR accumulators (FP64)
-//   nodes [nh]     halo nodes as double4: 3D {x, y, z, coef}, 2D {x, y, coef, -}
+//   p0    [nh]     node words {x, y}
+//   p1    [nh]     node words 3D {z, coef}, 2D {coef, -}
+//   ms    [nh]     row meta of owned slots: acc offset | diag rank << 16 |
+//                  row length << 24 (0 for the other slots)
 //   accb  [nown]   owned-node vector accumulator (FP64)
-//   ms    [nown]   owned-row meta: acc offset | diag rank << 16
+// Node data are read as 16-byte words: a quarter-wave's reads are
+// conflict-free when the slots differ mod 16, which the plan's slot
+// assignment aims for.
 struct Lds {
   double* acc;
-  double4* nodes;
-  double* accb;
+  double2* p0;
+  double2* p1;
   uint32_t* ms;
+  double* accb;
 };
 
 __device__ __forceinline__ Lds carve(char* s, int nacc, int nh, int nown) {
@@ -53,11 +60,13 @@ __device__ __forceinline__ Lds carve(char* s, int nacc, int nh, int nown) {
   size_t o = 0;
   L.acc = (double*)(s + o);
   o = a16(o + 8 * (size_t)nacc);
-  L.nodes = (double4*)(s + o);
-  o = a16(o + 32 * (size_t)nh);
-  L.accb = (double*)(s + o);
-  o = a16(o + 8 * (size_t)nown);
+  L.p0 = (double2*)(s + o);
+  o = a16(o + 16 * (size_t)nh);
+  L.p1 = (double2*)(s + o);
+  o = a16(o + 16 * (size_t)nh);
   L.ms = (uint32_t*)(s + o);
+  o = a16(o + 4 * (size_t)nh);
+  L.accb = (double*)(s + o);
   return L;
 }
 
@@ -68,30 +77,42 @@ __device__ __forceinline__ double ld(const T* p, int64_t i) {
 
 __device__ __forceinline__ int rec_local(uint64_t lo, int a) { return (int)((lo >> (10 * a)) & 1023u); }
 
-// In-row rank of vertex b in the row of owned vertex a (a != b).
+// In-row rank of vertex b in the row of owned vertex a (a != b): the 6-bit
+// field at bit 40 + 6 (a (NB-1) + j) of the record (a, b are unrolled
+// constants, so each field is one bit-field extract, or an align + extract
+// for the field that straddles two words).
 template <int NB>
-__device__ __forceinline__ int rec_rank(uint64_t lo, uint64_t hi, int a, int b) {
+__device__ __forceinline__ int rec_rank(const uint4& rc, int a, int b) {
   const int j = b - (b > a ? 1 : 0);
   const int bit = 40 + 6 * (a * (NB - 1) + j);
-  return bit < 64 ? (int)((lo >> bit) & 63u) : (int)((hi >> (bit - 64)) & 63u);
+  const int w = bit / 32, off = bit % 32;
+  const uint32_t lo = w == 1 ? rc.y : (w == 2 ? rc.z : rc.w);
+  if (off <= 26) return (int)__builtin_amdgcn_ubfe(lo, off, 6);
+  const uint32_t hi = w == 1 ? rc.z : rc.w;
+  return (int)__builtin_amdgcn_ubfe(__builtin_amdgcn_alignbit(hi, lo, off), 0, 6);
 }
 
 template <int DIM>
-__device__ __forceinline__ void node_xyz(const double4& n, double (&x)[DIM]);
+__device__ __forceinline__ void node_xyz(const double2& n0, const double2& n1, double (&x)[DIM]);
 template <>
-__device__ __forceinline__ void node_xyz<3>(const double4& n, double (&x)[3]) {
-  x[0] = n.x;
-  x[1] = n.y;
-  x[2] = n.z;
+__device__ __forceinline__ void node_xyz<3>(const double2& n0, const double2& n1, double (&x)[3]) {
+  x[0] = n0.x;
+  x[1] = n0.y;
+  x[2] = n1.x;
 }
 template <>
-__device__ __forceinline__ void node_xyz<2>(const double4& n, double (&x)[2]) {
-  x[0] = n.x;
-  x[1] = n.y;
+__device__ __forceinline__ void node_xyz<2>(const double2& n0, const double2&, double (&x)[2]) {
+  x[0] = n0.x;
+  x[1] = n0.y;
 }
 template <int DIM>
-__device__ __forceinline__ double node_coef(const double4& n) {
-  return DIM == 3 ? n.w : n.z;
+__device__ __forceinline__ double node_coef(const double2& n1) {
+  return DIM == 3 ? n1.y : n1.x;
+}
+// LDS address of the per-node FP64 scratch word (the coefficient word)
+template <int DIM>
+__device__ __forceinline__ double* node_coef_ptr(const Lds& L, int h) {
+  return (double*)(L.p1 + h) + (DIM == 3 ? 1 : 0);
 }
 
 // Cofactor vectors c[b] (grad phi_b = c[b] / det) and det J of a P1 simplex.
@@ -152,8 +173,12 @@ __device__ __forceinline__ double recip(double x) {
   return r;
 }
 
+__device__ __forceinline__ int meta_off(uint32_t m) { return (int)(m & 0xffffu); }
+__device__ __forceinline__ int meta_diag(uint32_t m) { return (int)((m >> 16) & 0xffu); }
+__device__ __forceinline__ int meta_len(uint32_t m) { return (int)(m >> 24); }
+
 // Stages the block's halo nodes (and optionally a nodal coefficient) into LDS
-// and zeroes the accumulators.
+// and zeroes the accumulators.  A slot with halo id -1 is empty.
 template <int NT, int DIM, typename CT, bool COEF>
 __device__ __forceinline__ void stage(const Lds& L, const int32_t* __restrict__ halo, const uint32_t* __restrict__ meta,
                                       int nacc, int nh, int nown, const double* __restrict__ X,
@@ -161,32 +186,35 @@ __device__ __forceinline__ void stage(const Lds& L, const int32_t* __restrict__ 
   for (int i = threadIdx.x; i < nacc; i += NT) L.acc[i] = 0.0;
   for (int h = threadIdx.x; h < nh; h += NT) {
     const int64_t g = halo[h];
-    const double cf = COEF ? ld(coef, g) : 0.0;
-    double4 n;
-    if (DIM == 3)
-      n = make_double4(X[g * 3], X[g * 3 + 1], X[g * 3 + 2], cf);
-    else
-      n = make_double4(X[g * 2], X[g * 2 + 1], cf, 0.0);
-    L.nodes[h] = n;
+    double2 n0 = make_double2(0.0, 0.0), n1 = make_double2(0.0, 0.0);
+    if (g >= 0) {
+      const double cf = COEF ? ld(coef, g) : 0.0;
+      if (DIM == 3) {
+        n0 = make_double2(X[g * 3], X[g * 3 + 1]);
+        n1 = make_double2(X[g * 3 + 2], cf);
+      } else {
+        n0 = make_double2(X[g * 2], X[g * 2 + 1]);
+        n1 = make_double2(cf, 0.0);
+      }
+    }
+    L.p0[h] = n0;
+    L.p1[h] = n1;
+    L.ms[h] = meta[h];
   }
-  for (int a = threadIdx.x; a < nown; a += NT) {
-    L.ms[a] = meta[a];
-    L.accb[a] = 0.0;
-  }
+  for (int a = threadIdx.x; a < nown; a += NT) L.accb[a] = 0.0;
 }
 
 // Sets the diagonal of every owned row to minus the sum of its off-diagonal
 // entries (exact for forms whose element rows sum to zero: sum_b c_b = 0).
 template <int NT>
-__device__ __forceinline__ void rowsum_diagonal(const Lds& L, int nown, int nacc) {
+__device__ __forceinline__ void rowsum_diagonal(const Lds& L, int nown) {
   for (int a = threadIdx.x; a < nown; a += NT) {
     const uint32_t m = L.ms[a];
-    const int b0 = (int)(m & 0xffffu);
-    const int b1 = a + 1 < nown ? (int)(L.ms[a + 1] & 0xffffu) : nacc;
-    if (b1 <= b0) continue;
-    const int d = b0 + (int)(m >> 16);
+    const int b0 = meta_off(m), len = meta_len(m);
+    if (len == 0) continue;
+    const int d = b0 + meta_diag(m);
     double s = 0.0;
-    for (int k = b0; k < b1; ++k) s += (k == d) ? 0.0 : L.acc[k];
+    for (int k = b0; k < b0 + len; ++k) s += (k == d) ? 0.0 : L.acc[k];
     L.acc[d] = -s;
   }
 }
@@ -205,23 +233,25 @@ __global__ __launch_bounds__(NT) void k_asm_matrix(PlanArgs p, const double* __r
   const int hoff = bi0.x, nh = bi0.y, nown = bi0.z, roff = bi0.w;
   const int nrec = bi1.x, runoff = bi1.y, nrun = bi1.z, nacc = bi1.w;
   const Lds L = carve(smem, nacc, nh, nown);
+  // the first record is independent of the staged nodes: load it first
+  uint4 nxt = threadIdx.x < nrec ? p.recs[roff + threadIdx.x] : make_uint4(0, 0, 0, 0);
   stage<NT, DIM, CT, FORM == 1>(L, p.halo + hoff, p.meta + hoff, nacc, nh, nown, X, coef);
   __syncthreads();
 
-  uint4 nxt = threadIdx.x < nrec ? p.recs[roff + threadIdx.x] : make_uint4(0, 0, 0, 0);
   for (int r = threadIdx.x; r < nrec; r += NT) {
     const uint4 rc = nxt;
     if (r + NT < nrec) nxt = p.recs[roff + r + NT];  // prefetch the next record
     const uint64_t lo = (uint64_t)rc.x | ((uint64_t)rc.y << 32);
-    const uint64_t hi = (uint64_t)rc.z | ((uint64_t)rc.w << 32);
     int l[NB];
     double x[NB][DIM], cf[NB];
+    uint32_t m[NB];
 #pragma unroll
     for (int a = 0; a < NB; ++a) {
       l[a] = rec_local(lo, a);
-      const double4 n = L.nodes[l[a]];
-      node_xyz<DIM>(n, x[a]);
-      cf[a] = node_coef<DIM>(n);
+      const double2 n0 = L.p0[l[a]], n1 = L.p1[l[a]];
+      m[a] = L.ms[l[a]];
+      node_xyz<DIM>(n0, n1, x[a]);
+      cf[a] = node_coef<DIM>(n1);
     }
     double c[NB][DIM], det;
     p1_geometry<DIM>(x, c, det);
@@ -238,35 +268,33 @@ __global__ __launch_bounds__(NT) void k_asm_matrix(PlanArgs p, const double* __r
       s = scale * adet;
     }
     bool own[NB];
-    uint32_t m[NB];
     bool all = true;
 #pragma unroll
     for (int a = 0; a < NB; ++a) {
       own[a] = l[a] < nown;
       all = all && own[a];
-      m[a] = own[a] ? L.ms[l[a]] : 0u;
     }
     if (__all(all)) {
       // every vertex owned in every lane: no branches (records are sorted so
       // that the block's interior elements form whole waves)
       if (!ROWSUM) {
 #pragma unroll
-        for (int a = 0; a < NB; ++a) atomicAdd(L.acc + (m[a] & 0xffffu) + (m[a] >> 16), s * q.M[a * NB + a]);
+        for (int a = 0; a < NB; ++a) atomicAdd(L.acc + meta_off(m[a]) + meta_diag(m[a]), s * q.M[a * NB + a]);
       }
 #pragma unroll
       for (int a = 0; a < NB; ++a)
 #pragma unroll
         for (int bb = a + 1; bb < NB; ++bb) {
           const double kab = FORM == 2 ? s * q.M[a * NB + bb] : s * dotc<DIM>(c[a], c[bb]);
-          atomicAdd(L.acc + (m[a] & 0xffffu) + rec_rank<NB>(lo, hi, a, bb), kab);
-          atomicAdd(L.acc + (m[bb] & 0xffffu) + rec_rank<NB>(lo, hi, bb, a), kab);
+          atomicAdd(L.acc + meta_off(m[a]) + rec_rank<NB>(rc, a, bb), kab);
+          atomicAdd(L.acc + meta_off(m[bb]) + rec_rank<NB>(rc, bb, a), kab);
         }
       continue;
     }
     if (!ROWSUM) {
 #pragma unroll
       for (int a = 0; a < NB; ++a)
-        if (own[a]) atomicAdd(L.acc + (m[a] & 0xffffu) + (m[a] >> 16), s * q.M[a * NB + a]);
+        if (own[a]) atomicAdd(L.acc + meta_off(m[a]) + meta_diag(m[a]), s * q.M[a * NB + a]);
     }
     // each symmetric pair once: K_ab = K_ba
 #pragma unroll
@@ -275,13 +303,13 @@ __global__ __launch_bounds__(NT) void k_asm_matrix(PlanArgs p, const double* __r
       for (int bb = a + 1; bb < NB; ++bb) {
         if (!(own[a] || own[bb])) continue;
         const double kab = FORM == 2 ? s * q.M[a * NB + bb] : s * dotc<DIM>(c[a], c[bb]);
-        if (own[a]) atomicAdd(L.acc + (m[a] & 0xffffu) + rec_rank<NB>(lo, hi, a, bb), kab);
-        if (own[bb]) atomicAdd(L.acc + (m[bb] & 0xffffu) + rec_rank<NB>(lo, hi, bb, a), kab);
+        if (own[a]) atomicAdd(L.acc + meta_off(m[a]) + rec_rank<NB>(rc, a, bb), kab);
+        if (own[bb]) atomicAdd(L.acc + meta_off(m[bb]) + rec_rank<NB>(rc, bb, a), kab);
       }
   }
   __syncthreads();
   if (ROWSUM) {
-    rowsum_diagonal<NT>(L, nown, nacc);
+    rowsum_diagonal<NT>(L, nown);
     __syncthreads();
   }
 
@@ -314,9 +342,9 @@ __global__ __launch_bounds__(NT) void k_asm_load(PlanArgs p, const double* __res
 #pragma unroll
     for (int a = 0; a < NB; ++a) {
       l[a] = rec_local(lo, a);
-      const double4 n = L.nodes[l[a]];
-      node_xyz<DIM>(n, x[a]);
-      cf[a] = node_coef<DIM>(n);
+      const double2 n0 = L.p0[l[a]], n1 = L.p1[l[a]];
+      node_xyz<DIM>(n0, n1, x[a]);
+      cf[a] = node_coef<DIM>(n1);
     }
     double c[NB][DIM], det;
     p1_geometry<DIM>(x, c, det);
@@ -340,7 +368,7 @@ __global__ __launch_bounds__(NT) void k_asm_load(PlanArgs p, const double* __res
 }
 
 // dkappa_c += sum_e W_c / |det| * sum_{a owned, b} g[slot(a, b)] c_a . c_b
-// (per-node accumulator: the 4th double of each LDS node, unused by this kernel)
+// (per-node accumulator: the coefficient word of each LDS node, unused by this kernel)
 template <int NB, typename GT, int NT>
 __global__ __launch_bounds__(NT) void k_kappa_adjoint(PlanArgs p, const double* __restrict__ X,
                                                      const GT* __restrict__ g, QuadArg q,
@@ -360,17 +388,15 @@ __global__ __launch_bounds__(NT) void k_kappa_adjoint(PlanArgs p, const double* 
     for (int t = lane; t < ru.z; t += kWave) L.acc[ru.y + t] = (double)g[(int64_t)ru.x + t];
   }
   __syncthreads();
-  double* acc_node = (double*)L.nodes;  // element 4*h + 3 of node h
   for (int r = threadIdx.x; r < nrec; r += NT) {
     const uint4 rc = p.recs[roff + r];
     const uint64_t lo = (uint64_t)rc.x | ((uint64_t)rc.y << 32);
-    const uint64_t hi = (uint64_t)rc.z | ((uint64_t)rc.w << 32);
     int l[NB];
     double x[NB][DIM];
 #pragma unroll
     for (int a = 0; a < NB; ++a) {
       l[a] = rec_local(lo, a);
-      node_xyz<DIM>(L.nodes[l[a]], x[a]);
+      node_xyz<DIM>(L.p0[l[a]], L.p1[l[a]], x[a]);
     }
     double c[NB][DIM], det;
     p1_geometry<DIM>(x, c, det);
@@ -379,27 +405,42 @@ __global__ __launch_bounds__(NT) void k_kappa_adjoint(PlanArgs p, const double* 
     for (int a = 0; a < NB; ++a) {
       if (l[a] >= nown) continue;
       const uint32_t m = L.ms[l[a]];
-      const double* row = L.acc + (m & 0xffffu);
-      t += row[m >> 16] * dotc<DIM>(c[a], c[a]);
+      const double* row = L.acc + meta_off(m);
+      t += row[meta_diag(m)] * dotc<DIM>(c[a], c[a]);
 #pragma unroll
       for (int bb = 0; bb < NB; ++bb) {
         if (bb == a) continue;
-        t += row[rec_rank<NB>(lo, hi, a, bb)] * dotc<DIM>(c[a], c[bb]);
+        t += row[rec_rank<NB>(rc, a, bb)] * dotc<DIM>(c[a], c[bb]);
       }
     }
     t *= recip(fabs(det));
 #pragma unroll
-    for (int a = 0; a < NB; ++a) atomicAdd(acc_node + 4 * l[a] + 3, q.W[a] * t);
+    for (int a = 0; a < NB; ++a) atomicAdd(node_coef_ptr<DIM>(L, l[a]), q.W[a] * t);
   }
   __syncthreads();
   for (int h = threadIdx.x; h < nh; h += NT) {
-    const double v = acc_node[4 * h + 3];
-    if (v != 0.0) atomicAdd(dkappa + p.halo[hoff + h], v);
+    const double v = *node_coef_ptr<DIM>(L, h);
+    const int32_t g = p.halo[hoff + h];
+    if (v != 0.0 && g >= 0) atomicAdd(dkappa + g, v);
   }
 }
 
 PlanArgs plan_args(const tm_plan_dev* d) {
   return PlanArgs{d->blk, d->halo, d->meta, (const uint4*)d->recs, (const int4*)d->runs, d->n_blocks};
+}
+
+// The plan may relabel an element's vertices, so the moments must not depend
+// on the labels: W equal, M's diagonal equal, M's off-diagonal equal.
+bool quad_is_symmetric(const tm_quad* q, int nb) {
+  double scale = fabs(q->wsum);
+  for (int i = 0; i < nb * nb; ++i) scale = fmax(scale, fabs(q->M[i]));
+  const double tol = 1e-12 * scale;
+  for (int a = 0; a < nb; ++a) {
+    if (fabs(q->W[a] - q->W[0]) > tol || fabs(q->M[a * nb + a] - q->M[0]) > tol) return false;
+    for (int b = 0; b < nb; ++b)
+      if (a != b && fabs(q->M[a * nb + b] - q->M[1]) > tol) return false;
+  }
+  return true;
 }
 
 QuadArg quad_arg(const tm_quad* q) {
@@ -457,6 +498,8 @@ extern "C" int tm_asm_p1_matrix(int form, int nb, int dim, const tm_plan_dev* pl
   TM_REQUIRE(plan && quad && coords && values, TM_ERR_ARG, "null argument");
   TM_REQUIRE((nb == 3 && dim == 2) || (nb == 4 && dim == 3), TM_ERR_ARG,
              "only P1 triangles (nb=3, dim=2) and tetrahedra (nb=4, dim=3) are supported");
+  TM_REQUIRE(quad_is_symmetric(quad, nb), TM_ERR_ARG,
+             "quadrature moments must be invariant under vertex relabelling");
   hipStream_t st = (hipStream_t)stream;
   QuadArg q = quad_arg(quad);
   if (value_dtype == TM_F32) {
@@ -472,6 +515,8 @@ extern "C" int tm_asm_p1_load(int form, int nb, int dim, const tm_plan_dev* plan
                               void* stream) {
   TM_REQUIRE(plan && quad && coords && b, TM_ERR_ARG, "null argument");
   TM_REQUIRE((nb == 3 && dim == 2) || (nb == 4 && dim == 3), TM_ERR_ARG, "only P1 simplices");
+  TM_REQUIRE(quad_is_symmetric(quad, nb), TM_ERR_ARG,
+             "quadrature moments must be invariant under vertex relabelling");
   hipStream_t st = (hipStream_t)stream;
   QuadArg q = quad_arg(quad);
   PlanArgs p = plan_args(plan);
@@ -493,6 +538,8 @@ extern "C" int tm_asm_p1_kappa_adjoint(int nb, int dim, const tm_plan_dev* plan,
                                        void* stream) {
   TM_REQUIRE(plan && quad && coords && g && dkappa, TM_ERR_ARG, "null argument");
   TM_REQUIRE((nb == 3 && dim == 2) || (nb == 4 && dim == 3), TM_ERR_ARG, "only P1 simplices");
+  TM_REQUIRE(quad_is_symmetric(quad, nb), TM_ERR_ARG,
+             "quadrature moments must be invariant under vertex relabelling");
   hipStream_t st = (hipStream_t)stream;
   QuadArg q = quad_arg(quad);
   PlanArgs p = plan_args(plan);

@@ -296,10 +296,9 @@ int make_blocks(const PlanCtx& c, Scratch& s, std::vector<int32_t> owned,
   B.n_own = nown;
   B.n_acc = (int32_t)n_acc;
   B.lds = (int32_t)lds;
-  B.halo.reserve(nh);
-  B.halo.insert(B.halo.end(), owned.begin(), owned.end());
-  B.halo.insert(B.halo.end(), others.begin(), others.end());
-  B.meta.assign(nh, 0u);
+  // row meta of owned node a (halo order): accumulator offset | diagonal rank
+  // << 16 | row length << 24
+  std::vector<uint32_t> meta0(nh, 0u);
   uint32_t lrp = 0;
   for (int a = 0; a < nown; ++a) {
     int32_t g = owned[a];
@@ -309,7 +308,8 @@ int make_blocks(const PlanCtx& c, Scratch& s, std::vector<int32_t> owned,
       dr = find_in_row(c.row_ptr, c.col_idx, g, g);
       TM_REQUIRE(dr >= 0 && dr < 256, TM_ERR_LIMIT, "row %d has no diagonal or rank >= 256", g);
     }
-    B.meta[a] = lrp | ((uint32_t)dr << 16);
+    TM_REQUIRE(len < 256, TM_ERR_LIMIT, "row %d has %d entries (at most 255 per row)", g, len);
+    meta0[a] = lrp | ((uint32_t)dr << 16) | ((uint32_t)len << 24);
     lrp += (uint32_t)len;
   }
   // Record order.  Elements whose vertices are all owned come first (a wave
@@ -340,39 +340,167 @@ int make_blocks(const PlanCtx& c, Scratch& s, std::vector<int32_t> owned,
       g0 = g1;
     }
   }
-  // element records: 4 x 10-bit local ids, then for each owned vertex a the
-  // in-row ranks of the other vertices (6 bits each, a-major, b ascending)
-  B.recs.reserve(elems.size() * 4);
-  B.rec_elem.reserve(elems.size());
-  for (int32_t e : elems) {
-    const int64_t* ce = c.conn + (int64_t)e * c.nb;
-    int32_t l[4] = {0, 0, 0, 0};
-    for (int a = 0; a < c.nb; ++a) l[a] = s.local[ce[a]];
-    uint64_t lo = (uint64_t)l[0] | ((uint64_t)l[1] << 10) | ((uint64_t)l[2] << 20) |
-                  ((uint64_t)l[3] << 30);
-    uint64_t hi = 0;
-    for (int a = 0; a < c.nb; ++a) {
-      if (l[a] >= nown) continue;
-      int j = 0;
-      for (int b = 0; b < c.nb; ++b) {
-        if (b == a) continue;
-        int32_t r = find_in_row(c.row_ptr, c.col_idx, ce[a], ce[b]);
-        TM_REQUIRE(r >= 0 && r < kMaxRank, TM_ERR_LIMIT,
-                   "row %lld: column %lld missing or rank >= %d", (long long)ce[a],
-                   (long long)ce[b], kMaxRank);
-        int k = a * (c.nb - 1) + j++;
-        int bit = 40 + 6 * k;
+  // Element records (16 bytes): nb x 10-bit LDS slots, then for each owned
+  // vertex a the in-row ranks of the other vertices (6 bits each, a-major, b
+  // ascending).  Two layout choices cut LDS bank conflicts of the kernels; both
+  // only relabel, so the assembled values do not change:
+  //  * vertex order per record: a wave issues one LDS atomic per ordered pair
+  //    (a, b) for all 64 lanes; the order is picked greedily, lane by lane, so
+  //    that each atomic's 8-byte addresses of a half-wave spread over the 32
+  //    bank pairs (P1 element matrices are symmetric under relabelling);
+  //  * LDS slots of the halo nodes: node data are read as 16-byte words, so a
+  //    quarter-wave's reads of one vertex position are conflict-free when
+  //    their slots differ mod 16; slots are coloured greedily (owned nodes stay
+  //    in [0, nown), the others in [nown, nh)).
+  const int nrec = (int)elems.size();
+  const int NB = c.nb;
+  std::vector<int32_t> vl((size_t)nrec * NB);  // halo-order local ids, final vertex order
+  std::vector<uint8_t> vr((size_t)nrec * 16, 0);  // ranks in final order, (a, j) a-major
+  {
+    int perms[24][4];
+    int np = 0;
+    int pv[4] = {0, 1, 2, 3};
+    do {
+      for (int k = 0; k < 4; ++k) perms[np][k] = pv[k];
+      ++np;
+    } while (std::next_permutation(pv, pv + NB));
+    // instruction order of the kernels' atomics: (a, b), (b, a) for a < b
+    int sx[12], sy[12], ns = 0;
+    for (int a = 0; a < NB; ++a)
+      for (int b = a + 1; b < NB; ++b) {
+        sx[ns] = a; sy[ns] = b; ++ns;
+        sx[ns] = b; sy[ns] = a; ++ns;
+      }
+    std::vector<int32_t> hist((size_t)ns * 2 * 32);
+    int rk[4][4];
+    int32_t l0[4];
+    for (int w0 = 0; w0 < nrec; w0 += 64) {
+      std::fill(hist.begin(), hist.end(), 0);
+      for (int r = w0; r < std::min(nrec, w0 + 64); ++r) {
+        const int64_t* ce = c.conn + (int64_t)elems[r] * NB;
+        for (int a = 0; a < NB; ++a) l0[a] = s.local[ce[a]];
+        for (int a = 0; a < NB; ++a) {
+          if (l0[a] >= nown) continue;
+          for (int b = 0; b < NB; ++b) {
+            if (b == a) continue;
+            int32_t rr = find_in_row(c.row_ptr, c.col_idx, ce[a], ce[b]);
+            TM_REQUIRE(rr >= 0 && rr < kMaxRank, TM_ERR_LIMIT,
+                       "row %lld: column %lld missing or rank >= %d", (long long)ce[a],
+                       (long long)ce[b], kMaxRank);
+            rk[a][b] = rr;
+          }
+        }
+        const int half = ((r - w0) >> 5) & 1;
+        int best = 0, best_sc = 1 << 30;
+        for (int q = 0; q < np; ++q) {
+          int sc = 0;
+          for (int k = 0; k < ns; ++k) {
+            const int a = perms[q][sx[k]], b = perms[q][sy[k]];
+            if (l0[a] >= nown) continue;
+            const int bank = (int)(((meta0[l0[a]] & 0xffffu) + (uint32_t)rk[a][b]) & 31u);
+            sc += hist[((size_t)k * 2 + half) * 32 + bank];
+          }
+          if (sc < best_sc) {
+            best_sc = sc;
+            best = q;
+          }
+        }
+        const int* pq = perms[best];
+        for (int k = 0; k < ns; ++k) {
+          const int a = pq[sx[k]], b = pq[sy[k]];
+          if (l0[a] >= nown) continue;
+          const int bank = (int)(((meta0[l0[a]] & 0xffffu) + (uint32_t)rk[a][b]) & 31u);
+          ++hist[((size_t)k * 2 + half) * 32 + bank];
+        }
+        for (int a = 0; a < NB; ++a) {
+          vl[(size_t)r * NB + a] = l0[pq[a]];
+          if (l0[pq[a]] >= nown) continue;
+          int j = 0;
+          for (int b = 0; b < NB; ++b)
+            if (b != a) vr[(size_t)r * 16 + a * (NB - 1) + j++] = (uint8_t)rk[pq[a]][pq[b]];
+        }
+      }
+    }
+  }
+  std::vector<int32_t> slot(nh);
+  {
+    // groups: (quarter-wave, vertex position); membership lists per node
+    const int ngrp = ((nrec + 15) / 16) * NB;
+    std::vector<std::vector<int32_t>> memb(nh);
+    for (int r = 0; r < nrec; ++r)
+      for (int a = 0; a < NB; ++a) {
+        const int g = (r / 16) * NB + a;
+        auto& m = memb[vl[(size_t)r * NB + a]];
+        if (m.empty() || m.back() != g) m.push_back(g);
+      }
+    std::vector<int32_t> gcol((size_t)ngrp * 16, 0);
+    auto colour = [&](int lo, int hi) {
+      int cap[16];
+      for (int k = 0; k < 16; ++k) cap[k] = 0;
+      for (int p = lo; p < hi; ++p) ++cap[p & 15];
+      std::vector<int32_t> order(hi - lo);
+      std::iota(order.begin(), order.end(), lo);
+      std::stable_sort(order.begin(), order.end(),
+                       [&](int32_t x, int32_t y) { return memb[x].size() > memb[y].size(); });
+      std::vector<int32_t> col(hi - lo);
+      for (int32_t x : order) {
+        int best = -1, best_sc = 1 << 30;
+        for (int k = 0; k < 16; ++k) {
+          if (cap[k] == 0) continue;
+          int sc = 0;
+          for (int32_t g : memb[x]) sc += gcol[(size_t)g * 16 + k];
+          if (sc < best_sc) {
+            best_sc = sc;
+            best = k;
+          }
+        }
+        --cap[best];
+        col[x - lo] = best;
+        for (int32_t g : memb[x]) ++gcol[(size_t)g * 16 + best];
+      }
+      // positions p in [lo, hi) with p % 16 == colour, in increasing order
+      int nextp[16];
+      for (int k = 0; k < 16; ++k) {
+        int p = lo;
+        while (p < hi && (p & 15) != k) ++p;
+        nextp[k] = p;
+      }
+      for (int x = lo; x < hi; ++x) {
+        const int k = col[x - lo];
+        slot[x] = nextp[k];
+        nextp[k] += 16;
+      }
+    };
+    colour(0, nown);
+    colour(nown, (int)nh);
+  }
+  B.halo.assign(nh, -1);
+  B.meta.assign(nh, 0u);
+  for (int h = 0; h < nh; ++h) {
+    B.halo[slot[h]] = h < nown ? owned[h] : others[h - nown];
+    B.meta[slot[h]] = meta0[h];
+  }
+  B.recs.reserve((size_t)nrec * 4);
+  B.rec_elem.reserve(nrec);
+  for (int r = 0; r < nrec; ++r) {
+    uint64_t lo = 0, hi = 0;
+    for (int a = 0; a < NB; ++a) lo |= (uint64_t)slot[vl[(size_t)r * NB + a]] << (10 * a);
+    for (int a = 0; a < NB; ++a) {
+      if (vl[(size_t)r * NB + a] >= nown) continue;
+      for (int j = 0; j < NB - 1; ++j) {
+        const uint64_t v = vr[(size_t)r * 16 + a * (NB - 1) + j];
+        const int bit = 40 + 6 * (a * (NB - 1) + j);
         if (bit < 64)
-          lo |= (uint64_t)r << bit;
+          lo |= v << bit;
         else
-          hi |= (uint64_t)r << (bit - 64);
+          hi |= v << (bit - 64);
       }
     }
     B.recs.push_back((uint32_t)lo);
     B.recs.push_back((uint32_t)(lo >> 32));
     B.recs.push_back((uint32_t)hi);
     B.recs.push_back((uint32_t)(hi >> 32));
-    B.rec_elem.push_back(e);
+    B.rec_elem.push_back(elems[r]);
   }
   // runs of consecutive owned rows -> contiguous value ranges
   int a = 0;

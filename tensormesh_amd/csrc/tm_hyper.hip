@@ -212,7 +212,7 @@ __global__ __launch_bounds__(kThreads) void k_nh_assemble(PlanArgs p, const doub
         int j = 0;
 #pragma unroll
         for (int bb = 0; bb < 4; ++bb) {
-          const int slot = base + (bb == a ? (int)(m >> 16) : rec_rank4(lo, hi, a, j));
+          const int slot = base + (bb == a ? (int)((m >> 16) & 0xffu) : rec_rank4(lo, hi, a, j));
           if (bb != a) ++j;
           const double gg = mu * (k.g[a][0] * k.g[bb][0] + k.g[a][1] * k.g[bb][1] + k.g[a][2] * k.g[bb][2]);
           double* blk = L.acc + 9 * slot;

@@ -1,8 +1,10 @@
 """Float64 CPU emulation of the block-plan assembly kernel (test helper).
 
 Decodes the exported plan exactly as the HIP kernel does and checks every
-index against its buffer: local ids < halo size, accumulator offsets inside
-the block, each CSR value written by exactly one run.
+index against its buffer: node slots < halo size, accumulator offsets inside
+the block, each CSR value written by exactly one run.  A record may list its
+element's vertices in any order (the plan relabels them to spread LDS bank
+accesses), so the element matrix is evaluated in the record's order.
 """
 import numpy as np
 
@@ -30,26 +32,29 @@ def emulate_matrix(topo, arrs, element_matrix):
         hoff, nh, nown, roff, nrec, runoff, nrun, nacc = (int(v) for v in blk[b])
         assert 0 < nh <= 1024 and 0 < nown <= nh
         halo = arrs["halo"][hoff:hoff + nh]
-        meta = arrs["meta"][hoff:hoff + nown].astype(np.int64)
+        meta = arrs["meta"][hoff:hoff + nh].astype(np.int64)
+        assert (halo[:nown] >= 0).all()
+        assert (meta[nown:] == 0).all()
         acc = np.zeros(nacc)
         for r in range(nrec):
             loc, rank = decode(recs[roff + r], nb)
             assert max(loc) < nh
             e = int(arrs["rec_elem"][roff + r])
-            assert list(halo[loc]) == list(topo.conn[e])
+            assert sorted(halo[loc]) == sorted(topo.conn[e])
             K = element_matrix(e, topo.points[halo[loc]])
             for a in range(nb):
                 if loc[a] >= nown:
                     continue
-                base, dr = int(meta[loc[a]] & 0xFFFF), int(meta[loc[a]] >> 16)
-                assert base + dr < nacc
+                m = int(meta[loc[a]])
+                base, dr, ln = m & 0xFFFF, (m >> 16) & 0xFF, m >> 24
+                assert dr < ln and base + ln <= nacc
                 acc[base + dr] += K[a, a]
                 j = 0
                 for bb in range(nb):
                     if bb == a:
                         continue
                     k = rank(a, j)
-                    assert base + k < nacc
+                    assert k < ln and k != dr
                     acc[base + k] += K[a, bb]
                     j += 1
         for q in range(nrun):

@@ -16,6 +16,7 @@ from tensormesh_amd.mesh import kuhn_cube_arrays, structured_rectangle_arrays
 from tests.plan_emulator import emulate_matrix
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ERR_ARG = -1  # TM_ERR_ARG
 
 
 def test_library_exports_every_declared_symbol():
@@ -111,3 +112,27 @@ def test_plan_limit_error_is_reported():
     topo = P1Topology(pts, conn, len(pts), "tetra")
     with pytest.raises(_lib.TMError, match="single row"):
         topo.host_plan("scalar", max_rows=8, lds_budget=256)
+
+
+def test_assembly_rejects_label_dependent_quadrature():
+    """Plan records may list an element's vertices in any order, so the entry
+    points refuse moments that depend on the labels (checked on the host,
+    before any device work)."""
+    topo = P1Topology(*kuhn_cube_arrays(2)[:2], 27, "tetra")
+    L = _lib.lib()
+    plan = _lib.PlanDev()
+    fake = ctypes.c_void_p(64)
+    good = topo.quad(2)
+    bad = topo.quad(2)
+    bad.W[0] *= 1.5
+    for q in (bad,):
+        rc = L.tm_asm_p1_matrix(_lib.FORM_KAPPA_LAPLACE, 4, 3, ctypes.byref(plan), fake, fake, _lib.F32, 1.0,
+                                ctypes.byref(q), fake, _lib.F32, None)
+        assert rc == ERR_ARG and b"relabelling" in L.tm_last_error()
+        rc = L.tm_asm_p1_load(_lib.LOAD_CONST, 4, 3, ctypes.byref(plan), fake, None, _lib.F64, 1.0,
+                              ctypes.byref(q), fake, None)
+        assert rc == ERR_ARG
+    # symmetric moments pass the check (an empty plan launches nothing)
+    rc = L.tm_asm_p1_matrix(_lib.FORM_KAPPA_LAPLACE, 4, 3, ctypes.byref(plan), fake, fake, _lib.F32, 1.0,
+                            ctypes.byref(good), fake, _lib.F32, None)
+    assert rc == 0
