@@ -875,8 +875,9 @@ __global__ __launch_bounds__(NT) void k_sym0_down(int64_t n, const int32_t* __re
   const int sl = threadIdx.x % S;
   const tmx::RowChunk rc = tmx::row_chunk(n);
   for (int64_t i = rc.r0 + threadIdx.x / S; i < rc.r1; i += NT / S) {
+    const float di = sl == 0 ? dr[i] : 0.0f;  // issued before the gathers
     const float acc = row_dot_f<S>(rp, ci, av, dr, i, sl);
-    if (sl == 0) xs[i] = float2{dr[i] * inv, acc * inv};
+    if (sl == 0) xs[i] = float2{di * inv, acc * inv};
   }
 }
 
@@ -941,11 +942,18 @@ __global__ __launch_bounds__(NT) void k_sym0_up(int64_t n, const int32_t* __rest
   const int sl = threadIdx.x % S;
   const tmx::RowChunk rc = tmx::row_chunk(n);
   for (int64_t i = rc.r0 + threadIdx.x / S; i < rc.r1; i += NT / S) {
+    // the row's own terms do not depend on the product: issue them first
+    float2 x = float2{0.0f, 0.0f};
+    float wu = 0.0f, di = 0.0f;
+    if (sl == 0) {
+      x = xs[i];
+      wu = w[i] * ua[agg[i]];
+      di = d[i];
+    }
     const float acc = row_dot_f<S>(rp, ci, av, t, i, sl);
     if (sl == 0) {
-      const float2 x = xs[i];
-      const float y = x.x * aa + x.y * ab + w[i] * ua[agg[i]] + acc;
-      z[i] = s * (double)d[i] * (double)y;
+      const float y = x.x * aa + x.y * ab + wu + acc;
+      z[i] = s * (double)di * (double)y;
     }
   }
 }
@@ -1176,6 +1184,25 @@ __global__ __launch_bounds__(NTF) void k_fcg_spmv(int64_t n, const int32_t* __re
 #pragma unroll
   for (int k = 0; k < 2 * M + 2; ++k) loc[k] = 0.0;
   for (int64_t row = rc.r0 + threadIdx.x / S; row < rc.r1; row += stride * R) {
+    // the row's own vector entries do not depend on the product: load them
+    // before the gathers so their latency overlaps
+    double zi[R], ri[R], qi[R][M], pi[R][M];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int64_t i = row + r * stride;
+      zi[r] = ri[r] = 0.0;
+#pragma unroll
+      for (int j = 0; j < M; ++j) qi[r][j] = pi[r][j] = 0.0;
+      if (!START && sl == 0 && i < rc.r1) {
+        zi[r] = w.z[i];
+        ri[r] = w.r[i];
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+          qi[r][j] = w.Q[(int64_t)j * n + i];
+          if (M > 1) pi[r][j] = w.P[(int64_t)j * n + i];
+        }
+      }
+    }
     double o[R][1];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -1189,15 +1216,14 @@ __global__ __launch_bounds__(NTF) void k_fcg_spmv(int64_t n, const int32_t* __re
       if (sl == 0 && i < rc.r1) {
         w.w[i] = o[r][0];
         if (!START) {
-          const double zi = w.z[i], ri = w.r[i];
 #pragma unroll
           for (int j = 0; j < M; ++j) {
-            loc[j] += zi * w.Q[(int64_t)j * n + i];
+            loc[j] += zi[r] * qi[r][j];
             // with one direction (p_old, r) = 0: alpha made the new residual orthogonal to it
-            if (M > 1) loc[M + j] += w.P[(int64_t)j * n + i] * ri;
+            if (M > 1) loc[M + j] += pi[r][j] * ri[r];
           }
-          loc[2 * M] += zi * ri;
-          loc[2 * M + 1] += zi * o[r][0];
+          loc[2 * M] += zi[r] * ri[r];
+          loc[2 * M + 1] += zi[r] * o[r][0];
         }
       }
     }
