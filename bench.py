@@ -77,8 +77,8 @@ def learned_leg(n: int, seeds, log_fn):
     if not os.path.exists(wpath):
         return {"skipped": f"no trained weights at {os.path.relpath(wpath)}"}
     t0 = time.perf_counter()
-    prob = PoissonCube(n, dev, n_levels=4)
     net = GNNPreconditioner.load(wpath, dev)
+    prob = PoissonCube(n, dev, n_levels=net.n_levels)
     solver = LearnedFCG(net, prob.hier, prob.csr, chunk=8, m=1)
     log_fn(f"learned: {n}^3 set-up {time.perf_counter() - t0:.1f}s, levels {[l.n for l in prob.hier.levels]}")
     # warm-up (kernel loading, graph capture) on a training-range seed
@@ -248,6 +248,24 @@ def main():
     kappa = torch.tensor(kappa_np, dtype=torch.float32, device=dev)
     pd = {"kappa": kappa}
     E, N, nnz, nb = len(conn), len(pts), asm.nnz, 4
+    # the same pattern built on the GPU from device connectivity, checked against the host one
+    from tensormesh_amd.assemble.topology import device_element_slots, device_pattern
+    conn_dev = torch.from_numpy(conn).to(dev)
+    device_pattern(conn_dev, N)   # warm-up (code-object load)
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    rp_d, ci_d, pwork = device_pattern(conn_dev, N)
+    torch.cuda.synchronize(dev)
+    t_pattern_dev = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    slots_d = device_element_slots(conn_dev, rp_d, ci_d, pwork, N)
+    torch.cuda.synchronize(dev)
+    t_slots_dev = time.perf_counter() - t0
+    pattern_exact = bool(np.array_equal(rp_d.cpu().numpy(), asm.topology.row_ptr)
+                         and np.array_equal(ci_d.cpu().numpy(), asm.topology.col_idx))
+    log(f"device pattern {t_pattern_dev * 1e3:.1f} ms (bit-exact vs host: {pattern_exact}), "
+        f"element slots {t_slots_dev * 1e3:.1f} ms")
+    del rp_d, ci_d, pwork, slots_d, conn_dev
 
     if args.calibrate_copy:
         src = torch.empty(args.calibrate_copy << 18, dtype=torch.float32, device=dev).uniform_()
@@ -291,7 +309,9 @@ def main():
                                f"kappa=exp(GRF l=0.1 sigma=1 seed 0), kappa-Laplace assembly into CSR",
                    "elements": E, "nodes": N, "nnz": nnz, "values_stored": "f32",
                    "geometry_and_accumulation": "f64", "setup_pattern_s": round(t_pattern, 2),
-                   "setup_plan_s": round(t_plan, 2)},
+                   "setup_plan_s": round(t_plan, 2), "setup_pattern_device_s": round(t_pattern_dev, 4),
+                   "setup_element_slots_device_s": round(t_slots_dev, 4),
+                   "pattern_device_bit_exact": pattern_exact},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
                      "frac": round(achieved / 8000.0, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": b_asm},

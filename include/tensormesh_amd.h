@@ -58,6 +58,21 @@ int tm_pattern_rowptr(const int64_t* conn, int64_t n_elem, int nb, int64_t n_poi
 int tm_pattern_colidx(const int64_t* conn, int64_t n_elem, int nb, int64_t n_points,
                       const int32_t* row_ptr, int32_t* col_idx);
 
+/* The same pattern and P_E built on the device from device connectivity
+ * (conn [E, nb] int64): bit-identical to tm_pattern_rowptr / tm_pattern_colidx
+ * / tm_element_slots.  work holds tm_pattern_dev_work_size(E, nb, N) bytes and
+ * is shared by the three calls.  tm_pattern_dev_rows writes row_ptr[N+1] and
+ * returns after one host synchronisation (row_ptr[N] is nnz); the caller then
+ * sizes col_idx[nnz] for tm_pattern_dev_cols.  Rows are limited to 1024
+ * element-vertex incidences. */
+int64_t tm_pattern_dev_work_size(int64_t n_elem, int nb, int64_t n_points);
+int tm_pattern_dev_rows(const int64_t* conn, int64_t n_elem, int nb, int64_t n_points, void* work,
+                        int32_t* row_ptr, void* stream);
+int tm_pattern_dev_cols(const int64_t* conn, int64_t n_elem, int nb, int64_t n_points, const void* work,
+                        const int32_t* row_ptr, int32_t* col_idx, void* stream);
+int tm_element_slots_dev(const int64_t* conn, int64_t n_elem, int nb, const int32_t* row_ptr,
+                         const int32_t* col_idx, void* work, int64_t n_points, int64_t* slots, void* stream);
+
 /* Element->edge map P_E[e*nb*nb + i*nb + j] = slot of (conn[e,i], conn[e,j]).
  * Replaces element_assembler.py:858-866 (eids_csr[elem_u, elem_v]). */
 int tm_element_slots(const int64_t* conn, int64_t n_elem, int nb, const int32_t* row_ptr,
@@ -149,6 +164,39 @@ int tm_spmv(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const voi
  * (condense.py:294-300). */
 int tm_gather(int64_t n, const int32_t* idx, const void* src, void* dst, int elem_bytes,
               void* stream);
+
+/* Device condensation layout (Condenser._compute_layout, condense.py:182-220)
+ * of a canonical CSR matrix [n x n] and a Dirichlet mask (uint8, nonzero =
+ * Dirichlet).  tm_condense_count fills local[n] (an inner DOF's rank among
+ * the inner DOFs, an outer DOF's rank among the outer DOFs, both in increasing
+ * global order) and, when sizes is given, the device int64 sizes[4] = {n_inner,
+ * n_outer, nnz_ii, nnz_io}; work holds tm_condense_work_size(n) bytes and must
+ * be kept for tm_condense_fill, which writes, in the original edge order:
+ * rp_ii[n_inner+1], ci_ii[nnz_ii] (local columns) and e_ii[nnz_ii] (edge ids)
+ * of the inner-inner block, the same for the inner-outer block, inner_dof /
+ * outer_dof (global id of each local DOF), optionally coo_ii[2][nnz_ii] and
+ * coo_io[2][nnz_io] (int64 local row and column) and the uint8 edge masks
+ * is_inner_edge[nnz] / is_ou2in_edge[nnz]. */
+int64_t tm_condense_work_size(int64_t n);
+int tm_condense_count(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const uint8_t* dirichlet,
+                      void* work, int32_t* local, int64_t* sizes, void* stream);
+int tm_condense_fill(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const uint8_t* dirichlet,
+                     const void* work, const int32_t* local, int32_t* rp_ii, int32_t* ci_ii, int32_t* e_ii,
+                     int32_t* rp_io, int32_t* ci_io, int32_t* e_io, int32_t* inner_dof, int32_t* outer_dof,
+                     int64_t* coo_ii, int64_t* coo_io, uint8_t* is_inner_edge, uint8_t* is_ou2in_edge,
+                     void* stream);
+/* Condensed right-hand side f_i - K_io u_o (Condenser.__call__ / condense_rhs,
+ * condense.py:294-309, 388-427): out[l] = rhs[inner_dof[l]] - (K_io u_outer)[l]
+ * with K_io given by rp_io/ci_io and values v_io (f32 or f64); u_outer may be
+ * NULL (zero Dirichlet values).  rhs, u_outer and out are f64. */
+int tm_condense_rhs(int64_t n_inner, const int32_t* inner_dof, const double* rhs, const int32_t* rp_io,
+                    const int32_t* ci_io, const void* v_io, int v_dtype, const double* u_outer, double* out,
+                    void* stream);
+/* Full solution from the inner one (Condenser.recover, condense.py:429-458):
+ * out[i] = u_outer[local[i]] on Dirichlet DOFs (0 when u_outer is NULL, which
+ * is Condenser.prolong), u_inner[local[i]] elsewhere; f64. */
+int tm_condense_recover(int64_t n, const uint8_t* dirichlet, const int32_t* local, const double* u_inner,
+                        const double* u_outer, double* out, void* stream);
 
 /* Jacobi-PCG for SPD CSR A: stops when ||b - A x||_2 <= max(rtol ||b||_2, atol)
  * (recursive residual) or after maxiter iterations.  x holds the initial guess

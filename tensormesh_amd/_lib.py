@@ -83,6 +83,17 @@ _SIGS = {
                                         ctypes.POINTER(Quad), c_void_p, c_void_p]),
     "tm_spmv": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "tm_gather": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+    "tm_pattern_dev_work_size": (c_i64, [c_i64, c_int, c_i64]),
+    "tm_pattern_dev_rows": (c_int, [c_void_p, c_i64, c_int, c_i64, c_void_p, c_void_p, c_void_p]),
+    "tm_pattern_dev_cols": (c_int, [c_void_p, c_i64, c_int, c_i64, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "tm_element_slots_dev": (c_int, [c_void_p, c_i64, c_int, c_void_p, c_void_p, c_void_p, c_i64, c_void_p,
+                                     c_void_p]),
+    "tm_condense_work_size": (c_i64, [c_i64]),
+    "tm_condense_count": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "tm_condense_fill": (c_int, [c_i64] + [c_void_p] * 17 + [c_void_p]),
+    "tm_condense_rhs": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
+                                c_void_p, c_void_p]),
+    "tm_condense_recover": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "tm_spmv_bsr3": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "tm_pcg_jacobi_bsr3": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                                    c_double, c_double, c_i64, c_int, ctypes.POINTER(c_i64),

@@ -197,3 +197,37 @@ def topology_for(points: torch.Tensor, conn: torch.Tensor, element_type: str) ->
 
 def clear_topology_cache():
     _KEEP.clear()
+
+
+def device_pattern(conn: torch.Tensor, n_points: int):
+    """Canonical CSR pattern of device connectivity ``conn [E, nb]`` built on
+    the GPU (``tm_pattern_dev_*``): ``(row_ptr, col_idx, work)`` with int32
+    ``row_ptr [N+1]`` / ``col_idx [nnz]``, bit-identical to the host pattern
+    (``element_assembler.py:838-857``); ``work`` serves
+    :func:`device_element_slots`.  One host synchronisation (the size of
+    ``col_idx``)."""
+    _lib.require_gpu(conn, "connectivity")
+    c = conn.detach().to(torch.int64).contiguous()
+    E, nb = int(c.shape[0]), int(c.shape[1])
+    L = _lib.lib()
+    dev, st = c.device, _lib.stream_of(c.device)
+    work = torch.empty(max(1, int(L.tm_pattern_dev_work_size(E, nb, n_points))), dtype=torch.uint8, device=dev)
+    row_ptr = torch.empty(n_points + 1, dtype=torch.int32, device=dev)
+    _lib.check(L.tm_pattern_dev_rows(_lib.ptr(c), E, nb, n_points, _lib.ptr(work), _lib.ptr(row_ptr), st))
+    nnz = int(row_ptr[n_points].item())
+    col_buf = torch.empty(max(1, nnz), dtype=torch.int32, device=dev)
+    _lib.check(L.tm_pattern_dev_cols(_lib.ptr(c), E, nb, n_points, _lib.ptr(work), _lib.ptr(row_ptr),
+                                     _lib.ptr(col_buf), st))
+    return row_ptr, col_buf[:nnz], work
+
+
+def device_element_slots(conn: torch.Tensor, row_ptr: torch.Tensor, col_idx: torch.Tensor, work: torch.Tensor,
+                         n_points: int) -> torch.Tensor:
+    """``P_E`` on the GPU (``element_assembler.py:858-866``): int64
+    ``[E * nb * nb]`` positions of every element pair in the pattern."""
+    c = conn.detach().to(torch.int64).contiguous()
+    E, nb = int(c.shape[0]), int(c.shape[1])
+    slots = torch.empty(max(1, E * nb * nb), dtype=torch.int64, device=c.device)
+    _lib.check(_lib.lib().tm_element_slots_dev(_lib.ptr(c), E, nb, _lib.ptr(row_ptr), _lib.ptr(col_idx),
+                                               _lib.ptr(work), n_points, _lib.ptr(slots), _lib.stream_of(c.device)))
+    return slots[:E * nb * nb]

@@ -20,7 +20,8 @@ PKG = os.path.dirname(HERE)
 LIB = os.path.join(PKG, "libtensormesh_amd.so")
 OBJ = os.path.join(PKG, "_build")
 
-HIP_SOURCES = ["tm_assemble.hip", "tm_krylov.hip", "tm_hyper.hip", "tm_batched.hip", "tm_gnn.hip"]
+HIP_SOURCES = ["tm_assemble.hip", "tm_krylov.hip", "tm_hyper.hip", "tm_batched.hip", "tm_gnn.hip",
+               "tm_condense.hip", "tm_pattern.hip"]
 CXX_SOURCES = ["tm_host.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -841,15 +841,18 @@ __device__ __forceinline__ float row_dot_f(const int32_t* __restrict__ rp, const
                                            int sl) {
   const int b = rp[row], e = rp[row + 1];
   float acc = 0.0f;
-  int k = b + sl;
-  for (; k + 3 * S < e; k += 4 * S) {
-    int c[4];
+  constexpr int U = S <= 4 ? 4 : 2;  // entries per lane per batch, loads issued together
+  for (int k0 = b + sl; k0 < e; k0 += U * S) {
+    int c[U];
+    bool ok[U];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) c[u] = ci[k + u * S];
+    for (int u = 0; u < U; ++u) {
+      ok[u] = k0 + u * S < e;
+      c[u] = ok[u] ? ci[k0 + u * S] : 0;
+    }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) acc += av[k + u * S] * v[c[u]];
+    for (int u = 0; u < U; ++u) acc += ok[u] ? av[k0 + u * S] * v[c[u]] : 0.0f;
   }
-  for (; k < e; k += S) acc += av[k] * v[ci[k]];
 #pragma unroll
   for (int o = S / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kWave);
   return acc;
@@ -1742,7 +1745,7 @@ int fcg_start_m(int64_t n, const int32_t* rp, const int32_t* ci, const float* v,
                 const float* d, FcgWork w, double rtol, double atol, int64_t maxiter, hipStream_t st) {
   const int G = fcg_grid(n);
   TM_HIP(hipMemsetAsync(w.f, 0, sizeof(FcgFlags), st));
-  hipLaunchKernelGGL((k_fcg_spmv<8, M, true>), dim3(G), dim3(NTF), 0, st, n, rp, ci, v, x, w);
+  hipLaunchKernelGGL((k_fcg_spmv<4, M, true>), dim3(G), dim3(NTF), 0, st, n, rp, ci, v, x, w);
   hipLaunchKernelGGL(k_fcg_init<M>, dim3(G), dim3(NTF), 0, st, n, b, d, w);
   hipLaunchKernelGGL(k_fcg_ctl0, dim3(1), dim3(NT), 0, st, w, n, G, rtol * rtol, atol * atol, (long long)maxiter);
   TM_HIP(hipGetLastError());
@@ -1769,7 +1772,7 @@ template <int M>
 int fcg_iterate_m(int64_t n, const int32_t* rp, const int32_t* ci, const float* v, double* x, const float* d,
                   FcgWork w, hipStream_t st) {
   const int G = fcg_grid(n);
-  hipLaunchKernelGGL((k_fcg_spmv<8, M, false>), dim3(G), dim3(NTF), 0, st, n, rp, ci, v, x, w);
+  hipLaunchKernelGGL((k_fcg_spmv<4, M, false>), dim3(G), dim3(NTF), 0, st, n, rp, ci, v, x, w);
   hipLaunchKernelGGL(k_fcg_scalars<M>, dim3(1), dim3(NT), 0, st, w, G);
   hipLaunchKernelGGL(k_fcg_update<M>, dim3(G), dim3(NTF), 0, st, n, x, d, w);
   hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(NT), 0, st, w, n, G);

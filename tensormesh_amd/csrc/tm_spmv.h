@@ -5,32 +5,34 @@
 
 namespace tmx {
 
-// out[i] += sum over entries of one block row, 4-deep unrolled so a lane keeps
-// several independent (col, value, x) load chains in flight.
-template <typename VT, int S, int BS>
+// out[i] += sum over entries of one block row.  Each lane takes entries
+// sl, sl + S, sl + 2S, ... in batches of U: the U column loads of a batch are
+// issued together (predicated past the row end: those lanes load nothing and
+// add zero), then the U value and x loads, so a lane keeps U independent
+// chains in flight instead of walking its entries one by one.
+template <typename VT, int S, int BS, int U = (S <= 4 ? 4 : 2)>
 __device__ __forceinline__ void row_accum(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
                                           const VT* __restrict__ v, const double* __restrict__ x, int64_t row,
                                           int sl, double (&out)[BS]) {
   const int b = rp[row], e = rp[row + 1];
-  int k = b + sl;
-  for (; k + 3 * S < e; k += 4 * S) {
-    int64_t c[4];
+  for (int k0 = b + sl; k0 < e; k0 += U * S) {
+    int64_t c[U];
+    bool ok[U];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) c[u] = ci[k + u * S];
+    for (int u = 0; u < U; ++u) {
+      ok[u] = k0 + u * S < e;
+      c[u] = ok[u] ? ci[k0 + u * S] : 0;
+    }
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < U; ++u)
 #pragma unroll
       for (int i = 0; i < BS; ++i)
 #pragma unroll
-        for (int j = 0; j < BS; ++j)
-          out[i] += (double)v[(int64_t)(k + u * S) * BS * BS + i * BS + j] * x[c[u] * BS + j];
-  }
-  for (; k < e; k += S) {
-    const int64_t c = ci[k];
-#pragma unroll
-    for (int i = 0; i < BS; ++i)
-#pragma unroll
-      for (int j = 0; j < BS; ++j) out[i] += (double)v[(int64_t)k * BS * BS + i * BS + j] * x[c * BS + j];
+        for (int j = 0; j < BS; ++j) {
+          const double a = ok[u] ? (double)v[(int64_t)(k0 + u * S) * BS * BS + i * BS + j] : 0.0;
+          const double xv = ok[u] ? x[c[u] * BS + j] : 0.0;
+          out[i] += a * xv;
+        }
   }
 }
 

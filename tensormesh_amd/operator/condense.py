@@ -6,10 +6,15 @@ rhs=None) -> (K_ii, f_i - K_io u_o)``, ``condense_rhs``, ``recover``,
 ``restrict``, ``prolong``, ``update_dirichlet``.  The layout (inner / outer
 DOF masks, monotone local numbering, inner-inner and inner-outer edge
 masks, ``_compute_layout`` ``:182-220``) is computed once on the matrix's
-device and cached; it is bit-identical to the reference's.  On GPU
-matrices the per-call work is two value gathers (HIP) and one SpMV for
-``K_io u_o``; ``K_ii`` keeps the canonical CSR order, so it carries its CSR
-view into ``solve`` without conversion.
+device and cached; it is bit-identical to the reference's.  For a scalar
+GPU matrix in canonical CSR order the layout comes from the HIP
+condensation kernels (``tm_condense_count`` / ``tm_condense_fill``: prefix
+sums over the rows, one host read of the four sizes), and the per-call work
+is two value gathers plus one kernel for ``f_i - K_io u_o``; ``recover`` and
+``prolong`` are one kernel each.  ``K_ii`` keeps the canonical CSR order, so
+it carries its CSR view into ``solve`` without conversion.  Calls that need
+gradients with respect to the right-hand side, the values or ``u`` take the
+differentiable path (value gathers with autograd, torch indexing).
 
 Deviation: the condensed right-hand side keeps the wider of the rhs and
 matrix dtypes (an FP64 load vector stays FP64 next to FP32 values).
@@ -21,6 +26,7 @@ from typing import Optional
 import torch
 import torch.nn as nn
 
+from .. import _lib
 from ..sparse.matrix import SparseMatrix
 from ..sparse.ops import CSR, Gather
 
@@ -61,7 +67,60 @@ class Condenser(nn.Module):
     def update_dirichlet(self, dirichlet_value: torch.Tensor):
         self.dirichlet_value = self._normalize_value(dirichlet_value)
 
+    def _device_layout(self, matrix: SparseMatrix) -> bool:
+        """Layout from the HIP condensation kernels; False when the matrix is
+        not a scalar canonical-CSR GPU matrix."""
+        dev = matrix.device
+        if dev.type != "cuda" or matrix.block_size != 1 or matrix._csr is None or matrix._perm is not None:
+            return False
+        csr = matrix._csr
+        L = _lib.lib()
+        st = _lib.stream_of(dev)
+        n, nnz = matrix.shape[0], int(csr.col_idx.numel())
+        i32, i64, u8 = torch.int32, torch.int64, torch.uint8
+        dirm = self.dirichlet_mask.to(dev).to(u8).contiguous()
+        work = torch.empty(int(L.tm_condense_work_size(n)), dtype=u8, device=dev)
+        local = torch.empty(n, dtype=i32, device=dev)
+        sizes = torch.empty(4, dtype=i64, device=dev)
+        _lib.check(L.tm_condense_count(n, _lib.ptr(csr.row_ptr), _lib.ptr(csr.col_idx), _lib.ptr(dirm),
+                                       _lib.ptr(work), _lib.ptr(local), _lib.ptr(sizes), st))
+        n_inner, n_outer, nnz_ii, nnz_io = (int(v) for v in sizes.cpu())
+
+        # at least one element each, so every pointer is valid (an empty
+        # tensor's data_ptr() is 0); the attributes are views of length k
+        sz = dict(rp_ii=(n_inner + 1, i32), ci_ii=(nnz_ii, i32), e_ii=(nnz_ii, i32), rp_io=(n_inner + 1, i32),
+                  ci_io=(nnz_io, i32), e_io=(nnz_io, i32), inner_dof=(n_inner, i32), outer_dof=(n_outer, i32),
+                  coo_ii=(2 * nnz_ii, i64), coo_io=(2 * nnz_io, i64), m_ii=(nnz, u8), m_io=(nnz, u8))
+        full = {k: torch.empty(max(1, c), dtype=dt, device=dev) for k, (c, dt) in sz.items()}
+        _lib.check(L.tm_condense_fill(n, _lib.ptr(csr.row_ptr), _lib.ptr(csr.col_idx), _lib.ptr(dirm),
+                                      _lib.ptr(work), _lib.ptr(local),
+                                      *(_lib.ptr(full[k]) for k in ("rp_ii", "ci_ii", "e_ii", "rp_io", "ci_io",
+                                                                    "e_io", "inner_dof", "outer_dof", "coo_ii",
+                                                                    "coo_io", "m_ii", "m_io")), st))
+        v = {k: full[k][:c] for k, (c, _) in sz.items()}
+        rp_ii, ci_ii, e_ii, rp_io, ci_io, e_io = (v[k] for k in ("rp_ii", "ci_ii", "e_ii", "rp_io", "ci_io", "e_io"))
+        inner_dof, outer_dof, coo_ii, coo_io, m_ii, m_io = (v[k] for k in ("inner_dof", "outer_dof", "coo_ii",
+                                                                            "coo_io", "m_ii", "m_io"))
+        self.inner_row, self.inner_col = coo_ii[:nnz_ii], coo_ii[nnz_ii:]
+        self.ou2in_row, self.ou2in_col = coo_io[:nnz_io], coo_io[nnz_io:]
+        self.is_inner_edge, self.is_ou2in_edge = m_ii.view(torch.bool), m_io.view(torch.bool)
+        self.is_inner_dof = dirm == 0
+        self.is_outer_dof = dirm != 0
+        self.inner_shape = (n_inner, n_inner)
+        self.ou2in_shape = (n_inner, n_outer)
+        self.n_inner_dof, self.n_outer_dof, self.n_dof = n_inner, n_outer, n
+        self.layout_signature = matrix.layout_signature
+        self._dev = dict(
+            ii=e_ii, ii_long=e_ii.to(i64), io=e_io, io_long=e_io.to(i64),
+            csr_ii=CSR(rp_ii, ci_ii, n_inner, n_inner), csr_io=CSR(rp_io, ci_io, n_inner, n_outer),
+            dirm=dirm, local=local, inner_dof=inner_dof, outer_dof=outer_dof,
+            inner_dof_buf=full["inner_dof"], ci_io_buf=full["ci_io"], rp_io_buf=full["rp_io"],
+        )
+        return True
+
     def _compute_layout(self, matrix: SparseMatrix):
+        if self._device_layout(matrix):
+            return
         dev = matrix.device
         edge_u, edge_v = matrix.row, matrix.col
         n_dof = matrix.shape[0]
@@ -130,8 +189,28 @@ class Condenser(nn.Module):
         self.K_ou2in = K_ou2in
         return K_inner, self._rhs(rhs, K_inner.edata)
 
+    def _kernel_ok(self, *ts) -> bool:
+        d = self._dev
+        return (d is not None and "inner_dof" in d and all(t.is_cuda for t in ts)
+                and not (torch.is_grad_enabled() and any(t.requires_grad for t in ts)))
+
     def _rhs(self, rhs: torch.Tensor, values: torch.Tensor) -> torch.Tensor:
         dt = torch.promote_types(rhs.dtype, values.dtype)
+        if rhs.dim() == 1 and self._kernel_ok(rhs, values, self.dirichlet_value.to(values.device)):
+            d = self._dev
+            u_o = self.dirichlet_value.to(device=values.device, dtype=torch.float64).contiguous()
+            self.dirichlet_value = self.dirichlet_value.to(device=values.device, dtype=dt)
+            r64 = rhs.to(device=values.device, dtype=torch.float64).contiguous()
+            out = torch.empty(max(1, self.n_inner_dof), dtype=torch.float64, device=values.device)
+            csr_io = d["csr_io"]
+            has_io = csr_io.col_idx.numel() > 0
+            _lib.check(_lib.lib().tm_condense_rhs(self.n_inner_dof, _lib.ptr(d["inner_dof_buf"]), _lib.ptr(r64),
+                                                  _lib.ptr(d["rp_io_buf"]), _lib.ptr(d["ci_io_buf"]),
+                                                  _lib.ptr(self.K_ou2in.values) if has_io else _lib.ptr(None),
+                                                  _lib.dtype_code(self.K_ou2in.values),
+                                                  _lib.ptr(u_o) if has_io else _lib.ptr(None),
+                                                  _lib.ptr(out), _lib.stream_of(values.device)))
+            return out[:self.n_inner_dof].to(dt)
         rhs = rhs.to(device=values.device, dtype=dt)
         self.dirichlet_value = self.dirichlet_value.to(device=values.device, dtype=dt)
         minus = (self.K_ou2in @ self.dirichlet_value).to(dt)
@@ -144,8 +223,20 @@ class Condenser(nn.Module):
             "call the Condenser on a matrix first; condense_rhs reuses the cached K_io block"
         return self._rhs(rhs, self.K_ou2in.edata)
 
+    def _recover_kernel(self, u: torch.Tensor, u_outer: Optional[torch.Tensor]) -> torch.Tensor:
+        d = self._dev
+        u64 = u.detach().to(torch.float64).contiguous()
+        uo = None if u_outer is None else u_outer.to(device=u.device, dtype=torch.float64).contiguous()
+        out = torch.empty(self.n_dof, dtype=torch.float64, device=u.device)
+        _lib.check(_lib.lib().tm_condense_recover(self.n_dof, _lib.ptr(d["dirm"]), _lib.ptr(d["local"]),
+                                                  _lib.ptr(u64), _lib.ptr(uo) if self.n_outer_dof else
+                                                  _lib.ptr(None), _lib.ptr(out), _lib.stream_of(u.device)))
+        return out.to(u.dtype)
+
     def recover(self, u: torch.Tensor) -> torch.Tensor:
         assert u.shape[0] == self.n_inner_dof, f"the shape of u must be [{self.n_inner_dof}, ...], but got {u.shape}"
+        if u.dim() == 1 and u.is_floating_point() and self._kernel_ok(u) and self.n_inner_dof > 0:
+            return self._recover_kernel(u, self.dirichlet_value)
         shape = list(u.shape)
         shape[0] = self.n_dof
         out = torch.zeros(shape, dtype=u.dtype, device=u.device)
@@ -162,6 +253,9 @@ class Condenser(nn.Module):
         return f[~self.dirichlet_mask.to(f.device)]
 
     def prolong(self, f_inner: torch.Tensor) -> torch.Tensor:
+        if (f_inner.dim() == 1 and f_inner.is_floating_point() and self._kernel_ok(f_inner)
+                and f_inner.shape[0] == self.n_inner_dof and self.n_inner_dof > 0):
+            return self._recover_kernel(f_inner, None)
         mask = self.dirichlet_mask.to(f_inner.device)
         n_inner = int((~mask).sum())
         assert f_inner.shape[0] == n_inner, f"the shape of f_inner must be [{n_inner}, ...], but got {f_inner.shape}"
