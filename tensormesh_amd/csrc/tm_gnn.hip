@@ -1038,23 +1038,36 @@ __global__ __launch_bounds__(NT) void k_cosine_grad(int64_t n, const double* __r
 // Level set-up for one matrix
 // ---------------------------------------------------------------------------
 // d_i = A_ii^-1/2 (1 where the diagonal is missing or not positive)
+// Row kernels of the level set-up: G lanes per row, so a wave's loads and
+// stores cover G consecutive entries of each of its rows (coalesced); the
+// outer loop is uniform over the workgroup, so the shuffles see whole groups.
+constexpr int kRowG = 16;
+
 __global__ __launch_bounds__(NT) void k_level_diag(int64_t n, const int32_t* __restrict__ rp,
                                                    const int32_t* __restrict__ ci, const float* __restrict__ v,
                                                    float* __restrict__ d) {
-  for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+  const int sub = threadIdx.x % kRowG;
+  for (int64_t i0 = blockIdx.x * (int64_t)(NT / kRowG); i0 < n; i0 += (int64_t)gridDim.x * (NT / kRowG)) {
+    const int64_t i = i0 + threadIdx.x / kRowG;
     double a = 0.0;
-    for (int k = rp[i]; k < rp[i + 1]; ++k)
-      if (ci[k] == i) a += (double)v[k];
-    d[i] = a > 0.0 ? (float)(1.0 / sqrt(a)) : 1.0f;
+    if (i < n)
+      for (int k = rp[i] + sub; k < rp[i + 1]; k += kRowG)
+        if (ci[k] == i) a += (double)v[k];
+#pragma unroll
+    for (int o = kRowG / 2; o > 0; o >>= 1) a += __shfl_xor(a, o, kRowG);
+    if (sub == 0 && i < n) d[i] = a > 0.0 ? (float)(1.0 / sqrt(a)) : 1.0f;
   }
 }
 
 __global__ __launch_bounds__(NT) void k_level_normalize(int64_t n, const int32_t* __restrict__ rp,
                                                         const int32_t* __restrict__ ci, const float* __restrict__ v,
                                                         const float* __restrict__ d, float* __restrict__ vn) {
-  for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+  const int sub = threadIdx.x % kRowG;
+  for (int64_t i0 = blockIdx.x * (int64_t)(NT / kRowG); i0 < n; i0 += (int64_t)gridDim.x * (NT / kRowG)) {
+    const int64_t i = i0 + threadIdx.x / kRowG;
+    if (i >= n) continue;
     const float di = d[i];
-    for (int k = rp[i]; k < rp[i + 1]; ++k) vn[k] = v[k] * di * d[ci[k]];
+    for (int k = rp[i] + sub; k < rp[i + 1]; k += kRowG) vn[k] = v[k] * di * d[ci[k]];
   }
 }
 
@@ -1095,10 +1108,15 @@ __global__ __launch_bounds__(NT) void k_level_prolong(int64_t nnz_b, const int32
 
 __global__ __launch_bounds__(NT) void k_row_sums(int64_t n, const int32_t* __restrict__ rp,
                                                  const float* __restrict__ v, float* __restrict__ out) {
-  for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+  const int sub = threadIdx.x % kRowG;
+  for (int64_t i0 = blockIdx.x * (int64_t)(NT / kRowG); i0 < n; i0 += (int64_t)gridDim.x * (NT / kRowG)) {
+    const int64_t i = i0 + threadIdx.x / kRowG;
     double a = 0.0;
-    for (int k = rp[i]; k < rp[i + 1]; ++k) a += (double)v[k];
-    out[i] = (float)a;
+    if (i < n)
+      for (int k = rp[i] + sub; k < rp[i + 1]; k += kRowG) a += (double)v[k];
+#pragma unroll
+    for (int o = kRowG / 2; o > 0; o >>= 1) a += __shfl_xor(a, o, kRowG);
+    if (sub == 0 && i < n) out[i] = (float)a;
   }
 }
 
@@ -1527,7 +1545,7 @@ extern "C" int tm_gnn_level_prolong(int64_t nnz_b, const int32_t* seg, const int
 extern "C" int tm_gnn_row_sums(int64_t n, const int32_t* row_ptr, const float* values, float* out, void* stream) {
   TM_REQUIRE(n >= 0 && row_ptr && values && out, TM_ERR_ARG, "null argument");
   if (n == 0) return TM_OK;
-  hipLaunchKernelGGL(k_row_sums, dim3(grid_of(n, NT, 8192)), dim3(NT), 0, (hipStream_t)stream, n, row_ptr, values,
+  hipLaunchKernelGGL(k_row_sums, dim3(grid_of(n, NT / kRowG, 8192)), dim3(NT), 0, (hipStream_t)stream, n, row_ptr, values,
                      out);
   TM_HIP(hipGetLastError());
   return TM_OK;
@@ -1678,7 +1696,7 @@ extern "C" int tm_gnn_level_setup(int64_t n, const int32_t* row_ptr, const int32
   TM_REQUIRE(n >= 0 && row_ptr && col_idx && values && d && values_normalized, TM_ERR_ARG, "null argument");
   if (n == 0) return TM_OK;
   hipStream_t st = (hipStream_t)stream;
-  const int g = grid_of(n, NT, 8192);
+  const int g = grid_of(n, NT / kRowG, 8192);
   hipLaunchKernelGGL(k_level_diag, dim3(g), dim3(NT), 0, st, n, row_ptr, col_idx, values, d);
   hipLaunchKernelGGL(k_level_normalize, dim3(g), dim3(NT), 0, st, n, row_ptr, col_idx, values, d, values_normalized);
   TM_HIP(hipGetLastError());
