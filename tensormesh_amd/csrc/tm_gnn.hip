@@ -31,6 +31,7 @@ constexpr int F = 64;          // feature width
 constexpr int XP = 2 * F + 8;  // LDS pitch (BF16 elements) of a [h | a] row
 constexpr int GP = F + 8;      // LDS pitch of a 64-wide BF16 row
 constexpr int TILE = 64;       // node rows per tile (4 waves x 16 rows)
+constexpr int64_t kSmallLevel = 65536;  // inference layers below this many rows use 16-row tiles
 constexpr int NT = 256;
 constexpr int NTF = 512;  // flexible-CG vector kernels
 constexpr int kMaxParts = 1024;
@@ -205,6 +206,79 @@ __global__ __launch_bounds__(NT) void k_mp(int64_t n, const int32_t* __restrict_
     const int i = wv * 16 + c / 8, k8 = (c % 8) * 8;
     if (r0 + i < n) *(uint4*)&out[(r0 + i) * F + k8] = *(const uint4*)&X[i * XP + k8];
   }
+}
+
+// Inference layer for small levels (few rows, so k_mp's 64-row tiles leave
+// most CUs idle and each lane walks ~7 batches of dependent gathers): 16-row
+// tiles, 16 lanes per row with 4 features each, 8 neighbour rows in flight.
+// Each feature's message is summed over the row's entries in CSR order and
+// rounded like k_mp, and the MFMA sees the same [h | a] tile, so the result
+// is bit-identical to k_mp<false>; wave w forms output columns 16w..16w+15.
+constexpr int TS = 16;  // rows per tile
+__global__ __launch_bounds__(NT) void k_mp_small(int64_t n, const int32_t* __restrict__ rp,
+                                                 const int32_t* __restrict__ ci, const float* __restrict__ av,
+                                                 const uint16_t* __restrict__ h, const uint16_t* __restrict__ wt,
+                                                 const float* __restrict__ bias, int n_lin,
+                                                 uint16_t* __restrict__ out, const int* stop) {
+  if (stopped(stop)) return;
+  __shared__ __align__(16) uint16_t X[TS * XP];
+  const int tid = threadIdx.x, wv = tid / kWave, l = tid % kWave;
+  const int64_t r0 = (int64_t)blockIdx.x * TS;
+  const int ri = tid / 16, f0 = (tid % 16) * 4;
+  const int64_t row = r0 + ri;
+  float a[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  uint2 hv = uint2{0u, 0u};
+  if (row < n) {
+    hv = *(const uint2*)&h[row * F + f0];
+    const int kb = rp[row], ke = rp[row + 1];
+    for (int k = kb; k < ke; k += 8) {
+      int c[8];
+      float v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const bool in = k + q < ke;
+        c[q] = in ? ci[k + q] : (int)row;
+        v[q] = in ? av[k + q] : 0.0f;
+      }
+      uint2 hq[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) hq[q] = *(const uint2*)&h[(int64_t)c[q] * F + f0];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        a[0] += v[q] * bf2f((uint16_t)(hq[q].x & 0xffffu));
+        a[1] += v[q] * bf2f((uint16_t)(hq[q].x >> 16));
+        a[2] += v[q] * bf2f((uint16_t)(hq[q].y & 0xffffu));
+        a[3] += v[q] * bf2f((uint16_t)(hq[q].y >> 16));
+      }
+    }
+  }
+  *(uint2*)&X[ri * XP + f0] = hv;
+  *(uint2*)&X[ri * XP + F + f0] = uint2{pack2(a[0], a[1]), pack2(a[2], a[3])};
+  __syncthreads();
+  // wave wv: columns jt = wv of the 16-row block
+  f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  {
+    const int ar = l % 16, ak = 8 * (l / 16), jt = wv;
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      const bf16x8 xa = lds8(&X[ar * XP + kk * 32 + ak]);
+      acc = mfma(xa, __builtin_bit_cast(bf16x8, *(const uint4*)&wt[(jt * 16 + (l % 16)) * 2 * F + kk * 32 + ak]),
+                 acc);
+    }
+  }
+  __syncthreads();  // every wave has read X before the epilogue overwrites it
+  {
+    const int j = wv * 16 + (l % 16);
+    const float bj = bias[j];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = 4 * (l / 16) + v;
+      const float u = acc[v] + bj;
+      X[i * XP + j] = f2bf(bf2f(X[i * XP + j]) + act(u, j, n_lin));
+    }
+  }
+  __syncthreads();
+  if (row < n) *(uint2*)&out[row * F + f0] = *(const uint2*)&X[ri * XP + f0];
 }
 
 // ---------------------------------------------------------------------------
@@ -1436,6 +1510,9 @@ extern "C" int tm_gnn_mp_forward(int64_t n, const int32_t* row_ptr, const int32_
   if (save_msg)
     hipLaunchKernelGGL(k_mp<true>, dim3((unsigned)tiles), dim3(NT), 0, st, n, row_ptr, col_idx, values, h, w_t,
                        bias, n_linear, out, save_msg, save_pre, stop);
+  else if (n < kSmallLevel)
+    hipLaunchKernelGGL(k_mp_small, dim3((unsigned)((n + TS - 1) / TS)), dim3(NT), 0, st, n, row_ptr, col_idx, values,
+                       h, w_t, bias, n_linear, out, stop);
   else
     hipLaunchKernelGGL(k_mp<false>, dim3((unsigned)tiles), dim3(NT), 0, st, n, row_ptr, col_idx, values, h, w_t,
                        bias, n_linear, out, save_msg, save_pre, stop);

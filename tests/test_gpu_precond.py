@@ -98,6 +98,30 @@ def test_mp_layer_forward(cuda):
     assert np.abs(pre.double().cpu().numpy() - u).max() <= 2 ** -7 * np.abs(u).max()
 
 
+def test_small_level_layer_matches_tiled_layer(cuda):
+    """Inference on levels under 65536 rows runs 16-row tiles (k_mp_small); the
+    training path (activations saved) runs the 64-row tiles: same bits."""
+    print("case precond mp small-vs-tiled n=8", flush=True)
+    Ai, H, hier, levels, values, csr, lk = _problem(8, 2, cuda)
+    L = _lib.lib()
+    g = np.random.default_rng(5)
+    for lev in hier.levels[:2]:
+        assert lev.n < 65536
+        ht = torch.tensor(og.bf16_round(g.standard_normal((lev.n, 64))), dtype=torch.bfloat16, device=cuda)
+        wt = torch.tensor(og.bf16_round(g.standard_normal((128, 64)) / 8.0), dtype=torch.bfloat16,
+                          device=cuda).t().contiguous()
+        bt = torch.tensor(g.standard_normal(64).astype(np.float32), device=cuda)
+        outs = []
+        for save in (True, False):
+            out = torch.empty_like(ht)
+            msg, pre = (torch.empty_like(ht), torch.empty_like(ht)) if save else (None, None)
+            _lib.check(L.tm_gnn_mp_forward(lev.n, _lib.ptr(lev.row_ptr), _lib.ptr(lev.col), _lib.ptr(lev.vn),
+                                           _lib.ptr(ht), _lib.ptr(wt), _lib.ptr(bt), 32, _lib.ptr(out),
+                                           _lib.ptr(msg), _lib.ptr(pre), None, _lib.stream_of(cuda)))
+            outs.append(out)
+        assert torch.equal(outs[0].view(torch.int16), outs[1].view(torch.int16)), f"level with {lev.n} rows"
+
+
 def test_network_forward(cuda):
     print("case precond forward n=8", flush=True)
     Ai, H, hier, levels, values, csr, lk = _problem(8, 3, cuda)
