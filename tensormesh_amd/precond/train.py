@@ -188,13 +188,13 @@ def main(argv=None):
     dev = torch.device("cuda:0")
     schedule = tuple(int(s) for s in args.schedule.split(","))
     t0 = time.perf_counter()
-    prob = PoissonCube(args.n, dev, n_levels=max(args.levels, max(schedule) + 1, args.bottom + 1))
-    log(f"{args.n}^3: {prob.n_inner} unknowns, levels {[l.n for l in prob.hier.levels]} "
-        f"({time.perf_counter() - t0:.1f}s)")
     net = (GNNPreconditioner.load(args.init, dev) if args.init
            else GNNPreconditioner(schedule, args.n_linear, dev, homogeneous=not args.general,
                                   symmetric=not (args.nonsymmetric or args.general),
                                   bottom=None if args.bottom < 0 else args.bottom))
+    prob = PoissonCube(args.n, dev, n_levels=max(args.levels, net.n_levels))
+    log(f"{args.n}^3: {prob.n_inner} unknowns, levels {[l.n for l in prob.hier.levels]} "
+        f"({time.perf_counter() - t0:.1f}s)")
     report = {"n": args.n, "schedule": list(net.schedule), "n_linear": net.n_linear,
               "homogeneous": net.homogeneous, "symmetric": net.symmetric, "bottom": net.bottom}
     if args.train > 0 and args.steps > 0:
