@@ -324,7 +324,9 @@ int make_blocks(const PlanCtx& c, Scratch& s, std::vector<int32_t> owned,
       for (int a = 0; a < c.nb; ++a)
         if (s.local[c.conn[(int64_t)elems[i] * c.nb + a]] < nown) mask |= 1u << a;
       const uint32_t full = (1u << c.nb) - 1;
-      bym[i] = {mask == full ? 0u : 16u - (uint32_t)__builtin_popcount(mask) * 4u + mask, elems[i]};
+      // records list their owned vertices first (below), so the owned count
+      // alone fixes a record's branch pattern in the kernel
+      bym[i] = {mask == full ? 0u : 16u - (uint32_t)__builtin_popcount(mask) * 4u, elems[i]};
     }
     std::stable_sort(bym.begin(), bym.end(),
                      [](const auto& x, const auto& y) { return x.first < y.first; });
@@ -391,8 +393,15 @@ int make_blocks(const PlanCtx& c, Scratch& s, std::vector<int32_t> owned,
           }
         }
         const int half = ((r - w0) >> 5) & 1;
-        int best = 0, best_sc = 1 << 30;
+        int n_own_v = 0;
+        for (int a = 0; a < NB; ++a) n_own_v += l0[a] < nown ? 1 : 0;
+        int best = -1, best_sc = 1 << 30;
         for (int q = 0; q < np; ++q) {
+          // owned vertices take the first positions: every lane of a wave of
+          // such records then follows the same branches in the kernel
+          bool owned_first = true;
+          for (int x = 0; x < NB; ++x) owned_first = owned_first && ((l0[perms[q][x]] < nown) == (x < n_own_v));
+          if (!owned_first) continue;
           int sc = 0;
           for (int k = 0; k < ns; ++k) {
             const int a = perms[q][sx[k]], b = perms[q][sy[k]];
