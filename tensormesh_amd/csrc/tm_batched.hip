@@ -12,11 +12,14 @@
 // workgroup leaves the loop, so the kernel always terminates.  The launch is
 // asynchronous (no host synchronisation) and can be captured in a hipGraph.
 #include "tm_device.h"
+#include "tm_spmv.h"
 
 namespace {
 
 using tmx::kWave;
-constexpr int kThreads = 256;
+// One workgroup of 16 waves per CU: the barrier keeps one arrival per CU while
+// the memory-bound phases get enough waves to hide their gather latency.
+constexpr int kThreads = 1024;
 
 struct BatchWork {
   double* r;
@@ -168,9 +171,9 @@ __global__ __launch_bounds__(kThreads) void k_batched_pcg(int64_t n, int n_tiles
       const int r0 = tile_row0[t], r1 = tile_row0[t + 1];
       double loc = 0.0;
       for (int i = r0 + threadIdx.x / S; i < r1; i += per) {
-        double s = 0.0;
-        for (int kk = rp[i] + sl; kk < rp[i + 1]; kk += S) s += (double)v[kk] * w.p[ci[kk]];
-        s = sub_sum<S>(s);
+        double o[1] = {0.0};
+        tmx::row_accum<VT, S, 1>(rp, ci, v, w.p, i, sl, o);  // loads of a lane's entries issued together
+        const double s = sub_sum<S>(o[0]);
         if (sl == 0) {
           w.q[i] = s;
           loc += s * w.p[i];

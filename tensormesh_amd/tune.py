@@ -24,9 +24,9 @@ from typing import Dict, Iterable, Optional
 import torch
 
 DEFAULT_SPACE = {
-    "rows_per_tile": (512, 1024, 2048),
+    "rows_per_tile": (1024, 2048, 4096),
     "lanes": (2, 4, 8),
-    "blocks_per_cu": (1, 2),
+    "blocks_per_cu": (1,),
     "plan_rows": (None,),
 }
 
