@@ -218,8 +218,8 @@ def main():
                     help="MiB of a plain device copy run before the timed region (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(see the sharding design in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the assembly and solver kernels here run on a single "
+              "device (no multi-GPU path); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

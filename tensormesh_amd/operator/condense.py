@@ -71,9 +71,11 @@ class Condenser(nn.Module):
         """Layout from the HIP condensation kernels; False when the matrix is
         not a scalar canonical-CSR GPU matrix."""
         dev = matrix.device
-        if dev.type != "cuda" or matrix.block_size != 1 or matrix._csr is None or matrix._perm is not None:
+        if dev.type != "cuda" or matrix.block_size != 1:
             return False
-        csr = matrix._csr
+        csr = matrix.csr()   # built once for a COO matrix; the solve reuses it
+        if matrix._perm is not None:
+            return False
         L = _lib.lib()
         st = _lib.stream_of(dev)
         n, nnz = matrix.shape[0], int(csr.col_idx.numel())

@@ -5,7 +5,7 @@ side, iterate, flexible-CG work) so that ``chunk`` iterations - the network's
 ~14 kernels plus the three flexible-CG kernels each - are captured once in a
 hipGraph and replayed; the host reads the device status after every replay.
 
-Fallback rules (the converged residual is the judge):
+Fallback rules (the true residual decides convergence):
 
 * breakdown (``p.Aq <= 0`` or non-finite), the iteration cap, or stagnation
   (the recursive residual shrinks by less than ``stall_factor`` over
