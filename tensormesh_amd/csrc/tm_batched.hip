@@ -21,6 +21,9 @@ using tmx::kWave;
 // the memory-bound phases get enough waves to hide their gather latency.
 constexpr int kThreads = 1024;
 
+constexpr int kMaxBarBlocks = 1024;  // arrival slots of the grid barrier
+constexpr int kSlot = 16;            // uint32 stride of an arrival slot (64 B: one line each)
+
 struct BatchWork {
   double* r;
   double* p;
@@ -32,43 +35,57 @@ struct BatchWork {
   double* bb;      // [n_tiles]
   int* status;     // [n_mesh] 0 running, 1 converged, 2 breakdown, 3 max iterations (caller buffer)
   int* iters;      // [n_mesh] (caller buffer)
-  unsigned* bar;   // [0] arrival count, [1] generation
+  unsigned* go;    // [kSlot]: [0] epoch released by workgroup 0
+  unsigned* arrive;  // [kMaxBarBlocks][kSlot]: epoch each workgroup has reached
   int* ctl;        // [0] active systems, [1] barrier timeout flag
 };
 
-// Every wave orders its stores before the workgroup barrier (workgroup-scope
-// release); then one wave per workgroup publishes them at agent scope (the
-// release writes back the XCD's L2, which holds every wave's stores), arrives,
-// waits, and drops stale cached lines (agent-scope acquire invalidates the
-// CU's L1 and the XCD's L2) before the workgroup barrier lets the other waves
-// read.  One fence pair per workgroup instead of one per wave: 25 instead of
-// 80 us per barrier with 256 workgroups of 16 waves (measured on MI355X).
-__device__ __forceinline__ bool grid_barrier(unsigned* bar, int* ctl, unsigned nblocks) {
+// Grid barrier.  Every wave orders its stores before the workgroup barrier
+// (workgroup-scope release); one wave per workgroup then publishes them at
+// agent scope (the release writes back the XCD's L2, which holds every wave's
+// stores) and stores the barrier's epoch in the workgroup's own arrival slot.
+// Workgroup 0 waits until every slot holds the epoch, then releases the epoch
+// in `go`; every other workgroup waits for `go`.  After the wait one wave per
+// workgroup drops stale cached lines (agent-scope acquire: the CU's L1 and
+// the XCD's L2) before the workgroup barrier lets the other waves read.
+// Measured on MI355X with 256 workgroups of 16 waves: 5.7 us per barrier,
+// against 80 us for one atomic counter with fences in every wave.  Every wait
+// is bounded: after 2 s the timeout flag is raised and every workgroup exits.
+__device__ __forceinline__ bool timed_out(const uint64_t t0, int* ctl) {
+  if (__hip_atomic_load(ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return true;
+  if (wall_clock64() - t0 > 200000000ull) {  // wall_clock64 runs at 100 MHz on CDNA: 2 s
+    __hip_atomic_store(ctl + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+  }
+  return false;
+}
+
+__device__ __forceinline__ bool grid_barrier(const BatchWork& w, unsigned nblocks, unsigned& epoch) {
+  ++epoch;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __syncthreads();
   if (threadIdx.x == 0) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    const unsigned gen = __hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned arrived = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1;
-    if (arrived == nblocks) {
-      __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(bar + 1, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      const uint64_t t0 = wall_clock64();
-      // wall_clock64 runs at a fixed 100 MHz on CDNA: 2e8 ticks = 2 s
-      while (__hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-        if (__hip_atomic_load(ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-        if (wall_clock64() - t0 > 200000000ull) {
-          __hip_atomic_store(ctl + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(2);
-      }
+    __hip_atomic_store(w.arrive + (size_t)blockIdx.x * kSlot, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (blockIdx.x == 0) {
+    const uint64_t t0 = wall_clock64();
+    for (unsigned b = threadIdx.x; b < nblocks; b += blockDim.x)
+      while (__hip_atomic_load(w.arrive + (size_t)b * kSlot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch)
+        if (timed_out(t0, w.ctl)) break;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+      __hip_atomic_store(w.go, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
+  } else if (threadIdx.x == 0) {
+    const uint64_t t0 = wall_clock64();
+    while (__hip_atomic_load(w.go, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch)
+      if (timed_out(t0, w.ctl)) break;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
-  return __hip_atomic_load(ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+  return __hip_atomic_load(w.ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
 }
 
 template <int S>
@@ -118,6 +135,7 @@ __global__ __launch_bounds__(kThreads) void k_batched_pcg(int64_t n, int n_tiles
   __shared__ double red[kThreads / kWave];
   __shared__ double sh[8];
   const unsigned G = gridDim.x;
+  unsigned epoch = 0;
   const int sl = threadIdx.x % S;
   const int per = kThreads / S;
 
@@ -151,7 +169,7 @@ __global__ __launch_bounds__(kThreads) void k_batched_pcg(int64_t n, int n_tiles
       w.status[k] = 0;
       w.iters[k] = 0;
     }
-  if (!grid_barrier(w.bar, w.ctl, G)) return;
+  if (!grid_barrier(w, G, epoch)) return;
   // initial convergence test (designated tile = first tile of each system)
   for (int t = blockIdx.x; t < n_tiles; t += G) {
     const int k = tile_mesh[t];
@@ -164,7 +182,7 @@ __global__ __launch_bounds__(kThreads) void k_batched_pcg(int64_t n, int n_tiles
       atomicSub(w.ctl, 1);
     }
   }
-  if (!grid_barrier(w.bar, w.ctl, G)) return;
+  if (!grid_barrier(w, G, epoch)) return;
 
   for (long long it = 0; it < maxiter; ++it) {
     if (__hip_atomic_load(w.ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= 0) break;
@@ -187,7 +205,7 @@ __global__ __launch_bounds__(kThreads) void k_batched_pcg(int64_t n, int n_tiles
       const double tot = tmx::block_sum<kThreads>(loc, red);
       if (threadIdx.x == 0) w.pq[t] = tot;
     }
-    if (!grid_barrier(w.bar, w.ctl, G)) return;
+    if (!grid_barrier(w, G, epoch)) return;
     // ---- phase B: alpha, x += alpha p, r -= alpha q, (r, z), (r, r)
     for (int t = blockIdx.x; t < n_tiles; t += G) {
       const int k = tile_mesh[t];
@@ -214,7 +232,7 @@ __global__ __launch_bounds__(kThreads) void k_batched_pcg(int64_t n, int n_tiles
         w.rr[t] = s_rr;
       }
     }
-    if (!grid_barrier(w.bar, w.ctl, G)) return;
+    if (!grid_barrier(w, G, epoch)) return;
     // ---- phase C: convergence per system, p = z + beta p
     for (int t = blockIdx.x; t < n_tiles; t += G) {
       const int k = tile_mesh[t];
@@ -243,14 +261,15 @@ __global__ __launch_bounds__(kThreads) void k_batched_pcg(int64_t n, int n_tiles
       const int r0 = tile_row0[t], r1 = tile_row0[t + 1];
       for (int i = r0 + threadIdx.x; i < r1; i += kThreads) w.p[i] = w.dinv[i] * w.r[i] + beta * w.p[i];
     }
-    if (!grid_barrier(w.bar, w.ctl, G)) return;
+    if (!grid_barrier(w, G, epoch)) return;
   }
 }
 
+// Clears the barrier state (a kernel, so a captured graph holds no memset).
 __global__ void k_batched_reset(BatchWork w, int n_mesh) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    w.bar[0] = 0u;
-    w.bar[1] = 0u;
+  for (int b = threadIdx.x; b < kMaxBarBlocks; b += blockDim.x) w.arrive[(size_t)b * kSlot] = 0u;
+  if (threadIdx.x == 0) {
+    w.go[0] = 0u;
     w.ctl[0] = n_mesh;
     w.ctl[1] = 0;
   }
@@ -272,8 +291,9 @@ BatchWork carve(const tm_batched_args* a) {
   w.bb = w.rr + n_tiles;
   w.status = a->mesh_status;
   w.iters = a->mesh_iters;
-  w.bar = (unsigned*)(w.bb + n_tiles);
-  w.ctl = (int*)(w.bar + 2);
+  w.go = (unsigned*)(w.bb + n_tiles);
+  w.arrive = w.go + kSlot;
+  w.ctl = (int*)(w.arrive + (size_t)kMaxBarBlocks * kSlot);
   return w;
 }
 
@@ -289,8 +309,9 @@ int launch_batched(const tm_batched_args* a, int blocks, hipStream_t st) {
   TM_REQUIRE(cap > 0, TM_ERR_HIP, "occupancy query returned no resident blocks");
   if (blocks <= 0 || blocks > cap) blocks = cap;
   if (blocks > a->n_tiles) blocks = a->n_tiles;
+  if (blocks > kMaxBarBlocks) blocks = kMaxBarBlocks;
   BatchWork w = carve(a);
-  hipLaunchKernelGGL(k_batched_reset, dim3(1), dim3(64), 0, st, w, a->n_mesh);
+  hipLaunchKernelGGL(k_batched_reset, dim3(1), dim3(256), 0, st, w, a->n_mesh);
   hipLaunchKernelGGL(kern, dim3(blocks), dim3(kThreads), 0, st, a->n_rows, a->n_tiles, a->n_mesh, a->row_ptr,
                      a->col_idx, (const VT*)a->values, a->b, a->x, a->tile_row0, a->tile_mesh, a->mesh_tile0,
                      a->rtol * a->rtol, (long long)a->maxiter, w, a->mesh_relres);
@@ -312,7 +333,7 @@ int dispatch_lanes(const tm_batched_args* a, int lanes, int blocks, hipStream_t 
 }  // namespace
 
 extern "C" int64_t tm_batched_pcg_work_size(int64_t n_rows, int32_t n_tiles) {
-  return 8 * (4 * n_rows + 5 * (int64_t)n_tiles) + 64;
+  return 8 * (4 * n_rows + 5 * (int64_t)n_tiles) + 4 * (int64_t)kSlot * (kMaxBarBlocks + 1) + 64;
 }
 
 extern "C" int tm_batched_pcg(const tm_batched_args* a, int lanes, int blocks, void* stream) {
