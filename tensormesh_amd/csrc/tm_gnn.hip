@@ -1378,45 +1378,50 @@ __global__ __launch_bounds__(NT) void k_fcg_ctl0(FcgWork w, int64_t n, int npart
   }
 }
 
-// Single workgroup: betas and alpha from the SpMV partials (one wave per
-// group of slots), written to the scalar block for the update kernel.
-template <int M>
-__global__ __launch_bounds__(NT) void k_fcg_scalars(FcgWork w, int nparts) {
-  if (w.f->status != 0) return;
-  __shared__ double sh[2 * M + 2];
-  const int wv = threadIdx.x / kWave, l = threadIdx.x % kWave;
-  for (int k = wv; k < 2 * M + 2; k += NT / kWave) {
-    const int slot = k < M ? S_ZQ + k : (k < 2 * M ? S_PR + (k - M) : (k == 2 * M ? S_ZR : S_ZW));
-    double a = 0.0;
-    for (int b = l; b < nparts; b += kWave) a += w.part[slot * kMaxParts + b];
-    a = tmx::wave_sum(a);
-    if (l == 0) sh[k] = a;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double pr = sh[2 * M], pq = sh[2 * M + 1];
-    for (int j = 0; j < M; ++j) {
-      const double beta = sh[j] / w.s->pq[j];
-      w.s->beta[j] = beta;
-      pr -= beta * sh[M + j];
-      pq += beta * (beta * w.s->pq[j] - 2.0 * sh[j]);
-    }
-    const bool ok = pq > 0.0 && __builtin_isfinite(pq) && __builtin_isfinite(pr);
-    w.s->alpha = ok ? pr / pq : 0.0;
-    w.s->pq_new = ok ? pq : -1.0;
-  }
-}
-
-// p, q into ring slot iter % m; x += alpha p, r -= alpha q; partials (r,r), (d r)^2
+// p, q into ring slot iter % m; x += alpha p, r -= alpha q; partials (r,r), (d r)^2.
+// Every workgroup first forms the betas and alpha from the SpMV partials (one
+// wave per slot, lanes strided over the partials in a fixed order, so every
+// workgroup holds the same bits); workgroup 0 records them for k_fcg_check.
 template <int M>
 __global__ __launch_bounds__(NTF) void k_fcg_update(int64_t n, double* __restrict__ x, const float* __restrict__ d,
-                                                    FcgWork w) {
+                                                    FcgWork w, int nparts) {
   if (w.f->status != 0) return;
-  const double alpha = w.s->alpha;
-  const bool ok = w.s->pq_new > 0.0;
+  __shared__ double sh[2 * M + 2];
+  __shared__ double sc[M + 2];
+  {
+    const int wv = threadIdx.x / kWave, l = threadIdx.x % kWave;
+    for (int k = wv; k < 2 * M + 2; k += NTF / kWave) {
+      const int slot = k < M ? S_ZQ + k : (k < 2 * M ? S_PR + (k - M) : (k == 2 * M ? S_ZR : S_ZW));
+      double a = 0.0;
+      for (int b = l; b < nparts; b += kWave) a += w.part[slot * kMaxParts + b];
+      a = tmx::wave_sum(a);
+      if (l == 0) sh[k] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double pr = sh[2 * M], pq = sh[2 * M + 1];
+      for (int j = 0; j < M; ++j) {
+        const double bj = sh[j] / w.s->pq[j];
+        sc[j] = bj;
+        pr -= bj * sh[M + j];
+        pq += bj * (bj * w.s->pq[j] - 2.0 * sh[j]);
+      }
+      const bool okq = pq > 0.0 && __builtin_isfinite(pq) && __builtin_isfinite(pr);
+      sc[M] = okq ? pr / pq : 0.0;
+      sc[M + 1] = okq ? pq : -1.0;
+      if (blockIdx.x == 0) {
+        for (int j = 0; j < M; ++j) w.s->beta[j] = sc[j];
+        w.s->alpha = sc[M];
+        w.s->pq_new = sc[M + 1];
+      }
+    }
+    __syncthreads();
+  }
+  const double alpha = sc[M];
+  const bool ok = sc[M + 1] > 0.0;
   double beta[M];
 #pragma unroll
-  for (int j = 0; j < M; ++j) beta[j] = w.s->beta[j];
+  for (int j = 0; j < M; ++j) beta[j] = sc[j];
   const int slot = w.f->iter % M;
   double loc[2] = {0.0, 0.0};
   if (ok) {
@@ -1868,8 +1873,7 @@ int fcg_iterate_m(int64_t n, const int32_t* rp, const int32_t* ci, const float* 
                   FcgWork w, hipStream_t st) {
   const int G = fcg_grid(n);
   hipLaunchKernelGGL((k_fcg_spmv<4, M, false>), dim3(G), dim3(NTF), 0, st, n, rp, ci, v, x, w);
-  hipLaunchKernelGGL(k_fcg_scalars<M>, dim3(1), dim3(NT), 0, st, w, G);
-  hipLaunchKernelGGL(k_fcg_update<M>, dim3(G), dim3(NTF), 0, st, n, x, d, w);
+  hipLaunchKernelGGL(k_fcg_update<M>, dim3(G), dim3(NTF), 0, st, n, x, d, w, G);
   hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(NT), 0, st, w, n, G);
   TM_HIP(hipGetLastError());
   return TM_OK;

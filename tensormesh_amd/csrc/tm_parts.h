@@ -14,8 +14,18 @@ __device__ __forceinline__ void sum_parts(const double* part, int stride, const 
   if (threadIdx.x < kWave) {
 #pragma unroll
     for (int i = 0; i < NS; ++i) {
+      // eight loads in flight per lane; the adds keep the order k ascending
+      const double* p = part + which[i] * stride;
       double s = 0.0;
-      for (int k = threadIdx.x; k < n; k += kWave) s += part[which[i] * stride + k];
+      int k = threadIdx.x;
+      for (; k + 7 * kWave < n; k += 8 * kWave) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = p[k + u * kWave];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+      }
+      for (; k < n; k += kWave) s += p[k];
       s = wave_sum(s);
       if (threadIdx.x == 0) sh[i] = s;
     }
@@ -25,14 +35,30 @@ __device__ __forceinline__ void sum_parts(const double* part, int stride, const 
   for (int i = 0; i < NS; ++i) out[i] = sh[i];
 }
 
+// All NS block sums in one pass: wave sums, one workgroup barrier, then
+// lane i < NS adds the wave sums of value i in wave order (the same order as
+// one block_sum per value, so the partials are unchanged bit for bit).
 template <int NT, int NS>
 __device__ __forceinline__ void write_parts(double* part, int stride, const int (&which)[NS], double (&loc)[NS]) {
-  __shared__ double red[NT / kWave];
+  __shared__ double red[NT / kWave][NS];
+  double v[NS];
 #pragma unroll
-  for (int i = 0; i < NS; ++i) {
-    const double b = block_sum<NT>(loc[i], red);
-    if (threadIdx.x == 0) part[which[i] * stride + blockIdx.x] = b;
+  for (int i = 0; i < NS; ++i) v[i] = wave_sum(loc[i]);
+  if (threadIdx.x % kWave == 0)
+#pragma unroll
+    for (int i = 0; i < NS; ++i) red[threadIdx.x / kWave][i] = v[i];
+  __syncthreads();
+  if (threadIdx.x < NS) {
+    double b = 0.0;
+#pragma unroll
+    for (int w = 0; w < NT / kWave; ++w) b += red[w][threadIdx.x];
+    int slot = which[0];
+#pragma unroll
+    for (int i = 1; i < NS; ++i)
+      if ((int)threadIdx.x == i) slot = which[i];
+    part[slot * stride + blockIdx.x] = b;
   }
+  __syncthreads();  // red may be reused by a following call
 }
 
 }  // namespace tmx
