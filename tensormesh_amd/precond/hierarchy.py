@@ -140,17 +140,27 @@ class DeviceLevel:
 class Hierarchy:
     """Device copy of the aggregation levels of one mesh graph."""
 
-    def __init__(self, host_levels: List[HostLevel], device: torch.device):
+    def __init__(self, host_levels: List[HostLevel], device: torch.device, level0_csr=None):
+        """``level0_csr``: optional device ``(row_ptr, col)`` of the level-0
+        matrix (int32, the same pattern as ``host_levels[0]``) to share with
+        the solver instead of a second copy, so one iteration streams one
+        column array for A and A^ (at 96^3 that is 50 MB less per iteration)."""
         self.device = torch.device(device)
         t = lambda a: None if a is None else torch.from_numpy(np.ascontiguousarray(a)).to(self.device)
-        self.levels = [DeviceLevel(h.n, int(h.col.size), t(h.row_ptr), t(h.col), t(h.agg), t(h.child_ptr),
-                                   t(h.child), t(h.perm), t(h.seg), b_row_ptr=t(h.b_row_ptr), b_col=t(h.b_col),
-                                   b_perm=t(h.b_perm), b_seg=t(h.b_seg)) for h in host_levels]
+        self.levels = []
+        for k, h in enumerate(host_levels):
+            rp, col = (level0_csr if (k == 0 and level0_csr is not None) else (t(h.row_ptr), t(h.col)))
+            if k == 0 and level0_csr is not None:
+                assert rp.dtype == torch.int32 and col.dtype == torch.int32 and rp.numel() == h.n + 1 \
+                    and col.numel() == h.col.size, "level-0 CSR does not match the hierarchy's pattern"
+            self.levels.append(DeviceLevel(h.n, int(h.col.size), rp, col, t(h.agg), t(h.child_ptr), t(h.child),
+                                           t(h.perm), t(h.seg), b_row_ptr=t(h.b_row_ptr), b_col=t(h.b_col),
+                                           b_perm=t(h.b_perm), b_seg=t(h.b_seg)))
 
     @classmethod
     def from_csr(cls, coords: np.ndarray, row_ptr: np.ndarray, col: np.ndarray, n_levels: int,
-                 device: torch.device) -> "Hierarchy":
-        return cls(build_host_levels(coords, row_ptr, col, n_levels), device)
+                 device: torch.device, level0_csr=None) -> "Hierarchy":
+        return cls(build_host_levels(coords, row_ptr, col, n_levels), device, level0_csr)
 
     @property
     def n_levels(self) -> int:

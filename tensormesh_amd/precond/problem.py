@@ -44,7 +44,8 @@ class PoissonCube:
         self.n_inner = Ki.shape[0]
         rp = self.csr.row_ptr.cpu().numpy()
         ci = self.csr.col_idx.cpu().numpy()
-        self.hier = Hierarchy.from_csr(pts[~bnd], rp, ci, n_levels, self.device)
+        self.hier = Hierarchy.from_csr(pts[~bnd], rp, ci, n_levels, self.device,
+                                       level0_csr=(self.csr.row_ptr, self.csr.col_idx))
 
     def kappa(self, seed: int) -> torch.Tensor:
         length, sigma = field_params(seed)
