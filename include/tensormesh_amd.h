@@ -351,18 +351,21 @@ int tm_gnn_unpool_decode(int64_t n, int64_t n_coarse, const float* xs, const int
                          void* stream);
 /* Symmetric network with a layer pair on level 0 (inference).  The level-0
  * features of the rank-one encoder output stay in the span of x = d r / s and
- * A^x, so the finest level reduces to scalar CSR products:
- *  - tm_gnn_sym0_down writes xs = (x, A^x) from dr = d r [n] (FP32; with
- *    dr_ready = 0 it is formed from r first, otherwise it is the dr of the
- *    flexible-CG view); tm_gnn_encode_pool with r = NULL,
- *    nparts = 0, we = [a; b], be = 0 then pools x (x) a + (A^x) (x) b;
+ * A^x, so the finest level reduces to scalar CSR products.  Both calls take
+ * A's own FP32 values (A^ = D A D with D = diag d is applied as d (A (d v)),
+ * so no normalised copy is streamed):
+ *  - tm_gnn_sym0_down writes xs = (x, A^x) from u = d^2 r [n] (FP32; with
+ *    u_ready = 0 it is formed from r first, otherwise it is the u the
+ *    flexible-CG update wrote into the view's dr buffer); tm_gnn_encode_pool
+ *    with r = NULL, nparts = 0, we = [a; b], be = 0 then pools
+ *    x (x) a + (A^x) (x) b;
  *  - tm_gnn_sym0_up takes the level-1 up-path features hc, the vectors a, b and
  *    c3 = (a.a, a.b, b.b), and writes z; ua, ub [n_coarse] and t [n] are scratch. */
-int tm_gnn_sym0_down(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values_normalized,
-                     const double* r, const float* d, float* dr, int dr_ready, const double* scale, float* xs,
+int tm_gnn_sym0_down(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values,
+                     const double* r, const float* d, float* u, int u_ready, const double* scale, float* xs,
                      const int* stop, void* stream);
 int tm_gnn_sym0_up(int64_t n, int64_t n_coarse, const int32_t* row_ptr, const int32_t* col_idx,
-                   const float* values_normalized, const float* xs, const int32_t* agg, const float* w,
+                   const float* values, const float* xs, const int32_t* agg, const float* w,
                    const uint16_t* hc, const float* va, const float* vb, const float* c3, const float* d,
                    const double* scale, float* ua, float* ub, float* t, double* z, const int* stop, void* stream);
 /* Finest-level prolongation B = A^ P^: b_values[s] = sum of A^_k w_col(k) over
@@ -416,7 +419,7 @@ typedef struct {
   int32_t* iter;
   int32_t nparts;
   int32_t reserved;
-  float* dr; /* d r in FP32, refreshed with r by the start and update kernels */
+  float* dr; /* d^2 r in FP32 (the u of tm_gnn_sym0_down), refreshed with r by the start and update kernels */
 } tm_fcg_view;
 int64_t tm_fcg_work_size(int64_t n, int m);
 int tm_fcg_layout(int64_t n, int m, void* work, tm_fcg_view* view);

@@ -901,11 +901,14 @@ __global__ __launch_bounds__(NT) void k_unpool_decode(int64_t n, const float2* _
 //   y_i = u_i . a + (A^ (u . b))_i,   u = x (x) a + (A^x) (x) b + w hc[agg]
 // (hc: level-1 features of the up path).  So the whole finest level is two
 // scalar CSR products and a few per-node scalars:
-//   k_sym0_down : xs_i = (x_i, (A^x)_i),  x = d r / s
+//   k_sym0_down : xs_i = (x_i, (A^x)_i),  x = d r / s, from u = d^2 r and A's
+//                 own values: (A^x)_i = d_i (A u)_i / s (A^ = D A D, D = diag d)
 //   (k_encode_pool with xs as input pools x (x) a + (A^x) (x) b)
 //   k_row_dot2  : ua[J] = hc[J] . a,  ub[J] = hc[J] . b
-//   k_sym0_up   : t_j = x_j ab + (A^x)_j bb + w_j ub[agg_j];
-//                 y_i = x_i aa + (A^x)_i ab + w_i ua[agg_i] + (A^ t)_i,  z = s d y
+//   k_sym0_up   : t_j = d_j (x_j ab + (A^x)_j bb + w_j ub[agg_j])  (pre-scaled by d);
+//                 y_i = x_i aa + (A^x)_i ab + w_i ua[agg_i] + d_i (A t)_i,  z = s d y
+// Using A's values (which the flexible-CG SpMV streams anyway) instead of a
+// second, normalised copy keeps one value array per iteration in the caches.
 // where aa = a.a, ab = a.b, bb = b.b.
 // ---------------------------------------------------------------------------
 // (A^ v)_i for float v on 4-lane groups with the unrolled row loop
@@ -932,29 +935,29 @@ __device__ __forceinline__ float row_dot_f(const int32_t* __restrict__ rp, const
   return acc;
 }
 
-// xs_i = (x_i, (A^x)_i), x = d r / s from dr = d r (FP32, written by the
-// flexible-CG update) or, without it, from r and d
+// u = d^2 r (FP32) when the flexible-CG update has not written it
 __global__ __launch_bounds__(NT) void k_sym0_dr(int64_t n, const double* __restrict__ r, const float* __restrict__ d,
-                                                float* __restrict__ dr, const int* stop) {
+                                                float* __restrict__ u, const int* stop) {
   if (stopped(stop)) return;
   for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT)
-    dr[i] = (float)((double)d[i] * r[i]);
+    u[i] = (float)((double)d[i] * ((double)d[i] * r[i]));
 }
 
-// xs_i = (x_i, (A^x)_i), x = d r / s, from dr = d r (FP32)
+// xs_i = (x_i, (A^x)_i), x = d r / s, from u = d^2 r (FP32) and A's values av
 template <int S>
 __global__ __launch_bounds__(NT) void k_sym0_down(int64_t n, const int32_t* __restrict__ rp,
                                                   const int32_t* __restrict__ ci, const float* __restrict__ av,
-                                                  const float* __restrict__ dr, const double* __restrict__ s_in,
-                                                  float2* __restrict__ xs, const int* stop) {
+                                                  const float* __restrict__ d, const float* __restrict__ u,
+                                                  const double* __restrict__ s_in, float2* __restrict__ xs,
+                                                  const int* stop) {
   if (stopped(stop)) return;
   const float inv = (float)(1.0 / *s_in);
   const int sl = threadIdx.x % S;
   const tmx::RowChunk rc = tmx::row_chunk(n);
   for (int64_t i = rc.r0 + threadIdx.x / S; i < rc.r1; i += NT / S) {
-    const float di = sl == 0 ? dr[i] : 0.0f;  // issued before the gathers
-    const float acc = row_dot_f<S>(rp, ci, av, dr, i, sl);
-    if (sl == 0) xs[i] = float2{di * inv, acc * inv};
+    const float ui = sl == 0 ? u[i] : 0.0f, di = sl == 0 ? d[i] : 1.0f;  // issued before the gathers
+    const float acc = row_dot_f<S>(rp, ci, av, u, i, sl);
+    if (sl == 0) xs[i] = float2{ui / di * inv, di * acc * inv};
   }
 }
 
@@ -996,12 +999,12 @@ __global__ __launch_bounds__(NT) void k_row_dot2(int64_t nc, const uint16_t* __r
 __global__ __launch_bounds__(NT) void k_sym0_t(int64_t n, const float2* __restrict__ xs,
                                                const int32_t* __restrict__ agg, const float* __restrict__ w,
                                                const float* __restrict__ ub, const float* __restrict__ c3,
-                                               float* __restrict__ t, const int* stop) {
+                                               const float* __restrict__ d, float* __restrict__ t, const int* stop) {
   if (stopped(stop)) return;
   const float ab = c3[1], bb = c3[2];
   for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
     const float2 x = xs[i];
-    t[i] = x.x * ab + x.y * bb + w[i] * ub[agg[i]];
+    t[i] = d[i] * (x.x * ab + x.y * bb + w[i] * ub[agg[i]]);
   }
 }
 
@@ -1029,7 +1032,7 @@ __global__ __launch_bounds__(NT) void k_sym0_up(int64_t n, const int32_t* __rest
     }
     const float acc = row_dot_f<S>(rp, ci, av, t, i, sl);
     if (sl == 0) {
-      const float y = x.x * aa + x.y * ab + wu + acc;
+      const float y = x.x * aa + x.y * ab + wu + di * acc;
       z[i] = s * (double)di * (double)y;
     }
   }
@@ -1352,7 +1355,7 @@ __global__ __launch_bounds__(NTF) void k_fcg_init(int64_t n, const double* __res
     loc[0] += ri * ri;
     loc[1] += b[i] * b[i];
     const double dr = (double)d[i] * ri;
-    w.dr[i] = (float)dr;
+    w.dr[i] = (float)((double)d[i] * dr);  // u = d^2 r for the fused finest level
     loc[2] += dr * dr;
   }
   const int which[3] = {S_RR, S_BB, S_SS};
@@ -1439,7 +1442,7 @@ __global__ __launch_bounds__(NTF) void k_fcg_update(int64_t n, double* __restric
       w.r[i] = ri;
       loc[0] += ri * ri;
       const double dr = (double)d[i] * ri;
-      w.dr[i] = (float)dr;
+      w.dr[i] = (float)((double)d[i] * dr);  // u = d^2 r for the fused finest level
       loc[1] += dr * dr;
     }
   }
@@ -1578,38 +1581,37 @@ extern "C" int tm_gnn_unpool_decode(int64_t n, int64_t n_coarse, const float* xs
   return TM_OK;
 }
 
-extern "C" int tm_gnn_sym0_down(int64_t n, const int32_t* row_ptr, const int32_t* col_idx,
-                                const float* values_normalized, const double* r, const float* d, float* dr,
-                                int dr_ready, const double* scale, float* xs, const int* stop, void* stream) {
-  TM_REQUIRE(n >= 0 && row_ptr && col_idx && values_normalized && d && dr && scale && xs, TM_ERR_ARG,
-             "null argument");
-  TM_REQUIRE(dr_ready || r, TM_ERR_ARG, "sym0_down: r needed when dr is not ready");
+extern "C" int tm_gnn_sym0_down(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values,
+                                const double* r, const float* d, float* u, int u_ready, const double* scale,
+                                float* xs, const int* stop, void* stream) {
+  TM_REQUIRE(n >= 0 && row_ptr && col_idx && values && d && u && scale && xs, TM_ERR_ARG, "null argument");
+  TM_REQUIRE(u_ready || r, TM_ERR_ARG, "sym0_down: r needed when u is not ready");
   if (n == 0) return TM_OK;
   hipStream_t st = (hipStream_t)stream;
-  if (!dr_ready)
-    hipLaunchKernelGGL(k_sym0_dr, dim3(grid_of(n, NT, 8192)), dim3(NT), 0, st, n, r, d, dr, stop);
+  if (!u_ready)
+    hipLaunchKernelGGL(k_sym0_dr, dim3(grid_of(n, NT, 8192)), dim3(NT), 0, st, n, r, d, u, stop);
   hipLaunchKernelGGL(k_sym0_down<4>, dim3(grid_of(n, NT / 4 * 4, 4096)), dim3(NT), 0, st, n, row_ptr, col_idx,
-                     values_normalized, dr, scale, (float2*)xs, stop);
+                     values, d, u, scale, (float2*)xs, stop);
   TM_HIP(hipGetLastError());
   return TM_OK;
 }
 
 extern "C" int tm_gnn_sym0_up(int64_t n, int64_t n_coarse, const int32_t* row_ptr, const int32_t* col_idx,
-                              const float* values_normalized, const float* xs, const int32_t* agg, const float* w,
+                              const float* values, const float* xs, const int32_t* agg, const float* w,
                               const uint16_t* hc, const float* va, const float* vb, const float* c3,
                               const float* d, const double* scale, float* ua, float* ub, float* t, double* z,
                               const int* stop, void* stream) {
-  TM_REQUIRE(n >= 0 && n_coarse >= 0 && row_ptr && col_idx && values_normalized && xs && agg && w && hc && va &&
-                 vb && c3 && d && scale && ua && ub && t && z,
+  TM_REQUIRE(n >= 0 && n_coarse >= 0 && row_ptr && col_idx && values && xs && agg && w && hc && va && vb && c3 &&
+                 d && scale && ua && ub && t && z,
              TM_ERR_ARG, "null argument");
   if (n == 0) return TM_OK;
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_row_dot2, dim3(grid_of(n_coarse, NT / 8, 8192)), dim3(NT), 0, st, n_coarse, hc, va, vb, ua,
                      ub, stop);
-  hipLaunchKernelGGL(k_sym0_t, dim3(grid_of(n, NT, 8192)), dim3(NT), 0, st, n, (const float2*)xs, agg, w, ub, c3, t,
-                     stop);
+  hipLaunchKernelGGL(k_sym0_t, dim3(grid_of(n, NT, 8192)), dim3(NT), 0, st, n, (const float2*)xs, agg, w, ub, c3, d,
+                     t, stop);
   hipLaunchKernelGGL(k_sym0_up<4>, dim3(grid_of(n, NT / 4 * 4, 4096)), dim3(NT), 0, st, n, row_ptr, col_idx,
-                     values_normalized, (const float2*)xs, agg, w, ua, t, c3, d, scale, z, stop);
+                     values, (const float2*)xs, agg, w, ua, t, c3, d, scale, z, stop);
   TM_HIP(hipGetLastError());
   return TM_OK;
 }

@@ -344,14 +344,15 @@ class Program:
                 check(L.tm_gnn_scale(self.n0, ss_ptr, nparts, scale_p, st))
             lv1 = hier.levels[1]
             s0 = net.sym0
-            check(L.tm_gnn_sym0_down(self.n0, ptr(lv0.row_ptr), ptr(lv0.col), ptr(lv0.vn), _addr(r), ptr(lv0.d),
+            # A's own values with d-scaled vectors (A^ = D A D): the same value array as the solver's SpMV
+            check(L.tm_gnn_sym0_down(self.n0, ptr(lv0.row_ptr), ptr(lv0.col), ptr(lv0.values), _addr(r), ptr(lv0.d),
                                      ptr(self.sym_dr) if dr is None else _addr(dr), 0 if dr is None else 1,
                                      scale_p, ptr(self.xs), stop_p, st))
             check(L.tm_gnn_encode_pool(self.n0, lv1.n, ptr(lv1.child_ptr), ptr(lv1.child), ptr(lv0.w), _addr(None),
                                        ptr(lv0.d), _addr(None), _addr(None), 0, ptr(s0["ab"]), ptr(s0["zero"]),
                                        ptr(self.bufs[self.ops[1][3]]), ptr(self.xs), scale_p, stop_p, st))
             self._run_ops(self.ops[2:-2], 2, train, stop_p, st)
-            check(L.tm_gnn_sym0_up(self.n0, lv1.n, ptr(lv0.row_ptr), ptr(lv0.col), ptr(lv0.vn), ptr(self.xs),
+            check(L.tm_gnn_sym0_up(self.n0, lv1.n, ptr(lv0.row_ptr), ptr(lv0.col), ptr(lv0.values), ptr(self.xs),
                                    ptr(lv0.agg), ptr(lv0.w), ptr(self.bufs[self.ops[-2][2]]), ptr(s0["ab"]),
                                    ptr(s0["ab"][F:]), ptr(s0["c3"]), ptr(lv0.d), scale_p, ptr(self.sym_ua),
                                    ptr(self.sym_ub), ptr(self.sym_t), _addr(zt), stop_p, st))
