@@ -386,7 +386,8 @@ int tm_gnn_energy_loss(int64_t n, const int32_t* row_ptr, const int32_t* col_idx
  * along y), from that call's partial sums. */
 int tm_gnn_cosine_grad(int64_t n, const double* r, const float* d, const double* scale, const double* part,
                        int nparts, float* g_y, void* stream);
-/* Level values of one matrix: d = diag^-1/2, values_normalized = D A D. */
+/* Level values of one matrix: d = diag^-1/2, values_normalized = D A D
+ * (values_normalized may be NULL: then only d is formed). */
 int tm_gnn_level_setup(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values, float* d,
                        float* values_normalized, void* stream);
 /* Galerkin coarse values: coarse[s] = sum of fine[perm[k]], k in [seg[s], seg[s+1]). */

@@ -1777,12 +1777,14 @@ extern "C" int tm_gnn_cosine_grad(int64_t n, const double* r, const float* d, co
 
 extern "C" int tm_gnn_level_setup(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const float* values,
                                   float* d, float* values_normalized, void* stream) {
-  TM_REQUIRE(n >= 0 && row_ptr && col_idx && values && d && values_normalized, TM_ERR_ARG, "null argument");
+  TM_REQUIRE(n >= 0 && row_ptr && col_idx && values && d, TM_ERR_ARG, "null argument");
   if (n == 0) return TM_OK;
   hipStream_t st = (hipStream_t)stream;
   const int g = grid_of(n, NT / kRowG, 8192);
   hipLaunchKernelGGL(k_level_diag, dim3(g), dim3(NT), 0, st, n, row_ptr, col_idx, values, d);
-  hipLaunchKernelGGL(k_level_normalize, dim3(g), dim3(NT), 0, st, n, row_ptr, col_idx, values, d, values_normalized);
+  if (values_normalized)
+    hipLaunchKernelGGL(k_level_normalize, dim3(g), dim3(NT), 0, st, n, row_ptr, col_idx, values, d,
+                       values_normalized);
   TM_HIP(hipGetLastError());
   return TM_OK;
 }

@@ -65,7 +65,8 @@ class LearnedFCG:
         """Matrix values (canonical CSR order of ``csr``) and the log-coefficient feature."""
         self.values.copy_(values.detach().to(torch.float32))
         self.lk.copy_(lk.detach().to(torch.float32))
-        self.hier.set_matrix(self.values)
+        # the fused symmetric finest level reads A and d only
+        self.hier.set_matrix(self.values, level0_normalized=not (self.fused and self.prog.fusable_sym0))
 
     def _iterate_once(self):
         L = _lib.lib()

@@ -146,6 +146,7 @@ class Hierarchy:
         the solver instead of a second copy, so one iteration streams one
         column array for A and A^ (at 96^3 that is 50 MB less per iteration)."""
         self.device = torch.device(device)
+        self.level0_ready = False
         t = lambda a: None if a is None else torch.from_numpy(np.ascontiguousarray(a)).to(self.device)
         self.levels = []
         for k, h in enumerate(host_levels):
@@ -166,8 +167,14 @@ class Hierarchy:
     def n_levels(self) -> int:
         return len(self.levels)
 
-    def set_matrix(self, values: torch.Tensor):
-        """Per-matrix set-up from the level-0 FP32 values (canonical CSR order)."""
+    def set_matrix(self, values: torch.Tensor, level0_normalized: bool = True):
+        """Per-matrix set-up from the level-0 FP32 values (canonical CSR order).
+
+        ``level0_normalized=False`` skips the level-0 products that only the
+        unfused network paths and training read (the normalised values
+        ``A^`` at level 0, ``B = A^ P^`` and ``A^ 1``); the fused symmetric
+        finest level works from ``A`` and ``d``.  ``level0_ready`` records
+        which set-up ran."""
         _lib.require_gpu(values, "matrix values")
         L = _lib.lib()
         st = _lib.stream_of(self.device)
@@ -188,15 +195,17 @@ class Hierarchy:
                 lev.d = torch.empty(lev.n, dtype=torch.float32, device=self.device)
             if lev.vn is None or lev.vn.numel() != lev.nnz:
                 lev.vn = torch.empty(lev.nnz, dtype=torch.float32, device=self.device)
+            skip_vn = k == 0 and not level0_normalized
             check(L.tm_gnn_level_setup(lev.n, ptr(lev.row_ptr), ptr(lev.col), ptr(lev.values), ptr(lev.d),
-                                       ptr(lev.vn), st))
+                                       ptr(None) if skip_vn else ptr(lev.vn), st))
         for k in range(len(self.levels) - 1):
             lev, nxt = self.levels[k], self.levels[k + 1]
             if lev.w is None:
                 lev.w = torch.empty(lev.n, dtype=torch.float32, device=self.device)
             check(L.tm_gnn_pool_weights(lev.n, ptr(lev.agg), ptr(lev.d), ptr(nxt.d), ptr(lev.w), st))
         l0 = self.levels[0]
-        if l0.b_row_ptr is not None:
+        self.level0_ready = bool(level0_normalized)
+        if l0.b_row_ptr is not None and level0_normalized:
             if l0.b_values is None:
                 l0.b_values = torch.empty(l0.b_col.numel(), dtype=torch.float32, device=self.device)
                 l0.row_sums = torch.empty(l0.n, dtype=torch.float32, device=self.device)

@@ -339,7 +339,11 @@ class Program:
         scale_p = ptr(self.scale) if scale is None else _addr(scale)
         we, be = net.enc()
         zt = self.z if z is None else z
-        if fused and self.fusable_sym0 and not train and not want_y:
+        sym0_path = fused and self.fusable_sym0 and not train and not want_y
+        if not sym0_path and not hier.level0_ready:
+            raise RuntimeError("this network path needs the level-0 normalised values: call "
+                               "Hierarchy.set_matrix(values) with level0_normalized=True")
+        if sym0_path:
             if nparts > 0:   # scale from the partials (the flexible-CG kernels store it themselves)
                 check(L.tm_gnn_scale(self.n0, ss_ptr, nparts, scale_p, st))
             lv1 = hier.levels[1]
