@@ -344,7 +344,7 @@ def main():
         torch.cuda.synchronize()
         t_solve = ev0.elapsed_time(ev1) / 1e3
         true_rel = ((fi.double() - Ki @ xs).norm() / fi.double().norm()).item()
-        b_iter = 8 * nnzi + 4 * (ni + 1) + 104 * ni
+        b_iter = 8 * nnzi + 4 * (ni + 1) + 96 * ni
         out["solve"] = {"method": "jacobi-pcg", "n_inner": ni, "nnz_inner": nnzi, "iterations": it,
                         "converged": ok, "relres_recursive": rel, "relres_true_fp32_matrix": true_rel,
                         "time_to_1e-8_s": round(t_solve, 4), "ms_per_iter": round(t_solve / max(it, 1) * 1e3, 4),

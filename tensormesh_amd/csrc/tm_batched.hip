@@ -192,10 +192,11 @@ __global__ __launch_bounds__(kThreads) void k_batched_pcg(int64_t n, int n_tiles
       const int k = tile_mesh[t];
       if (w.status[k] != 0) continue;
       const int r0 = tile_row0[t], r1 = tile_row0[t + 1];
+      const int lim = rp[r1];
       double loc = 0.0;
       for (int i = r0 + threadIdx.x / S; i < r1; i += per) {
         double o[1] = {0.0};
-        tmx::row_accum<VT, S, 1>(rp, ci, v, w.p, i, sl, o);  // loads of a lane's entries issued together
+        tmx::row_accum<VT, S, 1>(rp, ci, v, w.p, i, sl, lim, o);  // loads of a lane's entries issued together
         const double s = sub_sum<S>(o[0]);
         if (sl == 0) {
           w.q[i] = s;

@@ -1277,6 +1277,7 @@ __global__ __launch_bounds__(NTF) void k_fcg_spmv(int64_t n, const int32_t* __re
   constexpr int64_t stride = NTF / S;
   const int sl = threadIdx.x % S;
   const tmx::RowChunk rc = tmx::row_chunk(n);
+  const int lim = rp[rc.r1];
   const double* src = START ? x : w.z;
   double loc[2 * M + 2];
 #pragma unroll
@@ -1305,7 +1306,7 @@ __global__ __launch_bounds__(NTF) void k_fcg_spmv(int64_t n, const int32_t* __re
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       o[r][0] = 0.0;
-      if (row + r * stride < rc.r1) tmx::row_accum<float, S, 1>(rp, ci, v, src, row + r * stride, sl, o[r]);
+      if (row + r * stride < rc.r1) tmx::row_accum<float, S, 1>(rp, ci, v, src, row + r * stride, sl, lim, o[r]);
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {

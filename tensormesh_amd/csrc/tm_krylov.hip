@@ -3,9 +3,10 @@
 // Matrix values are FP32 (or FP64), vectors and every reduction FP64.  One PCG
 // iteration is three stream-ordered kernels:
 //   K1  q = A p and (p, q)               -> alpha = rz / pq
-//   K2  x += alpha p, r -= alpha q, z = D^-1 r, (r, z) and (r, r)
+//   K2  r -= alpha q, z = D^-1 r, (r, z) and (r, r)
 //                                        -> beta, convergence test
-//   K3  p = D^-1 r + beta p
+//   K3  x += alpha p, p = D^-1 r + beta p
+// (x is updated in K3, which reads p anyway: one vector pass fewer.)
 // Each kernel writes one partial sum per workgroup; the next kernel, which
 // needs the scalar, sums the partials in a fixed order (so the result does not
 // depend on scheduling and no atomics or grid-wide fences are needed).
@@ -32,12 +33,17 @@ constexpr int kMaxBlocks = 1024;
 // Device-side PCG state.  Scalars are produced by one designated workgroup of
 // a kernel and only consumed by later kernels (kernel boundaries order them).
 struct PcgScalars {
-  double rz, rz_prev, bb, tol2, relres;
+  double rz, rz_prev, bb, tol2, relres, alpha;
 };
+// K3 runs iff K2 of the same iteration ran to the end: K1 clears `step` (its
+// workgroups never read it), K2 sets it (never read in K2), K3 only reads it.
+// K3 cannot use `status` for this, since its own workgroup 0 may set it while
+// other workgroups of the launch are still to start.
 struct PcgFlags {
   int status;  // 0 running, 1 converged, 2 breakdown (p.Ap <= 0), 3 max iterations
   int iter;
   long long maxiter;
+  int step;
 };
 
 struct PcgWork {
@@ -73,13 +79,14 @@ __global__ __launch_bounds__(kThreads) void k_spmv(int64_t nrows, const int32_t*
   const int sl = threadIdx.x % S;
   constexpr int64_t stride = kThreads / S;
   const RowChunk rc = row_chunk(nrows);
+  const int lim = rp[rc.r1];
   for (int64_t row = rc.r0 + threadIdx.x / S; row < rc.r1; row += stride * R) {
     double o[R][BS];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
 #pragma unroll
       for (int i = 0; i < BS; ++i) o[r][i] = 0.0;
-      if (row + r * stride < rc.r1) row_accum<VT, S, BS>(rp, ci, v, x, row + r * stride, sl, o[r]);
+      if (row + r * stride < rc.r1) row_accum<VT, S, BS>(rp, ci, v, x, row + r * stride, sl, lim, o[r]);
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -157,6 +164,7 @@ __global__ __launch_bounds__(kThreads) void k_pcg_ctl0(PcgWork w, int nparts, do
     w.s->relres = t[2] > 0.0 ? sqrt(t[1] / t[2]) : 0.0;
     w.f->iter = 0;
     w.f->maxiter = maxiter;
+    w.f->step = 0;
     w.f->status = (t[1] <= w.s->tol2) ? 1 : 0;
   }
 }
@@ -166,11 +174,13 @@ template <typename VT, int S, int BS>
 __global__ __launch_bounds__(kThreads) void k_pcg_spmv(int64_t nrows, const int32_t* __restrict__ rp,
                                                       const int32_t* __restrict__ ci, const VT* __restrict__ v,
                                                       PcgWork w) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) w.f->step = 0;
   if (w.f->status != 0) return;
-  constexpr int R = 2;
+  constexpr int R = 1;
   const int sl = threadIdx.x % S;
   constexpr int64_t stride = kThreads / S;
   const RowChunk rc = row_chunk(nrows);
+  const int lim = rp[rc.r1];
   double loc[1] = {0.0};
   for (int64_t row = rc.r0 + threadIdx.x / S; row < rc.r1; row += stride * R) {
     double o[R][BS];
@@ -178,7 +188,7 @@ __global__ __launch_bounds__(kThreads) void k_pcg_spmv(int64_t nrows, const int3
     for (int r = 0; r < R; ++r) {
 #pragma unroll
       for (int i = 0; i < BS; ++i) o[r][i] = 0.0;
-      if (row + r * stride < rc.r1) row_accum<VT, S, BS>(rp, ci, v, w.p, row + r * stride, sl, o[r]);
+      if (row + r * stride < rc.r1) row_accum<VT, S, BS>(rp, ci, v, w.p, row + r * stride, sl, lim, o[r]);
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -196,8 +206,8 @@ __global__ __launch_bounds__(kThreads) void k_pcg_spmv(int64_t nrows, const int3
   write_parts<1>(w.part, which, loc);
 }
 
-// K2: alpha = rz / pq, x += alpha p, r -= alpha q, partials (r, z), (r, r)
-__global__ __launch_bounds__(kThreads) void k_pcg_update(int64_t n, double* __restrict__ x, PcgWork w, int nparts) {
+// K2: alpha = rz / pq, r -= alpha q, partials (r, z), (r, r)
+__global__ __launch_bounds__(kThreads) void k_pcg_update(int64_t n, PcgWork w, int nparts) {
   if (w.f->status != 0) return;
   const int which_in[1] = {P_PQ};
   double pq[1];
@@ -207,12 +217,15 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update(int64_t n, double* __re
   const double alpha = ok ? rz / pq[0] : 0.0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     w.s->rz_prev = rz;
-    if (!ok) w.f->status = 2;
+    w.s->alpha = alpha;
+    if (!ok)
+      w.f->status = 2;
+    else
+      w.f->step = 1;
   }
   double loc[2] = {0.0, 0.0};
   if (ok) {
     for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-      x[i] += alpha * w.p[i];
       const double ri = w.r[i] - alpha * w.q[i];
       w.r[i] = ri;
       loc[0] += ri * w.dinv[i] * ri;
@@ -223,9 +236,9 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update(int64_t n, double* __re
   write_parts<2>(w.part, which, loc);
 }
 
-// K3: beta = rz_new / rz_old, convergence test, p = D^-1 r + beta p
-__global__ __launch_bounds__(kThreads) void k_pcg_dir(int64_t n, PcgWork w, int nparts) {
-  if (w.f->status != 0) return;
+// K3: x += alpha p; beta = rz_new / rz_old, convergence test, p = D^-1 r + beta p
+__global__ __launch_bounds__(kThreads) void k_pcg_dir(int64_t n, double* __restrict__ x, PcgWork w, int nparts) {
+  if (w.f->step == 0) return;
   const int which[2] = {P_RZ, P_RR};
   double t[2];
   sum_parts<2>(w.part, which, nparts, t);
@@ -242,10 +255,18 @@ __global__ __launch_bounds__(kThreads) void k_pcg_dir(int64_t n, PcgWork w, int 
     else if (it >= w.f->maxiter)
       w.f->status = 3;
   }
-  if (conv) return;
+  const double alpha = w.s->alpha;
+  if (conv) {
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+      x[i] += alpha * w.p[i];
+    return;
+  }
   const double beta = rz_new / w.s->rz_prev;
-  for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
-    w.p[i] = w.dinv[i] * w.r[i] + beta * w.p[i];
+  for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    const double pi = w.p[i];
+    x[i] += alpha * pi;
+    w.p[i] = w.dinv[i] * w.r[i] + beta * pi;
+  }
 }
 
 template <typename T>
@@ -282,8 +303,8 @@ int pcg_run(int64_t nrows, const int32_t* rp, const int32_t* ci, const VT* v, co
     if (hf.status != 0) break;
     for (int k = 0; k < chunk; ++k) {
       hipLaunchKernelGGL((k_pcg_spmv<VT, S, BS>), dim3(G), dim3(kThreads), 0, st, nrows, rp, ci, v, w);
-      hipLaunchKernelGGL(k_pcg_update, dim3(G), dim3(kThreads), 0, st, n, x, w, G);
-      hipLaunchKernelGGL(k_pcg_dir, dim3(G), dim3(kThreads), 0, st, n, w, G);
+      hipLaunchKernelGGL(k_pcg_update, dim3(G), dim3(kThreads), 0, st, n, w, G);
+      hipLaunchKernelGGL(k_pcg_dir, dim3(G), dim3(kThreads), 0, st, n, x, w, G);
     }
     TM_HIP(hipGetLastError());
     if (check_every <= 0 && chunk < 256) chunk *= 2;

@@ -1,20 +1,61 @@
 // CSR row kernels shared by the Krylov solvers: S lanes per (block) row,
 // 4-deep unrolled accumulation, XCD-contiguous row slabs.
 #pragma once
+#include <type_traits>
+
 #include "tm_device.h"
 
 namespace tmx {
 
-// out[i] += sum over entries of one block row.  Each lane takes entries
-// sl, sl + S, sl + 2S, ... in batches of U: the U column loads of a batch are
-// issued together (predicated past the row end: those lanes load nothing and
-// add zero), then the U value and x loads, so a lane keeps U independent
-// chains in flight instead of walking its entries one by one.
+typedef int32_t i32x4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));
+
+// out[i] += sum over entries of one block row.  lim is an entry index at or
+// past the row's end and at most nnz (the caller's rp[last row + 1]): no load
+// reaches lim.
+//
+// FP32 scalar rows (S <= 4): lane sl takes the 4 consecutive entries from
+// b + 4 sl (then + 4 S, ...), read as one 16-byte column load and one 16-byte
+// value load (dword-aligned types; gfx950 serves unaligned multi-dword global
+// loads), so one load instruction of a wave covers whole consecutive cache
+// lines.  Entries past the row end are masked (the load may run into the next
+// row, never to lim); a group of 4 that would reach lim falls back to
+// single loads.  A lane adds its entries in order.
+//
+// Other rows: each lane takes entries sl, sl + S, sl + 2S, ... in batches of U:
+// the U column loads of a batch are issued together (predicated past the row
+// end: those lanes load nothing and add zero), then the U value and x loads,
+// so a lane keeps U independent chains in flight.
 template <typename VT, int S, int BS, int U = (S <= 4 ? 4 : 2)>
 __device__ __forceinline__ void row_accum(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
                                           const VT* __restrict__ v, const double* __restrict__ x, int64_t row,
-                                          int sl, double (&out)[BS]) {
+                                          int sl, int lim, double (&out)[BS]) {
   const int b = rp[row], e = rp[row + 1];
+  if constexpr (std::is_same<VT, float>::value && BS == 1 && S <= 4) {
+    for (int k0 = b + 4 * sl; k0 < e; k0 += 4 * S) {
+      int c[4];
+      float a[4];
+      if (k0 + 4 <= lim) {
+        const i32x4u cv = *(const i32x4u*)(ci + k0);
+        const f32x4u av = *(const f32x4u*)(v + k0);
+        c[0] = cv.x, c[1] = cv.y, c[2] = cv.z, c[3] = cv.w;
+        a[0] = av.x, a[1] = av.y, a[2] = av.z, a[3] = av.w;
+      } else {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          c[u] = k0 + u < e ? ci[k0 + u] : 0;
+          a[u] = k0 + u < e ? v[k0 + u] : 0.0f;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const bool ok = k0 + u < e;
+        const double xv = x[ok ? c[u] : 0];
+        out[0] += ok ? (double)a[u] * xv : 0.0;
+      }
+    }
+    return;
+  }
   for (int k0 = b + sl; k0 < e; k0 += U * S) {
     int64_t c[U];
     bool ok[U];
