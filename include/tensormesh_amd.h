@@ -238,6 +238,18 @@ int tm_pcg_jacobi_bsr3(int64_t n_block_rows, const int32_t* row_ptr, const int32
 int tm_asm_nh(const tm_plan_dev* plan, const double* coords, const double* u, const void* E_elem, int E_dtype,
               double E0, double nu, void* kvals, int kv_dtype, double* R, double* energy, void* stream);
 
+/* Isotropic linear-elastic stiffness of P1 tetrahedra
+ * (LinearElasticityElementAssembler, assemble/builtin.py:103-179, with
+ * voigt_shape_grad / voigt_stiffness, functional/elasticity.py:132-285):
+ * block (a, b) = vol B_a^T C B_b, i.e.
+ *   K_ab[i][k] = vol (mu (g_a . g_b) d_ik + lam g_a,i g_b,k + mu g_a,k g_b,i),
+ * mu = E/(2(1+nu)), lam = E nu/((1+nu)(1-2nu)), E per element (E_elem f32/f64)
+ * or the constant E0 (E_elem NULL).  Writes kvals[nnz*9] (f32 or f64) as 3x3
+ * row-major blocks on the scalar pattern (from_block_coo order, row = a,
+ * the u basis function; col = b).  Same plan as tm_asm_nh. */
+int tm_asm_linear_elastic(const tm_plan_dev* plan, const double* coords, const void* E_elem, int E_dtype, double E0,
+                          double nu, void* kvals, int kv_dtype, void* stream);
+
 /* Matrix-free tangent action Y = K(u) lam (f64 [N*3], every entry written) and,
  * when dE != NULL, dE[e] += lam . dR/dE_e (caller zeroes dE): the adjoint of
  * the residual assembly w.r.t. nodal displacements and per-element E. */

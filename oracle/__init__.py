@@ -27,4 +27,10 @@ How each function is pinned (beyond restating the reference):
 * ``hyperelastic.*`` - neo-Hookean density of ``builtin.py:285-315`` with
   closed-form P and dP/dF; pinned by central finite differences (h = 1e-6)
   and by torch autograd of the energy, and psi(F = I) = 0.
+* ``elasticity.*`` - Voigt ``B^T C B`` of ``builtin.py:173-179`` and
+  ``functional/elasticity.py:132-255``; pinned by the rigid-body modes
+  (translations and rotations, which a transposed block fails for
+  lam != mu), the constant-strain patch energy
+  ``u^T K u = |Omega| (lam tr(eps)^2 + 2 mu eps:eps)`` and equality with the
+  neo-Hookean tangent at u = 0 (tests/test_elasticity.py).
 """

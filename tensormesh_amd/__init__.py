@@ -7,10 +7,11 @@ with hand-written gfx950 HIP kernels reached through the C ABI of
 from .mesh import Mesh, kuhn_cube, structured_rectangle
 from .assemble import (ElementAssembler, NodeAssembler, LaplaceElementAssembler, MassElementAssembler,
                        KappaLaplaceElementAssembler, SourceNodeAssembler, const_node_assembler,
-                       func_node_assembler, NeoHookeanModel)
+                       func_node_assembler, NeoHookeanModel, LinearElasticityElementAssembler)
 from .sparse import SparseMatrix
 from .operator import Condenser
 
 __all__ = ["Mesh", "kuhn_cube", "structured_rectangle", "ElementAssembler", "NodeAssembler",
            "LaplaceElementAssembler", "MassElementAssembler", "KappaLaplaceElementAssembler",
-           "SourceNodeAssembler", "const_node_assembler", "func_node_assembler", "NeoHookeanModel", "SparseMatrix", "Condenser"]
+           "SourceNodeAssembler", "const_node_assembler", "func_node_assembler", "NeoHookeanModel",
+           "LinearElasticityElementAssembler", "SparseMatrix", "Condenser"]

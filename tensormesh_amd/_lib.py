@@ -100,6 +100,8 @@ _SIGS = {
                                    ctypes.POINTER(c_double), c_void_p]),
     "tm_asm_nh": (c_int, [ctypes.POINTER(PlanDev), c_void_p, c_void_p, c_void_p, c_int, c_double, c_double,
                           c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "tm_asm_linear_elastic": (c_int, [ctypes.POINTER(PlanDev), c_void_p, c_void_p, c_int, c_double, c_double,
+                                      c_void_p, c_int, c_void_p]),
     "tm_nh_apply": (c_int, [ctypes.POINTER(PlanDev), c_void_p, c_void_p, c_void_p, c_int, c_double, c_double,
                             c_void_p, c_void_p, c_void_p, c_void_p]),
     "tm_batched_pcg_work_size": (c_i64, [c_i64, ctypes.c_int32]),

@@ -4,3 +4,4 @@ from .builtin import (LaplaceElementAssembler, MassElementAssembler, KappaLaplac
                       SourceNodeAssembler, const_node_assembler, func_node_assembler)
 from .topology import P1Topology, topology_for, clear_topology_cache
 from .hyperelastic import NeoHookeanModel, BlockCondenser
+from .elasticity import LinearElasticityElementAssembler

@@ -1,9 +1,11 @@
 """Ready-made forms (``tensormesh/assemble/builtin.py``): Laplace (``:25-63``),
-mass (``:65-101``), constant and function loads (``:886-969``)."""
+mass (``:65-101``), linear elasticity (``:103-179``, in ``elasticity.py``),
+constant and function loads (``:886-969``)."""
 from __future__ import annotations
 
 from .element_assembler import ElementAssembler
 from .node_assembler import NodeAssembler
+from .elasticity import LinearElasticityElementAssembler  # noqa: F401  (reference module layout)
 
 
 class LaplaceElementAssembler(ElementAssembler):

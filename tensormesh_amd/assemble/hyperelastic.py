@@ -67,6 +67,27 @@ class _Energy(torch.autograd.Function):
         return (g * R).to(u.dtype), None, None
 
 
+def block_matrix(asm: ElementAssembler, kv: torch.Tensor, dev) -> SparseMatrix:
+    """``[3N, 3N]`` matrix from 3x3 blocks ``kv[nnz*9]`` on ``asm``'s scalar pattern:
+    COO in the ``SparseMatrix.from_block_coo`` order (edge-major, then the block
+    row-major: row = node*3 + i, col = node*3 + k) with a block-CSR view for
+    ``@`` / ``solve``.  The expanded indices are cached on ``asm`` per device."""
+    key = ("_bcoo", dev)
+    c = asm.__dict__.get("_bcoo_cache")
+    if c is None or c[0] != key:
+        e0, e1 = asm.edges[0].to(dev), asm.edges[1].to(dev)
+        i = torch.arange(3, device=dev)
+        rr = (e0[:, None, None] * 3 + i[None, :, None]).expand(-1, 3, 3).reshape(-1)
+        cc = (e1[:, None, None] * 3 + i[None, None, :]).expand(-1, 3, 3).reshape(-1)
+        c = (key, rr, cc)
+        asm.__dict__["_bcoo_cache"] = c
+    rp, ci = asm.topology.csr(dev)
+    m = SparseMatrix._from_csr(kv, c[1], c[2], (3 * asm.n_points, 3 * asm.n_points),
+                               CSR(rp, ci, asm.n_points, asm.n_points), symmetric=True)
+    m.block_size = 3
+    return m
+
+
 class NeoHookeanModel(ElementAssembler):
     value_dtype = torch.float32
 
@@ -160,20 +181,7 @@ class NeoHookeanModel(ElementAssembler):
         return self._block_matrix(kv, u.device)
 
     def _block_matrix(self, kv, dev) -> SparseMatrix:
-        key = ("_bcoo", dev)
-        c = self.__dict__.get("_bcoo_cache")
-        if c is None or c[0] != key:
-            e0, e1 = self.edges[0].to(dev), self.edges[1].to(dev)
-            i = torch.arange(3, device=dev)
-            rr = (e0[:, None, None] * 3 + i[None, :, None]).expand(-1, 3, 3).reshape(-1)
-            cc = (e1[:, None, None] * 3 + i[None, None, :]).expand(-1, 3, 3).reshape(-1)
-            c = (key, rr, cc)
-            self.__dict__["_bcoo_cache"] = c
-        rp, ci = self.topology.csr(dev)
-        m = SparseMatrix._from_csr(kv, c[1], c[2], (3 * self.n_points, 3 * self.n_points),
-                                   CSR(rp, ci, self.n_points, self.n_points), symmetric=True)
-        m.block_size = 3
-        return m
+        return block_matrix(self, kv, dev)
 
     def newton(self, u0: torch.Tensor, fixed: torch.Tensor, E=None, rtol: float = 1e-8, max_iter: int = 30,
                lin_rtol: float = 1e-10, lin_maxiter: int = 20000, verbose: bool = False):

@@ -7,6 +7,12 @@
 //   P g_a           = mu F g_a + (lam ln J - mu) w_a
 //   K_ab[i][k] / V  = mu (g_a . g_b) d_ik + lam w_a,i w_b,k - (lam ln J - mu) w_a,k w_b,i
 //   psi             = mu/2 (|F|^2 - 3) - mu ln J + lam/2 (ln J)^2
+//
+// Isotropic linear elasticity is the same tangent at u = 0 (F = I, J = 1,
+// ln J = 0, w_a = g_a):
+//   K_ab[i][k] / V  = mu (g_a . g_b) d_ik + lam g_a,i g_b,k + mu g_a,k g_b,i
+// i.e. B_a^T C B_b with the Voigt strain-displacement matrices and Hooke's
+// C; the LIN variants take that path without reading a displacement.
 #include "tm_device.h"
 
 namespace {
@@ -63,6 +69,7 @@ struct Kin {
   double vol, J, lnJ;
 };
 
+template <bool LIN = false>
 __device__ __forceinline__ void kinematics(const double (&x)[4][3], const double (&u)[4][3], Kin& k) {
   double e1[3], e2[3], e3[3];
 #pragma unroll
@@ -90,6 +97,19 @@ __device__ __forceinline__ void kinematics(const double (&x)[4][3], const double
     for (int a = 0; a < 4; ++a) k.g[a][d] = c[a][d] * idet;
   }
   k.vol = fabs(det) / 6.0;
+  if (LIN) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) k.F[i][j] = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) k.w[a][i] = k.g[a][i];
+    k.J = 1.0;
+    k.lnJ = 0.0;
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -135,7 +155,7 @@ __device__ __forceinline__ void stage_v(const LdsV& L, const int32_t* halo, cons
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       L.xs[d * nh + h] = X[g * 3 + d];
-      L.us[d * nh + h] = U[g * 3 + d];
+      if (U) L.us[d * nh + h] = U[g * 3 + d];
     }
   }
   for (int a = threadIdx.x; a < nown; a += kThreads) {
@@ -146,8 +166,9 @@ __device__ __forceinline__ void stage_v(const LdsV& L, const int32_t* halo, cons
   }
 }
 
-// Tangent values (optional), residual (optional) and energy (optional).
-template <typename ET, typename OT, bool WK, bool WR>
+// Tangent values (optional), residual (optional) and energy (optional).  LIN:
+// linear-elastic stiffness (u = 0; U is not read, only WK is meaningful).
+template <typename ET, typename OT, bool WK, bool WR, bool LIN = false>
 __global__ __launch_bounds__(kThreads) void k_nh_assemble(PlanArgs p, const double* __restrict__ X,
                                                          const double* __restrict__ U, const ET* __restrict__ E,
                                                          double E0, double nu, OT* __restrict__ kvals,
@@ -160,7 +181,7 @@ __global__ __launch_bounds__(kThreads) void k_nh_assemble(PlanArgs p, const doub
   const int hoff = bi0.x, nh = bi0.y, nown = bi0.z, roff = bi0.w;
   const int nrec = bi1.x, runoff = bi1.y, nrun = bi1.z, nacc = bi1.w;
   const LdsV L = carve_v(smem, nacc, nh, nown);
-  stage_v<ET>(L, p.halo + hoff, p.meta + hoff, nacc, nh, nown, X, U, WK);
+  stage_v<ET>(L, p.halo + hoff, p.meta + hoff, nacc, nh, nown, X, LIN ? nullptr : U, WK);
   __syncthreads();
   const double mu_f = 1.0 / (2.0 * (1.0 + nu)), lam_f = nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
   double en = 0.0;
@@ -176,11 +197,11 @@ __global__ __launch_bounds__(kThreads) void k_nh_assemble(PlanArgs p, const doub
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         x[a][d] = L.xs[d * nh + l[a]];
-        u[a][d] = L.us[d * nh + l[a]];
+        u[a][d] = LIN ? 0.0 : L.us[d * nh + l[a]];
       }
     }
     Kin k;
-    kinematics(x, u, k);
+    kinematics<LIN>(x, u, k);
     const double Ee = elem_E(E, E0, p.rec_elem[roff + r]);
     const double mu = mu_f * Ee, lam = lam_f * Ee;
     const double s = lam * k.lnJ - mu;
@@ -381,6 +402,30 @@ extern "C" int tm_asm_nh(const tm_plan_dev* plan, const double* coords, const do
   const float* E = (const float*)E_elem;
   if (kv_dtype == TM_F32) return nh_dispatch(plan, coords, u, E, E0, nu, (float*)kvals, R, energy, st);
   return nh_dispatch(plan, coords, u, E, E0, nu, (double*)kvals, R, energy, st);
+}
+
+extern "C" int tm_asm_linear_elastic(const tm_plan_dev* plan, const double* coords, const void* E_elem, int E_dtype,
+                                     double E0, double nu, void* kvals, int kv_dtype, void* stream) {
+  TM_REQUIRE(plan && coords && kvals, TM_ERR_ARG, "null argument");
+  TM_REQUIRE(nu > -1.0 && nu < 0.5, TM_ERR_ARG, "Poisson ratio %g outside (-1, 0.5)", nu);
+  TM_REQUIRE(kv_dtype == TM_F32 || kv_dtype == TM_F64, TM_ERR_ARG, "stiffness values must be f32 or f64");
+  hipStream_t st = (hipStream_t)stream;
+  const PlanArgs p = plan_args(plan);
+  if (E_elem == nullptr || E_dtype == TM_F64) {
+    const double* E = (const double*)E_elem;
+    if (kv_dtype == TM_F32)
+      return launch(k_nh_assemble<double, float, true, false, true>, plan, plan->lds_bytes, st, p, coords,
+                    (const double*)nullptr, E, E0, nu, (float*)kvals, (double*)nullptr, (double*)nullptr);
+    return launch(k_nh_assemble<double, double, true, false, true>, plan, plan->lds_bytes, st, p, coords,
+                  (const double*)nullptr, E, E0, nu, (double*)kvals, (double*)nullptr, (double*)nullptr);
+  }
+  TM_REQUIRE(E_dtype == TM_F32, TM_ERR_ARG, "E must be f32 or f64");
+  const float* E = (const float*)E_elem;
+  if (kv_dtype == TM_F32)
+    return launch(k_nh_assemble<float, float, true, false, true>, plan, plan->lds_bytes, st, p, coords,
+                  (const double*)nullptr, E, E0, nu, (float*)kvals, (double*)nullptr, (double*)nullptr);
+  return launch(k_nh_assemble<float, double, true, false, true>, plan, plan->lds_bytes, st, p, coords,
+                (const double*)nullptr, E, E0, nu, (double*)kvals, (double*)nullptr, (double*)nullptr);
 }
 
 extern "C" int tm_nh_apply(const tm_plan_dev* plan, const double* coords, const double* u, const void* E_elem,
