@@ -200,6 +200,63 @@ def batched_leg(n_mesh: int, log_fn):
             "setup_s": round(t_setup, 1)}
 
 
+def hyperelastic_leg(n, log_fn):
+    """Neo-Hookean 64^3 leg: tangent + residual assembly, linear-elastic stiffness and the
+    adjoint action (HIP events on the launch stream), then Newton with block Jacobi-PCG."""
+    import numpy as np
+    import torch
+    import tensormesh_amd as tm
+    from tensormesh_amd.mesh import Mesh, kuhn_cube_arrays
+    dev = torch.device("cuda:0")
+    pts, conn, bnd = kuhn_cube_arrays(n)
+    E_np = 1.0 + np.random.default_rng(1).uniform(-0.1, 0.1, len(conn))
+    mesh = Mesh(torch.from_numpy(pts), {"tetra": torch.from_numpy(conn)}, {"is_boundary": torch.from_numpy(bnd)})
+    Et = torch.tensor(E_np, dtype=torch.float32, device=dev)
+    model = tm.NeoHookeanModel.from_mesh(mesh, E=Et, nu=0.3).to(dev)
+    lin = tm.LinearElasticityElementAssembler.from_assembler(model, E=Et, nu=0.3).to(dev)
+    z = torch.tensor(pts[:, 2], device=dev)
+    fixed = (z == 0) | (z == 1)
+    u0 = torch.zeros((len(pts), 3), dtype=torch.float64, device=dev)
+    u0[:, 2] = 0.1 * z
+    E, N, nnz = len(conn), len(pts), model.nnz
+    stream = torch.cuda.current_stream(dev)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn, reps=10):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        for _ in range(reps):
+            fn()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1) / 1e3 / reps
+
+    t_tr = timed(lambda: model._launch(u0, Et, want_k=True, want_r=True))
+    t_lin = timed(lambda: lin.stiffness(E=Et))
+    lam = torch.rand((N, 3), dtype=torch.float64, device=dev)
+    t_adj = timed(lambda: model._tangent_action(u0, Et, lam, want_dE=True))
+    # conn + coords + u + E + row_ptr + col_idx + 3x3 f32 blocks + residual
+    b_tr = 16 * E + 24 * N + 24 * N + 4 * E + 4 * (N + 1) + 4 * nnz + 36 * nnz + 24 * N
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    u, info = model.newton(u0, fixed, rtol=1e-8)
+    torch.cuda.synchronize()
+    t_newton = time.perf_counter() - t0
+    log_fn(f"hyperelastic {n}^3: tangent+residual {t_tr * 1e6:.1f} us, linear stiffness {t_lin * 1e6:.1f} us, "
+           f"adjoint action {t_adj * 1e6:.1f} us; Newton {info['iterations']} its in {t_newton:.2f}s")
+    return {"workload": f"compressible neo-Hookean, {n}^3 Kuhn grid x 6 P1 tets, E=1+U(-0.1,0.1) per element "
+                        f"(seed 1), nu=0.3, bottom clamped, top u_z=0.1, Newton from u_z=0.1 z",
+            "elements": E, "nodes": N, "block_nnz": nnz,
+            "tangent_residual_us": round(t_tr * 1e6, 1), "tangent_residual_melem_per_s": round(E / t_tr / 1e6, 1),
+            "tangent_residual_GBps": round(b_tr / t_tr / 1e9, 1), "tangent_residual_bytes": b_tr,
+            "linear_elastic_stiffness_us": round(t_lin * 1e6, 1), "adjoint_action_us": round(t_adj * 1e6, 1),
+            "newton_iterations": info["iterations"], "newton_converged": bool(info["converged"]),
+            "newton_relres": info["residual_history"][-1], "pcg_iterations": info["linear_iterations"],
+            "newton_s": round(t_newton, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,6 +268,8 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=48)
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-hyper", action="store_true")
+    ap.add_argument("--hyper-n", type=int, default=64)
     ap.add_argument("--batched-meshes", type=int, default=256)
     ap.add_argument("--learned-n", type=int, default=96)
     ap.add_argument("--learned-seeds", type=int, default=8)
@@ -365,6 +424,9 @@ def main():
         bt = out["batched"]
         log(f"batched: {bt['graph_s'] * 1e3:.1f} ms for all systems ({bt['speedup_vs_loop']}x the loop), "
             f"iterations median {bt['median_iters']}, max {bt['max_iters']}")
+
+    if not args.no_hyper:
+        out["hyperelastic"] = hyperelastic_leg(args.hyper_n, log)
 
     if not args.no_cpu:
         log(f"cpu baseline on {args.cpu_n}^3 ...")
