@@ -8,6 +8,12 @@
 //   K_ab[i][k] / V  = mu (g_a . g_b) d_ik + lam w_a,i w_b,k - (lam ln J - mu) w_a,k w_b,i
 //   psi             = mu/2 (|F|^2 - 3) - mu ln J + lam/2 (ln J)^2
 //
+// Only the off-diagonal blocks are accumulated per element: every element's
+// gradients sum to zero (and so do its w_a), so sum_b K_ab = 0 and each owned
+// row's diagonal block is minus the sum of its off-diagonal blocks, formed
+// once per row after the element loop (27 instead of 36 LDS atomics per
+// owned vertex).
+//
 // Isotropic linear elasticity is the same tangent at u = 0 (F = I, J = 1,
 // ln J = 0, w_a = g_a):
 //   K_ab[i][k] / V  = mu (g_a . g_b) d_ik + lam g_a,i g_b,k + mu g_a,k g_b,i
@@ -228,13 +234,16 @@ __global__ __launch_bounds__(kThreads) void k_nh_assemble(PlanArgs p, const doub
         }
       }
       if (WK) {
+        // off-diagonal blocks only: the diagonal block is minus their row sum
+        // (sum_b g_b = 0, so sum_b K_ab = 0), formed after the loop
         const uint32_t m = L.ms[l[a]];
         const int base = (int)(m & 0xffffu);
         int j = 0;
 #pragma unroll
         for (int bb = 0; bb < 4; ++bb) {
-          const int slot = base + (bb == a ? (int)((m >> 16) & 0xffu) : rec_rank4(lo, hi, a, j));
-          if (bb != a) ++j;
+          if (bb == a) continue;
+          const int slot = base + rec_rank4(lo, hi, a, j);
+          ++j;
           const double gg = mu * (k.g[a][0] * k.g[bb][0] + k.g[a][1] * k.g[bb][1] + k.g[a][2] * k.g[bb][2]);
           double* blk = L.acc + 9 * slot;
 #pragma unroll
@@ -254,6 +263,16 @@ __global__ __launch_bounds__(kThreads) void k_nh_assemble(PlanArgs p, const doub
   }
   __syncthreads();
   if (WK) {
+    for (int t = threadIdx.x; t < 9 * nown; t += kThreads) {
+      const int a = t / 9, c = t % 9;
+      const uint32_t m = L.ms[a];
+      const int base = (int)(m & 0xffffu), dr = (int)((m >> 16) & 0xffu), len = (int)(m >> 24);
+      if (len == 0) continue;
+      double sum = 0.0;
+      for (int k = 0; k < len; ++k) sum += (k == dr) ? 0.0 : L.acc[9 * (base + k) + c];
+      L.acc[9 * (base + dr) + c] = -sum;
+    }
+    __syncthreads();
     const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     for (int q = w; q < nrun; q += kThreads / kWave) {
       const int4 ru = p.runs[runoff + q];
