@@ -9,6 +9,7 @@ namespace tmx {
 
 typedef int32_t i32x4u __attribute__((ext_vector_type(4), aligned(4)));
 typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef double f64x2u __attribute__((ext_vector_type(2), aligned(8)));
 
 // out[i] += sum over entries of one block row.  lim is an entry index at or
 // past the row's end and at most nnz (the caller's rp[last row + 1]): no load
@@ -25,7 +26,9 @@ typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));
 // Other rows: each lane takes entries sl, sl + S, sl + 2S, ... in batches of U:
 // the U column loads of a batch are issued together (predicated past the row
 // end: those lanes load nothing and add zero), then the U value and x loads,
-// so a lane keeps U independent chains in flight.
+// so a lane keeps U independent chains in flight.  FP32 3x3 blocks read their
+// 36 bytes as two 16-byte loads and one 4-byte load, and the three x entries
+// as one 16-byte and one 8-byte load (same products, same order).
 template <typename VT, int S, int BS, int U = (S <= 4 ? 4 : 2)>
 __device__ __forceinline__ void row_accum(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
                                           const VT* __restrict__ v, const double* __restrict__ x, int64_t row,
@@ -52,6 +55,38 @@ __device__ __forceinline__ void row_accum(const int32_t* __restrict__ rp, const 
         const bool ok = k0 + u < e;
         const double xv = x[ok ? c[u] : 0];
         out[0] += ok ? (double)a[u] * xv : 0.0;
+      }
+    }
+    return;
+  }
+  if constexpr (std::is_same<VT, float>::value && BS == 3) {
+    for (int k0 = b + sl; k0 < e; k0 += U * S) {
+      int64_t c[U];
+      bool ok[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        ok[u] = k0 + u * S < e;
+        c[u] = ok[u] ? ci[k0 + u * S] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (!ok[u]) continue;
+        const float* vb = v + (int64_t)(k0 + u * S) * 9;
+        const f32x4u a0 = *(const f32x4u*)vb;
+        const f32x4u a1 = *(const f32x4u*)(vb + 4);
+        const float a8 = vb[8];
+        const double* xb = x + c[u] * 3;
+        const f64x2u x01 = *(const f64x2u*)xb;
+        const double x2 = xb[2];
+        out[0] += (double)a0.x * x01.x;
+        out[0] += (double)a0.y * x01.y;
+        out[0] += (double)a0.z * x2;
+        out[1] += (double)a0.w * x01.x;
+        out[1] += (double)a1.x * x01.y;
+        out[1] += (double)a1.y * x2;
+        out[2] += (double)a1.z * x01.x;
+        out[2] += (double)a1.w * x01.y;
+        out[2] += (double)a8 * x2;
       }
     }
     return;
