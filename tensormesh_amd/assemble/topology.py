@@ -28,40 +28,43 @@ from ..element import ELEMENT_DIM, ELEMENT_NB, quadrature_moments
 PLAN_KINDS = {"scalar": (8, 36, 8), "vector3": (72, 56, 32)}
 
 
-def _default_plan_params(kind: str, dim: int) -> Tuple[int, int]:
-    """(max owned rows per block, LDS budget in bytes).
+def _default_plan_params(kind: str, dim: int) -> Tuple[int, int, int]:
+    """(max owned rows per block, LDS budget in bytes, lanes per block).
 
-    3D scalar: 160 rows in 40 KiB of LDS measured fastest on MI355X for the
-    128^3 Kuhn kappa-Laplace matrix (245 us vs 250-430 us for 224-512 rows
-    and 48-80 KiB): three such blocks of 512 lanes fit one CU, so the LDS
-    atomics of one block overlap the geometry of the others.
+    3D scalar: 80 rows in 20 KiB of LDS on 256-lane workgroups measured
+    fastest on MI355X for the 128^3 Kuhn kappa-Laplace matrix (233.5 / 234.0
+    us against 242.6 / 238.1 us for 160 rows in 40 KiB on 512 lanes, the
+    earlier default; 64 / 72 / 88 / 96 / 120 rows: 235-275 us; 224-512 rows
+    and 48-80 KiB: 250-430 us).  Eight such blocks fit one CU: blocks in
+    different phases (gathering their nodes, evaluating elements, writing
+    rows) overlap, which outweighs the 9 % more element records.
 
     3x3 blocks (64^3 neo-Hookean): 64 rows in 64 KiB.  40 rows in 40 KiB
     measured 5 % faster for the tangent + residual and 8 % faster for the
     linear-elastic stiffness, but 31 % slower for the adjoint action
     (217 vs 165 us), so the larger blocks stay the default.
     """
+    threads = os.environ.get("TM_ASM_THREADS")
     if kind == "vector3":
         return (int(os.environ.get("TM_ASM_MAX_ROWS_VEC", 64)),
-                int(os.environ.get("TM_ASM_LDS_BUDGET", 64 * 1024 - 64)))
+                int(os.environ.get("TM_ASM_LDS_BUDGET", 64 * 1024 - 64)), int(threads or 512))
     if dim == 3:
-        return (int(os.environ.get("TM_ASM_MAX_ROWS", 160)),
-                int(os.environ.get("TM_ASM_LDS_BUDGET", 40 * 1024)))
+        return (int(os.environ.get("TM_ASM_MAX_ROWS", 80)),
+                int(os.environ.get("TM_ASM_LDS_BUDGET", 20 * 1024)), int(threads or 256))
     return (int(os.environ.get("TM_ASM_MAX_ROWS_2D", 640)),
-            int(os.environ.get("TM_ASM_LDS_BUDGET", 64 * 1024 - 64)))
+            int(os.environ.get("TM_ASM_LDS_BUDGET", 64 * 1024 - 64)), int(threads or 512))
 
 
 class DevicePlan:
     """Device copy of an exported block plan (keeps the tensors alive)."""
 
-    def __init__(self, host: Dict[str, np.ndarray], info: np.ndarray, device: torch.device):
+    def __init__(self, host: Dict[str, np.ndarray], info: np.ndarray, device: torch.device, threads: int = 512):
         self.tensors = {k: torch.from_numpy(v).to(device) for k, v in host.items()}
         self.info = info
         t = self.tensors
         self.struct = _lib.PlanDev(
             t["blk"].data_ptr(), t["halo"].data_ptr(), t["meta"].data_ptr(), t["recs"].data_ptr(),
-            t["rec_elem"].data_ptr(), t["runs"].data_ptr(), int(info[0]), int(info[4]),
-            int(os.environ.get("TM_ASM_THREADS", 512)), 0)
+            t["rec_elem"].data_ptr(), t["runs"].data_ptr(), int(info[0]), int(info[4]), int(threads), 0)
 
     @property
     def n_blocks(self) -> int:
@@ -136,7 +139,7 @@ class P1Topology:
     # ---- plans ----------------------------------------------------------------
     def host_plan(self, kind: str = "scalar", max_rows: Optional[int] = None,
                   lds_budget: Optional[int] = None):
-        rows0, budget0 = _default_plan_params(kind, self.dim)
+        rows0, budget0, _ = _default_plan_params(kind, self.dim)
         rows = int(max_rows or rows0)
         budget = int(lds_budget or budget0)
         key = (kind, rows, budget)
@@ -164,11 +167,11 @@ class P1Topology:
     def device_plan(self, device, kind: str = "scalar", max_rows: Optional[int] = None,
                     lds_budget: Optional[int] = None) -> DevicePlan:
         device = torch.device(device)
-        rows0, budget0 = _default_plan_params(kind, self.dim)
-        key = (device, kind, int(max_rows or rows0), int(lds_budget or budget0))
+        rows0, budget0, threads = _default_plan_params(kind, self.dim)
+        key = (device, kind, int(max_rows or rows0), int(lds_budget or budget0), threads)
         if key not in self._dev_plans:
             arrs, info = self.host_plan(kind, max_rows, lds_budget)
-            self._dev_plans[key] = DevicePlan(arrs, info, device)
+            self._dev_plans[key] = DevicePlan(arrs, info, device, threads)
         return self._dev_plans[key]
 
     def quad(self, order: int) -> _lib.Quad:
