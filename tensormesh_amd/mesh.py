@@ -146,6 +146,28 @@ class Mesh(nn.Module):
     def register_point_data(self, key: str, value: torch.Tensor):
         self.point_data[key] = value
 
+    # ---- files (mesh.py:267-325, 694-740; gmsh .msh and .vtu without meshio) ----
+    def save(self, file_name: str, file_format: Optional[str] = None) -> "Mesh":
+        """Write ``.msh`` (gmsh ASCII 2.2) or ``.vtu`` (ASCII XML); returns ``self``."""
+        from . import mesh_io
+        mesh_io.write(file_name, self.points.detach().cpu().numpy(),
+                      {k: v.detach().cpu().numpy() for k, v in self.cells.items()},
+                      {k: v.detach().cpu().numpy() for k, v in self.point_data.items()}, file_format)
+        return self
+
+    to_file = save
+
+    @classmethod
+    def read(cls, file_name: str, file_format: Optional[str] = None, reorder: bool = False) -> "Mesh":
+        """Read ``.msh`` (gmsh ASCII 2.2 / 4.1) or ``.vtu`` (ASCII XML) triangle / tetra meshes.
+        ``reorder`` is accepted for the reference's signature; P1 simplices need none."""
+        from . import mesh_io
+        pts, cells, pd = mesh_io.read(file_name, file_format)
+        return cls(torch.from_numpy(pts), {k: torch.from_numpy(v) for k, v in cells.items()},
+                   {k: torch.from_numpy(v) for k, v in pd.items()})
+
+    from_file = read
+
     # ---- generators (gmsh substitutes) --------------------------------------
     @staticmethod
     def gen_rectangle(chara_length: float = 0.1, jitter: float = 0.0, seed: int = 0,
