@@ -142,6 +142,15 @@ int tm_asm_p1_matrix(int form, int nb, int dim, const tm_plan_dev* plan, const d
                      const void* coef, int coef_dtype, double scale, const tm_quad* quad,
                      void* values, int value_dtype, void* stream);
 
+/* General constant-coefficient P1 bilinear form (a user forward(u, gradu, v,
+ * gradv, scalars) recognised as U^T D V, U = (u, gradu), V = (v, gradv); the
+ * reference evaluates such a forward under vmap, element_assembler.py:243-466):
+ *   K_ab = int (phi_a, grad phi_a)^T D (phi_b, grad phi_b),  a = row (u), b = col (v),
+ * D[(dim+1)^2] f64 host array, row-major, index 0 = value, 1.. = gradient.
+ * No symmetry is assumed.  values [nnz] f32 or f64, every entry written once. */
+int tm_asm_p1_general(int nb, int dim, const tm_plan_dev* plan, const double* coords, const double* D,
+                      const tm_quad* quad, void* values, int value_dtype, void* stream);
+
 /* Load vector b[N] (f64) (NodeAssembler.__call__, node_assembler.py:242-409):
  * TM_LOAD_CONST: c * int phi_a;  TM_LOAD_NODAL: int (sum_c f_c phi_c) phi_a. */
 int tm_asm_p1_load(int form, int nb, int dim, const tm_plan_dev* plan, const double* coords,

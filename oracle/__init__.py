@@ -17,7 +17,11 @@ How each function is pinned (beyond restating the reference):
   ``projector.py:95-130``; pinned by single-element known answers (unit
   right triangle, Kuhn tetrahedron), ``K 1 = 0``, symmetry / PSD,
   ``1^T M 1 = |Omega|``, ``b = M f`` and the O(h^2) convergence to the exact
-  solution of the 2D quickstart problem.
+  solution of the 2D quickstart problem.  ``element_matrices_general``
+  (constant ``U^T D V`` integrands) is pinned by reducing to the Laplace and
+  mass matrices for ``D = diag(0, I)`` and ``D = e_0 e_0^T``, by
+  ``K(D^T) = K(D)^T`` (the u basis is the row) and by the exact integral of
+  a linear field's convection term (tests/test_general_form.py).
 * ``condense.*`` - ``condense.py:182-309, 388-458``; pinned by the 4x4
   tridiagonal fixture of ``tests/operator/test_condenser.py:48-99`` and
   dense block elimination.

@@ -78,6 +78,22 @@ def element_matrices(points, conn, element_type: str, form: str, kappa=None, ord
     return out
 
 
+def element_matrices_general(points, conn, element_type: str, D, order: int = 2) -> np.ndarray:
+    """Per-element ``[E, nb, nb]`` of the constant bilinear integrand ``U_a^T D V_b``.
+
+    At every quadrature point ``U_a = (phi_a(q), grad phi_a)`` for the u basis
+    function ``a`` (the row, ``element_assembler.py:371-398, 838-851``) and
+    ``V_b`` likewise for the v basis function ``b``; the products are summed
+    with jxw (``_integrate``, ``:196-221``).
+    """
+    conn = np.asarray(conn)
+    phi, grad, jxw = geometry(points, conn, element_type, order)            # [Q, nb], [E, Q, nb, d], [E, Q]
+    E, Q, nb, d = grad.shape
+    U = np.concatenate([np.broadcast_to(phi[None, :, :, None], (E, Q, nb, 1)), grad], -1)   # [E, Q, nb, d+1]
+    D = np.asarray(D, dtype=np.float64)
+    return np.einsum("eqai,ij,eqbj,eq->eab", U, D, U, jxw)
+
+
 def reduce(pe: np.ndarray, local: np.ndarray, nnz: int) -> np.ndarray:
     """``index_add_`` of ``local.ravel()`` into ``nnz`` slots (float64 accumulator)."""
     return np.bincount(np.asarray(pe), weights=np.asarray(local, dtype=np.float64).ravel(),

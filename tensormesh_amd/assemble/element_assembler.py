@@ -19,6 +19,7 @@ from __future__ import annotations
 import inspect
 from typing import Callable, Dict, Mapping, Optional, Union
 
+import numpy as np
 import torch
 import torch.nn as nn
 
@@ -129,6 +130,16 @@ class ElementAssembler(nn.Module):
             float(scale), self._quad, _lib.ptr(values), _lib.dtype_code(values), _lib.stream_of(dev)))
         return values
 
+    def _launch_general(self, points64, D: np.ndarray, out_dtype) -> torch.Tensor:
+        dev = points64.device
+        plan = self._plan(dev)
+        values = torch.empty(self.topology.nnz, dtype=out_dtype, device=dev)
+        Dc = np.ascontiguousarray(D, dtype=np.float64)
+        _lib.check(_lib.lib().tm_asm_p1_general(
+            self.topology.nb, self.topology.dim, plan.struct, _lib.ptr(points64), _lib.ptr(Dc), self._quad,
+            _lib.ptr(values), _lib.dtype_code(values), _lib.stream_of(dev)))
+        return values
+
     def _launch_kappa_adjoint(self, points64, g) -> torch.Tensor:
         dev = points64.device
         plan = self._plan(dev)
@@ -198,6 +209,11 @@ class ElementAssembler(nn.Module):
         if form is not None:
             kind, scale, key = form
             p64 = points.detach().to(torch.float64).contiguous()
+            if kind == _lib.FORM_GENERAL:
+                values = self._launch_general(p64, scale, self.value_dtype)
+                e = self.topology.edges_on(dev)
+                return SparseMatrix._from_csr(values, e[0], e[1], (self.n_points, self.n_points),
+                                              self.csr(dev), symmetric=bool(np.array_equal(scale, scale.T)))
             if kind == _lib.FORM_KAPPA_LAPLACE:
                 coef = point_data[key].to(dev)
                 if coef.requires_grad:

@@ -94,8 +94,55 @@ def classify_bilinear(fn: Callable, dim: int, point_data: Dict, scalar_data: Dic
         return None
     if s == {"u", "v"}:
         r = _probe(fn, names, build(None), lambda a: a["u"] * a["v"])
-        return None if r is None else (_lib.FORM_MASS, r, None)
+        if r is not None:
+            return (_lib.FORM_MASS, r, None)
+    if s and s <= {"u", "gradu", "v", "gradv"} and s & {"u", "gradu"} and s & {"v", "gradv"}:
+        D = probe_general(fn, names, sc, dim)
+        if D is not None:
+            return (_lib.FORM_GENERAL, D, None)
     return None
+
+
+def probe_general(fn: Callable, names, sc: Dict, dim: int) -> Optional[np.ndarray]:
+    """``D`` with ``fn = U^T D V`` for ``U = (u, gradu)``, ``V = (v, gradv)``, or ``None``.
+
+    ``D_ij`` is ``fn`` at the unit vectors ``U = e_i``, ``V = e_j``; three random
+    pairs must then satisfy ``fn(U, V) = U^T D V`` to 1e-12 of ``|U|^T |D| |V|``
+    (so ``fn`` is bilinear with constant coefficients).
+    """
+    dd = dim + 1
+
+    def call(U, V):
+        a = dict(sc)
+        for n in names:
+            if n == "u":
+                a[n] = U[0]
+            elif n == "gradu":
+                a[n] = U[1:]
+            elif n == "v":
+                a[n] = V[0]
+            elif n == "gradv":
+                a[n] = V[1:]
+        out = fn(*[a[n] for n in names])
+        if not torch.is_tensor(out) or out.numel() != 1:
+            raise ValueError("not a scalar integrand")
+        return float(out.reshape(()))
+
+    try:
+        I = torch.eye(dd, dtype=torch.float64)
+        D = np.array([[call(I[i], I[j]) for j in range(dd)] for i in range(dd)])
+        if not np.isfinite(D).all() or not D.any():
+            return None
+        gen = torch.Generator().manual_seed(20240601)
+        for _ in range(3):
+            U = torch.rand(dd, generator=gen, dtype=torch.float64) - 0.5
+            V = torch.rand(dd, generator=gen, dtype=torch.float64) - 0.5
+            u, v = U.numpy(), V.numpy()
+            if abs(call(U, V) - u @ D @ v) > _TOL * (np.abs(u) @ np.abs(D) @ np.abs(v)):
+                return None
+    except Exception:
+        return None
+    return D
 
 
 def classify_linear(fn: Callable, point_data: Dict, scalar_data: Dict) -> Optional[Tuple[int, float, Optional[str]]]:

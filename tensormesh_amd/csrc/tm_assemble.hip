@@ -320,6 +320,89 @@ __global__ __launch_bounds__(NT) void k_asm_matrix(PlanArgs p, const double* __r
   }
 }
 
+// Integrand U_a^T D V_b with U = (phi, grad phi) of the row (u) basis
+// function a and V of the column (v) basis function b, D constant
+// ((DIM+1)^2 row-major; index 0 = value, 1.. = gradient components).  With
+// g = c / det (c = cofactor vectors) and the quadrature moments wsum, W, M:
+//   K_ab = |det| D00 M_ab + sgn(det) W (c_a . D_g0 + D_0g . c_b) + wsum/|det| c_a^T D_gg c_b.
+// No symmetry and no zero row sum are assumed: every (owned a, b) entry is
+// accumulated.
+struct GenD {
+  double d[16];
+};
+
+template <int NB, typename OT, int NT>
+__global__ __launch_bounds__(NT) void k_asm_general(PlanArgs p, const double* __restrict__ X, GenD D, QuadArg q,
+                                                   OT* __restrict__ values) {
+  constexpr int DIM = NB - 1, DD = DIM + 1;
+  extern __shared__ __align__(16) char smem[];
+  const int b = tmx::xcd_remap(blockIdx.x, p.nblk);
+  const int4 bi0 = *(const int4*)(p.blk + 8 * b);
+  const int4 bi1 = *(const int4*)(p.blk + 8 * b + 4);
+  const int hoff = bi0.x, nh = bi0.y, nown = bi0.z, roff = bi0.w;
+  const int nrec = bi1.x, runoff = bi1.y, nrun = bi1.z, nacc = bi1.w;
+  const Lds L = carve(smem, nacc, nh, nown);
+  stage<NT, DIM, float, false>(L, p.halo + hoff, p.meta + hoff, nacc, nh, nown, X, (const float*)nullptr);
+  __syncthreads();
+  for (int r = threadIdx.x; r < nrec; r += NT) {
+    const uint4 rc = p.recs[roff + r];
+    const uint64_t lo = (uint64_t)rc.x | ((uint64_t)rc.y << 32);
+    int l[NB];
+    double x[NB][DIM];
+    uint32_t m[NB];
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+      l[a] = rec_local(lo, a);
+      node_xyz<DIM>(L.p0[l[a]], L.p1[l[a]], x[a]);
+      m[a] = L.ms[l[a]];
+    }
+    double c[NB][DIM], det;
+    p1_geometry<DIM>(x, c, det);
+    const double adet = fabs(det);
+    const double sgg = q.wsum / adet, sw = det < 0.0 ? -q.W[0] : q.W[0];
+    const double mdiag = D.d[0] * q.M[0] * adet, moff = D.d[0] * q.M[1] * adet;
+    double vb[NB];  // sgn W (D_0g . c_b)
+#pragma unroll
+    for (int bb = 0; bb < NB; ++bb) {
+      double t = 0.0;
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) t += D.d[1 + j] * c[bb][j];
+      vb[bb] = sw * t;
+    }
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+      if (l[a] >= nown) continue;
+      double ta[DIM];  // (c_a^T D_gg)_j
+      double ua = 0.0; // c_a . D_g0
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) {
+        double t = 0.0;
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) t += c[a][i] * D.d[(1 + i) * DD + 1 + j];
+        ta[j] = t;
+        ua += c[a][j] * D.d[(1 + j) * DD];
+      }
+      ua *= sw;
+      const int base = meta_off(m[a]);
+#pragma unroll
+      for (int bb = 0; bb < NB; ++bb) {
+        double gg = 0.0;
+#pragma unroll
+        for (int j = 0; j < DIM; ++j) gg += ta[j] * c[bb][j];
+        const double v = (a == bb ? mdiag : moff) + ua + vb[bb] + sgg * gg;
+        const int slot = bb == a ? meta_diag(m[a]) : rec_rank<NB>(rc, a, bb);
+        atomicAdd(L.acc + base + slot, v);
+      }
+    }
+  }
+  __syncthreads();
+  const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  for (int k = w; k < nrun; k += NT / kWave) {
+    const int4 ru = p.runs[runoff + k];
+    for (int t = lane; t < ru.z; t += kWave) values[(int64_t)ru.x + t] = (OT)L.acc[ru.y + t];
+  }
+}
+
 // FORM: 0 constant load c, 1 nodal f.
 template <int NB, int FORM, typename CT, int NT>
 __global__ __launch_bounds__(NT) void k_asm_load(PlanArgs p, const double* __restrict__ X,
@@ -508,6 +591,27 @@ extern "C" int tm_asm_p1_matrix(int form, int nb, int dim, const tm_plan_dev* pl
   }
   if (nb == 3) return dispatch_matrix<3>(form, plan, coords, coef, coef_dtype, scale, q, (double*)values, st);
   return dispatch_matrix<4>(form, plan, coords, coef, coef_dtype, scale, q, (double*)values, st);
+}
+
+extern "C" int tm_asm_p1_general(int nb, int dim, const tm_plan_dev* plan, const double* coords, const double* D,
+                                 const tm_quad* quad, void* values, int value_dtype, void* stream) {
+  TM_REQUIRE(plan && quad && coords && D && values, TM_ERR_ARG, "null argument");
+  TM_REQUIRE((nb == 3 && dim == 2) || (nb == 4 && dim == 3), TM_ERR_ARG,
+             "only P1 triangles (nb=3, dim=2) and tetrahedra (nb=4, dim=3) are supported");
+  TM_REQUIRE(quad_is_symmetric(quad, nb), TM_ERR_ARG,
+             "quadrature moments must be invariant under vertex relabelling");
+  TM_REQUIRE(value_dtype == TM_F32 || value_dtype == TM_F64, TM_ERR_ARG, "values must be f32 or f64");
+  hipStream_t st = (hipStream_t)stream;
+  const QuadArg q = quad_arg(quad);
+  GenD g{};
+  for (int i = 0; i < (dim + 1) * (dim + 1); ++i) g.d[i] = D[i];
+  const PlanArgs p = plan_args(plan);
+  if (value_dtype == TM_F32) {
+    if (nb == 3) TM_BY_THREADS(plan, (k_asm_general<3, float, NT>), st, p, coords, g, q, (float*)values);
+    TM_BY_THREADS(plan, (k_asm_general<4, float, NT>), st, p, coords, g, q, (float*)values);
+  }
+  if (nb == 3) TM_BY_THREADS(plan, (k_asm_general<3, double, NT>), st, p, coords, g, q, (double*)values);
+  TM_BY_THREADS(plan, (k_asm_general<4, double, NT>), st, p, coords, g, q, (double*)values);
 }
 
 extern "C" int tm_asm_p1_load(int form, int nb, int dim, const tm_plan_dev* plan, const double* coords,

@@ -71,17 +71,37 @@ def test_scaled_and_f64_coefficient(cuda):
     assert np.abs(K.values.cpu().numpy() - ref).max() <= 1e-13 * np.abs(ref).max()
 
 
-def test_unrecognised_form_uses_generic_path(cuda):
+def test_reaction_diffusion_form_runs_the_general_kernel(cuda):
     m, pts, conn, bnd, et = _mesh("tri", 3)
 
     class Reaction(tm.ElementAssembler):
         def forward(self, gradu, gradv, u, v):
             return gradu @ gradv + 3.0 * u * v
 
-    K = Reaction.from_mesh(m).to(cuda)(m.points.to(cuda))
+    asm = Reaction.from_mesh(m).to(cuda)
+    assert asm._classify(asm.forward, {}, {}, {})[0] == tm._lib.FORM_GENERAL
+    asm.value_dtype = torch.float64
+    K = asm(m.points.to(cuda))
     u, v, pe = op.pattern_and_slots(conn, len(pts))
     ref = oa.reduce(pe, oa.element_matrices(pts, conn, et, "laplace") + 3 * oa.element_matrices(pts, conn, et, "mass"),
                     len(u))
+    assert np.abs(K.values.cpu().numpy() - ref).max() < 1e-12
+
+
+def test_unrecognised_form_uses_generic_path(cuda):
+    """A coefficient that varies in space (here through the coordinates) is not a
+    constant U^T D V: the generic torch path evaluates it on the device in FP64."""
+    m, pts, conn, bnd, et = _mesh("tri", 3)
+
+    class Variable(tm.ElementAssembler):
+        def forward(self, gradu, gradv, x):
+            return (1.0 + x[0]) * (gradu @ gradv)
+
+    asm = Variable.from_mesh(m).to(cuda)
+    assert asm._classify(asm.forward, {"x": m.points}, {}, {}) is None
+    K = asm(m.points.to(cuda))
+    u, v, pe = op.pattern_and_slots(conn, len(pts))
+    ref = oa.reduce(pe, oa.element_matrices(pts, conn, et, "kappa_laplace", kappa=1.0 + pts[:, 0]), len(u))
     assert np.abs(K.values.cpu().numpy() - ref).max() < 1e-12
 
 
