@@ -219,25 +219,14 @@ __device__ __forceinline__ void rowsum_diagonal(const Lds& L, int nown) {
   }
 }
 
-// FORM: 0 laplace, 1 kappa-laplace (nodal coef), 2 mass; all multiplied by scale.
-template <int NB, int FORM, typename CT, typename OT, int NT>
-__global__ __launch_bounds__(NT) void k_asm_matrix(PlanArgs p, const double* __restrict__ X,
-                                                  const CT* __restrict__ coef, double scale, QuadArg q,
-                                                  OT* __restrict__ values) {
+// The element loop of one block: each lane evaluates records r = tid, tid + NT,
+// ... (nxt = the lane's first record, already loaded) and adds the owned rows
+// of each element matrix into L.acc with ds_add_f64.
+template <int NB, int FORM, int NT>
+__device__ __forceinline__ void asm_records(const Lds& L, const PlanArgs& p, int roff, int nrec, int nown, uint4 nxt,
+                                            double scale, const QuadArg& q) {
   constexpr int DIM = NB - 1;
-  constexpr bool ROWSUM = FORM != 2;  // diagonal from the zero row sum
-  extern __shared__ __align__(16) char smem[];
-  const int b = tmx::xcd_remap(blockIdx.x, p.nblk);
-  const int4 bi0 = *(const int4*)(p.blk + 8 * b);
-  const int4 bi1 = *(const int4*)(p.blk + 8 * b + 4);
-  const int hoff = bi0.x, nh = bi0.y, nown = bi0.z, roff = bi0.w;
-  const int nrec = bi1.x, runoff = bi1.y, nrun = bi1.z, nacc = bi1.w;
-  const Lds L = carve(smem, nacc, nh, nown);
-  // the first record is independent of the staged nodes: load it first
-  uint4 nxt = threadIdx.x < nrec ? p.recs[roff + threadIdx.x] : make_uint4(0, 0, 0, 0);
-  stage<NT, DIM, CT, FORM == 1>(L, p.halo + hoff, p.meta + hoff, nacc, nh, nown, X, coef);
-  __syncthreads();
-
+  constexpr bool ROWSUM = FORM != 2;
   for (int r = threadIdx.x; r < nrec; r += NT) {
     const uint4 rc = nxt;
     if (r + NT < nrec) nxt = p.recs[roff + r + NT];  // prefetch the next record
@@ -307,6 +296,28 @@ __global__ __launch_bounds__(NT) void k_asm_matrix(PlanArgs p, const double* __r
         if (own[bb]) atomicAdd(L.acc + meta_off(m[bb]) + rec_rank<NB>(rc, bb, a), kab);
       }
   }
+}
+
+// FORM: 0 laplace, 1 kappa-laplace (nodal coef), 2 mass; all multiplied by scale.
+template <int NB, int FORM, typename CT, typename OT, int NT>
+__global__ __launch_bounds__(NT) void k_asm_matrix(PlanArgs p, const double* __restrict__ X,
+                                                  const CT* __restrict__ coef, double scale, QuadArg q,
+                                                  OT* __restrict__ values) {
+  constexpr int DIM = NB - 1;
+  constexpr bool ROWSUM = FORM != 2;  // diagonal from the zero row sum
+  extern __shared__ __align__(16) char smem[];
+  const int b = tmx::xcd_remap(blockIdx.x, p.nblk);
+  const int4 bi0 = *(const int4*)(p.blk + 8 * b);
+  const int4 bi1 = *(const int4*)(p.blk + 8 * b + 4);
+  const int hoff = bi0.x, nh = bi0.y, nown = bi0.z, roff = bi0.w;
+  const int nrec = bi1.x, runoff = bi1.y, nrun = bi1.z, nacc = bi1.w;
+  const Lds L = carve(smem, nacc, nh, nown);
+  // the first record is independent of the staged nodes: load it first
+  uint4 nxt = threadIdx.x < nrec ? p.recs[roff + threadIdx.x] : make_uint4(0, 0, 0, 0);
+  stage<NT, DIM, CT, FORM == 1>(L, p.halo + hoff, p.meta + hoff, nacc, nh, nown, X, coef);
+  __syncthreads();
+
+  asm_records<NB, FORM, NT>(L, p, roff, nrec, nown, nxt, scale, q);
   __syncthreads();
   if (ROWSUM) {
     rowsum_diagonal<NT>(L, nown);
