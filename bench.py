@@ -323,7 +323,7 @@ def reference_elasticity_leg(sizes, log_fn):
         load = tm.const_node_assembler(1.0).from_mesh(mesh).to(dev)
         free = mesh.points[:, 2].to(dev) > 0.0
         cond = BlockCondenser(asm, free)
-        asm.topology.device_plan(dev, "vector3")
+        asm.stiffness()   # builds the block plan
         torch.cuda.synchronize(dev)
         t_setup = time.perf_counter() - t0
 
