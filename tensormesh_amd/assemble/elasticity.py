@@ -31,8 +31,8 @@ from .hyperelastic import block_matrix
 
 
 # Block size of the stiffness plan: 40 rows in 40 KiB measured fastest for the u = 0 kernel
-# (216.3 us at 64^3 against 236.1 us with the 64-row plan of the neo-Hookean kernels, whose
-# adjoint action prefers the larger blocks; 32 / 48 rows: 224.2 / 222.2 us).
+# (216.3 us at 64^3 against 236.1 us with the 64-row plan that the neo-Hookean adjoint action
+# prefers; 32 / 48 rows: 224.2 / 222.2 us).  The same plan as the neo-Hookean tangent.
 LIN_PLAN_ROWS = 40
 LIN_PLAN_LDS = 40 * 1024 - 64
 

@@ -39,10 +39,9 @@ def _default_plan_params(kind: str, dim: int) -> Tuple[int, int, int]:
     different phases (gathering their nodes, evaluating elements, writing
     rows) overlap, which outweighs the 9 % more element records.
 
-    3x3 blocks (64^3 neo-Hookean): 64 rows in 64 KiB.  40 rows in 40 KiB
-    measured 5 % faster for the tangent + residual and 8 % faster for the
-    linear-elastic stiffness, but 31 % slower for the adjoint action
-    (217 vs 165 us), so the larger blocks stay the default.
+    3x3 blocks (64^3 neo-Hookean): 64 rows in 64 KiB, the adjoint action's
+    best (165 vs 217 us at 40 rows).  The tangent / residual and the
+    linear-elastic stiffness ask for 40-row plans explicitly (faster there).
     """
     threads = os.environ.get("TM_ASM_THREADS")
     if kind == "vector3":
