@@ -164,9 +164,11 @@ int tm_asm_p1_kappa_adjoint(int nb, int dim, const tm_plan_dev* plan, const doub
                             const void* g, int g_dtype, const tm_quad* quad, double* dkappa,
                             void* stream);
 
-/* y = A x for CSR A (values f32 or f64), x and y f64.  Reference: SparseMatrix
- * @ (matrix.py:172-176), SparseMVTorch (mm/torch_mm.py:32-48). */
-int tm_spmv(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const void* values,
+/* y = A x for CSR A (values f32 or f64), x and y f64.  nnz = row_ptr[n], given
+ * by the caller (it picks the lanes per row; the call never synchronises the
+ * stream).  Reference: SparseMatrix @ (matrix.py:172-176), SparseMVTorch
+ * (mm/torch_mm.py:32-48). */
+int tm_spmv(int64_t n, int64_t nnz, const int32_t* row_ptr, const int32_t* col_idx, const void* values,
             int value_dtype, const double* x, double* y, void* stream);
 
 /* dst[i] = src[idx[i]] (4- or 8-byte elements): Condenser value gathers
@@ -222,9 +224,10 @@ int tm_pcg_jacobi(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, con
 /* y = A x for a block-CSR matrix with 3x3 blocks: values[nnz*9], block k at
  * values[9k..9k+8] row-major, on the scalar pattern (row_ptr, col_idx) of the
  * nodes; x and y have 3*n_block_rows entries (dof = node*3 + component).  This is
- * the storage order of SparseMatrix.from_block_coo (sparse/matrix.py:325-375). */
-int tm_spmv_bsr3(int64_t n_block_rows, const int32_t* row_ptr, const int32_t* col_idx, const void* values,
-                 int value_dtype, const double* x, double* y, void* stream);
+ * the storage order of SparseMatrix.from_block_coo (sparse/matrix.py:325-375).
+ * nnz = row_ptr[n_block_rows] (blocks), given by the caller as for tm_spmv. */
+int tm_spmv_bsr3(int64_t n_block_rows, int64_t nnz, const int32_t* row_ptr, const int32_t* col_idx,
+                 const void* values, int value_dtype, const double* x, double* y, void* stream);
 
 /* Jacobi-PCG on a 3x3 block-CSR matrix (point Jacobi on the block diagonals);
  * semantics as tm_pcg_jacobi with n = 3*n_block_rows unknowns; work must hold

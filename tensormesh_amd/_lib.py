@@ -84,7 +84,7 @@ _SIGS = {
                                c_double, ctypes.POINTER(Quad), c_void_p, c_void_p]),
     "tm_asm_p1_kappa_adjoint": (c_int, [c_int, c_int, ctypes.POINTER(PlanDev), c_void_p, c_void_p, c_int,
                                         ctypes.POINTER(Quad), c_void_p, c_void_p]),
-    "tm_spmv": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "tm_spmv": (c_int, [c_i64, c_i64, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "tm_gather": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "tm_pattern_dev_work_size": (c_i64, [c_i64, c_int, c_i64]),
     "tm_pattern_dev_rows": (c_int, [c_void_p, c_i64, c_int, c_i64, c_void_p, c_void_p, c_void_p]),
@@ -97,7 +97,7 @@ _SIGS = {
     "tm_condense_rhs": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
                                 c_void_p, c_void_p]),
     "tm_condense_recover": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
-    "tm_spmv_bsr3": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "tm_spmv_bsr3": (c_int, [c_i64, c_i64, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "tm_pcg_jacobi_bsr3": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                                    c_double, c_double, c_i64, c_int, ctypes.POINTER(c_i64),
                                    ctypes.POINTER(c_double), c_void_p]),

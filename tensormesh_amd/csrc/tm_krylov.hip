@@ -372,26 +372,23 @@ int pcg_dispatch(int64_t n, int64_t nnz, const int32_t* rp, const int32_t* ci, c
 
 }  // namespace
 
-extern "C" int tm_spmv(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const void* values,
-                       int value_dtype, const double* x, double* y, void* stream) {
-  TM_REQUIRE(n >= 0 && row_ptr && col_idx && values && x && y, TM_ERR_ARG, "null argument");
+extern "C" int tm_spmv(int64_t n, int64_t nnz, const int32_t* row_ptr, const int32_t* col_idx,
+                       const void* values, int value_dtype, const double* x, double* y, void* stream) {
+  TM_REQUIRE(n >= 0 && nnz >= 0 && row_ptr && col_idx && values && x && y, TM_ERR_ARG, "null argument");
   if (n == 0) return TM_OK;
-  int32_t nnz_h = 0;
+  const int64_t nnz_h = nnz;
   hipStream_t st = (hipStream_t)stream;
-  TM_HIP(hipMemcpyAsync(&nnz_h, row_ptr + n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-  TM_HIP(hipStreamSynchronize(st));
   if (value_dtype == TM_F32) return spmv_dispatch<float, 1>(n, nnz_h, row_ptr, col_idx, (const float*)values, x, y, st);
   return spmv_dispatch<double, 1>(n, nnz_h, row_ptr, col_idx, (const double*)values, x, y, st);
 }
 
-extern "C" int tm_spmv_bsr3(int64_t n_block_rows, const int32_t* row_ptr, const int32_t* col_idx,
+extern "C" int tm_spmv_bsr3(int64_t n_block_rows, int64_t nnz, const int32_t* row_ptr, const int32_t* col_idx,
                             const void* values, int value_dtype, const double* x, double* y, void* stream) {
-  TM_REQUIRE(n_block_rows >= 0 && row_ptr && col_idx && values && x && y, TM_ERR_ARG, "null argument");
+  TM_REQUIRE(n_block_rows >= 0 && nnz >= 0 && row_ptr && col_idx && values && x && y, TM_ERR_ARG,
+             "null argument");
   if (n_block_rows == 0) return TM_OK;
-  int32_t nnz_h = 0;
+  const int64_t nnz_h = nnz;
   hipStream_t st = (hipStream_t)stream;
-  TM_HIP(hipMemcpyAsync(&nnz_h, row_ptr + n_block_rows, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-  TM_HIP(hipStreamSynchronize(st));
   if (value_dtype == TM_F32)
     return spmv_dispatch<float, 3>(n_block_rows, nnz_h, row_ptr, col_idx, (const float*)values, x, y, st);
   return spmv_dispatch<double, 3>(n_block_rows, nnz_h, row_ptr, col_idx, (const double*)values, x, y, st);

@@ -40,7 +40,7 @@ def _spmv_raw(csr: CSR, values: torch.Tensor, x: torch.Tensor, block_size: int =
     y = torch.empty(csr.n_rows * block_size, dtype=torch.float64, device=values.device)
     fn = _lib.lib().tm_spmv if block_size == 1 else _lib.lib().tm_spmv_bsr3
     assert block_size in (1, 3), f"block size {block_size} not supported"
-    _lib.check(fn(csr.n_rows, _lib.ptr(csr.row_ptr), _lib.ptr(csr.col_idx),
+    _lib.check(fn(csr.n_rows, csr.nnz, _lib.ptr(csr.row_ptr), _lib.ptr(csr.col_idx),
                   _lib.ptr(values.detach().contiguous()), _lib.dtype_code(values),
                   _lib.ptr(x64), _lib.ptr(y), _lib.stream_of(values.device)))
     return y
