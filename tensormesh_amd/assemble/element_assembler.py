@@ -151,16 +151,7 @@ class ElementAssembler(nn.Module):
         return dk
 
     def _classify(self, fn, point_data, scalar_data, element_data):
-        key = None
-        try:
-            code = getattr(fn, "__func__", fn).__code__
-            owner = getattr(fn, "__self__", None)
-            attrs = tuple(sorted((k, v) for k, v in (vars(owner).items() if owner is not None else [])
-                                 if isinstance(v, (int, float, bool, str))))
-            key = (code, id(owner), attrs, tuple(sorted((k, float(v)) for k, v in scalar_data.items())),
-                   tuple(sorted((k, v.dim()) for k, v in point_data.items())), tuple(sorted(element_data)))
-        except Exception:
-            key = None
+        key = forms.form_cache_key(fn, point_data, scalar_data, element_data)
         if key is not None and key in self._form_cache:
             return self._form_cache[key]
         r = forms.classify_bilinear(fn, self.dimension, point_data, scalar_data, element_data)

@@ -62,6 +62,49 @@ def _scalars(scalar_data):
     return {k: (v if torch.is_tensor(v) else torch.tensor(v, dtype=torch.float64)) for k, v in scalar_data.items()}
 
 
+def _value_key(v):
+    """Hashable stand-in for a value a ``forward`` may read: scalars and small
+    tensors / arrays by value, anything else by identity."""
+    if isinstance(v, (bool, int, float, str)) or v is None:
+        return v
+    if torch.is_tensor(v) and v.numel() <= 64:
+        return ("t", str(v.dtype), tuple(v.shape), tuple(v.detach().cpu().reshape(-1).tolist()))
+    if isinstance(v, np.ndarray) and v.size <= 64:
+        return ("a", str(v.dtype), v.shape, tuple(v.reshape(-1).tolist()))
+    return ("id", id(v))
+
+
+def form_cache_key(fn: Callable, point_data: Dict, scalar_data: Dict, element_data: Dict):
+    """Key under which a classification of ``fn`` may be reused, or ``None``.
+
+    It covers what decides the result: the function object (for a bound method
+    the function and its instance), the instance's scalar and small-tensor
+    attributes, the closure's values and the module globals it names (by value,
+    so a changed coefficient is probed again), the scalar data and the
+    point-data ranks.
+    """
+    try:
+        func = getattr(fn, "__func__", fn)
+        owner = getattr(fn, "__self__", None)
+        attrs = ()
+        if owner is not None:
+            attrs = tuple(sorted((k, _value_key(v)) for k, v in vars(owner).items()
+                                 if isinstance(v, (bool, int, float, str)) or
+                                 (torch.is_tensor(v) and v.numel() <= 64)))
+        cells = tuple(_value_key(c.cell_contents) for c in (getattr(func, "__closure__", None) or ()))
+        defaults = tuple(_value_key(d) for d in (getattr(func, "__defaults__", None) or ()))
+        glb = getattr(func, "__globals__", {})
+        names = tuple((n, _value_key(glb[n])) for n in getattr(getattr(func, "__code__", None), "co_names", ())
+                      if n in glb)
+        key = (func, id(owner), attrs, cells, defaults, names,
+               tuple(sorted((k, _value_key(v)) for k, v in scalar_data.items())),
+               tuple(sorted((k, v.dim()) for k, v in point_data.items())), tuple(sorted(element_data)))
+        hash(key)
+        return key
+    except Exception:
+        return None
+
+
 def classify_bilinear(fn: Callable, dim: int, point_data: Dict, scalar_data: Dict,
                       element_data: Dict) -> Optional[Tuple[int, float, Optional[str]]]:
     """Return ``(form, scale, coefficient key)`` or ``None``."""
@@ -85,7 +128,8 @@ def classify_bilinear(fn: Callable, dim: int, point_data: Dict, scalar_data: Dic
     s = set(rest)
     if s == {"gradu", "gradv"}:
         r = _probe(fn, names, build(None), lambda a: torch.dot(a["gradu"], a["gradv"]))
-        return None if r is None else (_lib.FORM_LAPLACE, r, None)
+        if r is not None:
+            return (_lib.FORM_LAPLACE, r, None)
     if len(rest) == 3 and {"gradu", "gradv"} <= s:
         k = (s - {"gradu", "gradv"}).pop()
         if k in point_data and point_data[k].dim() == 1:

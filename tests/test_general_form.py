@@ -98,3 +98,39 @@ def test_generic_path_matches_oracle(et):
     ref = oa.reduce(pe, oa.element_matrices_general(pts, conn, et, D), len(eu))
     assert np.array_equal(K.row.numpy(), eu) and np.array_equal(K.col.numpy(), ev)
     assert np.abs(K.values.double().numpy() - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_form_cache_follows_coefficients():
+    """A classification is reused only while what the forward reads is unchanged:
+    closures, instance attributes, small module-level tensors."""
+    pts, conn = _case("triangle")
+    mesh = tm.Mesh(torch.from_numpy(pts), {"triangle": torch.from_numpy(conn)})
+    asm = tm.LaplaceElementAssembler.from_mesh(mesh)
+
+    def scaled(c):
+        return lambda gradu, gradv: c * (gradu @ gradv)
+
+    assert asm._classify(scaled(2.0), {}, {}, {})[1] == 2.0
+    assert asm._classify(scaled(3.0), {}, {}, {})[1] == 3.0
+    A = torch.eye(2, dtype=torch.float64)
+
+    def aniso(gradu, gradv):
+        return gradu @ A @ gradv
+
+    r1 = asm._classify(aniso, {}, {}, {})
+    A[0, 1] = 0.5
+    r2 = asm._classify(aniso, {}, {}, {})
+    assert r1 == (tm._lib.FORM_LAPLACE, 1.0, None)                 # isotropic: the Laplace kernel
+    assert r2[0] == tm._lib.FORM_GENERAL and r2[1][1, 2] == 0.5 and r2[1][2, 1] == 0.0
+
+    class Scaled(tm.ElementAssembler):
+        def __post_init__(self, c=1.0):
+            self.c = c
+
+        def forward(self, gradu, gradv):
+            return self.c * (gradu @ gradv)
+
+    s = Scaled.from_mesh(mesh, c=2.0)
+    assert s._classify(s.forward, {}, {}, {})[1] == 2.0
+    s.c = 5.0
+    assert s._classify(s.forward, {}, {}, {})[1] == 5.0

@@ -78,3 +78,27 @@ def test_general_kernel_equals_laplace_kernel(cuda, et):
     lap = asm._launch_matrix(tm._lib.FORM_LAPLACE, p64, None, 1.0, torch.float64)
     gen = asm._launch_general(p64, np.diag([0.0] + [1.0] * d), torch.float64)
     assert (gen - lap).abs().max().item() <= 1e-13 * lap.abs().max().item()
+
+
+@pytest.mark.parametrize("et", ["triangle", "tetra"])
+def test_anisotropic_diffusion_runs_the_general_kernel(cuda, et):
+    print(f"case anisotropic diffusion {et}", flush=True)
+    pts, conn = _case(et, seed=9)
+    d = pts.shape[1]
+    A = np.eye(d) + 0.3 * np.random.default_rng(10).standard_normal((d, d))
+    At = torch.tensor(A)
+
+    class Aniso(tm.ElementAssembler):
+        def forward(self, gradu, gradv):
+            return gradu @ At.to(gradu.dtype) @ gradv
+
+    mesh = tm.Mesh(torch.from_numpy(pts), {et: torch.from_numpy(conn)})
+    asm = Aniso.from_mesh(mesh).to(cuda)
+    asm.value_dtype = torch.float64
+    assert asm._classify(asm.forward, {}, {}, {})[0] == tm._lib.FORM_GENERAL
+    K = asm(mesh.points.to(cuda))
+    D = np.zeros((d + 1, d + 1))
+    D[1:, 1:] = A
+    eu, ev, pe = op.pattern_and_slots(conn, len(pts))
+    ref = oa.reduce(pe, oa.element_matrices_general(pts, conn, et, D), len(eu))
+    assert _row_scaled(K.values.cpu().numpy(), ref, eu) <= 1e-12
