@@ -221,6 +221,18 @@ int tm_pcg_jacobi(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, con
                   double atol, int64_t maxiter, int check_every, int64_t* iters, double* relres,
                   void* stream);
 
+/* Jacobi-preconditioned BiCGSTAB (right preconditioning) for square, possibly
+ * non-symmetric CSR systems: the reference's lse_solver bicgstab_py
+ * (sparse/solve/torch_solve.py:130-248) with device-side scalars, FP64 vectors
+ * and reductions.  Stops when ||b - A x|| <= max(rtol ||b||, atol); x holds
+ * the initial guess on entry.  nnz = row_ptr[n] (picks the lanes per row).
+ * work: tm_bicgstab_work_size(n) bytes.  Returns TM_ERR_NOTCONV on breakdown
+ * or when maxiter is reached (x is the last iterate, *relres its residual). */
+int64_t tm_bicgstab_work_size(int64_t n);
+int tm_bicgstab_jacobi(int64_t n, int64_t nnz, const int32_t* row_ptr, const int32_t* col_idx, const void* values,
+                       int value_dtype, const double* b, double* x, void* work, double rtol, double atol,
+                       int64_t maxiter, int check_every, int64_t* iters, double* relres, void* stream);
+
 /* y = A x for a block-CSR matrix with 3x3 blocks: values[nnz*9], block k at
  * values[9k..9k+8] row-major, on the scalar pattern (row_ptr, col_idx) of the
  * nodes; x and y have 3*n_block_rows entries (dof = node*3 + component).  This is

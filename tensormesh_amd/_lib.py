@@ -139,6 +139,10 @@ _SIGS = {
     "tm_fcg_start": (c_int, [c_i64, c_int] + [c_void_p] * 7 + [c_double, c_double, c_i64, c_void_p]),
     "tm_fcg_iterate": (c_int, [c_i64, c_int] + [c_void_p] * 7),
     "tm_pcg_work_size": (c_i64, [c_i64]),
+    "tm_bicgstab_work_size": (c_i64, [c_i64]),
+    "tm_bicgstab_jacobi": (c_int, [c_i64, c_i64, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                                   c_double, c_double, c_i64, c_int, ctypes.POINTER(c_i64),
+                                   ctypes.POINTER(c_double), c_void_p]),
     "tm_pcg_jacobi": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                               c_double, c_double, c_i64, c_int, ctypes.POINTER(c_i64),
                               ctypes.POINTER(c_double), c_void_p]),

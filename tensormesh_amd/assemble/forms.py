@@ -132,10 +132,11 @@ def classify_bilinear(fn: Callable, dim: int, point_data: Dict, scalar_data: Dic
             return (_lib.FORM_LAPLACE, r, None)
     if len(rest) == 3 and {"gradu", "gradv"} <= s:
         k = (s - {"gradu", "gradv"}).pop()
-        if k in point_data and point_data[k].dim() == 1:
+        if k in point_data:
+            if point_data[k].dim() != 1:
+                return None
             r = _probe(fn, names, build(None), lambda a: a[k] * torch.dot(a["gradu"], a["gradv"]))
             return None if r is None else (_lib.FORM_KAPPA_LAPLACE, r, k)
-        return None
     if s == {"u", "v"}:
         r = _probe(fn, names, build(None), lambda a: a["u"] * a["v"])
         if r is not None:

@@ -370,6 +370,254 @@ int pcg_dispatch(int64_t n, int64_t nnz, const int32_t* rp, const int32_t* ci, c
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Jacobi-preconditioned BiCGSTAB (right preconditioning, M = diag(A)) for
+// non-symmetric systems: the algorithm of sparse/solve/torch_solve.py:130-248
+// with device-side scalars.  One iteration is five kernels:
+//   B1  v = A phat, (r0h, v)
+//   B2  alpha = rho / (r0h, v); s = r - alpha v, shat = M^-1 s
+//   B3  t = A shat, (t, t), (t, s)
+//   B4  omega = (t, s) / (t, t); x += alpha phat + omega shat; r = s - omega t;
+//       (r, r), (r0h, r)
+//   B5  convergence test; beta = (rho_new / rho)(alpha / omega);
+//       p = r + beta (p - omega v), phat = M^-1 p
+// Every scalar is summed from the producer's partials by each consumer in a
+// fixed order; the values a later kernel needs (rho, alpha) are stored by
+// workgroup 0 of the kernel that formed them, and never overwritten in the
+// kernel that reads them.
+struct BcScalars {
+  double rho, alpha, bb, tol2, relres;
+};
+struct BcWork {
+  double *r, *r0h, *p, *v, *s, *t, *phat, *shat, *minv, *part;
+  BcScalars* sc;
+  PcgFlags* f;
+};
+constexpr int B_R0V = 0, B_TT = 1, B_TS = 2, B_RR = 3, B_RHO = 4, B_BB = 5;
+
+BcWork carve_bc(void* work, int64_t n) {
+  BcWork w;
+  double* d = (double*)work;
+  w.r = d;
+  w.r0h = d + n;
+  w.p = d + 2 * n;
+  w.v = d + 3 * n;
+  w.s = d + 4 * n;
+  w.t = d + 5 * n;
+  w.phat = d + 6 * n;
+  w.shat = d + 7 * n;
+  w.minv = d + 8 * n;
+  w.part = d + 9 * n;
+  w.sc = (BcScalars*)(w.part + 6 * kMaxBlocks);
+  w.f = (PcgFlags*)(w.sc + 1);
+  return w;
+}
+
+// y = A x with one partial (a, y) per workgroup (a = r0h for B1, y = t -> (t,t), (t,s) for B3)
+template <typename VT, int S, bool TWO>
+__global__ __launch_bounds__(kThreads) void k_bc_spmv(int64_t nrows, const int32_t* __restrict__ rp,
+                                                     const int32_t* __restrict__ ci, const VT* __restrict__ v,
+                                                     const double* __restrict__ x, double* __restrict__ y,
+                                                     const double* __restrict__ a, BcWork w, int slot0) {
+  if (w.f->status != 0) return;
+  const int sl = threadIdx.x % S;
+  constexpr int64_t stride = kThreads / S;
+  const RowChunk rc = row_chunk(nrows);
+  const int lim = rp[rc.r1];
+  double loc[2] = {0.0, 0.0};
+  for (int64_t row = rc.r0 + threadIdx.x / S; row < rc.r1; row += stride) {
+    double o[1] = {0.0};
+    row_accum<VT, S, 1>(rp, ci, v, x, row, sl, lim, o);
+    sub_reduce<S, 1>(o);
+    if (sl == 0) {
+      y[row] = o[0];
+      if (TWO) {
+        loc[0] += o[0] * o[0];
+        loc[1] += o[0] * a[row];
+      } else {
+        loc[0] += o[0] * a[row];
+      }
+    }
+  }
+  if (TWO) {
+    const int which[2] = {slot0, slot0 + 1};
+    write_parts<2>(w.part, which, loc);
+  } else {
+    double l1[1] = {loc[0]};
+    const int which[1] = {slot0};
+    write_parts<1>(w.part, which, l1);
+  }
+}
+
+// r = b - v (v = A x0), r0h = r, p = r, phat = M^-1 r; partials (r,r), (r0h,r), (b,b)
+__global__ __launch_bounds__(kThreads) void k_bc_init(int64_t n, const double* __restrict__ b, BcWork w) {
+  double loc[3] = {0.0, 0.0, 0.0};
+  for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    const double bi = b[i];
+    const double ri = bi - w.v[i];
+    w.r[i] = ri;
+    w.r0h[i] = ri;
+    w.p[i] = ri;
+    w.phat[i] = w.minv[i] * ri;
+    loc[0] += ri * ri;
+    loc[1] += ri * ri;
+    loc[2] += bi * bi;
+  }
+  const int which[3] = {B_RR, B_RHO, B_BB};
+  write_parts<3>(w.part, which, loc);
+}
+
+__global__ __launch_bounds__(kThreads) void k_bc_ctl0(BcWork w, int nparts, double rtol2, double atol2,
+                                                     long long maxiter) {
+  const int which[3] = {B_RR, B_RHO, B_BB};
+  double t[3];
+  sum_parts<3>(w.part, which, nparts, t);
+  if (threadIdx.x == 0) {
+    w.sc->bb = t[2];
+    w.sc->tol2 = fmax(rtol2 * t[2], atol2);
+    w.sc->relres = t[2] > 0.0 ? sqrt(t[0] / t[2]) : 0.0;
+    w.f->iter = 0;
+    w.f->maxiter = maxiter;
+    w.f->step = 0;
+    w.f->status = (t[0] <= w.sc->tol2) ? 1 : 0;
+  }
+}
+
+// B2: alpha = rho / (r0h, v) with rho = (r0h, r) of the current r; s, shat
+__global__ __launch_bounds__(kThreads) void k_bc_s(int64_t n, BcWork w, int nparts) {
+  if (w.f->status != 0) return;
+  const int which[2] = {B_R0V, B_RHO};
+  double t[2];
+  sum_parts<2>(w.part, which, nparts, t);
+  const double denom = t[0], rho = t[1];
+  const bool ok = denom != 0.0 && __builtin_isfinite(denom) && __builtin_isfinite(rho);
+  const double alpha = ok ? rho / denom : 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    w.sc->rho = rho;
+    w.sc->alpha = alpha;
+    if (!ok) w.f->status = 2;
+  }
+  if (!ok) return;
+  for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    const double si = w.r[i] - alpha * w.v[i];
+    w.s[i] = si;
+    w.shat[i] = w.minv[i] * si;
+  }
+}
+
+// B4: omega, x and r updates; partials (r,r), (r0h,r).  `step` tells B5 that
+// this kernel ran (B5 cannot rely on the status word, see PcgFlags).
+__global__ __launch_bounds__(kThreads) void k_bc_x(int64_t n, double* __restrict__ x, BcWork w, int nparts) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) w.f->step = 0;
+  if (w.f->status != 0) return;
+  const int which_in[2] = {B_TT, B_TS};
+  double t[2];
+  sum_parts<2>(w.part, which_in, nparts, t);
+  const double omega = t[0] != 0.0 ? t[1] / t[0] : 0.0;
+  const double alpha = w.sc->alpha;
+  double loc[2] = {0.0, 0.0};
+  for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    const double si = w.s[i];
+    x[i] += alpha * w.phat[i] + omega * w.shat[i];
+    const double ri = si - omega * w.t[i];
+    w.r[i] = ri;
+    loc[0] += ri * ri;
+    loc[1] += w.r0h[i] * ri;
+  }
+  const int which[2] = {B_RR, B_RHO};
+  write_parts<2>(w.part, which, loc);
+  if (blockIdx.x == 0 && threadIdx.x == 0) w.f->step = 1;
+}
+
+// B5: convergence test and the new direction
+__global__ __launch_bounds__(kThreads) void k_bc_dir(int64_t n, BcWork w, int nparts) {
+  if (w.f->step == 0) return;
+  const int which[4] = {B_RR, B_RHO, B_TT, B_TS};
+  double t[4];
+  sum_parts<4>(w.part, which, nparts, t);
+  const double rr = t[0], rho_new = t[1];
+  const double omega = t[2] != 0.0 ? t[3] / t[2] : 0.0;
+  const bool conv = rr <= w.sc->tol2;
+  const bool stall = !conv && (omega == 0.0 || !__builtin_isfinite(rho_new));
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const int it = w.f->iter + 1;
+    w.f->iter = it;
+    w.sc->relres = w.sc->bb > 0.0 ? sqrt(rr / w.sc->bb) : 0.0;
+    if (conv)
+      w.f->status = 1;
+    else if (stall)
+      w.f->status = 2;
+    else if (it >= w.f->maxiter)
+      w.f->status = 3;
+  }
+  if (conv || stall) return;
+  const double beta = (rho_new / w.sc->rho) * (w.sc->alpha / omega);
+  for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    const double pi = w.r[i] + beta * (w.p[i] - omega * w.v[i]);
+    w.p[i] = pi;
+    w.phat[i] = w.minv[i] * pi;
+  }
+}
+
+template <typename VT, int S>
+int bc_run(int64_t n, const int32_t* rp, const int32_t* ci, const VT* v, const double* b, double* x, BcWork w,
+           double rtol, double atol, int64_t maxiter, int check_every, int64_t* iters, double* relres,
+           hipStream_t st) {
+  const int G = grid_for(n, kThreads * 2);
+  hipLaunchKernelGGL((k_diag_inv<VT, 1>), dim3(grid_for(n, kThreads)), dim3(kThreads), 0, st, n, rp, ci, v,
+                     w.minv);
+  hipLaunchKernelGGL((k_spmv<VT, S, 1, 2>), dim3(G), dim3(kThreads), 0, st, n, rp, ci, v, x, w.v);
+  hipLaunchKernelGGL(k_bc_init, dim3(G), dim3(kThreads), 0, st, n, b, w);
+  hipLaunchKernelGGL(k_bc_ctl0, dim3(1), dim3(kThreads), 0, st, w, G, rtol * rtol, atol * atol, (long long)maxiter);
+  TM_HIP(hipGetLastError());
+  PcgFlags hf{};
+  BcScalars hs{};
+  int chunk = check_every > 0 ? check_every : 16;
+  for (;;) {
+    TM_HIP(hipMemcpyAsync(&hf, w.f, sizeof(hf), hipMemcpyDeviceToHost, st));
+    TM_HIP(hipStreamSynchronize(st));
+    if (hf.status != 0) break;
+    for (int k = 0; k < chunk; ++k) {
+      hipLaunchKernelGGL((k_bc_spmv<VT, S, false>), dim3(G), dim3(kThreads), 0, st, n, rp, ci, v, w.phat, w.v,
+                         w.r0h, w, B_R0V);
+      hipLaunchKernelGGL(k_bc_s, dim3(G), dim3(kThreads), 0, st, n, w, G);
+      hipLaunchKernelGGL((k_bc_spmv<VT, S, true>), dim3(G), dim3(kThreads), 0, st, n, rp, ci, v, w.shat, w.t,
+                         w.s, w, B_TT);
+      hipLaunchKernelGGL(k_bc_x, dim3(G), dim3(kThreads), 0, st, n, x, w, G);
+      hipLaunchKernelGGL(k_bc_dir, dim3(G), dim3(kThreads), 0, st, n, w, G);
+    }
+    TM_HIP(hipGetLastError());
+    if (check_every <= 0 && chunk < 256) chunk *= 2;
+  }
+  TM_HIP(hipMemcpyAsync(&hs, w.sc, sizeof(hs), hipMemcpyDeviceToHost, st));
+  TM_HIP(hipStreamSynchronize(st));
+  if (iters) *iters = hf.iter;
+  if (relres) *relres = hs.relres;
+  if (hf.status == 2) {
+    tmx::set_error("BiCGSTAB breakdown at iteration %d (relres %.3e)", hf.iter, hs.relres);
+    return TM_ERR_NOTCONV;
+  }
+  if (hf.status == 3) {
+    tmx::set_error("BiCGSTAB did not converge in %lld iterations (relres %.3e)", (long long)maxiter, hs.relres);
+    return TM_ERR_NOTCONV;
+  }
+  return TM_OK;
+}
+
+template <typename VT>
+int bc_dispatch(int64_t n, int64_t nnz, const int32_t* rp, const int32_t* ci, const VT* v, const double* b, double* x,
+                BcWork w, double rtol, double atol, int64_t maxiter, int check_every, int64_t* iters, double* relres,
+                hipStream_t st) {
+  switch (lanes_for(n, nnz)) {
+    case 2: return bc_run<VT, 2>(n, rp, ci, v, b, x, w, rtol, atol, maxiter, check_every, iters, relres, st);
+    case 4: return bc_run<VT, 4>(n, rp, ci, v, b, x, w, rtol, atol, maxiter, check_every, iters, relres, st);
+    case 8: return bc_run<VT, 8>(n, rp, ci, v, b, x, w, rtol, atol, maxiter, check_every, iters, relres, st);
+    case 16: return bc_run<VT, 16>(n, rp, ci, v, b, x, w, rtol, atol, maxiter, check_every, iters, relres, st);
+    default: return bc_run<VT, 32>(n, rp, ci, v, b, x, w, rtol, atol, maxiter, check_every, iters, relres, st);
+  }
+}
+
 }  // namespace
 
 extern "C" int tm_spmv(int64_t n, int64_t nnz, const int32_t* row_ptr, const int32_t* col_idx,
@@ -445,4 +693,24 @@ extern "C" int tm_pcg_jacobi_bsr3(int64_t n_block_rows, const int32_t* row_ptr, 
                                   maxiter, check_every, iters, relres, st);
   return pcg_dispatch<double, 3>(n_block_rows, nnz_h, row_ptr, col_idx, (const double*)values, b, x, w, rtol, atol,
                                  maxiter, check_every, iters, relres, st);
+}
+
+extern "C" int64_t tm_bicgstab_work_size(int64_t n) {
+  return (int64_t)sizeof(double) * (9 * n + 6 * kMaxBlocks) + (int64_t)sizeof(BcScalars) +
+         (int64_t)sizeof(PcgFlags) + 64;
+}
+
+extern "C" int tm_bicgstab_jacobi(int64_t n, int64_t nnz, const int32_t* row_ptr, const int32_t* col_idx,
+                                  const void* values, int value_dtype, const double* b, double* x, void* work,
+                                  double rtol, double atol, int64_t maxiter, int check_every, int64_t* iters,
+                                  double* relres, void* stream) {
+  TM_REQUIRE(n > 0 && nnz >= 0 && row_ptr && col_idx && values && b && x && work, TM_ERR_ARG, "null argument");
+  TM_REQUIRE(maxiter > 0, TM_ERR_ARG, "maxiter must be positive");
+  hipStream_t st = (hipStream_t)stream;
+  BcWork w = carve_bc(work, n);
+  if (value_dtype == TM_F32)
+    return bc_dispatch<float>(n, nnz, row_ptr, col_idx, (const float*)values, b, x, w, rtol, atol, maxiter,
+                              check_every, iters, relres, st);
+  return bc_dispatch<double>(n, nnz, row_ptr, col_idx, (const double*)values, b, x, w, rtol, atol, maxiter,
+                             check_every, iters, relres, st);
 }

@@ -23,7 +23,7 @@ import scipy.sparse
 import scipy.sparse.linalg
 import torch
 
-from .ops import CSR, SpMV, PCGSolve
+from .ops import CSR, SpMV, PCGSolve, BiCGSTABSolve
 
 
 class SparseMatrix:
@@ -253,9 +253,11 @@ class SparseMatrix:
               **kwargs) -> torch.Tensor:
         """Solve ``A x = b``.
 
-        GPU tensors: Jacobi-preconditioned CG (the matrix must be SPD); stops
+        GPU tensors: Jacobi-preconditioned CG for symmetric (SPD) matrices,
+        Jacobi-preconditioned BiCGSTAB (the reference's ``lse_solver``) for
+        matrices known to be non-symmetric or with ``method="bicgstab"``; stops
         when ``||b - A x|| <= max(tol ||b||, atol)``.  ``backend`` may be
-        ``None``/``"hip"``; ``method`` ``None``/``"cg"``/``"pcg"``.
+        ``None``/``"hip"``; ``method`` ``None``/``"cg"``/``"pcg"``/``"bicgstab"``.
         CPU tensors: SciPy SuperLU.  Differentiable w.r.t. values and b.
         """
         assert self.shape[0] == self.shape[1], f"solve needs a square matrix, got {self.shape}"
@@ -272,9 +274,21 @@ class SparseMatrix:
             return torch.from_numpy(np.asarray(x)).to(torch.promote_types(self.dtype, b.dtype))
         if backend not in (None, "hip", "mi355x", "auto"):
             raise ValueError(f"backend {backend!r} is not available for GPU tensors (use 'hip')")
-        if method not in (None, "cg", "pcg", "jacobi-pcg", "auto"):
-            raise ValueError(f"method {method!r} is not available on the GPU (Jacobi-PCG only)")
+        if method not in (None, "cg", "pcg", "jacobi-pcg", "bicgstab", "auto"):
+            raise ValueError(f"method {method!r} is not available on the GPU (Jacobi-PCG or BiCGSTAB)")
         info: dict = {}
+        if method == "bicgstab" or (method in (None, "auto") and self.symmetric is False):
+            if self.block_size != 1:
+                raise ValueError("BiCGSTAB covers scalar CSR matrices")
+            v, r, c = self._canonical()
+            x = BiCGSTABSolve.apply(v, b.to(self.device), self.csr(), r, c, float(tol), float(atol), int(maxiter),
+                                    info)
+            self.last_solve_info = info
+            if verbose:
+                print(f"[tensormesh_amd] solve: n={self.shape[0]}, nnz={self.nnz}, dtype={self.dtype}, "
+                      f"device={self.device}, backend=hip, method=bicgstab, "
+                      f"iterations={info.get('iterations')}, relres={info.get('relres'):.3e}")
+            return x
         if self.block_size != 1:
             x = PCGSolve.apply(self.values, b.to(self.device), self._csr, self.row_indices, self.col_indices,
                                float(tol), float(atol), int(maxiter), info, self.block_size)

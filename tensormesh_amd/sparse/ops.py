@@ -132,6 +132,56 @@ class PCGSolve(torch.autograd.Function):
         return gv, gb, None, None, None, None, None, None, None, None
 
 
+def bicgstab_raw(csr: CSR, values: torch.Tensor, b: torch.Tensor, x0: Optional[torch.Tensor] = None,
+                 rtol: float = 1e-8, atol: float = 0.0, maxiter: int = 10000, check_every: int = 0,
+                 raise_on_fail: bool = False):
+    """Jacobi-preconditioned BiCGSTAB on the device (square CSR, any symmetry);
+    returns ``(x [f64], iterations, relres, converged)``."""
+    _lib.require_gpu(values, "matrix values")
+    dev = values.device
+    n = csr.n_rows
+    b64 = b.detach().to(torch.float64).contiguous()
+    x = (torch.zeros(n, dtype=torch.float64, device=dev) if x0 is None
+         else x0.detach().to(torch.float64).clone().contiguous())
+    L = _lib.lib()
+    work = torch.empty(int(L.tm_bicgstab_work_size(n)) // 8 + 1, dtype=torch.float64, device=dev)
+    it = ctypes.c_int64(0)
+    rr = ctypes.c_double(0.0)
+    st = L.tm_bicgstab_jacobi(n, csr.nnz, _lib.ptr(csr.row_ptr), _lib.ptr(csr.col_idx),
+                              _lib.ptr(values.detach().contiguous()), _lib.dtype_code(values), _lib.ptr(b64),
+                              _lib.ptr(x), _lib.ptr(work), float(rtol), float(atol), int(maxiter), int(check_every),
+                              ctypes.byref(it), ctypes.byref(rr), _lib.stream_of(dev))
+    converged = st == _lib.TM_OK
+    if not converged:
+        if st != _lib.TM_ERR_NOTCONV or raise_on_fail:
+            _lib.check(st)
+        warnings.warn(f"BiCGSTAB did not converge: {L.tm_last_error().decode()}")
+    return x, int(it.value), float(rr.value), converged
+
+
+class BiCGSTABSolve(torch.autograd.Function):
+    """``x = A^-1 b`` by BiCGSTAB; backward ``lam = A^-T g`` by BiCGSTAB on the
+    transposed CSR, ``dL/dvalues = -lam[row] x[col]`` (``torch_sla_solve.py:102-136``)."""
+
+    @staticmethod
+    def forward(ctx, values, b, csr: CSR, row, col, rtol, atol, maxiter, info: dict):
+        x, it, rel, ok = bicgstab_raw(csr, values, b, None, rtol, atol, maxiter)
+        info.update(iterations=it, relres=rel, converged=ok, method="bicgstab")
+        ctx.csr, ctx.tol = csr, (rtol, atol, maxiter)
+        ctx.save_for_backward(values, x, row, col)
+        return x.to(torch.promote_types(values.dtype, b.dtype))
+
+    @staticmethod
+    def backward(ctx, g):
+        values, x, row, col = ctx.saved_tensors
+        rtol, atol, maxiter = ctx.tol
+        csr_t, vt = _transpose_csr(ctx.csr, values)
+        lam, _, _, _ = bicgstab_raw(csr_t, vt, g, None, rtol, atol, maxiter)
+        gv = (-lam[row] * x[col]).to(values.dtype) if ctx.needs_input_grad[0] else None
+        gb = lam.to(g.dtype) if ctx.needs_input_grad[1] else None
+        return gv, gb, None, None, None, None, None, None, None
+
+
 def gather_raw(src: torch.Tensor, idx: torch.Tensor) -> torch.Tensor:
     _lib.require_gpu(src, "gather source")
     src = src.contiguous()

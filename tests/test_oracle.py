@@ -277,3 +277,23 @@ def test_kappa_gradient_is_finite_difference(jittered_tri):
         kp[c] += 1e-6
         km[c] -= 1e-6
         assert abs((L(kp) - L(km)) / 2e-6 - dk[c]) < 1e-7
+
+
+def test_bicgstab_oracle_pins():
+    """BiCGSTAB (torch_solve.py:130-248 order): non-symmetric systems against SuperLU,
+    one step on a diagonal system, and no step for b = 0."""
+    rng = np.random.default_rng(11)
+    n = 400
+    B = scipy.sparse.random(n, n, density=0.02, random_state=11, format="csr")
+    d = np.asarray(abs(B).sum(1)).ravel() + 1.0
+    A = (B + scipy.sparse.diags(d)).tocsr()
+    assert abs(A - A.T).max() > 0.1                                  # genuinely non-symmetric
+    b = rng.standard_normal(n)
+    x, it, rel = osv.bicgstab(A, b, rtol=1e-12)
+    assert rel <= 1e-12 and np.linalg.norm(x - osv.direct(A, b)) <= 1e-10 * np.linalg.norm(x)
+    D = scipy.sparse.diags(np.linspace(1.0, 3.0, 50)).tocsr()
+    bd = rng.standard_normal(50)
+    x, it, rel = osv.bicgstab(D, bd, rtol=1e-14)
+    assert it == 1 and np.allclose(x, bd / np.linspace(1.0, 3.0, 50), rtol=1e-14, atol=0)
+    x, it, rel = osv.bicgstab(A, np.zeros(n))
+    assert it == 0 and not x.any()
