@@ -255,7 +255,8 @@ class SparseMatrix:
 
         GPU tensors: Jacobi-preconditioned CG for symmetric (SPD) matrices,
         Jacobi-preconditioned BiCGSTAB (the reference's ``lse_solver``) for
-        matrices known to be non-symmetric or with ``method="bicgstab"``; stops
+        non-symmetric ones (assembled matrices carry the flag; others are
+        checked once) or with ``method="bicgstab"``; stops
         when ``||b - A x|| <= max(tol ||b||, atol)``.  ``backend`` may be
         ``None``/``"hip"``; ``method`` ``None``/``"cg"``/``"pcg"``/``"bicgstab"``.
         CPU tensors: SciPy SuperLU.  Differentiable w.r.t. values and b.
@@ -277,7 +278,7 @@ class SparseMatrix:
         if method not in (None, "cg", "pcg", "jacobi-pcg", "bicgstab", "auto"):
             raise ValueError(f"method {method!r} is not available on the GPU (Jacobi-PCG or BiCGSTAB)")
         info: dict = {}
-        if method == "bicgstab" or (method in (None, "auto") and self.symmetric is False):
+        if method == "bicgstab" or (method in (None, "auto") and self.block_size == 1 and not self.is_symmetric()):
             if self.block_size != 1:
                 raise ValueError("BiCGSTAB covers scalar CSR matrices")
             v, r, c = self._canonical()
