@@ -1,2 +1,3 @@
 from .matrix import SparseMatrix
 from .ops import CSR, pcg_raw
+from .nonlinear import nonlinear_solve, NonLinearSolveFunction

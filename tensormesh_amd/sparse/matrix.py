@@ -144,7 +144,9 @@ class SparseMatrix:
         return self
 
     def transpose(self) -> "SparseMatrix":
-        return SparseMatrix(self.values, self.col_indices, self.row_indices, (self.shape[1], self.shape[0]))
+        t = SparseMatrix(self.values, self.col_indices, self.row_indices, (self.shape[1], self.shape[0]))
+        t.symmetric = self.symmetric
+        return t
 
     @property
     def T(self):
