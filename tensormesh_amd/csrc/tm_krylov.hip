@@ -484,6 +484,20 @@ __global__ __launch_bounds__(kThreads) void k_bc_ctl0(BcWork w, int nparts, doub
   }
 }
 
+// After the recursion reports convergence: with v = A x and r = b - v
+// recomputed by k_bc_init, test the true residual (keeps the iteration count);
+// a miss restarts the recursion from this r (residual replacement).
+__global__ __launch_bounds__(kThreads) void k_bc_recheck(BcWork w, int nparts) {
+  const int which[1] = {B_RR};
+  double t[1];
+  sum_parts<1>(w.part, which, nparts, t);
+  if (threadIdx.x == 0) {
+    w.sc->relres = w.sc->bb > 0.0 ? sqrt(t[0] / w.sc->bb) : 0.0;
+    w.f->step = 0;
+    w.f->status = (t[0] <= w.sc->tol2) ? 1 : (w.f->iter >= w.f->maxiter ? 3 : 0);
+  }
+}
+
 // B2: alpha = rho / (r0h, v) with rho = (r0h, r) of the current r; s, shat
 __global__ __launch_bounds__(kThreads) void k_bc_s(int64_t n, BcWork w, int nparts) {
   if (w.f->status != 0) return;
@@ -574,9 +588,20 @@ int bc_run(int64_t n, const int32_t* rp, const int32_t* ci, const VT* v, const d
   PcgFlags hf{};
   BcScalars hs{};
   int chunk = check_every > 0 ? check_every : 16;
+  int restarts = 0;
   for (;;) {
     TM_HIP(hipMemcpyAsync(&hf, w.f, sizeof(hf), hipMemcpyDeviceToHost, st));
     TM_HIP(hipStreamSynchronize(st));
+    if (hf.status == 1 && hf.iter > 0 && restarts < 3) {
+      // the recursive residual met the tolerance: check the true one (x is final here)
+      ++restarts;
+      hipLaunchKernelGGL((k_spmv<VT, S, 1, 2>), dim3(G), dim3(kThreads), 0, st, n, rp, ci, v, x, w.v);
+      hipLaunchKernelGGL(k_bc_init, dim3(G), dim3(kThreads), 0, st, n, b, w);
+      hipLaunchKernelGGL(k_bc_recheck, dim3(1), dim3(kThreads), 0, st, w, G);
+      TM_HIP(hipGetLastError());
+      TM_HIP(hipMemcpyAsync(&hf, w.f, sizeof(hf), hipMemcpyDeviceToHost, st));
+      TM_HIP(hipStreamSynchronize(st));
+    }
     if (hf.status != 0) break;
     for (int k = 0; k < chunk; ++k) {
       hipLaunchKernelGGL((k_bc_spmv<VT, S, false>), dim3(G), dim3(kThreads), 0, st, n, rp, ci, v, w.phat, w.v,

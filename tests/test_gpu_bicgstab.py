@@ -74,6 +74,11 @@ def test_convection_diffusion_solve(cuda):
     A = Ki.to_scipy_coo().tocsr()
     xd = scipy.sparse.linalg.spsolve(A.tocsc(), fi.double().cpu().numpy())
     assert np.linalg.norm(x.cpu().numpy() - xd) <= 1e-8 * np.linalg.norm(xd)
+    # the reported residual is the true one, b - A x recomputed after convergence
+    bh = fi.double().cpu().numpy()
+    true_rel = np.linalg.norm(bh - A @ x.cpu().numpy()) / np.linalg.norm(bh)
+    assert true_rel <= 1e-10 * 1.01
+    assert abs(Ki.last_solve_info["relres"] - true_rel) <= 1e-3 * true_rel + 1e-16
 
 
 def test_bicgstab_trivial_systems(cuda):
