@@ -26,11 +26,14 @@ def csr(values, row, col, n):
 
 
 def diagonal(A: scipy.sparse.csr_matrix):
+    """Jacobi diagonal: a missing entry is 1 (``torch_solve.py:24-59``) and so
+    is a stored one below 1e-12 in magnitude (``torch_solve.py:177-182``)."""
     d = np.ones(A.shape[0])
     coo = A.tocoo()
     m = coo.row == coo.col
     d[coo.row[m]] = 0.0
     np.add.at(d, coo.row[m], coo.data[m])
+    d[np.abs(d) < 1e-12] = 1.0
     return d
 
 

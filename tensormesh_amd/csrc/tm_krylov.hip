@@ -40,7 +40,8 @@ struct PcgScalars {
 // K3 cannot use `status` for this, since its own workgroup 0 may set it while
 // other workgroups of the launch are still to start.
 struct PcgFlags {
-  int status;  // 0 running, 1 converged, 2 breakdown (p.Ap <= 0), 3 max iterations
+  int status;  // 0 running, 1 converged, 2 breakdown (p.Ap <= 0), 3 max iterations,
+               // 4 true residual missed after the restarts (BiCGSTAB)
   int iter;
   long long maxiter;
   int step;
@@ -129,8 +130,10 @@ __global__ __launch_bounds__(kThreads) void k_diag_inv(int64_t nrows, const int3
         }
         found = true;
       }
+    // a stored diagonal below 1e-12 in magnitude counts as 1, as a missing one
+    // does (the Jacobi set-up of bicgstab_py, torch_solve.py:177-182)
 #pragma unroll
-    for (int c = 0; c < BS; ++c) dinv[i * BS + c] = 1.0 / d[c];
+    for (int c = 0; c < BS; ++c) dinv[i * BS + c] = 1.0 / (fabs(d[c]) < 1e-12 ? 1.0 : d[c]);
   }
 }
 
@@ -486,15 +489,16 @@ __global__ __launch_bounds__(kThreads) void k_bc_ctl0(BcWork w, int nparts, doub
 
 // After the recursion reports convergence: with v = A x and r = b - v
 // recomputed by k_bc_init, test the true residual (keeps the iteration count);
-// a miss restarts the recursion from this r (residual replacement).
-__global__ __launch_bounds__(kThreads) void k_bc_recheck(BcWork w, int nparts) {
+// a miss restarts the recursion from this r (residual replacement) while
+// `restart` is set, and otherwise ends the solve with status 4.
+__global__ __launch_bounds__(kThreads) void k_bc_recheck(BcWork w, int nparts, int restart) {
   const int which[1] = {B_RR};
   double t[1];
   sum_parts<1>(w.part, which, nparts, t);
   if (threadIdx.x == 0) {
     w.sc->relres = w.sc->bb > 0.0 ? sqrt(t[0] / w.sc->bb) : 0.0;
     w.f->step = 0;
-    w.f->status = (t[0] <= w.sc->tol2) ? 1 : (w.f->iter >= w.f->maxiter ? 3 : 0);
+    w.f->status = (t[0] <= w.sc->tol2) ? 1 : (!restart ? 4 : (w.f->iter >= w.f->maxiter ? 3 : 0));
   }
 }
 
@@ -592,12 +596,14 @@ int bc_run(int64_t n, const int32_t* rp, const int32_t* ci, const VT* v, const d
   for (;;) {
     TM_HIP(hipMemcpyAsync(&hf, w.f, sizeof(hf), hipMemcpyDeviceToHost, st));
     TM_HIP(hipStreamSynchronize(st));
-    if (hf.status == 1 && hf.iter > 0 && restarts < 3) {
-      // the recursive residual met the tolerance: check the true one (x is final here)
+    if (hf.status == 1 && hf.iter > 0) {
+      // the recursive residual met the tolerance: check the true one (x is
+      // final here); after three restarts a miss ends the solve
+      const int restart = restarts < 3 ? 1 : 0;
       ++restarts;
       hipLaunchKernelGGL((k_spmv<VT, S, 1, 2>), dim3(G), dim3(kThreads), 0, st, n, rp, ci, v, x, w.v);
       hipLaunchKernelGGL(k_bc_init, dim3(G), dim3(kThreads), 0, st, n, b, w);
-      hipLaunchKernelGGL(k_bc_recheck, dim3(1), dim3(kThreads), 0, st, w, G);
+      hipLaunchKernelGGL(k_bc_recheck, dim3(1), dim3(kThreads), 0, st, w, G, restart);
       TM_HIP(hipGetLastError());
       TM_HIP(hipMemcpyAsync(&hf, w.f, sizeof(hf), hipMemcpyDeviceToHost, st));
       TM_HIP(hipStreamSynchronize(st));
@@ -625,6 +631,11 @@ int bc_run(int64_t n, const int32_t* rp, const int32_t* ci, const VT* v, const d
   }
   if (hf.status == 3) {
     tmx::set_error("BiCGSTAB did not converge in %lld iterations (relres %.3e)", (long long)maxiter, hs.relres);
+    return TM_ERR_NOTCONV;
+  }
+  if (hf.status == 4) {
+    tmx::set_error("BiCGSTAB: the true residual misses the tolerance after %d restarts (relres %.3e)",
+                   restarts - 1, hs.relres);
     return TM_ERR_NOTCONV;
   }
   return TM_OK;

@@ -171,9 +171,22 @@ class SparseMatrix:
         out = torch.zeros(self.shape[0 if dim == 1 else 1], dtype=self.values.dtype, device=self.device)
         return out.index_add(0, idx, self.values)
 
+    @staticmethod
+    def _sum_symmetry(a: Optional[bool], b: Optional[bool]) -> Optional[bool]:
+        """Symmetry flag of a sum on one layout: symmetric + symmetric is
+        symmetric, symmetric + non-symmetric is not, anything else is checked
+        when needed."""
+        if a is True and b is True:
+            return True
+        if (a is True and b is False) or (a is False and b is True):
+            return False
+        return None
+
     def __add__(self, other):
         if isinstance(other, SparseMatrix) and self.has_same_layout(other):
-            return self._wrap(self.values + other.values)
+            m = self._wrap(self.values + other.values)
+            m.symmetric = self._sum_symmetry(self.symmetric, other.symmetric)
+            return m
         if isinstance(other, SparseMatrix):
             return SparseMatrix(torch.cat([self.values, other.values]),
                                 torch.cat([self.row_indices, other.row_indices]),
@@ -227,11 +240,21 @@ class SparseMatrix:
         return v, self._canon_rc[0], self._canon_rc[1]
 
     def is_symmetric(self) -> bool:
+        """Pattern symmetric and ``max |v - v^T| <= tol * max |v|`` with
+        ``tol = 32 eps`` of the value type: duplicate COO entries are summed
+        with atomics in no fixed order, so the two triangles of a symmetric
+        input can differ in the last bits."""
         if self.symmetric is None:
             v, r, c = self._canonical()
             t = SparseMatrix(v, c, r, self.shape)
             tv, tr, tc = t._canonical()
-            self.symmetric = bool(torch.equal(tr, r) and torch.equal(tc, c) and torch.equal(tv, v))
+            sym = bool(torch.equal(tr, r) and torch.equal(tc, c))
+            if sym and v.numel() > 0:
+                vd, tvd = v.detach(), tv.detach()
+                tol = 32.0 * torch.finfo(vd.dtype).eps if vd.is_floating_point() else 0.0
+                scale = float(vd.abs().max())
+                sym = float((vd - tvd).abs().max()) <= tol * scale
+            self.symmetric = sym
         return self.symmetric
 
     # ---- products and solves -------------------------------------------------------
@@ -258,7 +281,8 @@ class SparseMatrix:
         GPU tensors: Jacobi-preconditioned CG for symmetric (SPD) matrices,
         Jacobi-preconditioned BiCGSTAB (the reference's ``lse_solver``) for
         non-symmetric ones (assembled matrices carry the flag; others are
-        checked once) or with ``method="bicgstab"``; stops
+        checked once), with ``method="bicgstab"``, and for symmetric matrices
+        on which CG breaks down (``p.Ap <= 0``: not positive definite); stops
         when ``||b - A x|| <= max(tol ||b||, atol)``.  ``backend`` may be
         ``None``/``"hip"``; ``method`` ``None``/``"cg"``/``"pcg"``/``"bicgstab"``.
         CPU tensors: SciPy SuperLU.  Differentiable w.r.t. values and b.
@@ -298,10 +322,19 @@ class SparseMatrix:
         else:
             v, r, c = self._canonical()
             x = PCGSolve.apply(v, b.to(self.device), self.csr(), r, c, float(tol), float(atol), int(maxiter), info)
+            if info.get("breakdown") and method in (None, "auto"):
+                # symmetric but not positive definite: the reference picks a
+                # general solver for such systems (spd=False, README.md:132);
+                # here that is BiCGSTAB
+                pcg_info = dict(info)
+                info = {}
+                x = BiCGSTABSolve.apply(v, b.to(self.device), self.csr(), r, c, float(tol), float(atol),
+                                        int(maxiter), info)
+                info["after_pcg_breakdown"] = pcg_info
         self.last_solve_info = info
         if verbose:
             print(f"[tensormesh_amd] solve: n={self.shape[0]}, nnz={self.nnz}, dtype={self.dtype}, "
-                  f"device={self.device}, backend=hip, method=jacobi-pcg, "
+                  f"device={self.device}, backend=hip, method={info.get('method')}, "
                   f"iterations={info.get('iterations')}, relres={info.get('relres'):.3e}")
         return x
 

@@ -87,8 +87,10 @@ class SpMV(torch.autograd.Function):
 
 def pcg_raw(csr: CSR, values: torch.Tensor, b: torch.Tensor, x0: Optional[torch.Tensor] = None,
             rtol: float = 1e-8, atol: float = 0.0, maxiter: int = 10000, check_every: int = 0,
-            raise_on_fail: bool = False, block_size: int = 1):
-    """Jacobi-PCG on the device; returns ``(x [f64], iterations, relres, converged)``."""
+            raise_on_fail: bool = False, block_size: int = 1, status: Optional[dict] = None):
+    """Jacobi-PCG on the device; returns ``(x [f64], iterations, relres, converged)``.
+    ``status`` (optional) receives ``breakdown``: True when ``p.Ap <= 0`` ended
+    the solve (the matrix is not positive definite)."""
     _lib.require_gpu(values, "matrix values")
     dev = values.device
     n = csr.n_rows * block_size
@@ -105,18 +107,24 @@ def pcg_raw(csr: CSR, values: torch.Tensor, b: torch.Tensor, x0: Optional[torch.
                          float(atol), int(maxiter), int(check_every), ctypes.byref(it), ctypes.byref(rr),
                          _lib.stream_of(dev))
     converged = st == _lib.TM_OK
+    breakdown = False
     if not converged:
         if st != _lib.TM_ERR_NOTCONV or raise_on_fail:
             _lib.check(st)
-        warnings.warn(f"PCG did not converge: {L.tm_last_error().decode()}")
+        msg = L.tm_last_error().decode()
+        breakdown = msg.startswith("PCG breakdown")
+        warnings.warn(f"PCG did not converge: {msg}")
+    if status is not None:
+        status["breakdown"] = breakdown
     return x, int(it.value), float(rr.value), converged
 
 
 class PCGSolve(torch.autograd.Function):
     @staticmethod
     def forward(ctx, values, b, csr: CSR, row, col, rtol, atol, maxiter, info: dict, block_size: int = 1):
-        x, it, rel, ok = pcg_raw(csr, values, b, None, rtol, atol, maxiter, block_size=block_size)
-        info.update(iterations=it, relres=rel, converged=ok)
+        st: dict = {}
+        x, it, rel, ok = pcg_raw(csr, values, b, None, rtol, atol, maxiter, block_size=block_size, status=st)
+        info.update(iterations=it, relres=rel, converged=ok, breakdown=st["breakdown"], method="jacobi-pcg")
         ctx.csr, ctx.tol, ctx.bs = csr, (rtol, atol, maxiter), block_size
         ctx.save_for_backward(values, x, row, col)
         return x.to(torch.promote_types(values.dtype, b.dtype))

@@ -297,3 +297,18 @@ def test_bicgstab_oracle_pins():
     assert it == 1 and np.allclose(x, bd / np.linspace(1.0, 3.0, 50), rtol=1e-14, atol=0)
     x, it, rel = osv.bicgstab(A, np.zeros(n))
     assert it == 0 and not x.any()
+
+
+def test_bicgstab_oracle_zero_diagonal_guard():
+    """A stored diagonal below 1e-12 counts as 1 in the Jacobi set-up
+    (torch_solve.py:177-182): BiCGSTAB then solves a system with explicit zero
+    diagonals (without the guard its first M^-1 p is inf)."""
+    from tests.matrices import zero_diagonal_rows
+    A, pr = zero_diagonal_rows(4000, 9, 40)
+    assert np.all(A.diagonal()[pr] == 0.0) and all(i in set(A[i].indices) for i in pr)
+    d = osv.diagonal(A)
+    assert np.all(d[pr] == 1.0) and np.all(d[pr + 1] == 1.0)
+    b = np.random.default_rng(3).standard_normal(A.shape[0])
+    x, it, rel = osv.bicgstab(A, b, rtol=1e-12, maxiter=5000)
+    xd = osv.direct(A, b)
+    assert rel <= 1e-12 and np.linalg.norm(x - xd) <= 1e-10 * np.linalg.norm(xd)
