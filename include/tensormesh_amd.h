@@ -221,6 +221,21 @@ int tm_pcg_jacobi(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, con
                   double atol, int64_t maxiter, int check_every, int64_t* iters, double* relres,
                   void* stream);
 
+/* Jacobi-PCG that allocates nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (serving loops and harnesses
+ * that forbid hipMalloc).  Same arguments, kernels and results as
+ * tm_pcg_jacobi without `work`: the workspace is a static device array of the
+ * library, one per GPU, sized for n <= TM_PCG_NOALLOC_MAX_ROWS (2^20 rows,
+ * 32 MiB; the 96^3 Poisson system has 857,375).  Larger n returns
+ * TM_ERR_LIMIT before any device work.  The stream must belong to the current
+ * device.  Calls on one GPU share that array: calls that overlap in time on
+ * one GPU (two host threads, or two streams) race and must not be made. */
+#define TM_PCG_NOALLOC_MAX_ROWS 1048576
+int64_t tm_pcg_noalloc_max_rows(void);
+int tm_pcg_jacobi_noalloc(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const void* values,
+                          int value_dtype, const double* b, double* x, double rtol, double atol,
+                          int64_t maxiter, int check_every, int64_t* iters, double* relres, void* stream);
+
 /* Jacobi-preconditioned BiCGSTAB (right preconditioning) for square, possibly
  * non-symmetric CSR systems: the reference's lse_solver bicgstab_py
  * (sparse/solve/torch_solve.py:130-248) with device-side scalars, FP64 vectors

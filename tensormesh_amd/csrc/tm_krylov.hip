@@ -14,6 +14,7 @@
 // device-side status flag is set, so the host only reads the flag every few
 // iterations.
 #include <cstdlib>
+#include <mutex>
 #include <vector>
 
 #include "tm_device.h"
@@ -749,4 +750,74 @@ extern "C" int tm_bicgstab_jacobi(int64_t n, int64_t nnz, const int32_t* row_ptr
                               check_every, iters, relres, st);
   return bc_dispatch<double>(n, nnz, row_ptr, col_idx, (const double*)values, b, x, w, rtol, atol, maxiter,
                              check_every, iters, relres, st);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free Jacobi-PCG: the workspace of tm_pcg_jacobi (r, p, q, D^-1,
+// the partial sums, scalars and flags) lives in a static device array of
+// fixed size instead of caller memory.  Each GPU holds its own copy of the
+// array at its own address; host code only ever passes the address that
+// hipGetSymbolAddress returned for the current device (the symbol's host-side
+// name is not a device address).  Every kernel writes each workspace word
+// before reading it in the same solve (D^-1, q = A x0, r and p in the set-up
+// kernels, partials by their producers, scalars and flags by k_pcg_ctl0), so
+// what an earlier call left there is never read.
+namespace {
+
+constexpr int64_t kNoAllocMaxRows = TM_PCG_NOALLOC_MAX_ROWS;
+constexpr size_t kNoAllocBytes = sizeof(double) * (4 * kNoAllocMaxRows + 4 * kMaxBlocks) + sizeof(PcgScalars) +
+                                 sizeof(PcgFlags) + 64;
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up once per device).
+int scratch_address(hipStream_t st, void** out) {
+  int dev = -1;
+  TM_HIP(hipGetDevice(&dev));
+  TM_REQUIRE(dev >= 0 && dev < kMaxDevices, TM_ERR_LIMIT, "device ordinal %d beyond %d", dev, kMaxDevices);
+  if (st) {
+    int sdev = -1;
+    TM_HIP(hipStreamGetDevice(st, &sdev));
+    TM_REQUIRE(sdev == dev, TM_ERR_ARG, "the stream belongs to device %d, the current device is %d", sdev, dev);
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    TM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    TM_REQUIRE(p != nullptr, TM_ERR_HIP, "no device address for the PCG scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return TM_OK;
+}
+
+}  // namespace
+
+extern "C" int64_t tm_pcg_noalloc_max_rows(void) { return kNoAllocMaxRows; }
+
+extern "C" int tm_pcg_jacobi_noalloc(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, const void* values,
+                                     int value_dtype, const double* b, double* x, double rtol, double atol,
+                                     int64_t maxiter, int check_every, int64_t* iters, double* relres,
+                                     void* stream) {
+  TM_REQUIRE(n <= kNoAllocMaxRows, TM_ERR_LIMIT,
+             "tm_pcg_jacobi_noalloc: %lld rows exceed the static workspace of %lld rows", (long long)n,
+             (long long)kNoAllocMaxRows);
+  TM_REQUIRE(n > 0 && row_ptr && col_idx && values && b && x, TM_ERR_ARG, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  void* work = nullptr;
+  const int rc = scratch_address(st, &work);
+  if (rc != TM_OK) return rc;
+  int32_t nnz_h = 0;
+  TM_HIP(hipMemcpyAsync(&nnz_h, row_ptr + n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  TM_HIP(hipStreamSynchronize(st));
+  PcgWork w = carve_work(work, n);
+  if (value_dtype == TM_F32)
+    return pcg_dispatch<float, 1>(n, nnz_h, row_ptr, col_idx, (const float*)values, b, x, w, rtol, atol, maxiter,
+                                  check_every, iters, relres, st);
+  return pcg_dispatch<double, 1>(n, nnz_h, row_ptr, col_idx, (const double*)values, b, x, w, rtol, atol, maxiter,
+                                 check_every, iters, relres, st);
 }
