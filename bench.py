@@ -129,10 +129,19 @@ def learned_leg(n: int, seeds, log_fn):
                         "gemm_gflop_per_apply": round(flops / 1e9, 3),
                         "mfma_tflops": round(flops / t_app / 1e12, 2),
                         "mfma_util_vs_2500_tflops_bf16": round(flops / t_app / 2.5e15, 5)},
+            "seeds": len(rows),
             "median_speedup_vs_jacobi": float(np.median(sp)),
+            "p10_speedup_vs_jacobi": float(np.percentile(sp, 10)),
+            "min_speedup_vs_jacobi": float(sp.min()),
             "median_jacobi_iters": float(np.median([r_["jacobi_iters"] for r_ in rows])),
             "median_fcg_iters": float(np.median([r_["fcg_iters"] for r_ in rows])),
-            "fallbacks": int(sum(r_["fallback_iters"] > 0 for r_ in rows)), "rows": rows}
+            "max_fcg_iters": int(max(r_["fcg_iters"] for r_ in rows)),
+            "median_jacobi_s": float(np.median([r_["jacobi_s"] for r_ in rows])),
+            "median_fcg_s": float(np.median([r_["fcg_s"] for r_ in rows])),
+            "max_true_relres": float(max(r_["relres"] for r_ in rows)),
+            "fallbacks": int(sum(r_["fallback_iters"] > 0 for r_ in rows)),
+            "worst_rows": sorted(rows, key=lambda r_: r_["speedup"])[:5],
+            "speedups": [r_["speedup"] for r_ in rows]}
 
 
 def batched_leg(n_mesh: int, log_fn):
@@ -420,7 +429,7 @@ def main():
     ap.add_argument("--hyper-n", type=int, default=64)
     ap.add_argument("--batched-meshes", type=int, default=256)
     ap.add_argument("--learned-n", type=int, default=96)
-    ap.add_argument("--learned-seeds", type=int, default=8)
+    ap.add_argument("--learned-seeds", type=int, default=200)
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="MiB of a plain device copy run before the timed region (HBM counter calibration)")
     args = ap.parse_args()
@@ -564,8 +573,10 @@ def main():
         out["learned_fpcg"] = learned_leg(args.learned_n, seeds, log)
         lf = out["learned_fpcg"]
         if "median_speedup_vs_jacobi" in lf:
-            log(f"learned FCG: median speedup {lf['median_speedup_vs_jacobi']:.2f}x over Jacobi-PCG, "
-                f"iterations {lf['median_fcg_iters']} vs {lf['median_jacobi_iters']}")
+            log(f"learned FCG: {lf['seeds']} held-out seeds, speedup over Jacobi-PCG median "
+                f"{lf['median_speedup_vs_jacobi']:.2f}x, 10th percentile {lf['p10_speedup_vs_jacobi']:.2f}x, "
+                f"min {lf['min_speedup_vs_jacobi']:.2f}x; iterations {lf['median_fcg_iters']} vs "
+                f"{lf['median_jacobi_iters']}, fallbacks {lf['fallbacks']}")
 
     if not args.no_batched:
         out["batched"] = batched_leg(args.batched_meshes, log)

@@ -151,6 +151,24 @@ int tm_asm_p1_matrix(int form, int nb, int dim, const tm_plan_dev* plan, const d
 int tm_asm_p1_general(int nb, int dim, const tm_plan_dev* plan, const double* coords, const double* D,
                       const tm_quad* quad, void* values, int value_dtype, void* stream);
 
+/* Spatially varying bilinear forms (ElementAssembler.__call__ with a forward
+ * reading x, point data, their gradients or element data,
+ * element_assembler.py:332-346, 402-466): the integrand U_a^T D(q) V_b,
+ * U = (phi, grad phi) of the row basis function, is given by per-element
+ * moments in the element's vertex order, moments[E][(nb+dim)^2] f64 =
+ * {MM[nb][nb] = sum_q w_q phi_a phi_b D00, P[nb][dim] = sum_q w_q phi_a D_0g,
+ *  Qm[nb][dim] = sum_q w_q D_g0 phi_b, S[dim][dim] = sum_q w_q D_gg}
+ * (w_q reference weights; the kernel supplies |det J| and grad phi).  values
+ * [nnz] f32 or f64 in canonical CSR order, each entry written once. */
+int tm_asm_p1_moments(int nb, int dim, const tm_plan_dev* plan, const double* coords,
+                      const double* moments, void* values, int value_dtype, void* stream);
+/* Spatially varying linear forms F0 v + F1 . grad v (NodeAssembler.__call__,
+ * node_assembler.py:242-409, e.g. func_node_assembler, builtin.py:929-969):
+ * moments[E][nb+dim] f64 = {L0[nb] = sum_q w_q phi_a F0, L1[dim] = sum_q w_q F1}
+ * in the element's vertex order; b[N] f64 is overwritten. */
+int tm_asm_p1_load_moments(int nb, int dim, const tm_plan_dev* plan, const double* coords,
+                           const double* moments, double* b, void* stream);
+
 /* Load vector b[N] (f64) (NodeAssembler.__call__, node_assembler.py:242-409):
  * TM_LOAD_CONST: c * int phi_a;  TM_LOAD_NODAL: int (sum_c f_c phi_c) phi_a. */
 int tm_asm_p1_load(int form, int nb, int dim, const tm_plan_dev* plan, const double* coords,

@@ -26,8 +26,15 @@ _A, _B = 0.5854101966249685, 0.1381966011250105
 _RULES = {
     ("triangle", 1): ([0.5], [[1 / 3, 1 / 3]]),
     ("triangle", 2): ([1 / 6] * 3, [[1 / 6, 1 / 6], [2 / 3, 1 / 6], [1 / 6, 2 / 3]]),
+    # order 3 (quadrature.py:94,103; Strang-Fix 4-point rule, one negative weight)
+    ("triangle", 3): ([-27 / 96, 25 / 96, 25 / 96, 25 / 96],
+                      [[1 / 3, 1 / 3], [0.2, 0.6], [0.6, 0.2], [0.2, 0.2]]),
     ("tetra", 1): ([1 / 6], [[0.25, 0.25, 0.25]]),
     ("tetra", 2): ([1 / 24] * 4, [[_A, _B, _B], [_B, _B, _B], [_B, _B, _A], [_B, _A, _B]]),
+    # order 3 (quadrature.py:144,153; Keast 5-point rule)
+    ("tetra", 3): ([-2 / 15, 3 / 40, 3 / 40, 3 / 40, 3 / 40],
+                   [[0.25, 0.25, 0.25], [0.5, 1 / 6, 1 / 6], [1 / 6, 1 / 6, 1 / 6], [1 / 6, 1 / 6, 0.5],
+                    [1 / 6, 0.5, 1 / 6]]),
 }
 
 
@@ -127,3 +134,56 @@ def kappa_gradient(points, conn, element_type, g_values: np.ndarray, pe: np.ndar
     s = np.einsum("eab,eqab,eq->eq", gg, dots, jxw)      # [E, Q]
     dk = np.einsum("eq,qc->ec", s, phi)
     return np.bincount(conn.ravel(), weights=dk.ravel(), minlength=n_points)
+
+
+def _at_quadrature(points, conn, element_type, order, fields):
+    """Geometry plus ``x`` and nodal fields (and their gradients, key ``grad<name>``)
+    interpolated to the quadrature points (``element_assembler.py:416-457``)."""
+    conn = np.asarray(conn)
+    phi, grad, jxw = geometry(points, conn, element_type, order)
+    X = np.asarray(points, dtype=np.float64)[conn]
+    at = {"x": np.einsum("ebd,qb->eqd", X, phi)}
+    for k, v in (fields or {}).items():
+        v = np.asarray(v, dtype=np.float64)[conn]                          # [E, nb, ...]
+        at[k] = np.einsum("eb...,qb->eq...", v, phi)
+        at["grad" + k] = np.einsum("eb...,eqbd->eq...d", v, grad)
+    return phi, grad, jxw, at
+
+
+def element_matrices_quadrature(points, conn, element_type: str, integrand, order: int = 2, fields=None,
+                                element_data=None) -> np.ndarray:
+    """Per-element ``[E, nb, nb]`` for an integrand evaluated at every (element,
+    quadrature point, a, b) - the reference's Map (``element_assembler.py:371-466``)
+    without vmap: ``integrand(u, gradu, v, gradv, at)`` receives ``u``/``v``
+    shaped ``[E, Q, nb, nb]``, ``gradu``/``gradv`` ``[E, Q, nb, nb, d]`` (a = row
+    basis function u, b = column basis function v) and ``at`` = the
+    coefficients at the quadrature points broadcast to ``[E, Q, 1, 1, ...]``
+    (``x``, ``fields``, ``grad<field>``; ``element_data`` per element)."""
+    phi, grad, jxw, at = _at_quadrature(points, conn, element_type, order, fields)
+    E, Q, nb, d = grad.shape
+    u = np.broadcast_to(phi[None, :, :, None], (E, Q, nb, nb))
+    v = np.broadcast_to(phi[None, :, None, :], (E, Q, nb, nb))
+    gu = np.broadcast_to(grad[:, :, :, None, :], (E, Q, nb, nb, d))
+    gv = np.broadcast_to(grad[:, :, None, :, :], (E, Q, nb, nb, d))
+    ab = {k: a.reshape(a.shape[:2] + (1, 1) + a.shape[2:]) for k, a in at.items()}
+    for k, a in (element_data or {}).items():
+        a = np.asarray(a, dtype=np.float64)
+        ab[k] = a.reshape((E, 1, 1, 1) + a.shape[1:])
+    f = integrand(u, gu, v, gv, ab)
+    return np.einsum("eqab,eq->eab", np.broadcast_to(f, (E, Q, nb, nb)), jxw)
+
+
+def load_vector_quadrature(points, conn, element_type: str, n_points: int, integrand, order: int = 2,
+                           fields=None) -> np.ndarray:
+    """``b_a = sum_e sum_q w_q |J| g(v = phi_a, gradv = grad phi_a, coefficients at q)``
+    (``node_assembler.py:242-409``, projector indices ``conn.flatten()``):
+    ``integrand(v [E, Q, nb], gradv [E, Q, nb, d], at)`` with ``at`` broadcast to
+    ``[E, Q, 1, ...]``."""
+    conn = np.asarray(conn)
+    phi, grad, jxw, at = _at_quadrature(points, conn, element_type, order, fields)
+    E, Q, nb, d = grad.shape
+    v = np.broadcast_to(phi[None], (E, Q, nb))
+    ab = {k: a.reshape(a.shape[:2] + (1,) + a.shape[2:]) for k, a in at.items()}
+    g = np.broadcast_to(integrand(v, grad, ab), (E, Q, nb))
+    be = np.einsum("eqa,eq->ea", g, jxw)
+    return np.bincount(conn.ravel(), weights=be.ravel(), minlength=n_points)

@@ -44,7 +44,8 @@ def const_node_assembler(c=1):
 
 
 def func_node_assembler(f=lambda x: x):
-    """``f_i = int f(x) phi_i`` (generic path: ``f`` is evaluated at quadrature points)."""
+    """``f_i = int f(x) phi_i`` (``builtin.py:929-969``; on GPU tensors the moments kernel with ``f`` evaluated
+    at the quadrature points)."""
 
     class FuncNodeAssembler(NodeAssembler):
         def __post_init__(self, f=f):

@@ -9,8 +9,11 @@ for parameters, ``edges`` ``[2, nnz]`` in canonical CSR order.
 
 On GPU tensors a recognised ``forward`` (see :mod:`.forms`) runs the
 owner-computes HIP kernel: FP64 geometry, FP64 on-chip accumulation, values
-stored as ``value_dtype`` (float32 unless set otherwise).  Other forms, and
-CPU tensors, run the generic name-dispatch path.  Unlike the reference, the
+stored as ``value_dtype`` (float32 unless set otherwise).  A ``forward`` that
+is bilinear in ``(u, gradu)`` and ``(v, gradv)`` with coefficients varying in
+space (``x``, point data and their gradients, element data) is probed at the
+quadrature points and runs the moments kernel.  Other forms, inputs that
+require gradients, and CPU tensors run the generic name-dispatch path.  Unlike the reference, the
 matrix's ``row``/``col`` are the assembler's own int64 ``edges`` rows on
 every call, so a Condenser's cached layout stays valid across assemblies.
 """
@@ -140,6 +143,15 @@ class ElementAssembler(nn.Module):
             _lib.ptr(values), _lib.dtype_code(values), _lib.stream_of(dev)))
         return values
 
+    def _launch_moments(self, points64, mom: torch.Tensor, out_dtype) -> torch.Tensor:
+        dev = points64.device
+        plan = self._plan(dev)
+        values = torch.empty(self.topology.nnz, dtype=out_dtype, device=dev)
+        _lib.check(_lib.lib().tm_asm_p1_moments(
+            self.topology.nb, self.topology.dim, plan.struct, _lib.ptr(points64), _lib.ptr(mom),
+            _lib.ptr(values), _lib.dtype_code(values), _lib.stream_of(dev)))
+        return values
+
     def _launch_kappa_adjoint(self, points64, g) -> torch.Tensor:
         dev = points64.device
         plan = self._plan(dev)
@@ -194,9 +206,24 @@ class ElementAssembler(nn.Module):
             self.to(dev)
 
         form = None
-        if dev.type == "cuda" and not points.requires_grad:
+        on_kernel = dev.type == "cuda" and not points.requires_grad
+        if on_kernel:
             pd = {k: v for k, v in point_data.items() if k != "x"}
             form = self._classify(fn, pd, scalar_data, element_data)
+        if form is None and on_kernel and not any(
+                torch.is_tensor(v) and v.requires_grad
+                for v in list(point_data.values()) + list(element_data.values()) + list(scalar_data.values())):
+            # coefficients that vary in space: per-element moments of the
+            # integrand, then the moments kernel
+            q = forms.probe_bilinear_q(fn, points, conn.to(dev), etype, self.quadrature_order, point_data,
+                                       element_data, scalar_data)
+            if q is not None:
+                mom, sym = q
+                values = self._launch_moments(points.detach().to(torch.float64).contiguous(), mom,
+                                              self.value_dtype)
+                e = self.topology.edges_on(dev)
+                return SparseMatrix._from_csr(values, e[0], e[1], (self.n_points, self.n_points),
+                                              self.csr(dev), symmetric=sym)
         if form is not None:
             kind, scale, key = form
             p64 = points.detach().to(torch.float64).contiguous()

@@ -212,6 +212,166 @@ def classify_linear(fn: Callable, point_data: Dict, scalar_data: Dict) -> Option
     return None
 
 
+# ------------------------------------------------ spatially varying forms ---
+
+_BASIS = ("u", "gradu", "v", "gradv")
+
+
+def _eval_points(fn, names, args, dims, basis, E: int, Q: int):
+    """``fn`` at every (element, quadrature point) with the basis arguments
+    replaced by the constants in ``basis``; returns ``[E, Q]`` or ``None`` when
+    ``fn`` does not give one scalar per point."""
+    a2 = [basis[n] if n in _BASIS else a for n, a in zip(names, args)]
+    d2 = [(None, None) if n in _BASIS else (d[0], d[1]) for n, d in zip(names, dims)]
+    has_e = any(d[0] is not None for d in d2)
+    has_q = any(d[1] is not None for d in d2)
+    out = _vmapped(fn, d2, 2)(*a2)
+    if not torch.is_tensor(out):
+        return None
+    want = (E if has_e else None, Q if has_q else None)
+    shape = tuple(x for x in want if x is not None)
+    if tuple(out.shape) != shape:
+        return None
+    if not has_e:
+        out = out.unsqueeze(0)
+    if not has_q:
+        out = out.unsqueeze(1)
+    return out.expand(E, Q)
+
+
+def _coefficient_args(fn, points, conn, element_type, order, point_data, element_data, scalar_data):
+    names = list(inspect.signature(fn).parameters)
+    phi, grad, _ = geometry_torch(points, conn, element_type, order)
+    args, dims = _args(names, phi, grad, point_data, element_data, scalar_data, conn, True)
+    return names, phi, args, dims
+
+
+def _chunks(conn, element_data, chunk):
+    E = conn.shape[0]
+    for s in range(0, E, chunk):
+        yield conn[s:s + chunk], {k: v[s:s + chunk] for k, v in element_data.items()}
+
+
+def probe_bilinear_q(fn: Callable, points: torch.Tensor, conn: torch.Tensor, element_type: str, order: int,
+                     point_data: Dict, element_data: Dict, scalar_data: Dict, chunk: int = 1 << 19):
+    """Per-element moments of a bilinear integrand whose coefficients vary in
+    space, for ``tm_asm_p1_moments``, or ``None``.
+
+    The reference evaluates ``forward`` at every (element, quadrature point,
+    a, b) under ``vmap`` (``element_assembler.py:371-466``).  For P1 elements
+    the integrand is ``U_a^T D(q) V_b`` whenever ``forward`` is bilinear in
+    ``U = (u, gradu)`` and ``V = (v, gradv)``: ``D(q)`` is ``forward`` at the
+    unit vectors (``(d+1)^2`` vectorised evaluations over all points), and
+    three random ``(U, V)`` pairs must reproduce ``U^T D V`` at every point to
+    1e-12.  Returns ``(moments [E, (nb+d)^2] float64, symmetric)``."""
+    names = list(inspect.signature(fn).parameters)
+    if not (set(names) & {"u", "gradu"} and set(names) & {"v", "gradv"}):
+        return None
+    d = points.shape[1]
+    dd = d + 1
+    dt, dev = torch.float64, points.device
+    w_np, q_np = quadrature(element_type, order)
+    w = torch.as_tensor(w_np, dtype=dt, device=dev)
+    gen = torch.Generator().manual_seed(20240602)
+    checks = [(torch.rand(dd, generator=gen, dtype=dt) - 0.5, torch.rand(dd, generator=gen, dtype=dt) - 0.5)
+              for _ in range(3)]
+    out, symmetric = [], True
+    try:
+        for c, ed in _chunks(conn, element_data, chunk):
+            names, phi, args, dims = _coefficient_args(fn, points, c, element_type, order, point_data, ed,
+                                                       scalar_data)
+            E, Q = c.shape[0], phi.shape[0]
+
+            def basis(U, V):
+                U, V = U.to(dev), V.to(dev)
+                return {"u": U[0], "gradu": U[1:], "v": V[0], "gradv": V[1:]}
+
+            I = torch.eye(dd, dtype=dt)
+            D = torch.empty((E, Q, dd, dd), dtype=dt, device=dev)
+            for i in range(dd):
+                for j in range(dd):
+                    f = _eval_points(fn, names, args, dims, basis(I[i], I[j]), E, Q)
+                    if f is None:
+                        return None
+                    D[:, :, i, j] = f
+            if not torch.isfinite(D).all():
+                return None
+            for U, V in checks:
+                f = _eval_points(fn, names, args, dims, basis(U, V), E, Q)
+                Ud, Vd = U.to(dev), V.to(dev)
+                ref = torch.einsum("i,eqij,j->eq", Ud, D, Vd)
+                bound = torch.einsum("i,eqij,j->eq", Ud.abs(), D.abs(), Vd.abs())
+                if f is None or ((f.to(dt) - ref).abs() > _TOL * bound.max().clamp_min(1e-300)).any():
+                    return None
+            symmetric = symmetric and bool(torch.equal(D, D.transpose(-1, -2)))
+            ph = phi.to(dt)
+            MM = torch.einsum("q,qa,qb,eq->eab", w, ph, ph, D[..., 0, 0])
+            P = torch.einsum("q,qa,eqj->eaj", w, ph, D[..., 0, 1:])
+            Qm = torch.einsum("q,eqj,qb->ebj", w, D[..., 1:, 0], ph)
+            S = torch.einsum("q,eqij->eij", w, D[..., 1:, 1:])
+            out.append(torch.cat([MM.reshape(E, -1), P.reshape(E, -1), Qm.reshape(E, -1), S.reshape(E, -1)], 1))
+    except ValueError:
+        raise
+    except Exception:
+        return None
+    return torch.cat(out, 0).contiguous(), symmetric
+
+
+def probe_linear_q(fn: Callable, points: torch.Tensor, conn: torch.Tensor, element_type: str, order: int,
+                   point_data: Dict, scalar_data: Dict, chunk: int = 1 << 19):
+    """Per-element moments of a linear integrand ``F0(q) v + F1(q) . gradv``
+    with spatially varying ``F0``, ``F1`` (``node_assembler.py:242-409``; e.g.
+    ``func_node_assembler``, ``builtin.py:929-969``) for
+    ``tm_asm_p1_load_moments``, or ``None``: ``F0`` and ``F1`` are ``forward``
+    at ``(v, gradv) = (1, 0)`` and ``(0, e_j)``, checked on three random pairs
+    to 1e-12.  Returns ``moments [E, nb + d]`` float64."""
+    names = list(inspect.signature(fn).parameters)
+    if not set(names) & {"v", "gradv"} or set(names) & {"u", "gradu"}:
+        return None
+    d = points.shape[1]
+    dt, dev = torch.float64, points.device
+    w_np, _ = quadrature(element_type, order)
+    w = torch.as_tensor(w_np, dtype=dt, device=dev)
+    gen = torch.Generator().manual_seed(20240603)
+    checks = [torch.rand(d + 1, generator=gen, dtype=dt) - 0.5 for _ in range(3)]
+    out = []
+    try:
+        for c, _ in _chunks(conn, {}, chunk):
+            names, phi, args, dims = _coefficient_args(fn, points, c, element_type, order, point_data, {},
+                                                       scalar_data)
+            E, Q = c.shape[0], phi.shape[0]
+
+            def basis(V):
+                V = V.to(dev)
+                return {"v": V[0], "gradv": V[1:]}
+
+            I = torch.eye(d + 1, dtype=dt)
+            F = torch.empty((E, Q, d + 1), dtype=dt, device=dev)
+            for j in range(d + 1):
+                f = _eval_points(fn, names, args, dims, basis(I[j]), E, Q)
+                if f is None:
+                    return None
+                F[:, :, j] = f
+            if not torch.isfinite(F).all():
+                return None
+            for V in checks:
+                f = _eval_points(fn, names, args, dims, basis(V), E, Q)
+                Vd = V.to(dev)
+                ref = torch.einsum("eqj,j->eq", F, Vd)
+                bound = torch.einsum("eqj,j->eq", F.abs(), Vd.abs())
+                if f is None or ((f.to(dt) - ref).abs() > _TOL * bound.max().clamp_min(1e-300)).any():
+                    return None
+            ph = phi.to(dt)
+            L0 = torch.einsum("q,qa,eq->ea", w, ph, F[..., 0])
+            L1 = torch.einsum("q,eqj->ej", w, F[..., 1:])
+            out.append(torch.cat([L0, L1], 1))
+    except ValueError:
+        raise
+    except Exception:
+        return None
+    return torch.cat(out, 0).contiguous()
+
+
 # ---------------------------------------------------------------- generic path ---
 
 def geometry_torch(points: torch.Tensor, conn: torch.Tensor, element_type: str, order: int):

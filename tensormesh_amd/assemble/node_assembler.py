@@ -5,8 +5,10 @@ Drop-in for ``tensormesh.NodeAssembler`` (``tensormesh/assemble/node_assembler.p
 ``from_assembler``, ``__call__(points=None, func=None, point_data=None,
 scalar_data=None, batch_size=1) -> Tensor[N * H]`` (``:242-409``).  On GPU
 tensors the recognised loads ``c * v`` and ``f * v`` (nodal ``f``) run the
-owner-computes HIP kernel (FP64 result); anything else runs the generic
-path.
+owner-computes HIP kernel (FP64 result); other loads linear in ``(v, gradv)``
+(coefficients from ``x``, point data and their gradients, as
+``func_node_assembler`` builds) run the moments kernel on per-element
+quadrature moments; anything else runs the generic path.
 """
 from __future__ import annotations
 
@@ -95,8 +97,23 @@ class NodeAssembler(nn.Module):
         conn = self.elements[etype]
         form = None
         pd = {k: v for k, v in point_data.items() if k != "x"}
-        if dev.type == "cuda" and not points.requires_grad and not any(v.requires_grad for v in pd.values()):
+        on_kernel = dev.type == "cuda" and not points.requires_grad and not any(
+            torch.is_tensor(v) and v.requires_grad for v in list(pd.values()) + list(scalar_data.values()))
+        if on_kernel:
             form = forms.classify_linear(fn, pd, scalar_data)
+            if form is None:
+                # spatially varying load (e.g. func_node_assembler): per-element
+                # moments of the integrand, then the moments kernel
+                mom = forms.probe_linear_q(fn, points, conn.to(dev), etype, self.quadrature_order, point_data,
+                                           scalar_data)
+                if mom is not None:
+                    p64 = points.detach().to(torch.float64).contiguous()
+                    plan = self.topology.device_plan(dev)
+                    out = torch.empty(self.n_points, dtype=torch.float64, device=dev)
+                    _lib.check(_lib.lib().tm_asm_p1_load_moments(
+                        self.topology.nb, self.topology.dim, plan.struct, _lib.ptr(p64), _lib.ptr(mom),
+                        _lib.ptr(out), _lib.stream_of(dev)))
+                    return out
         if form is not None:
             kind, scale, key = form
             p64 = points.detach().to(torch.float64).contiguous()

@@ -414,6 +414,137 @@ __global__ __launch_bounds__(NT) void k_asm_general(PlanArgs p, const double* __
   }
 }
 
+// Element-local vertex index of record vertex a (record bits 112-119).
+__device__ __forceinline__ int rec_orig(const uint4& rc, int a) { return (int)((rc.w >> (16 + 2 * a)) & 3u); }
+
+// Spatially varying bilinear forms U_a^T D(q) V_b (U = (phi, grad phi) of the
+// row basis function a, V of the column basis function b; D depends on the
+// element and the quadrature point).  The caller passes per-element moments
+// of the integrand in the element's own vertex order (w_q reference weights),
+// NM = (NB + DIM)^2 doubles per element:
+//   MM[NB][NB] = sum_q w_q phi_a(q) phi_b(q) D00(q)
+//   P [NB][DIM] = sum_q w_q phi_a(q) D_0g(q)
+//   Qm[NB][DIM] = sum_q w_q D_g0(q) phi_b(q)
+//   S [DIM][DIM] = sum_q w_q D_gg(q)
+// With grad phi = c / det (c = cofactor vectors) the element matrix is
+//   K_ab = |det| MM_ab + sgn(det) (P_a . c_b + c_a . Qm_b) + c_a^T S c_b / |det|.
+// The record's vertex order may differ from the element's (rec_orig maps it).
+template <int NB, typename OT, int NT>
+__global__ __launch_bounds__(NT) void k_asm_moments(PlanArgs p, const int32_t* __restrict__ rec_elem,
+                                                   const double* __restrict__ X, const double* __restrict__ mom,
+                                                   OT* __restrict__ values) {
+  constexpr int DIM = NB - 1, NM = (NB + DIM) * (NB + DIM);
+  extern __shared__ __align__(16) char smem[];
+  const int b = tmx::xcd_remap(blockIdx.x, p.nblk);
+  const int4 bi0 = *(const int4*)(p.blk + 8 * b);
+  const int4 bi1 = *(const int4*)(p.blk + 8 * b + 4);
+  const int hoff = bi0.x, nh = bi0.y, nown = bi0.z, roff = bi0.w;
+  const int nrec = bi1.x, runoff = bi1.y, nrun = bi1.z, nacc = bi1.w;
+  const Lds L = carve(smem, nacc, nh, nown);
+  stage<NT, DIM, float, false>(L, p.halo + hoff, p.meta + hoff, nacc, nh, nown, X, (const float*)nullptr);
+  __syncthreads();
+  for (int r = threadIdx.x; r < nrec; r += NT) {
+    const uint4 rc = p.recs[roff + r];
+    const uint64_t lo = (uint64_t)rc.x | ((uint64_t)rc.y << 32);
+    const double* m = mom + (int64_t)rec_elem[roff + r] * NM;
+    const double* MM = m;
+    const double* P = m + NB * NB;
+    const double* Qm = P + NB * DIM;
+    const double* S = Qm + NB * DIM;
+    int l[NB], o[NB];
+    double x[NB][DIM];
+    uint32_t ms[NB];
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+      l[a] = rec_local(lo, a);
+      o[a] = rec_orig(rc, a);
+      node_xyz<DIM>(L.p0[l[a]], L.p1[l[a]], x[a]);
+      ms[a] = L.ms[l[a]];
+    }
+    double c[NB][DIM], det;
+    p1_geometry<DIM>(x, c, det);
+    const double adet = fabs(det), inv = recip(adet), sg = det < 0.0 ? -1.0 : 1.0;
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+      if (l[a] >= nown) continue;
+      double ta[DIM];  // (c_a^T S)_j
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) {
+        double t = 0.0;
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) t += c[a][i] * S[i * DIM + j];
+        ta[j] = t;
+      }
+      const double* Pa = P + o[a] * DIM;
+      const int base = meta_off(ms[a]);
+#pragma unroll
+      for (int bb = 0; bb < NB; ++bb) {
+        const double* Qb = Qm + o[bb] * DIM;
+        double gg = 0.0, pc = 0.0, cq = 0.0;
+#pragma unroll
+        for (int j = 0; j < DIM; ++j) {
+          gg += ta[j] * c[bb][j];
+          pc += Pa[j] * c[bb][j];
+          cq += c[a][j] * Qb[j];
+        }
+        const double v = adet * MM[o[a] * NB + o[bb]] + sg * (pc + cq) + inv * gg;
+        const int slot = bb == a ? meta_diag(ms[a]) : rec_rank<NB>(rc, a, bb);
+        atomicAdd(L.acc + base + slot, v);
+      }
+    }
+  }
+  __syncthreads();
+  const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  for (int k = w; k < nrun; k += NT / kWave) {
+    const int4 ru = p.runs[runoff + k];
+    for (int t = lane; t < ru.z; t += kWave) values[(int64_t)ru.x + t] = (OT)L.acc[ru.y + t];
+  }
+}
+
+// Spatially varying linear forms F0(q) v + F1(q) . grad v with per-element
+// moments in the element's vertex order, NB + DIM doubles per element:
+//   L0[NB] = sum_q w_q phi_a(q) F0(q),  L1[DIM] = sum_q w_q F1(q)
+// so that b_a += |det| L0_a + sgn(det) c_a . L1.
+template <int NB, int NT>
+__global__ __launch_bounds__(NT) void k_asm_load_moments(PlanArgs p, const int32_t* __restrict__ rec_elem,
+                                                        const double* __restrict__ X,
+                                                        const double* __restrict__ mom, double* __restrict__ out) {
+  constexpr int DIM = NB - 1, NM = NB + DIM;
+  extern __shared__ __align__(16) char smem[];
+  const int b = tmx::xcd_remap(blockIdx.x, p.nblk);
+  const int4 bi0 = *(const int4*)(p.blk + 8 * b);
+  const int4 bi1 = *(const int4*)(p.blk + 8 * b + 4);
+  const int hoff = bi0.x, nh = bi0.y, nown = bi0.z, roff = bi0.w, nrec = bi1.x;
+  const Lds L = carve(smem, 0, nh, nown);
+  stage<NT, DIM, float, false>(L, p.halo + hoff, p.meta + hoff, 0, nh, nown, X, (const float*)nullptr);
+  __syncthreads();
+  for (int r = threadIdx.x; r < nrec; r += NT) {
+    const uint4 rc = p.recs[roff + r];
+    const uint64_t lo = (uint64_t)rc.x | ((uint64_t)rc.y << 32);
+    const double* m = mom + (int64_t)rec_elem[roff + r] * NM;
+    int l[NB];
+    double x[NB][DIM];
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+      l[a] = rec_local(lo, a);
+      node_xyz<DIM>(L.p0[l[a]], L.p1[l[a]], x[a]);
+    }
+    double c[NB][DIM], det;
+    p1_geometry<DIM>(x, c, det);
+    const double adet = fabs(det), sg = det < 0.0 ? -1.0 : 1.0;
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+      if (l[a] >= nown) continue;
+      double g = 0.0;
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) g += c[a][j] * m[NB + j];
+      atomicAdd(L.accb + l[a], adet * m[rec_orig(rc, a)] + sg * g);
+    }
+  }
+  __syncthreads();
+  for (int a = threadIdx.x; a < nown; a += NT) out[p.halo[hoff + a]] = L.accb[a];
+}
+
 // FORM: 0 constant load c, 1 nodal f.
 template <int NB, int FORM, typename CT, int NT>
 __global__ __launch_bounds__(NT) void k_asm_load(PlanArgs p, const double* __restrict__ X,
@@ -623,6 +754,34 @@ extern "C" int tm_asm_p1_general(int nb, int dim, const tm_plan_dev* plan, const
   }
   if (nb == 3) TM_BY_THREADS(plan, (k_asm_general<3, double, NT>), st, p, coords, g, q, (double*)values);
   TM_BY_THREADS(plan, (k_asm_general<4, double, NT>), st, p, coords, g, q, (double*)values);
+}
+
+extern "C" int tm_asm_p1_moments(int nb, int dim, const tm_plan_dev* plan, const double* coords,
+                                 const double* moments, void* values, int value_dtype, void* stream) {
+  TM_REQUIRE(plan && coords && moments && values && plan->rec_elem, TM_ERR_ARG, "null argument");
+  TM_REQUIRE((nb == 3 && dim == 2) || (nb == 4 && dim == 3), TM_ERR_ARG,
+             "only P1 triangles (nb=3, dim=2) and tetrahedra (nb=4, dim=3) are supported");
+  TM_REQUIRE(value_dtype == TM_F32 || value_dtype == TM_F64, TM_ERR_ARG, "values must be f32 or f64");
+  hipStream_t st = (hipStream_t)stream;
+  const PlanArgs p = plan_args(plan);
+  const int32_t* re = plan->rec_elem;
+  if (value_dtype == TM_F32) {
+    if (nb == 3) TM_BY_THREADS(plan, (k_asm_moments<3, float, NT>), st, p, re, coords, moments, (float*)values);
+    TM_BY_THREADS(plan, (k_asm_moments<4, float, NT>), st, p, re, coords, moments, (float*)values);
+  }
+  if (nb == 3) TM_BY_THREADS(plan, (k_asm_moments<3, double, NT>), st, p, re, coords, moments, (double*)values);
+  TM_BY_THREADS(plan, (k_asm_moments<4, double, NT>), st, p, re, coords, moments, (double*)values);
+}
+
+extern "C" int tm_asm_p1_load_moments(int nb, int dim, const tm_plan_dev* plan, const double* coords,
+                                      const double* moments, double* b, void* stream) {
+  TM_REQUIRE(plan && coords && moments && b && plan->rec_elem, TM_ERR_ARG, "null argument");
+  TM_REQUIRE((nb == 3 && dim == 2) || (nb == 4 && dim == 3), TM_ERR_ARG, "only P1 simplices");
+  hipStream_t st = (hipStream_t)stream;
+  const PlanArgs p = plan_args(plan);
+  if (nb == 3)
+    return launch_plan_kernel(k_asm_load_moments<3, 256>, 256, plan, st, p, plan->rec_elem, coords, moments, b);
+  return launch_plan_kernel(k_asm_load_moments<4, 256>, 256, plan, st, p, plan->rec_elem, coords, moments, b);
 }
 
 extern "C" int tm_asm_p1_load(int form, int nb, int dim, const tm_plan_dev* plan, const double* coords,

@@ -344,7 +344,8 @@ int make_blocks(const PlanCtx& c, Scratch& s, std::vector<int32_t> owned,
   }
   // Element records (16 bytes): nb x 10-bit LDS slots, then for each owned
   // vertex a the in-row ranks of the other vertices (6 bits each, a-major, b
-  // ascending).  Two layout choices cut LDS bank conflicts of the kernels; both
+  // ascending, from bit 40), and at bit 112 the element-local index of each
+  // record vertex (2 bits each).  Two layout choices cut LDS bank conflicts of the kernels; both
   // only relabel, so the assembled values do not change:
   //  * vertex order per record: a wave issues one LDS atomic per ordered pair
   //    (a, b) for all 64 lanes; the order is picked greedily, lane by lane, so
@@ -358,6 +359,7 @@ int make_blocks(const PlanCtx& c, Scratch& s, std::vector<int32_t> owned,
   const int NB = c.nb;
   std::vector<int32_t> vl((size_t)nrec * NB);  // halo-order local ids, final vertex order
   std::vector<uint8_t> vr((size_t)nrec * 16, 0);  // ranks in final order, (a, j) a-major
+  std::vector<uint8_t> vo((size_t)nrec, 0);       // element-local index of each record vertex, 2 bits each
   {
     int perms[24][4];
     int np = 0;
@@ -415,6 +417,9 @@ int make_blocks(const PlanCtx& c, Scratch& s, std::vector<int32_t> owned,
           }
         }
         const int* pq = perms[best];
+        uint8_t code = 0;
+        for (int a = 0; a < NB; ++a) code |= (uint8_t)(pq[a] << (2 * a));
+        vo[(size_t)r] = code;
         for (int k = 0; k < ns; ++k) {
           const int a = pq[sx[k]], b = pq[sy[k]];
           if (l0[a] >= nown) continue;
@@ -505,6 +510,9 @@ int make_blocks(const PlanCtx& c, Scratch& s, std::vector<int32_t> owned,
           hi |= v << (bit - 64);
       }
     }
+    // bits 112-119: record vertex a is vertex (code >> 2a) & 3 of the element
+    // (kernels that read per-element data in the element's own vertex order)
+    hi |= (uint64_t)vo[(size_t)r] << (112 - 64);
     B.recs.push_back((uint32_t)lo);
     B.recs.push_back((uint32_t)(lo >> 32));
     B.recs.push_back((uint32_t)hi);

@@ -11,8 +11,8 @@
 // needs the scalar, sums the partials in a fixed order (so the result does not
 // depend on scheduling and no atomics or grid-wide fences are needed).
 // Scalars live in device memory; kernels return immediately once the
-// device-side status flag is set, so the host only reads the flag every few
-// iterations.
+// device-side status flag is set, so the host only reads the flag (and the
+// residual) every few iterations, as many as the convergence rate predicts.
 #include <cstdlib>
 #include <mutex>
 #include <vector>
@@ -285,6 +285,33 @@ int grid_for(int64_t n, int per_block) {
   return (int)(g < 1 ? 1 : (g > kMaxBlocks ? kMaxBlocks : g));
 }
 
+// Iterations to enqueue before the host next reads the device status.  The
+// kernels of an iteration that starts after convergence return at once, but
+// each still costs a dispatch, so the chunk follows the convergence rate seen
+// over the last chunk: it ends at the predicted convergence iteration (at
+// least 1, at most 256; 16 for the first chunk, doubling while the residual
+// does not fall).  check_every > 0 fixes the chunk.
+struct Poller {
+  int check_every;
+  int chunk;
+  int it_prev = -1;
+  double rel_prev = 0.0;
+  explicit Poller(int ce) : check_every(ce), chunk(ce > 0 ? ce : 16) {}
+  int next(int it, double rel, double target) {
+    if (check_every > 0) return chunk;
+    int c = chunk < 256 ? 2 * chunk : 256;
+    if (it_prev >= 0 && it > it_prev && rel > 0.0 && rel_prev > 0.0 && rel < rel_prev && target > 0.0) {
+      const double rate = log(rel / rel_prev) / (double)(it - it_prev);  // < 0
+      const double rem = log(target / rel) / rate;                       // iterations left
+      c = rem < 1.0 ? 1 : (rem > 256.0 ? 256 : (int)ceil(rem));
+    }
+    it_prev = it;
+    rel_prev = rel;
+    chunk = c;
+    return c;
+  }
+};
+
 template <typename VT, int S, int BS>
 int pcg_run(int64_t nrows, const int32_t* rp, const int32_t* ci, const VT* v, const double* b, double* x, PcgWork w,
             double rtol, double atol, int64_t maxiter, int check_every, int64_t* iters, double* relres,
@@ -298,23 +325,28 @@ int pcg_run(int64_t nrows, const int32_t* rp, const int32_t* ci, const VT* v, co
   hipLaunchKernelGGL(k_pcg_init, dim3(G), dim3(kThreads), 0, st, n, b, w);
   hipLaunchKernelGGL(k_pcg_ctl0, dim3(1), dim3(kThreads), 0, st, w, G, rtol * rtol, atol * atol, (long long)maxiter);
   TM_HIP(hipGetLastError());
-  PcgFlags hf{};
-  PcgScalars hs{};
-  int chunk = check_every > 0 ? check_every : 16;
+  // scalars and flags are adjacent: one read gives the status and the residual
+  struct {
+    PcgScalars s;
+    PcgFlags f;
+  } h{};
+  static_assert(sizeof(h) == sizeof(PcgScalars) + sizeof(PcgFlags), "PCG state layout");
+  PcgFlags& hf = h.f;
+  PcgScalars& hs = h.s;
+  Poller poll(check_every);
+  int chunk = poll.chunk;
   for (;;) {
-    TM_HIP(hipMemcpyAsync(&hf, w.f, sizeof(hf), hipMemcpyDeviceToHost, st));
+    TM_HIP(hipMemcpyAsync(&h, w.s, sizeof(h), hipMemcpyDeviceToHost, st));
     TM_HIP(hipStreamSynchronize(st));
     if (hf.status != 0) break;
+    if (hf.iter > 0) chunk = poll.next(hf.iter, hs.relres, hs.bb > 0.0 ? sqrt(hs.tol2 / hs.bb) : 0.0);
     for (int k = 0; k < chunk; ++k) {
       hipLaunchKernelGGL((k_pcg_spmv<VT, S, BS>), dim3(G), dim3(kThreads), 0, st, nrows, rp, ci, v, w);
       hipLaunchKernelGGL(k_pcg_update, dim3(G), dim3(kThreads), 0, st, n, w, G);
       hipLaunchKernelGGL(k_pcg_dir, dim3(G), dim3(kThreads), 0, st, n, x, w, G);
     }
     TM_HIP(hipGetLastError());
-    if (check_every <= 0 && chunk < 256) chunk *= 2;
   }
-  TM_HIP(hipMemcpyAsync(&hs, w.s, sizeof(hs), hipMemcpyDeviceToHost, st));
-  TM_HIP(hipStreamSynchronize(st));
   if (iters) *iters = hf.iter;
   if (relres) *relres = hs.relres;
   if (hf.status == 2) {
@@ -590,12 +622,18 @@ int bc_run(int64_t n, const int32_t* rp, const int32_t* ci, const VT* v, const d
   hipLaunchKernelGGL(k_bc_init, dim3(G), dim3(kThreads), 0, st, n, b, w);
   hipLaunchKernelGGL(k_bc_ctl0, dim3(1), dim3(kThreads), 0, st, w, G, rtol * rtol, atol * atol, (long long)maxiter);
   TM_HIP(hipGetLastError());
-  PcgFlags hf{};
-  BcScalars hs{};
-  int chunk = check_every > 0 ? check_every : 16;
+  struct {
+    BcScalars s;
+    PcgFlags f;
+  } h{};
+  static_assert(sizeof(h) == sizeof(BcScalars) + sizeof(PcgFlags), "BiCGSTAB state layout");
+  PcgFlags& hf = h.f;
+  BcScalars& hs = h.s;
+  Poller poll(check_every);
+  int chunk = poll.chunk;
   int restarts = 0;
   for (;;) {
-    TM_HIP(hipMemcpyAsync(&hf, w.f, sizeof(hf), hipMemcpyDeviceToHost, st));
+    TM_HIP(hipMemcpyAsync(&h, w.sc, sizeof(h), hipMemcpyDeviceToHost, st));
     TM_HIP(hipStreamSynchronize(st));
     if (hf.status == 1 && hf.iter > 0) {
       // the recursive residual met the tolerance: check the true one (x is
@@ -606,8 +644,12 @@ int bc_run(int64_t n, const int32_t* rp, const int32_t* ci, const VT* v, const d
       hipLaunchKernelGGL(k_bc_init, dim3(G), dim3(kThreads), 0, st, n, b, w);
       hipLaunchKernelGGL(k_bc_recheck, dim3(1), dim3(kThreads), 0, st, w, G, restart);
       TM_HIP(hipGetLastError());
-      TM_HIP(hipMemcpyAsync(&hf, w.f, sizeof(hf), hipMemcpyDeviceToHost, st));
+      TM_HIP(hipMemcpyAsync(&h, w.sc, sizeof(h), hipMemcpyDeviceToHost, st));
       TM_HIP(hipStreamSynchronize(st));
+      poll = Poller(check_every);  // the recursion restarts from the true residual
+      chunk = poll.chunk;
+    } else if (hf.iter > 0) {
+      chunk = poll.next(hf.iter, hs.relres, hs.bb > 0.0 ? sqrt(hs.tol2 / hs.bb) : 0.0);
     }
     if (hf.status != 0) break;
     for (int k = 0; k < chunk; ++k) {
@@ -620,10 +662,7 @@ int bc_run(int64_t n, const int32_t* rp, const int32_t* ci, const VT* v, const d
       hipLaunchKernelGGL(k_bc_dir, dim3(G), dim3(kThreads), 0, st, n, w, G);
     }
     TM_HIP(hipGetLastError());
-    if (check_every <= 0 && chunk < 256) chunk *= 2;
   }
-  TM_HIP(hipMemcpyAsync(&hs, w.sc, sizeof(hs), hipMemcpyDeviceToHost, st));
-  TM_HIP(hipStreamSynchronize(st));
   if (iters) *iters = hf.iter;
   if (relres) *relres = hs.relres;
   if (hf.status == 2) {

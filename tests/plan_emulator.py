@@ -20,6 +20,12 @@ def decode(rec, nb):
     return loc, rank
 
 
+def vertex_order(rec, nb):
+    """Element-local index of each record vertex (bits 112-119, 2 bits each)."""
+    code = (int(rec[3]) >> 16) & 0xFF
+    return [(code >> (2 * a)) & 3 for a in range(nb)]
+
+
 def emulate_matrix(topo, arrs, element_matrix):
     """``element_matrix(e, x[nb, d]) -> [nb, nb]``; returns the CSR values."""
     nb = topo.nb
@@ -41,6 +47,9 @@ def emulate_matrix(topo, arrs, element_matrix):
             assert max(loc) < nh
             e = int(arrs["rec_elem"][roff + r])
             assert sorted(halo[loc]) == sorted(topo.conn[e])
+            order = vertex_order(recs[roff + r], nb)
+            assert sorted(order) == list(range(nb))
+            assert list(halo[loc]) == [int(topo.conn[e][o]) for o in order]
             K = element_matrix(e, topo.points[halo[loc]])
             for a in range(nb):
                 if loc[a] >= nown:
