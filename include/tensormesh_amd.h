@@ -134,6 +134,22 @@ typedef struct {
   double M[16];
 } tm_quad;
 
+/* Boundary-facet load vector (FacetAssembler.__call__,
+ * assemble/facet_assembler.py:123-277): for n_facets selected facets, each an
+ * element cells[f][nb] (int64 node ids) and the local index opposite[f] of the
+ * cell vertex off the facet, adds
+ *   b[node_a*H + h] += sum_q w_q |F| (F0[f][q][h] phi_a(q) + F1[f][q][h][:] . grad phi_a)
+ * where |F| is the facet Jacobian (edge length in 2D, twice the face area in
+ * 3D), phi_a(q) the cell basis at the facet point with facet barycentric
+ * coordinates q_bary[q][0..dim-1] (the facet vertices in ascending local
+ * order; 0 for the opposite vertex), q_w the facet rule's weights (summing to
+ * 1 on the unit interval, 1/2 on the reference triangle), F0 [n_facets][n_q][H]
+ * and F1 [n_facets][n_q][H][dim] f64 (F1 may be NULL), H <= 3, n_q <= 16.
+ * b [N*H] f64 is accumulated into (FP64 atomics), not overwritten. */
+int tm_facet_load(int nb, int dim, int64_t n_facets, const double* coords, const int64_t* cells,
+                  const int32_t* opposite, int n_q, const double* q_bary, const double* q_w, int H,
+                  const double* F0, const double* F1, double* b, void* stream);
+
 /* Stiffness / mass values in the canonical CSR order (ElementAssembler.__call__,
  * element_assembler.py:243-466, for the forms above).  coords [N, dim] f64;
  * coef [N] nodal (f32 or f64, NULL when unused); values [nnz] f32 or f64.

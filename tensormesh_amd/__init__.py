@@ -5,13 +5,13 @@ with hand-written gfx950 HIP kernels reached through the C ABI of
 ``libtensormesh_amd.so`` (``include/tensormesh_amd.h``).
 """
 from .mesh import Mesh, kuhn_cube, structured_rectangle
-from .assemble import (ElementAssembler, NodeAssembler, LaplaceElementAssembler, MassElementAssembler,
+from .assemble import (ElementAssembler, NodeAssembler, FacetAssembler, LaplaceElementAssembler, MassElementAssembler,
                        KappaLaplaceElementAssembler, SourceNodeAssembler, const_node_assembler,
                        func_node_assembler, NeoHookeanModel, LinearElasticityElementAssembler)
 from .sparse import SparseMatrix
 from .operator import Condenser
 
-__all__ = ["Mesh", "kuhn_cube", "structured_rectangle", "ElementAssembler", "NodeAssembler",
+__all__ = ["Mesh", "kuhn_cube", "structured_rectangle", "ElementAssembler", "NodeAssembler", "FacetAssembler",
            "LaplaceElementAssembler", "MassElementAssembler", "KappaLaplaceElementAssembler",
            "SourceNodeAssembler", "const_node_assembler", "func_node_assembler", "NeoHookeanModel",
            "LinearElasticityElementAssembler", "SparseMatrix", "Condenser"]

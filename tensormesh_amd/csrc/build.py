@@ -21,7 +21,7 @@ LIB = os.path.join(PKG, "libtensormesh_amd.so")
 OBJ = os.path.join(PKG, "_build")
 
 HIP_SOURCES = ["tm_assemble.hip", "tm_krylov.hip", "tm_hyper.hip", "tm_batched.hip", "tm_gnn.hip",
-               "tm_condense.hip", "tm_pattern.hip"]
+               "tm_condense.hip", "tm_pattern.hip", "tm_facet.hip"]
 CXX_SOURCES = ["tm_host.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

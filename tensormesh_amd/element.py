@@ -78,3 +78,31 @@ def quadrature_moments(element_type: str, order: int = 2):
     W = (w[:, None] * phi).sum(0)
     M = np.einsum("q,qa,qb->ab", w, phi, phi)
     return wsum, W, M
+
+
+# Gauss-Legendre rules on [0, 1] for the edges of triangles, indexed like the
+# reference's lin_quadrature (tensormesh/element/quadrature.py:13-66): "order"
+# n is the number of points (exact to degree 2n - 1), weights sum to 1.
+_GL = {
+    1: ([2.0], [0.0]),
+    2: ([1.0, 1.0], [-1.0 / np.sqrt(3.0), 1.0 / np.sqrt(3.0)]),
+    3: ([5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0], [-np.sqrt(0.6), 0.0, np.sqrt(0.6)]),
+}
+
+
+def facet_quadrature(element_type: str, order: int = 2):
+    """Quadrature on the facets of a P1 element as ``(weights [Q], bary [Q, d])``:
+    barycentric coordinates of each point with respect to the facet's ``d``
+    vertices.  Triangle facets (edges) use the Gauss-Legendre rule above
+    (weights sum to 1, the edge Jacobian is its length); tetrahedron facets use
+    the triangle rules of :func:`quadrature` (weights sum to 1/2, the face
+    Jacobian is twice its area) - the facet rules of
+    ``tensormesh/element/quadrature.py:321-410``."""
+    if element_type == "triangle":
+        if order not in _GL:
+            raise ValueError(f"facet quadrature order {order} not available for triangle (have {sorted(_GL)})")
+        w, p = _GL[order]
+        t = (np.asarray(p, dtype=np.float64) + 1.0) / 2.0
+        return np.asarray(w, dtype=np.float64) / 2.0, np.stack([1.0 - t, t], 1)
+    w, p = quadrature("triangle", order)
+    return w, np.concatenate([1.0 - p.sum(1, keepdims=True), p], 1)
