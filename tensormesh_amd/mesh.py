@@ -159,7 +159,8 @@ class Mesh(nn.Module):
 
     @classmethod
     def read(cls, file_name: str, file_format: Optional[str] = None, reorder: bool = False) -> "Mesh":
-        """Read ``.msh`` (gmsh ASCII 2.2 / 4.1) or ``.vtu`` (ASCII XML) triangle / tetra meshes.
+        """Read ``.msh`` (gmsh 2.2 / 4.1, ASCII or binary) or ``.vtu`` (XML: ASCII, base64 or
+        appended raw / base64 arrays, optionally zlib-compressed) triangle / tetra meshes.
         ``reorder`` is accepted for the reference's signature; P1 simplices need none."""
         from . import mesh_io
         pts, cells, pd = mesh_io.read(file_name, file_format)

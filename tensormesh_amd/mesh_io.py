@@ -1,4 +1,5 @@
-"""Mesh files without meshio: gmsh ``.msh`` (ASCII 2.2 and 4.1) and VTK ``.vtu`` (ASCII XML).
+"""Mesh files without meshio: gmsh ``.msh`` (2.2 and 4.1, ASCII and binary) and VTK ``.vtu``
+(XML with ASCII, base64 or appended raw / base64 data, optionally zlib-compressed).
 
 The reference reads and writes meshes through ``meshio.read`` / ``meshio.write``
 (``tensormesh/mesh/mesh.py:267-325, 694-740``); meshio is not available here,
@@ -70,15 +71,180 @@ def _sections(text: str) -> Dict[str, list]:
     return out
 
 
+class _Bytes:
+    """Cursor over a binary gmsh file (little- or big-endian, fixed sizes)."""
+
+    def __init__(self, data: bytes, path: str):
+        self.d, self.i, self.path = data, 0, path
+        self.end = "<"
+        self.szt = "Q"
+
+    def line(self) -> str:
+        j = self.d.index(b"\n", self.i)
+        out = self.d[self.i:j].decode("ascii", errors="replace").strip()
+        self.i = j + 1
+        return out
+
+    def skip_ws(self):
+        while self.i < len(self.d) and self.d[self.i:self.i + 1] in (b"\n", b"\r", b" "):
+            self.i += 1
+
+    def arr(self, code: str, n: int) -> np.ndarray:
+        dt = np.dtype(self.end + code)
+        nbytes = dt.itemsize * n
+        if self.i + nbytes > len(self.d):
+            raise ValueError(f"{self.path}: truncated binary data")
+        out = np.frombuffer(self.d, dtype=dt, count=n, offset=self.i)
+        self.i += nbytes
+        return out
+
+    def one(self, code: str) -> int:
+        return int(self.arr(code, 1)[0])
+
+    def expect_end(self, name: str):
+        self.skip_ws()
+        if self.line() != "$End" + name:
+            raise ValueError(f"{self.path}: ${name} is not closed where expected")
+
+    def skip_section(self, name: str):
+        j = self.d.find(b"$End" + name.encode(), self.i)
+        if j < 0:
+            raise ValueError(f"{self.path}: section ${name} is not closed")
+        self.i = j
+        self.line()
+
+
+def _read_msh_binary(path: str, data: bytes) -> MeshArrays:
+    """gmsh binary 2.2 and 4.1 (the gmsh file-format reference, sections
+    $Nodes, $Elements, $Entities, $NodeData)."""
+    c = _Bytes(data, path)
+    while c.line() != "$MeshFormat":
+        pass
+    head = c.line().split()
+    version, dsize = head[0], int(head[2])
+    if dsize not in (4, 8):
+        raise NotImplementedError(f"{path}: data size {dsize}")
+    c.szt = "Q" if dsize == 8 else "I"
+    one = np.frombuffer(data, "<i4", 1, c.i)[0]
+    if one != 1:
+        c.end = ">"
+        if np.frombuffer(data, ">i4", 1, c.i)[0] != 1:
+            raise ValueError(f"{path}: bad endianness marker")
+    c.i += 4
+    c.expect_end("MeshFormat")
+    v4 = version.startswith("4")
+    if not (v4 or version.startswith("2")):
+        raise NotImplementedError(f"{path}: gmsh format version {version}")
+    ids = pts = None
+    cells: Dict[str, list] = {}
+    node_data = []
+    while True:
+        c.skip_ws()
+        if c.i >= len(data):
+            break
+        name = c.line()
+        if not name.startswith("$"):
+            raise ValueError(f"{path}: unexpected content {name[:40]!r}")
+        name = name[1:]
+        if name == "Nodes" and v4:
+            nblk, n, _, _ = (int(v) for v in c.arr(c.szt, 4))
+            ids, pts, filled = np.empty(n, np.int64), np.empty((n, 3)), 0
+            for _ in range(nblk):
+                edim, _tag, param = (int(v) for v in c.arr("i", 3))
+                nin = c.one(c.szt)
+                if param:
+                    raise NotImplementedError(f"{path}: parametric node coordinates are not supported")
+                ids[filled:filled + nin] = c.arr(c.szt, nin)
+                pts[filled:filled + nin] = c.arr("d", 3 * nin).reshape(nin, 3)
+                filled += nin
+            c.expect_end(name)
+        elif name == "Nodes":
+            n = int(c.line())
+            rec = np.dtype([("tag", c.end + "i4"), ("x", c.end + "f8", 3)])
+            r = np.frombuffer(data, rec, n, c.i)
+            c.i += rec.itemsize * n
+            ids, pts = r["tag"].astype(np.int64), r["x"].astype(np.float64)
+            c.expect_end(name)
+        elif name == "Elements" and v4:
+            nblk, _, _, _ = (int(v) for v in c.arr(c.szt, 4))
+            for _ in range(nblk):
+                _edim, _tag, et = (int(v) for v in c.arr("i", 3))
+                nin = c.one(c.szt)
+                if et not in GMSH_NODES_PER_TYPE:
+                    raise NotImplementedError(f"{path}: gmsh element type {et}")
+                npe = GMSH_NODES_PER_TYPE[et]
+                blk = c.arr(c.szt, nin * (1 + npe)).reshape(nin, 1 + npe)
+                if et in GMSH_TYPES:
+                    cells.setdefault(GMSH_TYPES[et][0], []).extend(blk[:, 1:].reshape(-1).tolist())
+            c.expect_end(name)
+        elif name == "Elements":
+            total, done = int(c.line()), 0
+            while done < total:
+                et, nin, ntag = (int(v) for v in c.arr("i", 3))
+                if et not in GMSH_NODES_PER_TYPE:
+                    raise NotImplementedError(f"{path}: gmsh element type {et}")
+                npe = GMSH_NODES_PER_TYPE[et]
+                blk = c.arr("i", nin * (1 + ntag + npe)).reshape(nin, 1 + ntag + npe)
+                if et in GMSH_TYPES:
+                    cells.setdefault(GMSH_TYPES[et][0], []).extend(blk[:, 1 + ntag:].reshape(-1).tolist())
+                done += nin
+            c.expect_end(name)
+        elif name == "Entities" and v4:
+            npt, ncv, nsf, nvl = (int(v) for v in c.arr(c.szt, 4))
+            for _ in range(npt):
+                c.arr("i", 1), c.arr("d", 3)
+                c.arr("i", c.one(c.szt))
+            for _ in range(ncv + nsf + nvl):
+                c.arr("i", 1), c.arr("d", 6)
+                c.arr("i", c.one(c.szt))
+                c.arr("i", c.one(c.szt))
+            c.expect_end(name)
+        elif name == "NodeData":
+            hdr = []
+            ns = int(c.line())
+            hdr.append([c.line() for _ in range(ns)])
+            nr = int(c.line())
+            [c.line() for _ in range(nr)]
+            ni = int(c.line())
+            ints = [int(c.line()) for _ in range(ni)]
+            ncomp, count = ints[1], ints[2]
+            rec = np.dtype([("tag", c.end + "i4"), ("v", c.end + "f8", ncomp)])
+            r = np.frombuffer(data, rec, count, c.i)
+            c.i += rec.itemsize * count
+            node_data.append((hdr[0][0].strip().strip('"') if hdr[0] else f"data{len(node_data)}", ncomp,
+                              r["tag"].astype(np.int64), r["v"].reshape(count, ncomp).astype(np.float64)))
+            c.expect_end(name)
+        else:
+            c.skip_section(name)
+    if ids is None:
+        raise ValueError(f"{path}: no $Nodes section")
+    lookup = np.full(int(ids.max()) + 1, -1, np.int64)
+    lookup[ids] = np.arange(len(ids))
+    for k in cells:
+        cc = lookup[np.asarray(cells[k], np.int64)]
+        if (cc < 0).any():
+            raise ValueError(f"{path}: a {k} cell references an undefined node")
+        cells[k] = cc
+    point_data: Dict[str, np.ndarray] = {}
+    for nm, ncomp, tags, vals in node_data:
+        full = np.zeros((len(ids), ncomp))
+        full[lookup[tags]] = vals
+        point_data[nm] = full[:, 0] if ncomp == 1 else full
+    return _finish(pts, cells, point_data)
+
+
 def read_msh(path: str) -> MeshArrays:
-    with open(path, "r") as f:
-        sec = _sections(f.read())
-    if "MeshFormat" not in sec:
+    with open(path, "rb") as f:
+        raw = f.read()
+    i = raw.find(b"$MeshFormat")
+    if i < 0:
         raise ValueError(f"{path}: no $MeshFormat section")
+    head = raw[i:i + 200].split(b"\n")[1].split()
+    if len(head) >= 2 and head[1] == b"1":
+        return _read_msh_binary(path, raw)
+    sec = _sections(raw.decode("utf-8", errors="replace"))
     head = sec["MeshFormat"][0][0].split()
-    version, ftype = head[0], int(head[1])
-    if ftype != 0:
-        raise NotImplementedError(f"{path}: binary gmsh files are not supported (ASCII only)")
+    version = head[0]
     if version.startswith("2"):
         nodes = sec["Nodes"][0]
         n = int(nodes[0])
@@ -182,23 +348,116 @@ def write_msh(path: str, points: np.ndarray, cells: Dict[str, np.ndarray],
 
 
 # ---- VTU -------------------------------------------------------------------------------------
-def _ascii_array(node, dtype):
-    if node.get("format", "ascii") != "ascii":
-        raise NotImplementedError("only ASCII .vtu data arrays are supported")
-    return np.array((node.text or "").split(), dtype=dtype)
+# VTK XML data arrays (the VTK file-format reference): "ascii", inline "binary"
+# (base64 of a byte-count header and the data, encoded together or one after
+# the other), and "appended" (raw bytes or base64 after the "_" of
+# <AppendedData>, each array at its byte / character offset).  With
+# compressor="vtkZLibDataCompressor" the header is [number of blocks, block
+# size, last block size, compressed sizes...] and the data zlib blocks.
+_VTK_DT = {"Int8": "i1", "UInt8": "u1", "Int16": "i2", "UInt16": "u2", "Int32": "i4", "UInt32": "u4",
+           "Int64": "i8", "UInt64": "u8", "Float32": "f4", "Float64": "f8"}
+
+
+def _b64_chars(nbytes: int) -> int:
+    return 4 * ((nbytes + 2) // 3)
+
+
+class _VtuArrays:
+    def __init__(self, root, raw: Optional[bytes], text: Optional[str], path: str):
+        self.path = path
+        self.order = ">" if root.get("byte_order") == "BigEndian" else "<"
+        self.hdt = np.dtype(self.order + _VTK_DT[root.get("header_type", "UInt32")])
+        comp = root.get("compressor")
+        if comp not in (None, "", "vtkZLibDataCompressor"):
+            raise NotImplementedError(f"{path}: compressor {comp} is not supported (zlib only)")
+        self.zlib = comp == "vtkZLibDataCompressor"
+        self.raw, self.text = raw, text
+
+    def _inflate(self, header: np.ndarray, body: bytes) -> bytes:
+        import zlib
+        nblk = int(header[0])
+        sizes = [int(v) for v in header[3:3 + nblk]]
+        out, o = [], 0
+        for sz in sizes:
+            out.append(zlib.decompress(body[o:o + sz]))
+            o += sz
+        return b"".join(out)
+
+    def _from_raw(self, buf: bytes, off: int) -> bytes:
+        hs = self.hdt.itemsize
+        if not self.zlib:
+            n = int(np.frombuffer(buf, self.hdt, 1, off)[0])
+            return buf[off + hs:off + hs + n]
+        nblk = int(np.frombuffer(buf, self.hdt, 1, off)[0])
+        header = np.frombuffer(buf, self.hdt, 3 + nblk, off)
+        body = off + hs * (3 + nblk)
+        return self._inflate(header, buf[body:body + int(header[3:].sum())])
+
+    def _from_b64(self, t: str) -> bytes:
+        import base64
+        hs = self.hdt.itemsize
+        if not self.zlib:
+            hc = _b64_chars(hs)
+            n = int(np.frombuffer(base64.b64decode(t[:hc])[:hs], self.hdt)[0])
+            if t[hc - 1] == "=":                       # header and data encoded one after the other
+                return base64.b64decode(t[hc:hc + _b64_chars(n)])[:n]
+            return base64.b64decode(t[:_b64_chars(hs + n)])[hs:hs + n]
+        nblk = int(np.frombuffer(base64.b64decode(t[:_b64_chars(hs)])[:hs], self.hdt)[0])
+        hc = _b64_chars(hs * (3 + nblk))
+        header = np.frombuffer(base64.b64decode(t[:hc])[:hs * (3 + nblk)], self.hdt)
+        nbody = int(header[3:].sum())
+        return self._inflate(header, base64.b64decode(t[hc:hc + _b64_chars(nbody)]))
+
+    def get(self, node, dtype) -> np.ndarray:
+        dt = np.dtype(self.order + _VTK_DT[node.get("type", "Float64")])
+        fmt = node.get("format", "ascii")
+        if fmt == "ascii":
+            return np.array((node.text or "").split(), dtype=dtype)
+        if fmt == "binary":
+            buf = self._from_b64("".join((node.text or "").split()))
+        elif fmt == "appended":
+            off = int(node.get("offset", "0"))
+            if self.raw is not None:
+                buf = self._from_raw(self.raw, off)
+            elif self.text is not None:
+                buf = self._from_b64(self.text[off:])
+            else:
+                raise ValueError(f"{self.path}: appended data array without <AppendedData>")
+        else:
+            raise NotImplementedError(f"{self.path}: data array format {fmt}")
+        return np.frombuffer(buf, dt).astype(dtype)
+
+
+def _parse_vtu(path: str):
+    with open(path, "rb") as f:
+        raw_file = f.read()
+    raw = text = None
+    i = raw_file.find(b"<AppendedData")
+    if i >= 0:
+        gt = raw_file.index(b">", i)
+        enc = b'encoding="raw"' in raw_file[i:gt] or b"encoding='raw'" in raw_file[i:gt]
+        us = raw_file.index(b"_", gt)
+        j = raw_file.rfind(b"</AppendedData>")
+        if enc:
+            raw = raw_file[us + 1:j]
+        else:
+            text = "".join(raw_file[us + 1:j].decode("ascii").split())
+        raw_file = raw_file[:gt + 1] + raw_file[j:]
+    root = ET.fromstring(raw_file)
+    return root, _VtuArrays(root, raw, text, path)
 
 
 def read_vtu(path: str) -> MeshArrays:
-    root = ET.parse(path).getroot()
+    root, arrays = _parse_vtu(path)
     if root.get("type") != "UnstructuredGrid":
         raise ValueError(f"{path}: not a VTK UnstructuredGrid file")
     piece = root.find("UnstructuredGrid/Piece")
     npts = int(piece.get("NumberOfPoints"))
-    pts = _ascii_array(piece.find("Points/DataArray"), np.float64).reshape(npts, -1)
+    pts = arrays.get(piece.find("Points/DataArray"), np.float64).reshape(npts, -1)
     arr = {a.get("Name"): a for a in piece.find("Cells").findall("DataArray")}
-    conn = _ascii_array(arr["connectivity"], np.int64)
-    offs = _ascii_array(arr["offsets"], np.int64)
-    types = _ascii_array(arr["types"], np.int64)
+    conn = arrays.get(arr["connectivity"], np.int64)
+    offs = arrays.get(arr["offsets"], np.int64)
+    types = arrays.get(arr["types"], np.int64)
     starts = np.concatenate([[0], offs[:-1]])
     cells: Dict[str, list] = {}
     for t, s, e in zip(types, starts, offs):
@@ -208,7 +467,7 @@ def read_vtu(path: str) -> MeshArrays:
     pd = piece.find("PointData")
     if pd is not None:
         for a in pd.findall("DataArray"):
-            v = _ascii_array(a, np.float64)
+            v = arrays.get(a, np.float64)
             nc = int(a.get("NumberOfComponents", "1"))
             point_data[a.get("Name")] = v if nc == 1 else v.reshape(npts, nc)
     return _finish(pts, cells, point_data)
