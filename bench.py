@@ -35,32 +35,82 @@ def cpu_info():
     return model, max(1, len(sockets))
 
 
-def cpu_baseline(n_cpu: int, repeats: int = 5):
-    """Float64 oracle Map + Reduce (numpy, one thread) on an n_cpu^3 Kuhn grid.
-
-    One untimed warm-up, then the median of ``repeats`` timed assemblies
-    (about 10 s of CPU work at n_cpu = 48)."""
+def _cpu_phases(n: int, threads: int, repeats: int, solve: str, pcg_iters: int = 30):
+    """Torch-CPU float64 restatement of the reference pipeline on the n^3 Kuhn
+    grid with ``threads`` threads: pattern (SciPy, once), Map + Reduce and
+    condensation (median of ``repeats`` after one warm-up), Jacobi-PCG
+    (``solve`` = "full": to 1e-8 once; "iters": ``pcg_iters`` iterations)."""
     import numpy as np
-    from oracle import assemble as oa
+    import torch
+    import warnings
     from oracle import pattern as op
+    from oracle import torch_cpu as tc
     from tensormesh_amd.dataset import lognormal_kappa_cube
     from tensormesh_amd.mesh import kuhn_cube_arrays
-    pts, conn, _ = kuhn_cube_arrays(n_cpu)
-    kappa = lognormal_kappa_cube(n_cpu, 0.1, 1.0, 0)
+    torch.set_num_threads(threads)
+    pts, conn, bnd = kuhn_cube_arrays(n)
+    kappa = lognormal_kappa_cube(n, 0.1, 1.0, 0)
+    t0 = time.perf_counter()
     u, v, pe = op.pattern_and_slots(conn, len(pts))
-    times = []
+    t_pat = time.perf_counter() - t0
+    P, C, K, PE = (torch.from_numpy(a) for a in (pts, conn, kappa, pe))
+    med = lambda ts: sorted(ts)[len(ts) // 2]
+    ta = []
     for it in range(repeats + 1):
         t0 = time.perf_counter()
-        ke = oa.element_matrices(pts, conn, "tetra", "kappa_laplace", kappa=kappa)
-        oa.reduce(pe, ke, len(u))
+        vals = tc.assemble_kappa_laplace(P, C, K, PE, len(u))
         if it:
-            times.append(time.perf_counter() - t0)
-    t = sorted(times)[len(times) // 2]
+            ta.append(time.perf_counter() - t0)
+    row, col = torch.from_numpy(u.astype(np.int64)), torch.from_numpy(v.astype(np.int64))
+    mask = torch.from_numpy(bnd)
+    rhs = torch.ones(len(pts), dtype=torch.float64)
+    tcd = []
+    for it in range(repeats + 1):
+        t0 = time.perf_counter()
+        kv, kr, kc, ni, fi = tc.condense(vals, row, col, mask, rhs)
+        if it:
+            tcd.append(time.perf_counter() - t0)
+    rp = torch.zeros(ni + 1, dtype=torch.int64)
+    rp[1:] = torch.cumsum(torch.bincount(kr, minlength=ni), 0)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        A = torch.sparse_csr_tensor(rp, kc, kv, (ni, ni))
+    d = torch.zeros(ni, dtype=torch.float64)
+    dm = kr == kc
+    d[kr[dm]] = kv[dm]
+    t0 = time.perf_counter()
+    _, its = tc.jacobi_pcg(A, fi, 1.0 / d, rtol=1e-8, maxiter=100000 if solve == "full" else pcg_iters)
+    t_s = time.perf_counter() - t0
+    out = {"grid": f"{n}^3", "elements": len(conn), "threads": threads,
+           "pattern_s": round(t_pat, 3), "assembly_s": round(med(ta), 4),
+           "assembly_melem_per_s": round(len(conn) / med(ta) / 1e6, 4), "condense_s": round(med(tcd), 4),
+           "pcg_iterations": its, "pcg_s_per_iter": round(t_s / max(its, 1), 5)}
+    if solve == "full":
+        out["solve_to_1e-8_s"] = round(t_s, 3)
+    return out
+
+
+def cpu_baseline(n_cpu: int, n_single: int = 48, repeats: int = 3):
+    """The reference pipeline restated in float64 torch on the host CPU
+    (BASELINE plan): all threads of the process's affinity set on the n_cpu^3
+    grid, and one thread on the bounded n_single^3 sample; each phase timed
+    separately."""
+    import torch
+    k = len(os.sched_getaffinity(0))
+    prev = torch.get_num_threads()
+    try:
+        full = _cpu_phases(n_cpu, k, repeats, "full")
+        one = _cpu_phases(n_single, 1, repeats, "iters")
+    finally:
+        torch.set_num_threads(prev)
     model, sockets = cpu_info()
-    return {"value": round(len(conn) / t / 1e6, 4), "unit": "Melem/s", "cores": 1, "kind": "port",
-            "sample": f"float64 numpy oracle kappa-Laplace Map+Reduce on the {n_cpu}^3 Kuhn grid "
-                      f"({len(conn)} tets), median of {repeats} after 1 warm-up, pattern prebuilt",
-            "seconds_per_assembly": round(t, 4), "cpu": model, "sockets": sockets}
+    return {"value": full["assembly_melem_per_s"], "unit": "Melem/s", "cores": k, "kind": "port",
+            "sample": (f"float64 torch-CPU restatement of the reference (SciPy pattern, einsum Map, index_add "
+                       f"Reduce, boolean condensation, Jacobi-PCG with torch CSR), kappa-Laplace on the "
+                       f"{n_cpu}^3 Kuhn grid with {k} threads (assembly and condensation: median of {repeats} "
+                       f"after 1 warm-up; pattern and the solve to 1e-8 once), and on the {n_single}^3 grid "
+                       f"with 1 thread (30 PCG iterations)"),
+            "all_threads": full, "one_thread": one, "cpu": model, "sockets": sockets}
 
 
 def learned_leg(n: int, seeds, log_fn):
@@ -421,7 +471,7 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-n", type=int, default=48)
+    ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-hyper", action="store_true")
@@ -520,7 +570,7 @@ def main():
         "metric": "assembly Melem/s + HBM GB/s; time-to-1e-8 residual, learned-FPCG vs Jacobi-PCG",
         "value": round(E / t_step / 1e6, 2), "unit": "Melem/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(t_step * 1e3, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 compute, f32 values", "data": "synthetic",
         "config": {"workload": f"3D variable-coefficient Poisson, {n}^3 Kuhn grid x 6 P1 tets, "
                                f"kappa=exp(GRF l=0.1 sigma=1 seed 0), kappa-Laplace assembly into CSR",
                    "elements": E, "nodes": N, "nnz": nnz, "values_stored": "f32",
@@ -595,7 +645,12 @@ def main():
     if not args.no_cpu:
         log(f"cpu baseline on {args.cpu_n}^3 ...")
         out["cpu_baseline"] = cpu_baseline(args.cpu_n)
-        log(f"cpu baseline {out['cpu_baseline']['value']} Melem/s")
+        cb = out["cpu_baseline"]
+        log(f"cpu baseline ({cb['cores']} threads, {args.cpu_n}^3): pattern {cb['all_threads']['pattern_s']} s, "
+            f"assembly {cb['all_threads']['assembly_s']} s = {cb['value']} Melem/s, condense "
+            f"{cb['all_threads']['condense_s']} s, PCG {cb['all_threads']['pcg_iterations']} its in "
+            f"{cb['all_threads'].get('solve_to_1e-8_s')} s; 1 thread ({cb['one_thread']['grid']}): "
+            f"{cb['one_thread']['assembly_melem_per_s']} Melem/s")
     print(json.dumps(out), flush=True)
 
 

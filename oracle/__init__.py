@@ -31,6 +31,19 @@ How each function is pinned (beyond restating the reference):
 * ``hyperelastic.*`` - neo-Hookean density of ``builtin.py:285-315`` with
   closed-form P and dP/dF; pinned by central finite differences (h = 1e-6)
   and by torch autograd of the energy, and psi(F = I) = 0.
+* ``torch_cpu.*`` - the same Map / Reduce / condensation / Jacobi-PCG in
+  float64 torch on the CPU, used only as the timed CPU baseline of
+  ``bench.py``; its assembly is checked against ``assemble.element_matrices``
+  + ``assemble.reduce`` (tests/test_oracle.py).
+* ``facet.*`` - boundary-facet loads of ``facet_assembler.py:123-277,
+  368-515``; pinned by closed forms on the sides of the unit square and cube
+  (constant loads, ``int x_0``, tractions, zero-sum gradient fluxes, nodal
+  values ``c h``) and by the reference's selection rule (tests/test_facet.py).
+* ``assemble.element_matrices_quadrature`` / ``load_vector_quadrature`` -
+  the quadrature-level Map for spatially varying integrands; pinned by
+  ``kappa(centroid) |T| grad . grad`` for linear ``kappa``, the exact
+  barycentric integrals of ``x_0 phi_a phi_b`` (order-3 rule) and of
+  ``x_0 phi_a`` (tests/test_spatial_forms.py).
 * ``elasticity.*`` - Voigt ``B^T C B`` of ``builtin.py:173-179`` and
   ``functional/elasticity.py:132-255``; pinned by the rigid-body modes
   (translations and rotations, which a transposed block fails for

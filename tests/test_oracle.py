@@ -312,3 +312,31 @@ def test_bicgstab_oracle_zero_diagonal_guard():
     x, it, rel = osv.bicgstab(A, b, rtol=1e-12, maxiter=5000)
     xd = osv.direct(A, b)
     assert rel <= 1e-12 and np.linalg.norm(x - xd) <= 1e-10 * np.linalg.norm(xd)
+
+
+def test_torch_cpu_baseline_matches_oracle():
+    """The torch-CPU baseline's Map + Reduce and condensation equal the numpy oracle."""
+    import torch
+    from oracle import torch_cpu as tc
+    pts, conn, bnd = kuhn_cube_arrays(4)
+    kappa = np.exp(np.random.default_rng(2).standard_normal(len(pts)))
+    u, v, pe = op.pattern_and_slots(conn, len(pts))
+    got = tc.assemble_kappa_laplace(torch.tensor(pts), torch.tensor(conn), torch.tensor(kappa), torch.tensor(pe),
+                                    len(u), chunk=100).numpy()
+    ref = oa.reduce(pe, oa.element_matrices(pts, conn, "tetra", "kappa_laplace", kappa=kappa), len(u))
+    assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+    kv, kr, kc, ni, fi = tc.condense(torch.tensor(got), torch.tensor(u.astype(np.int64)),
+                                     torch.tensor(v.astype(np.int64)), torch.tensor(bnd),
+                                     torch.ones(len(pts), dtype=torch.float64))
+    rv, rr, rc, rni, rfi, _ = oc.condense(ref, u, v, bnd, np.ones(len(pts)))
+    assert ni == rni and np.array_equal(kr.numpy(), rr) and np.array_equal(kc.numpy(), rc)
+    assert np.abs(kv.numpy() - rv).max() <= 1e-13 * np.abs(rv).max() and np.allclose(fi.numpy(), rfi)
+    import scipy.sparse as sp
+    A = sp.csr_matrix((rv, (rr, rc)), shape=(ni, ni))
+    rp = torch.zeros(ni + 1, dtype=torch.int64)
+    rp[1:] = torch.cumsum(torch.bincount(kr, minlength=ni), 0)
+    At = torch.sparse_csr_tensor(rp, kc, kv, (ni, ni))
+    d = torch.tensor(A.diagonal())
+    x, it = tc.jacobi_pcg(At, fi, 1.0 / d, rtol=1e-10)
+    xr, itr, _ = osv.jacobi_pcg(A, rfi, rtol=1e-10)
+    assert abs(it - itr) <= 1 and np.abs(x.numpy() - xr).max() <= 1e-8 * np.abs(xr).max()
