@@ -36,6 +36,14 @@ extern "C" {
 #define TM_FORM_LAPLACE 0       /* scale * grad u . grad v                      */
 #define TM_FORM_KAPPA_LAPLACE 1 /* kappa(x) grad u . grad v, kappa nodal        */
 #define TM_FORM_MASS 2          /* scale * u v                                  */
+/* OR-ed into the form of tm_asm_p1_matrix / the flags of tm_asm_p1_general:
+ * deterministic assembly.  Every element-matrix entry of a row is rounded to
+ * a multiple of a power of two 2^k per row (2^(k+44) above the row's largest
+ * |entry|, so each entry moves by at most 2^-44 of that maximum) before it is
+ * accumulated; the FP64 sums are then exact, so the values do not depend on
+ * the order of the on-chip atomics and repeat bit for bit.  Costs a second
+ * pass over the block's elements. */
+#define TM_ASM_DETERMINISTIC 0x100
 #define TM_LOAD_CONST 0         /* c * v                                        */
 #define TM_LOAD_NODAL 1         /* f(x) v, f nodal (interpolated at quadrature)  */
 
@@ -165,7 +173,7 @@ int tm_asm_p1_matrix(int form, int nb, int dim, const tm_plan_dev* plan, const d
  * D[(dim+1)^2] f64 host array, row-major, index 0 = value, 1.. = gradient.
  * No symmetry is assumed.  values [nnz] f32 or f64, every entry written once. */
 int tm_asm_p1_general(int nb, int dim, const tm_plan_dev* plan, const double* coords, const double* D,
-                      const tm_quad* quad, void* values, int value_dtype, void* stream);
+                      const tm_quad* quad, void* values, int value_dtype, int flags, void* stream);
 
 /* Spatially varying bilinear forms (ElementAssembler.__call__ with a forward
  * reading x, point data, their gradients or element data,

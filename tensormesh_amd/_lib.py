@@ -19,6 +19,7 @@ TM_OK, TM_ERR_ARG, TM_ERR_HIP, TM_ERR_LIMIT, TM_ERR_ALLOC, TM_ERR_NOTCONV = 0, -
 FORM_LAPLACE, FORM_KAPPA_LAPLACE, FORM_MASS = 0, 1, 2
 FORM_GENERAL = 3   # Python-side tag: dispatched to tm_asm_p1_general
 LOAD_CONST, LOAD_NODAL = 0, 1
+ASM_DETERMINISTIC = 0x100   # TM_ASM_DETERMINISTIC
 F32, F64, BF16 = 0, 1, 2
 
 c_void_p = ctypes.c_void_p
@@ -79,7 +80,7 @@ _SIGS = {
     "tm_asm_p1_matrix": (c_int, [c_int, c_int, c_int, ctypes.POINTER(PlanDev), c_void_p, c_void_p, c_int,
                                  c_double, ctypes.POINTER(Quad), c_void_p, c_int, c_void_p]),
     "tm_asm_p1_general": (c_int, [c_int, c_int, ctypes.POINTER(PlanDev), c_void_p, c_void_p,
-                                  ctypes.POINTER(Quad), c_void_p, c_int, c_void_p]),
+                                  ctypes.POINTER(Quad), c_void_p, c_int, c_int, c_void_p]),
     "tm_asm_p1_load": (c_int, [c_int, c_int, c_int, ctypes.POINTER(PlanDev), c_void_p, c_void_p, c_int,
                                c_double, ctypes.POINTER(Quad), c_void_p, c_void_p]),
     "tm_asm_p1_kappa_adjoint": (c_int, [c_int, c_int, ctypes.POINTER(PlanDev), c_void_p, c_void_p, c_int,

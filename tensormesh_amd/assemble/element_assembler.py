@@ -53,6 +53,10 @@ class _KappaLaplace(torch.autograd.Function):
 
 class ElementAssembler(nn.Module):
     value_dtype: torch.dtype = torch.float32
+    # bit-identical values on every call (TM_ASM_DETERMINISTIC: entries rounded
+    # to a per-block power-of-two grid, at most 2^-44 of the block's largest
+    # entry, so the on-chip sums are exact and order-independent)
+    deterministic: bool = False
 
     def __init__(self, topology: P1Topology, elements: Dict[str, torch.Tensor], points: torch.Tensor,
                  quadrature_order: int = 2, *args, **kwargs):
@@ -128,6 +132,8 @@ class ElementAssembler(nn.Module):
         cdt = _lib.dtype_code(coef) if coef is not None else _lib.F32
         if coef is not None:
             coef = coef.contiguous()
+        if self.deterministic:
+            form |= _lib.ASM_DETERMINISTIC
         _lib.check(_lib.lib().tm_asm_p1_matrix(
             form, self.topology.nb, self.topology.dim, plan.struct, _lib.ptr(points64), _lib.ptr(coef), cdt,
             float(scale), self._quad, _lib.ptr(values), _lib.dtype_code(values), _lib.stream_of(dev)))
@@ -140,7 +146,8 @@ class ElementAssembler(nn.Module):
         Dc = np.ascontiguousarray(D, dtype=np.float64)
         _lib.check(_lib.lib().tm_asm_p1_general(
             self.topology.nb, self.topology.dim, plan.struct, _lib.ptr(points64), _lib.ptr(Dc), self._quad,
-            _lib.ptr(values), _lib.dtype_code(values), _lib.stream_of(dev)))
+            _lib.ptr(values), _lib.dtype_code(values), _lib.ASM_DETERMINISTIC if self.deterministic else 0,
+            _lib.stream_of(dev)))
         return values
 
     def _launch_moments(self, points64, mom: torch.Tensor, out_dtype) -> torch.Tensor:

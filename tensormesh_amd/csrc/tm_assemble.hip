@@ -219,12 +219,47 @@ __device__ __forceinline__ void rowsum_diagonal(const Lds& L, int nown) {
   }
 }
 
+// How an element-matrix entry of owned row `row` (its LDS slot) reaches its
+// LDS accumulator.
+struct AddEmit {  // ds_add_f64 (the order of the adds varies from run to run)
+  __device__ __forceinline__ void operator()(const Lds&, int, double* a, double v) const { atomicAdd(a, v); }
+};
+// Deterministic mode.  Every entry that an accumulator of row a receives is
+// rounded to a multiple of 2^k_a, with 2^(k_a + 44) above the largest |entry|
+// M_a of that row; sums of up to 512 such terms are integers times 2^k_a below
+// 2^(k_a + 53), so every FP64 partial sum is exact and the result does not
+// depend on the order of the LDS atomics.  An entry moves by at most
+// 2^(k_a - 1) <= 2^-44 M_a.  A first pass over the block's records (same
+// arithmetic) finds M_a; the per-row word L.accb (unused by the matrix
+// kernels) holds it, then k_a.
+struct MaxEmit {  // first pass: row maxima (non-negative doubles order like their bit patterns)
+  __device__ __forceinline__ void operator()(const Lds& L, int row, double*, double v) const {
+    atomicMax((unsigned long long*)(L.accb + row), (unsigned long long)__double_as_longlong(fabs(v)));
+  }
+};
+struct GridEmit {  // second pass: entries rounded to the row's grid
+  __device__ __forceinline__ void operator()(const Lds& L, int row, double* a, double v) const {
+    const int k = *(const int*)(L.accb + row);
+    atomicAdd(a, ldexp(rint(ldexp(v, -k)), k));
+  }
+};
+
+// Row maxima -> grid exponents k_a (in place), between the two passes.
+template <int NT>
+__device__ __forceinline__ void row_grids(const Lds& L, int nown) {
+  for (int a = threadIdx.x; a < nown; a += NT) {
+    const unsigned long long bits = *(const unsigned long long*)(L.accb + a);
+    const int e = (int)((bits >> 52) & 0x7ffull) - 1023;  // M_a in [2^e, 2^(e+1))
+    *(int*)(L.accb + a) = bits ? e + 1 - 44 : 0;
+  }
+}
+
 // The element loop of one block: each lane evaluates records r = tid, tid + NT,
 // ... (nxt = the lane's first record, already loaded) and adds the owned rows
-// of each element matrix into L.acc with ds_add_f64.
-template <int NB, int FORM, int NT>
+// of each element matrix into L.acc through `em` (ds_add_f64).
+template <int NB, int FORM, int NT, typename Emit>
 __device__ __forceinline__ void asm_records(const Lds& L, const PlanArgs& p, int roff, int nrec, int nown, uint4 nxt,
-                                            double scale, const QuadArg& q) {
+                                            double scale, const QuadArg& q, const Emit& em) {
   constexpr int DIM = NB - 1;
   constexpr bool ROWSUM = FORM != 2;
   for (int r = threadIdx.x; r < nrec; r += NT) {
@@ -268,22 +303,22 @@ __device__ __forceinline__ void asm_records(const Lds& L, const PlanArgs& p, int
       // that the block's interior elements form whole waves)
       if (!ROWSUM) {
 #pragma unroll
-        for (int a = 0; a < NB; ++a) atomicAdd(L.acc + meta_off(m[a]) + meta_diag(m[a]), s * q.M[a * NB + a]);
+        for (int a = 0; a < NB; ++a) em(L, l[a], L.acc + meta_off(m[a]) + meta_diag(m[a]), s * q.M[a * NB + a]);
       }
 #pragma unroll
       for (int a = 0; a < NB; ++a)
 #pragma unroll
         for (int bb = a + 1; bb < NB; ++bb) {
           const double kab = FORM == 2 ? s * q.M[a * NB + bb] : s * dotc<DIM>(c[a], c[bb]);
-          atomicAdd(L.acc + meta_off(m[a]) + rec_rank<NB>(rc, a, bb), kab);
-          atomicAdd(L.acc + meta_off(m[bb]) + rec_rank<NB>(rc, bb, a), kab);
+          em(L, l[a], L.acc + meta_off(m[a]) + rec_rank<NB>(rc, a, bb), kab);
+          em(L, l[bb], L.acc + meta_off(m[bb]) + rec_rank<NB>(rc, bb, a), kab);
         }
       continue;
     }
     if (!ROWSUM) {
 #pragma unroll
       for (int a = 0; a < NB; ++a)
-        if (own[a]) atomicAdd(L.acc + meta_off(m[a]) + meta_diag(m[a]), s * q.M[a * NB + a]);
+        if (own[a]) em(L, l[a], L.acc + meta_off(m[a]) + meta_diag(m[a]), s * q.M[a * NB + a]);
     }
     // each symmetric pair once: K_ab = K_ba
 #pragma unroll
@@ -292,14 +327,15 @@ __device__ __forceinline__ void asm_records(const Lds& L, const PlanArgs& p, int
       for (int bb = a + 1; bb < NB; ++bb) {
         if (!(own[a] || own[bb])) continue;
         const double kab = FORM == 2 ? s * q.M[a * NB + bb] : s * dotc<DIM>(c[a], c[bb]);
-        if (own[a]) atomicAdd(L.acc + meta_off(m[a]) + rec_rank<NB>(rc, a, bb), kab);
-        if (own[bb]) atomicAdd(L.acc + meta_off(m[bb]) + rec_rank<NB>(rc, bb, a), kab);
+        if (own[a]) em(L, l[a], L.acc + meta_off(m[a]) + rec_rank<NB>(rc, a, bb), kab);
+        if (own[bb]) em(L, l[bb], L.acc + meta_off(m[bb]) + rec_rank<NB>(rc, bb, a), kab);
       }
   }
 }
 
 // FORM: 0 laplace, 1 kappa-laplace (nodal coef), 2 mass; all multiplied by scale.
-template <int NB, int FORM, typename CT, typename OT, int NT>
+// DET: deterministic mode (entries rounded to per-row grids, see MaxEmit / GridEmit).
+template <int NB, int FORM, typename CT, typename OT, int NT, bool DET>
 __global__ __launch_bounds__(NT) void k_asm_matrix(PlanArgs p, const double* __restrict__ X,
                                                   const CT* __restrict__ coef, double scale, QuadArg q,
                                                   OT* __restrict__ values) {
@@ -317,7 +353,16 @@ __global__ __launch_bounds__(NT) void k_asm_matrix(PlanArgs p, const double* __r
   stage<NT, DIM, CT, FORM == 1>(L, p.halo + hoff, p.meta + hoff, nacc, nh, nown, X, coef);
   __syncthreads();
 
-  asm_records<NB, FORM, NT>(L, p, roff, nrec, nown, nxt, scale, q);
+  if (DET) {
+    asm_records<NB, FORM, NT>(L, p, roff, nrec, nown, nxt, scale, q, MaxEmit{});
+    __syncthreads();
+    row_grids<NT>(L, nown);
+    __syncthreads();
+    nxt = threadIdx.x < nrec ? p.recs[roff + threadIdx.x] : make_uint4(0, 0, 0, 0);
+    asm_records<NB, FORM, NT>(L, p, roff, nrec, nown, nxt, scale, q, GridEmit{});
+  } else {
+    asm_records<NB, FORM, NT>(L, p, roff, nrec, nown, nxt, scale, q, AddEmit{});
+  }
   __syncthreads();
   if (ROWSUM) {
     rowsum_diagonal<NT>(L, nown);
@@ -342,19 +387,11 @@ struct GenD {
   double d[16];
 };
 
-template <int NB, typename OT, int NT>
-__global__ __launch_bounds__(NT) void k_asm_general(PlanArgs p, const double* __restrict__ X, GenD D, QuadArg q,
-                                                   OT* __restrict__ values) {
+// Records of one block for k_asm_general: every (owned a, b) entry through `em`.
+template <int NB, int NT, typename Emit>
+__device__ __forceinline__ void general_records(const Lds& L, const PlanArgs& p, int roff, int nrec, int nown,
+                                                const GenD& D, const QuadArg& q, const Emit& em) {
   constexpr int DIM = NB - 1, DD = DIM + 1;
-  extern __shared__ __align__(16) char smem[];
-  const int b = tmx::xcd_remap(blockIdx.x, p.nblk);
-  const int4 bi0 = *(const int4*)(p.blk + 8 * b);
-  const int4 bi1 = *(const int4*)(p.blk + 8 * b + 4);
-  const int hoff = bi0.x, nh = bi0.y, nown = bi0.z, roff = bi0.w;
-  const int nrec = bi1.x, runoff = bi1.y, nrun = bi1.z, nacc = bi1.w;
-  const Lds L = carve(smem, nacc, nh, nown);
-  stage<NT, DIM, float, false>(L, p.halo + hoff, p.meta + hoff, nacc, nh, nown, X, (const float*)nullptr);
-  __syncthreads();
   for (int r = threadIdx.x; r < nrec; r += NT) {
     const uint4 rc = p.recs[roff + r];
     const uint64_t lo = (uint64_t)rc.x | ((uint64_t)rc.y << 32);
@@ -402,9 +439,33 @@ __global__ __launch_bounds__(NT) void k_asm_general(PlanArgs p, const double* __
         for (int j = 0; j < DIM; ++j) gg += ta[j] * c[bb][j];
         const double v = (a == bb ? mdiag : moff) + ua + vb[bb] + sgg * gg;
         const int slot = bb == a ? meta_diag(m[a]) : rec_rank<NB>(rc, a, bb);
-        atomicAdd(L.acc + base + slot, v);
+        em(L, l[a], L.acc + base + slot, v);
       }
     }
+  }
+}
+
+template <int NB, typename OT, int NT, bool DET>
+__global__ __launch_bounds__(NT) void k_asm_general(PlanArgs p, const double* __restrict__ X, GenD D, QuadArg q,
+                                                   OT* __restrict__ values) {
+  constexpr int DIM = NB - 1;
+  extern __shared__ __align__(16) char smem[];
+  const int b = tmx::xcd_remap(blockIdx.x, p.nblk);
+  const int4 bi0 = *(const int4*)(p.blk + 8 * b);
+  const int4 bi1 = *(const int4*)(p.blk + 8 * b + 4);
+  const int hoff = bi0.x, nh = bi0.y, nown = bi0.z, roff = bi0.w;
+  const int nrec = bi1.x, runoff = bi1.y, nrun = bi1.z, nacc = bi1.w;
+  const Lds L = carve(smem, nacc, nh, nown);
+  stage<NT, DIM, float, false>(L, p.halo + hoff, p.meta + hoff, nacc, nh, nown, X, (const float*)nullptr);
+  __syncthreads();
+  if (DET) {
+    general_records<NB, NT>(L, p, roff, nrec, nown, D, q, MaxEmit{});
+    __syncthreads();
+    row_grids<NT>(L, nown);
+    __syncthreads();
+    general_records<NB, NT>(L, p, roff, nrec, nown, D, q, GridEmit{});
+  } else {
+    general_records<NB, NT>(L, p, roff, nrec, nown, D, q, AddEmit{});
   }
   __syncthreads();
   const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
@@ -697,21 +758,32 @@ int launch_plan_kernel(K kernel, int nt, const tm_plan_dev* plan, hipStream_t st
     { constexpr int NT = 256; return launch_plan_kernel(KERNEL_NT, NT, plan, __VA_ARGS__); }                   \
   } while (0)
 
-template <int NB, typename OT>
+template <int NB, typename OT, bool DET>
 int dispatch_matrix(int form, const tm_plan_dev* plan, const double* X, const void* coef, int cdt,
                     double scale, QuadArg q, OT* values, hipStream_t st) {
   PlanArgs p = plan_args(plan);
   if (form == TM_FORM_LAPLACE)
-    TM_BY_THREADS(plan, (k_asm_matrix<NB, 0, float, OT, NT>), st, p, X, (const float*)nullptr, scale, q, values);
+    TM_BY_THREADS(plan, (k_asm_matrix<NB, 0, float, OT, NT, DET>), st, p, X, (const float*)nullptr, scale, q,
+                  values);
   if (form == TM_FORM_MASS)
-    TM_BY_THREADS(plan, (k_asm_matrix<NB, 2, float, OT, NT>), st, p, X, (const float*)nullptr, scale, q, values);
+    TM_BY_THREADS(plan, (k_asm_matrix<NB, 2, float, OT, NT, DET>), st, p, X, (const float*)nullptr, scale, q,
+                  values);
   if (form == TM_FORM_KAPPA_LAPLACE) {
     TM_REQUIRE(coef != nullptr, TM_ERR_ARG, "kappa-Laplace needs a nodal coefficient");
     if (cdt == TM_F32)
-      TM_BY_THREADS(plan, (k_asm_matrix<NB, 1, float, OT, NT>), st, p, X, (const float*)coef, scale, q, values);
-    TM_BY_THREADS(plan, (k_asm_matrix<NB, 1, double, OT, NT>), st, p, X, (const double*)coef, scale, q, values);
+      TM_BY_THREADS(plan, (k_asm_matrix<NB, 1, float, OT, NT, DET>), st, p, X, (const float*)coef, scale, q,
+                    values);
+    TM_BY_THREADS(plan, (k_asm_matrix<NB, 1, double, OT, NT, DET>), st, p, X, (const double*)coef, scale, q,
+                  values);
   }
   TM_REQUIRE(false, TM_ERR_ARG, "unknown form %d", form);
+}
+
+template <int NB, typename OT>
+int dispatch_matrix_det(int form, bool det, const tm_plan_dev* plan, const double* X, const void* coef, int cdt,
+                        double scale, QuadArg q, OT* values, hipStream_t st) {
+  if (det) return dispatch_matrix<NB, OT, true>(form, plan, X, coef, cdt, scale, q, values, st);
+  return dispatch_matrix<NB, OT, false>(form, plan, X, coef, cdt, scale, q, values, st);
 }
 
 }  // namespace
@@ -727,16 +799,20 @@ extern "C" int tm_asm_p1_matrix(int form, int nb, int dim, const tm_plan_dev* pl
              "quadrature moments must be invariant under vertex relabelling");
   hipStream_t st = (hipStream_t)stream;
   QuadArg q = quad_arg(quad);
+  const bool det = (form & TM_ASM_DETERMINISTIC) != 0;
+  form &= ~TM_ASM_DETERMINISTIC;
   if (value_dtype == TM_F32) {
-    if (nb == 3) return dispatch_matrix<3>(form, plan, coords, coef, coef_dtype, scale, q, (float*)values, st);
-    return dispatch_matrix<4>(form, plan, coords, coef, coef_dtype, scale, q, (float*)values, st);
+    if (nb == 3)
+      return dispatch_matrix_det<3>(form, det, plan, coords, coef, coef_dtype, scale, q, (float*)values, st);
+    return dispatch_matrix_det<4>(form, det, plan, coords, coef, coef_dtype, scale, q, (float*)values, st);
   }
-  if (nb == 3) return dispatch_matrix<3>(form, plan, coords, coef, coef_dtype, scale, q, (double*)values, st);
-  return dispatch_matrix<4>(form, plan, coords, coef, coef_dtype, scale, q, (double*)values, st);
+  if (nb == 3)
+    return dispatch_matrix_det<3>(form, det, plan, coords, coef, coef_dtype, scale, q, (double*)values, st);
+  return dispatch_matrix_det<4>(form, det, plan, coords, coef, coef_dtype, scale, q, (double*)values, st);
 }
 
 extern "C" int tm_asm_p1_general(int nb, int dim, const tm_plan_dev* plan, const double* coords, const double* D,
-                                 const tm_quad* quad, void* values, int value_dtype, void* stream) {
+                                 const tm_quad* quad, void* values, int value_dtype, int flags, void* stream) {
   TM_REQUIRE(plan && quad && coords && D && values, TM_ERR_ARG, "null argument");
   TM_REQUIRE((nb == 3 && dim == 2) || (nb == 4 && dim == 3), TM_ERR_ARG,
              "only P1 triangles (nb=3, dim=2) and tetrahedra (nb=4, dim=3) are supported");
@@ -748,12 +824,20 @@ extern "C" int tm_asm_p1_general(int nb, int dim, const tm_plan_dev* plan, const
   GenD g{};
   for (int i = 0; i < (dim + 1) * (dim + 1); ++i) g.d[i] = D[i];
   const PlanArgs p = plan_args(plan);
-  if (value_dtype == TM_F32) {
-    if (nb == 3) TM_BY_THREADS(plan, (k_asm_general<3, float, NT>), st, p, coords, g, q, (float*)values);
-    TM_BY_THREADS(plan, (k_asm_general<4, float, NT>), st, p, coords, g, q, (float*)values);
+  if (flags & TM_ASM_DETERMINISTIC) {
+    if (value_dtype == TM_F32) {
+      if (nb == 3) TM_BY_THREADS(plan, (k_asm_general<3, float, NT, true>), st, p, coords, g, q, (float*)values);
+      TM_BY_THREADS(plan, (k_asm_general<4, float, NT, true>), st, p, coords, g, q, (float*)values);
+    }
+    if (nb == 3) TM_BY_THREADS(plan, (k_asm_general<3, double, NT, true>), st, p, coords, g, q, (double*)values);
+    TM_BY_THREADS(plan, (k_asm_general<4, double, NT, true>), st, p, coords, g, q, (double*)values);
   }
-  if (nb == 3) TM_BY_THREADS(plan, (k_asm_general<3, double, NT>), st, p, coords, g, q, (double*)values);
-  TM_BY_THREADS(plan, (k_asm_general<4, double, NT>), st, p, coords, g, q, (double*)values);
+  if (value_dtype == TM_F32) {
+    if (nb == 3) TM_BY_THREADS(plan, (k_asm_general<3, float, NT, false>), st, p, coords, g, q, (float*)values);
+    TM_BY_THREADS(plan, (k_asm_general<4, float, NT, false>), st, p, coords, g, q, (float*)values);
+  }
+  if (nb == 3) TM_BY_THREADS(plan, (k_asm_general<3, double, NT, false>), st, p, coords, g, q, (double*)values);
+  TM_BY_THREADS(plan, (k_asm_general<4, double, NT, false>), st, p, coords, g, q, (double*)values);
 }
 
 extern "C" int tm_asm_p1_moments(int nb, int dim, const tm_plan_dev* plan, const double* coords,

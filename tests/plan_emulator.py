@@ -26,8 +26,22 @@ def vertex_order(rec, nb):
     return [(code >> (2 * a)) & 3 for a in range(nb)]
 
 
-def emulate_matrix(topo, arrs, element_matrix):
-    """``element_matrix(e, x[nb, d]) -> [nb, nb]``; returns the CSR values."""
+def det_grid(values):
+    """The deterministic mode's grid step for a row whose entries are
+    ``values``: 2^k with M = max|v| in [2^e, 2^(e+1)), k = e + 1 - 44."""
+    M = float(np.max(np.abs(values))) if len(values) else 0.0
+    if not M > 0.0:
+        return 1.0
+    e = int(np.frexp(M)[1]) - 1
+    return float(np.ldexp(1.0, e + 1 - 44))
+
+
+def emulate_matrix(topo, arrs, element_matrix, det=False, shuffle=None):
+    """``element_matrix(e, x[nb, d]) -> [nb, nb]``; returns the CSR values.
+
+    ``det``: the deterministic mode (entries rounded to the block's grid step
+    before they are summed); ``shuffle``: a numpy Generator that permutes the
+    order in which each block's entries are added (the atomics' order)."""
     nb = topo.nb
     blk = arrs["blk"].reshape(-1, 8)
     recs = arrs["recs"].reshape(-1, 4)
@@ -42,6 +56,7 @@ def emulate_matrix(topo, arrs, element_matrix):
         assert (halo[:nown] >= 0).all()
         assert (meta[nown:] == 0).all()
         acc = np.zeros(nacc)
+        contrib = []                                     # (accumulator index, value) in issue order
         for r in range(nrec):
             loc, rank = decode(recs[roff + r], nb)
             assert max(loc) < nh
@@ -57,15 +72,24 @@ def emulate_matrix(topo, arrs, element_matrix):
                 m = int(meta[loc[a]])
                 base, dr, ln = m & 0xFFFF, (m >> 16) & 0xFF, m >> 24
                 assert dr < ln and base + ln <= nacc
-                acc[base + dr] += K[a, a]
+                contrib.append((base + dr, K[a, a], loc[a]))
                 j = 0
                 for bb in range(nb):
                     if bb == a:
                         continue
                     k = rank(a, j)
                     assert k < ln and k != dr
-                    acc[base + k] += K[a, bb]
+                    contrib.append((base + k, K[a, bb], loc[a]))
                     j += 1
+        if shuffle is not None:
+            contrib = [contrib[i] for i in shuffle.permutation(len(contrib))]
+        if det:
+            rows = {}
+            for _, v, rw in contrib:
+                rows.setdefault(rw, []).append(v)
+            step = {rw: det_grid(vs) for rw, vs in rows.items()}
+        for i, v, rw in contrib:
+            acc[i] += np.rint(v / step[rw]) * step[rw] if det else v
         for q in range(nrun):
             vs, as_, ln, _ = (int(v) for v in runs[runoff + q])
             assert as_ + ln <= nacc

@@ -88,9 +88,11 @@ def test_reaction_diffusion_form_runs_the_general_kernel(cuda):
     assert np.abs(K.values.cpu().numpy() - ref).max() < 1e-12
 
 
-def test_unrecognised_form_uses_generic_path(cuda):
+def test_spatially_varying_form_runs_the_moments_kernel(cuda):
     """A coefficient that varies in space (here through the coordinates) is not a
-    constant U^T D V: the generic torch path evaluates it on the device in FP64."""
+    constant U^T D V: it is probed at the quadrature points and assembled by the
+    moments kernel (FP32 values: 1e-6 row-scaled; FP64 values: 1e-12); a forward
+    that is not bilinear runs the generic torch path in FP64."""
     m, pts, conn, bnd, et = _mesh("tri", 3)
 
     class Variable(tm.ElementAssembler):
@@ -99,10 +101,26 @@ def test_unrecognised_form_uses_generic_path(cuda):
 
     asm = Variable.from_mesh(m).to(cuda)
     assert asm._classify(asm.forward, {"x": m.points}, {}, {}) is None
-    K = asm(m.points.to(cuda))
     u, v, pe = op.pattern_and_slots(conn, len(pts))
     ref = oa.reduce(pe, oa.element_matrices(pts, conn, et, "kappa_laplace", kappa=1.0 + pts[:, 0]), len(u))
+    K = asm(m.points.to(cuda))
+    assert K.values.dtype == torch.float32
+    scale = np.zeros(len(pts))
+    np.maximum.at(scale, u, np.abs(ref))
+    assert (np.abs(K.values.double().cpu().numpy() - ref) / scale[u]).max() <= 1e-6
+    asm.value_dtype = torch.float64
+    K = asm(m.points.to(cuda))
     assert np.abs(K.values.cpu().numpy() - ref).max() < 1e-12
+
+    class NotBilinear(tm.ElementAssembler):
+        def forward(self, gradu, gradv, x):
+            return (1.0 + x[0]) * (gradu @ gradv) * (1.0 + 1e-3 * gradu[0] ** 2)
+
+    ref2 = oa.reduce(pe, oa.element_matrices_quadrature(
+        pts, conn, et, lambda uu, gu, vv, gv, at: (1.0 + at["x"][..., 0]) * (gu * gv).sum(-1)
+        * (1.0 + 1e-3 * gu[..., 0] ** 2)), len(u))
+    K = NotBilinear.from_assembler(asm)(m.points.to(cuda))
+    assert K.values.dtype == torch.float64 and np.abs(K.values.cpu().numpy() - ref2).max() < 1e-12
 
 
 @pytest.mark.parametrize("kind", ["tri", "tet"])

@@ -136,3 +136,26 @@ def test_assembly_rejects_label_dependent_quadrature():
     rc = L.tm_asm_p1_matrix(_lib.FORM_KAPPA_LAPLACE, 4, 3, ctypes.byref(plan), fake, fake, _lib.F32, 1.0,
                             ctypes.byref(good), fake, _lib.F32, None)
     assert rc == 0
+
+
+@pytest.mark.parametrize("name,arrays,et", list(_meshes()))
+def test_deterministic_mode_is_order_independent(name, arrays, et):
+    """The deterministic mode's arithmetic on the CPU: with entries rounded to
+    the row's power-of-two grid, every order of the additions gives the same
+    bits, and the values stay within a few 2^-44 of the row's largest entry of
+    the float64 oracle; without the rounding the orders disagree in the last
+    bits."""
+    pts, conn, _ = arrays
+    topo = P1Topology(pts, conn, len(pts), et)
+    arrs, info = topo.host_plan("scalar", max_rows=33)
+    kappa = np.exp(2.0 * np.random.default_rng(4).standard_normal(len(pts)))
+    f = _kernel_element_matrix(topo, "kappa_laplace", kappa)
+    u, v, pe = op.pattern_and_slots(conn, len(pts))
+    ref = oa.reduce(pe, oa.element_matrices(pts, conn, et, "kappa_laplace", kappa=kappa), len(u))
+    runs = [emulate_matrix(topo, arrs, f, det=True, shuffle=np.random.default_rng(s)) for s in range(3)]
+    assert all(np.array_equal(runs[0], r) for r in runs[1:])
+    scale = np.zeros(len(pts))
+    np.maximum.at(scale, u, np.abs(ref))
+    assert (np.abs(runs[0] - ref) / scale[u]).max() <= 64 * 2.0 ** -44
+    plain = [emulate_matrix(topo, arrs, f, shuffle=np.random.default_rng(s)) for s in range(3)]
+    assert not all(np.array_equal(plain[0], r) for r in plain[1:])
