@@ -93,8 +93,9 @@ class ElementAssembler(nn.Module):
 
     @property
     def projector(self):
-        """The reference's element->edge indices ``P_E`` (int64, element-major)."""
-        return torch.from_numpy(self.topology.element_slots())
+        """The reference's element->edge indices ``P_E`` (int64, element-major), on the
+        assembler's device (built there when the pattern was)."""
+        return self.topology.element_slots_on(self.edges.device)
 
     # ---- construction -----------------------------------------------------------
     @classmethod

@@ -74,9 +74,15 @@ class DevicePlan:
 
 
 class P1Topology:
-    """Pattern and plans of one P1 simplex connectivity (``triangle`` or ``tetra``)."""
+    """Pattern and plans of one P1 simplex connectivity (``triangle`` or ``tetra``).
 
-    def __init__(self, points: np.ndarray, conn: np.ndarray, n_points: int, element_type: str):
+    With ``device_conn`` (the connectivity on a GPU) the canonical pattern is
+    built on that GPU (:func:`device_pattern`) and the matrices' ``edges`` are
+    formed there; the host receives a copy of ``row_ptr`` / ``col_idx`` only
+    when a block plan is built (the plan builder runs on the host)."""
+
+    def __init__(self, points: np.ndarray, conn: np.ndarray, n_points: int, element_type: str,
+                 device_conn: Optional[torch.Tensor] = None):
         if element_type not in ELEMENT_NB:
             raise NotImplementedError(f"element type {element_type!r}: the HIP path covers P1 "
                                       f"'triangle' and 'tetra' meshes")
@@ -89,6 +95,22 @@ class P1Topology:
             f"connectivity must be [n_elements, {self.nb}], got {self.conn.shape}"
         self.n_elements = self.conn.shape[0]
         self.points = np.ascontiguousarray(points, dtype=np.float64)
+        self._host_plans: Dict[tuple, tuple] = {}
+        self._dev_plans: Dict[tuple, DevicePlan] = {}
+        self._dev_csr: Dict[torch.device, Tuple[torch.Tensor, torch.Tensor]] = {}
+        self._dev_edges: Dict[torch.device, torch.Tensor] = {}
+        self._slots: Optional[np.ndarray] = None
+        self._dev_slots: Dict[torch.device, torch.Tensor] = {}
+        self._row_ptr = self._col_idx = self._edges = None
+        self.pattern_device: Optional[torch.device] = None
+        if device_conn is not None and device_conn.is_cuda:
+            rp_d, ci_d, work = device_pattern(device_conn, self.n_points)
+            dev = device_conn.device
+            self.pattern_device = dev
+            self._dev_csr[dev] = (rp_d, ci_d)
+            self._dev_conn, self._dev_work = device_conn.detach().to(torch.int64).contiguous(), work
+            self.nnz = int(ci_d.numel())
+            return
         L = _lib.lib()
         rp = np.empty(self.n_points + 1, dtype=np.int32)
         _lib.check(L.tm_pattern_rowptr(_lib.ptr(self.conn), self.n_elements, self.nb, self.n_points,
@@ -96,16 +118,31 @@ class P1Topology:
         ci = np.empty(int(rp[-1]), dtype=np.int32)
         _lib.check(L.tm_pattern_colidx(_lib.ptr(self.conn), self.n_elements, self.nb, self.n_points,
                                        _lib.ptr(rp), _lib.ptr(ci)))
-        self.row_ptr = rp
-        self.col_idx = ci
+        self._row_ptr, self._col_idx = rp, ci
         self.nnz = int(rp[-1])
-        row = np.repeat(np.arange(self.n_points, dtype=np.int64), np.diff(rp))
-        self.edges = torch.from_numpy(np.stack([row, ci.astype(np.int64)], 0))
-        self._host_plans: Dict[tuple, tuple] = {}
-        self._dev_plans: Dict[tuple, DevicePlan] = {}
-        self._dev_csr: Dict[torch.device, Tuple[torch.Tensor, torch.Tensor]] = {}
-        self._dev_edges: Dict[torch.device, torch.Tensor] = {}
-        self._slots: Optional[np.ndarray] = None
+
+    @property
+    def row_ptr(self) -> np.ndarray:
+        if self._row_ptr is None:
+            self._row_ptr = self._dev_csr[self.pattern_device][0].cpu().numpy()
+        return self._row_ptr
+
+    @property
+    def col_idx(self) -> np.ndarray:
+        if self._col_idx is None:
+            self._col_idx = self._dev_csr[self.pattern_device][1].cpu().numpy()
+        return self._col_idx
+
+    @property
+    def edges(self) -> torch.Tensor:
+        """int64 ``[2, nnz]`` edges on the host."""
+        if self._edges is None:
+            if self.pattern_device is not None:
+                self._edges = self.edges_on(self.pattern_device).cpu()
+            else:
+                row = np.repeat(np.arange(self.n_points, dtype=np.int64), np.diff(self.row_ptr))
+                self._edges = torch.from_numpy(np.stack([row, self.col_idx.astype(np.int64)], 0))
+        return self._edges
 
     # ---- pattern ------------------------------------------------------------
     def edges_on(self, device) -> torch.Tensor:
@@ -115,7 +152,13 @@ class P1Topology:
         if device.type == "cpu":
             return self.edges
         if device not in self._dev_edges:
-            self._dev_edges[device] = self.edges.to(device)
+            if device == self.pattern_device:
+                rp, ci = self._dev_csr[device]
+                row = torch.repeat_interleave(torch.arange(self.n_points, device=device),
+                                              (rp[1:] - rp[:-1]).long(), output_size=self.nnz)
+                self._dev_edges[device] = torch.stack([row, ci.long()], 0)
+            else:
+                self._dev_edges[device] = self.edges.to(device)
         return self._dev_edges[device]
 
     def csr(self, device) -> Tuple[torch.Tensor, torch.Tensor]:
@@ -125,8 +168,21 @@ class P1Topology:
                                      torch.from_numpy(self.col_idx).to(device))
         return self._dev_csr[device]
 
+    def element_slots_on(self, device) -> torch.Tensor:
+        """``P_E`` on ``device`` (built there when the pattern was built on it)."""
+        device = torch.device(device)
+        if device == self.pattern_device:
+            if device not in self._dev_slots:
+                rp, ci = self._dev_csr[device]
+                self._dev_slots[device] = device_element_slots(self._dev_conn, rp, ci, self._dev_work,
+                                                               self.n_points)
+            return self._dev_slots[device]
+        return torch.from_numpy(self.element_slots()).to(device)
+
     def element_slots(self) -> np.ndarray:
         """The reference's ``P_E``: int64 ``[E * nb * nb]``, element-major then (i, j)."""
+        if self._slots is None and self.pattern_device is not None:
+            self._slots = self.element_slots_on(self.pattern_device).cpu().numpy()
         if self._slots is None:
             s = np.empty(self.n_elements * self.nb * self.nb, dtype=np.int64)
             _lib.check(_lib.lib().tm_element_slots(_lib.ptr(self.conn), self.n_elements, self.nb,
@@ -195,7 +251,7 @@ def topology_for(points: torch.Tensor, conn: torch.Tensor, element_type: str) ->
     hit = _KEEP.get(key)
     if hit is None:
         topo = P1Topology(points.detach().cpu().double().numpy(), conn.detach().cpu().numpy(),
-                          points.shape[0], element_type)
+                          points.shape[0], element_type, device_conn=conn if conn.is_cuda else None)
         # the cache holds the connectivity tensor so its address stays unique
         _KEEP[key] = (topo, conn)
         return topo
