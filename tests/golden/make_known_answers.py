@@ -18,6 +18,12 @@ against (no reference code is imported or run).
 * ``tet_order2_rule``: the 4-point rule of tensormesh/element/quadrature.py:143,152.
 * ``kuhn_nnz``: canonical-CSR sizes of the n^3 Kuhn grids, closed form
   N + 2 (3 n (n+1)^2 + 3 n^2 (n+1) + n^3).
+* ``block_coo_naive``: the two 2x2 blocks of tests/sparse/test_block_coo.py:10-24
+  and the 4x4 dense matrix they expand to.
+* ``block_coo_random``: seeded instances of the dense-placement check of
+  tests/sparse/test_block_coo.py:27-44 (block size, shape, block positions,
+  block values and the dense matrix with every block added at
+  ``[i*h:(i+1)*h, j*h:(j+1)*h]``).
 
     python tests/golden/make_known_answers.py
 """
@@ -64,6 +70,25 @@ def main():
     }
     out["kuhn_nnz"] = {str(n): int((n + 1) ** 3 + 2 * (3 * n * (n + 1) ** 2 + 3 * n * n * (n + 1) + n ** 3))
                        for n in (64, 96, 128)}
+    out["block_coo_naive"] = {
+        "source": "tests/sparse/test_block_coo.py:10-24",
+        "edata": [np.ones((2, 2)).tolist(), (2 * np.ones((2, 2))).tolist()], "row": [0, 1], "col": [0, 1],
+        "shape": [2, 2],
+        "dense": [[1, 1, 0, 0], [1, 1, 0, 0], [0, 0, 2, 2], [0, 0, 2, 2]],
+    }
+    rng = np.random.default_rng(20240605)
+    cases = []
+    for h, r in ((1, 7), (3, 11), (5, 4)):
+        nblk = int(rng.integers(1, r * r))
+        row = rng.integers(0, r, nblk)
+        col = rng.integers(0, r, nblk)               # repeated positions are added, as scipy's random allows
+        edata = rng.uniform(-1.0, 1.0, (nblk, h, h)).round(6)
+        dense = np.zeros((r * h, r * h))
+        for i, j, v in zip(row, col, edata):
+            dense[i * h:(i + 1) * h, j * h:(j + 1) * h] += v
+        cases.append({"h": h, "shape": [r, r], "row": row.tolist(), "col": col.tolist(),
+                      "edata": edata.tolist(), "dense": dense.tolist()})
+    out["block_coo_random"] = {"source": "tests/sparse/test_block_coo.py:27-44", "cases": cases}
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "known_answers.json")
     with open(path, "w") as fh:
         json.dump(out, fh, indent=1)

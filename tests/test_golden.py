@@ -51,3 +51,19 @@ def test_condenser_fixture():
     assert np.allclose(fi, g["f_inner_dirichlet_update"])
     full = oc.recover(osv.direct(osv.csr(kv, kr, kc, ni), fi), mask, np.array([0.0, 0.0, 0.0, 0.0]))
     assert np.allclose(full[mask], 0.0)
+
+
+def test_block_coo_fixtures():
+    """SparseMatrix.from_block_coo (sparse/matrix.py:325-375) against the
+    reference's block-COO fixtures: the two-block case and dense placement."""
+    import torch
+    import tensormesh_amd as tm
+    g = GOLD["block_coo_naive"]
+    m = tm.SparseMatrix.from_block_coo(torch.tensor(g["edata"]), torch.tensor(g["row"]), torch.tensor(g["col"]),
+                                       tuple(g["shape"]))
+    assert torch.equal(m.to_dense(), torch.tensor(g["dense"], dtype=torch.float32))
+    for c in GOLD["block_coo_random"]["cases"]:
+        m = tm.SparseMatrix.from_block_coo(torch.tensor(c["edata"], dtype=torch.float64), torch.tensor(c["row"]),
+                                           torch.tensor(c["col"]), tuple(c["shape"]))
+        assert m.shape == (c["shape"][0] * c["h"], c["shape"][1] * c["h"])
+        assert torch.allclose(m.to_dense(), torch.tensor(c["dense"], dtype=torch.float64), atol=1e-12, rtol=0)

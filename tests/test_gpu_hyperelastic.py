@@ -61,6 +61,13 @@ def test_tangent_blocks_and_bsr_products(cuda, vdt):
     ref = tm.SparseMatrix.from_block_coo(K.values.view(-1, 3, 3), model.edges[0], model.edges[1],
                                          (len(pts), len(pts)))
     assert torch.equal(ref.row, K.row) and torch.equal(ref.col, K.col)
+    # the reference's dense-placement check of block COO (tests/sparse/test_block_coo.py:27-44)
+    # on the assembled tangent: block k lands at [3 eu_k : 3 eu_k + 3, 3 ev_k : 3 ev_k + 3]
+    N = len(pts)
+    dense = np.zeros((3 * N, 3 * N))
+    for i, j, blk in zip(eu, ev, got):
+        dense[3 * i:3 * i + 3, 3 * j:3 * j + 3] += blk
+    assert np.array_equal(K.to_dense().double().cpu().numpy(), dense)
     x = torch.randn(3 * len(pts), dtype=torch.float64, device=cuda)
     y = (K @ x).cpu().numpy()
     yr = K.to_scipy_coo().tocsr() @ x.cpu().numpy()

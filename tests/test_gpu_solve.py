@@ -139,4 +139,16 @@ def test_quickstart_on_gpu(cuda):
     ue = (torch.sin(math.pi * x) * torch.sin(math.pi * y)).to(cuda)
     err = ((u - ue).norm() / ue.norm()).item()
     assert K.values.is_cuda and u.is_cuda
-    assert abs(err - 6.1866e-3) < 2e-5
+    # the float64 oracle on the same mesh (SciPy LU, the reference's CPU solve)
+    pts, conn, bnd = m.points.cpu().numpy(), m.cells["triangle"].cpu().numpy(), m.boundary_mask.cpu().numpy()
+    eu, ev, pe = op.pattern_and_slots(conn, len(pts))
+    Kr = oa.reduce(pe, oa.element_matrices(pts, conn, "triangle", "laplace"), len(eu))
+    br = oa.load_vector(pts, conn, "triangle", len(pts), f=f.cpu().numpy())
+    kv, kr, kc, ni, fi, _ = oc.condense(Kr, eu, ev, bnd, br)
+    ur = oc.recover(osv.direct(osv.csr(kv, kr, kc, ni), fi), bnd)
+    ue_np = ue.cpu().numpy()
+    err_ref = np.linalg.norm(ur - ue_np) / np.linalg.norm(ue_np)
+    assert abs(err - err_ref) <= 1e-3 * err_ref          # FP32 values on the GPU
+    # README.md:132-133 reports n = 431, nnz = 2859, error 3.135e-3 on gmsh's mesh;
+    # the substitute mesh of the same edge length: 429 DOFs, nnz 2839, 3.243e-3
+    assert ni == 429 and len(kv) == 2839 and abs(err_ref - 3.2429e-3) < 1e-7
