@@ -223,38 +223,51 @@ __global__ __launch_bounds__(kThreads) void k_nh_assemble(PlanArgs p, const doub
       const double psi = 0.5 * mu * (f2 - 3.0) - mu * k.lnJ + 0.5 * lam * k.lnJ * k.lnJ;
       en += 0.25 * owned * k.vol * psi;
     }
+    if (WR) {
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      if (l[a] >= nown) continue;
-      if (WR) {
+      for (int a = 0; a < 4; ++a) {
+        if (l[a] >= nown) continue;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
           const double pg = mu * (k.F[i][0] * k.g[a][0] + k.F[i][1] * k.g[a][1] + k.F[i][2] * k.g[a][2]) + s * k.w[a][i];
           atomicAdd(L.accr + 3 * l[a] + i, k.vol * pg);
         }
       }
-      if (WK) {
-        // off-diagonal blocks only: the diagonal block is minus their row sum
-        // (sum_b g_b = 0, so sum_b K_ab = 0), formed after the loop
-        const uint32_t m = L.ms[l[a]];
-        const int base = (int)(m & 0xffffu);
-        int j = 0;
+    }
+    if (WK) {
+      // Off-diagonal blocks only: the diagonal block is minus their row sum
+      // (sum_b g_b = 0, so sum_b K_ab = 0), formed after the loop.  K_ba is
+      // the transpose of K_ab, so each unordered pair is evaluated once and
+      // added to row a as K_ab and to row b as K_ab^T (each where owned).
+      const double muv = mu * k.vol, lamv = lam * k.vol, sv = s * k.vol;
 #pragma unroll
-        for (int bb = 0; bb < 4; ++bb) {
-          if (bb == a) continue;
-          const int slot = base + rec_rank4(lo, hi, a, j);
-          ++j;
-          const double gg = mu * (k.g[a][0] * k.g[bb][0] + k.g[a][1] * k.g[bb][1] + k.g[a][2] * k.g[bb][2]);
-          double* blk = L.acc + 9 * slot;
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int bb = a + 1; bb < 4; ++bb) {
+          const bool oa = l[a] < nown, ob = l[bb] < nown;
+          if (!(oa || ob)) continue;
+          const double gg = muv * (k.g[a][0] * k.g[bb][0] + k.g[a][1] * k.g[bb][1] + k.g[a][2] * k.g[bb][2]);
+          double v[3][3];
 #pragma unroll
           for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int kk = 0; kk < 3; ++kk) {
-              const double v = (i == kk ? gg : 0.0) + lam * k.w[a][i] * k.w[bb][kk] - s * k.w[a][kk] * k.w[bb][i];
-              atomicAdd(blk + 3 * i + kk, k.vol * v);
-            }
+            for (int kk = 0; kk < 3; ++kk)
+              v[i][kk] = (i == kk ? gg : 0.0) + lamv * k.w[a][i] * k.w[bb][kk] - sv * k.w[a][kk] * k.w[bb][i];
+          if (oa) {  // rank of bb in row a: bb is the (bb-1)-th vertex other than a
+            double* blk = L.acc + 9 * ((int)(L.ms[l[a]] & 0xffffu) + rec_rank4(lo, hi, a, bb - 1));
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+              for (int kk = 0; kk < 3; ++kk) atomicAdd(blk + 3 * i + kk, v[i][kk]);
+          }
+          if (ob) {  // rank of a in row bb: a < bb is the a-th vertex other than bb
+            double* blk = L.acc + 9 * ((int)(L.ms[l[bb]] & 0xffffu) + rec_rank4(lo, hi, bb, a));
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+              for (int kk = 0; kk < 3; ++kk) atomicAdd(blk + 3 * i + kk, v[kk][i]);
+          }
         }
-      }
     }
   }
   if (energy) {
