@@ -418,8 +418,8 @@ def reference_elasticity_leg(sizes, log_fn):
 def convection_diffusion_leg(m, log_fn):
     """Non-symmetric leg: -lap u + beta . grad u = 1 on the unit square (m x m x 2 P1 triangles,
     zero Dirichlet), assembled by the general constant-form kernel and solved by Jacobi-BiCGSTAB
-    to 1e-8 (set-up and one untimed pass first).  Deterministic assembly mode: the iteration
-    count is the same in every run."""
+    to 1e-8 (set-up and one untimed pass first).  Deterministic assembly mode for the matrix and
+    the load: the iteration count is the same in every run."""
     import torch
     import tensormesh_amd as tm
     from tensormesh_amd.mesh import structured_rectangle
@@ -435,6 +435,7 @@ def convection_diffusion_leg(m, log_fn):
     # deterministic assembly: the matrix repeats bit for bit, so does BiCGSTAB's iteration count
     asm.deterministic = True
     load = tm.const_node_assembler(1.0).from_mesh(mesh).to(dev)
+    load.deterministic = True
     cond = tm.Condenser(mesh.boundary_mask.to(dev))
 
     def run():
@@ -465,6 +466,79 @@ def convection_diffusion_leg(m, log_fn):
             "ms_per_iter": round(t_run / it * 1e3, 4), "iter_GBps_est": round(b_it / (t_run / it) / 1e9, 1)}
 
 
+def _event_time(fn, reps, dev):
+    """Mean seconds per call of ``fn`` between HIP events on the current stream (one untimed call first)."""
+    import torch
+    fn()
+    torch.cuda.synchronize(dev)
+    stream = torch.cuda.current_stream(dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize(dev)
+    return e0.elapsed_time(e1) / 1e3 / reps
+
+
+def forms_leg(mesh, asm, pd, t_default, log_fn):
+    """The other assembly paths on the headline mesh: the deterministic mode of the kappa-Laplace
+    kernel, a spatially varying bilinear form (moments kernel), a spatially varying load
+    (``func_node_assembler``) and a facet load over the boundary.  End-to-end call times include
+    the per-call probe of the user's ``forward`` on the quadrature points."""
+    import torch
+    import tensormesh_amd as tm
+    from tensormesh_amd.assemble import forms
+    dev = torch.device("cuda:0")
+    out = {}
+    asm.deterministic = True
+    try:
+        t_det = _event_time(lambda: asm(point_data=pd), 20, dev)
+    finally:
+        asm.deterministic = False
+    out["kappa_laplace_deterministic_us"] = round(t_det * 1e6, 1)
+    out["kappa_laplace_us"] = round(t_default * 1e6, 1)
+    log_fn(f"forms: deterministic kappa-Laplace {t_det * 1e6:.1f} us ({t_det / t_default:.2f}x the default)")
+
+    class VaryingDiffusion(tm.ElementAssembler):
+        def forward(self, gradu, gradv, x):
+            return (1.0 + x[0] * x[1]) * (gradu @ gradv) + x[2] * (gradu[0] * gradv[1])
+
+    vd = VaryingDiffusion.from_assembler(asm)
+    pts = vd.points.to(dev)
+    conn = vd.elements["tetra"].to(dev)
+    t_call = _event_time(lambda: vd(), 5, dev)
+    mom, _ = forms.probe_bilinear_q(vd.forward, pts, conn, "tetra", vd.quadrature_order, {"x": pts}, {}, {})
+    p64 = pts.to(torch.float64).contiguous()
+    t_kernel = _event_time(lambda: vd._launch_moments(p64, mom, vd.value_dtype), 20, dev)
+    E = conn.shape[0]
+    # moments (49 doubles per element) + coordinates + values, the kernel's algorithmic bytes
+    b_mom = 8 * 49 * E + 24 * pts.shape[0] + 4 * asm.nnz
+    out["moments_form_call_ms"] = round(t_call * 1e3, 2)
+    out["moments_kernel_us"] = round(t_kernel * 1e6, 1)
+    out["moments_kernel_GBps"] = round(b_mom / t_kernel / 1e9, 1)
+    log_fn(f"forms: spatially varying bilinear form {t_call * 1e3:.1f} ms per call, moments kernel "
+           f"{t_kernel * 1e6:.1f} us ({b_mom / t_kernel / 1e9:.0f} GB/s on its moments)")
+
+    load = tm.func_node_assembler(lambda x: torch.sin(3.0 * x[0]) * x[1] + x[2]).from_assembler(
+        tm.const_node_assembler(1.0).from_mesh(mesh)).to(dev)
+    t_load = _event_time(lambda: load(), 5, dev)
+    out["func_load_call_ms"] = round(t_load * 1e3, 2)
+
+    class Traction(tm.FacetAssembler):
+        def forward(self, v, x):
+            return (x[0] + 2.0 * x[1]) * v
+
+    fa = Traction.from_mesh(mesh).to(dev)
+    t_facet = _event_time(lambda: fa(), 5, dev)
+    out["facet_load_call_ms"] = round(t_facet * 1e3, 2)
+    out["boundary_facets"] = fa.n_facets
+    log_fn(f"forms: func_node load {t_load * 1e3:.1f} ms, facet load over {fa.n_facets} facets "
+           f"{t_facet * 1e3:.1f} ms per call")
+    del mom
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -479,6 +553,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-hyper", action="store_true")
     ap.add_argument("--no-refpoisson", action="store_true")
+    ap.add_argument("--no-forms", action="store_true")
     ap.add_argument("--hyper-n", type=int, default=64)
     ap.add_argument("--batched-meshes", type=int, default=256)
     ap.add_argument("--learned-n", type=int, default=96)
@@ -620,6 +695,9 @@ def main():
                         "pcg_iter_GBps": round(b_iter / (t_solve / max(it, 1)) / 1e9, 1),
                         "spmv_us": round(t_spmv * 1e6, 2), "spmv_GBps": round(b_spmv / t_spmv / 1e9, 1)}
         log(f"solve: {it} its, {t_solve:.3f}s, true relres {true_rel:.2e}, spmv {t_spmv * 1e6:.1f} us")
+
+    if not args.no_forms:
+        out["forms"] = forms_leg(mesh, asm, pd, t_step, log)
 
     if not args.no_learned:
         seeds = list(range(5000, 5000 + args.learned_seeds))

@@ -36,7 +36,8 @@ extern "C" {
 #define TM_FORM_LAPLACE 0       /* scale * grad u . grad v                      */
 #define TM_FORM_KAPPA_LAPLACE 1 /* kappa(x) grad u . grad v, kappa nodal        */
 #define TM_FORM_MASS 2          /* scale * u v                                  */
-/* OR-ed into the form of tm_asm_p1_matrix / the flags of tm_asm_p1_general:
+/* OR-ed into the form of tm_asm_p1_matrix / tm_asm_p1_load, or the flags of
+ * tm_asm_p1_general / tm_asm_p1_moments / tm_asm_p1_load_moments:
  * deterministic assembly.  Every element-matrix entry of a row is rounded to
  * a multiple of a power of two 2^k per row (2^(k+44) above the row's largest
  * |entry|, so each entry moves by at most 2^-44 of that maximum) before it is
@@ -185,13 +186,13 @@ int tm_asm_p1_general(int nb, int dim, const tm_plan_dev* plan, const double* co
  * (w_q reference weights; the kernel supplies |det J| and grad phi).  values
  * [nnz] f32 or f64 in canonical CSR order, each entry written once. */
 int tm_asm_p1_moments(int nb, int dim, const tm_plan_dev* plan, const double* coords,
-                      const double* moments, void* values, int value_dtype, void* stream);
+                      const double* moments, void* values, int value_dtype, int flags, void* stream);
 /* Spatially varying linear forms F0 v + F1 . grad v (NodeAssembler.__call__,
  * node_assembler.py:242-409, e.g. func_node_assembler, builtin.py:929-969):
  * moments[E][nb+dim] f64 = {L0[nb] = sum_q w_q phi_a F0, L1[dim] = sum_q w_q F1}
  * in the element's vertex order; b[N] f64 is overwritten. */
 int tm_asm_p1_load_moments(int nb, int dim, const tm_plan_dev* plan, const double* coords,
-                           const double* moments, double* b, void* stream);
+                           const double* moments, double* b, int flags, void* stream);
 
 /* Load vector b[N] (f64) (NodeAssembler.__call__, node_assembler.py:242-409):
  * TM_LOAD_CONST: c * int phi_a;  TM_LOAD_NODAL: int (sum_c f_c phi_c) phi_a. */

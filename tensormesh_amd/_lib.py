@@ -148,9 +148,9 @@ _SIGS = {
                               c_double, c_double, c_i64, c_int, ctypes.POINTER(c_i64),
                               ctypes.POINTER(c_double), c_void_p]),
     "tm_asm_p1_moments": (c_int, [c_int, c_int, ctypes.POINTER(PlanDev), c_void_p, c_void_p, c_void_p, c_int,
-                                  c_void_p]),
+                                  c_int, c_void_p]),
     "tm_asm_p1_load_moments": (c_int, [c_int, c_int, ctypes.POINTER(PlanDev), c_void_p, c_void_p, c_void_p,
-                                       c_void_p]),
+                                       c_int, c_void_p]),
     "tm_facet_load": (c_int, [c_int, c_int, c_i64, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int,
                               c_void_p, c_void_p, c_void_p, c_void_p]),
     "tm_pcg_noalloc_max_rows": (c_i64, []),

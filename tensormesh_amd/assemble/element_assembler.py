@@ -54,8 +54,8 @@ class _KappaLaplace(torch.autograd.Function):
 class ElementAssembler(nn.Module):
     value_dtype: torch.dtype = torch.float32
     # bit-identical values on every call (TM_ASM_DETERMINISTIC: entries rounded
-    # to a per-block power-of-two grid, at most 2^-44 of the block's largest
-    # entry, so the on-chip sums are exact and order-independent)
+    # to a per-row power-of-two grid, at most 2^-44 of the row's largest entry,
+    # so the on-chip sums are exact and order-independent)
     deterministic: bool = False
 
     def __init__(self, topology: P1Topology, elements: Dict[str, torch.Tensor], points: torch.Tensor,
@@ -157,7 +157,8 @@ class ElementAssembler(nn.Module):
         values = torch.empty(self.topology.nnz, dtype=out_dtype, device=dev)
         _lib.check(_lib.lib().tm_asm_p1_moments(
             self.topology.nb, self.topology.dim, plan.struct, _lib.ptr(points64), _lib.ptr(mom),
-            _lib.ptr(values), _lib.dtype_code(values), _lib.stream_of(dev)))
+            _lib.ptr(values), _lib.dtype_code(values), _lib.ASM_DETERMINISTIC if self.deterministic else 0,
+            _lib.stream_of(dev)))
         return values
 
     def _launch_kappa_adjoint(self, points64, g) -> torch.Tensor:

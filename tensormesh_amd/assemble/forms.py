@@ -239,9 +239,30 @@ def _eval_points(fn, names, args, dims, basis, E: int, Q: int):
     return out.expand(E, Q)
 
 
+def p1_gradients(X: torch.Tensor) -> torch.Tensor:
+    """``grad phi_b`` of P1 simplices ``X [E, nb, d]`` as ``[E, nb, d]``, in closed
+    form: the cofactor vectors of the edge matrix over its determinant (the
+    rows of ``J^-1``, as the HIP kernels compute them)."""
+    d = X.shape[2]
+    e = [X[:, k] - X[:, 0] for k in range(1, d + 1)]
+    if d == 2:
+        det = e[0][:, 0] * e[1][:, 1] - e[0][:, 1] * e[1][:, 0]
+        c = [torch.stack([e[1][:, 1], -e[1][:, 0]], 1), torch.stack([-e[0][:, 1], e[0][:, 0]], 1)]
+    else:
+        c = [torch.linalg.cross(e[1], e[2]), torch.linalg.cross(e[2], e[0]), torch.linalg.cross(e[0], e[1])]
+        det = (e[0] * c[0]).sum(1)
+    c0 = -sum(c)
+    return torch.stack([c0] + c, 1) / det[:, None, None]
+
+
 def _coefficient_args(fn, points, conn, element_type, order, point_data, element_data, scalar_data):
+    """The arguments of ``forward`` at every (element, quadrature point) for the
+    probes; the basis arguments are placeholders that the probes replace."""
     names = list(inspect.signature(fn).parameters)
-    phi, grad, _ = geometry_torch(points, conn, element_type, order)
+    w, q = quadrature(element_type, order)
+    phi = torch.as_tensor(shape_val(element_type, q), dtype=points.dtype, device=points.device)
+    grad = p1_gradients(points[conn])                                       # constant per element
+    grad = grad[:, None].expand(grad.shape[0], len(w), grad.shape[1], grad.shape[2])
     args, dims = _args(names, phi, grad, point_data, element_data, scalar_data, conn, True)
     return names, phi, args, dims
 
@@ -252,8 +273,35 @@ def _chunks(conn, element_data, chunk):
         yield conn[s:s + chunk], {k: v[s:s + chunk] for k, v in element_data.items()}
 
 
+def _bilinear_moment_map(w: np.ndarray, phi: np.ndarray, d: int) -> np.ndarray:
+    """``T [(d+1)^2 * Q, (nb+d)^2]`` with ``moments = D.reshape(E, -1) @ T`` for
+    ``D [E, d+1, d+1, Q]``: ``MM [nb][nb]``, ``P [nb][d]``, ``Qm [nb][d]``, ``S [d][d]``."""
+    Q, nb = phi.shape
+    dd = d + 1
+    T = np.zeros((Q, dd, dd, (nb + d) ** 2))
+    o_p, o_q, o_s = nb * nb, nb * nb + nb * d, nb * nb + 2 * nb * d
+    for q in range(Q):
+        for a in range(nb):
+            for b in range(nb):
+                T[q, 0, 0, a * nb + b] = w[q] * phi[q, a] * phi[q, b]
+            for j in range(d):
+                T[q, 0, 1 + j, o_p + a * d + j] = w[q] * phi[q, a]
+                T[q, 1 + j, 0, o_q + a * d + j] = w[q] * phi[q, a]
+        for i in range(d):
+            for j in range(d):
+                T[q, 1 + i, 1 + j, o_s + i * d + j] = w[q]
+    return T.transpose(1, 2, 0, 3).reshape(dd * dd * Q, -1)
+
+
+def _chunk_for(points: torch.Tensor, chunk: Optional[int]) -> int:
+    """Elements per probe pass: large on the GPU (fewer, larger kernels), small on the host."""
+    if chunk is not None:
+        return int(chunk)
+    return 1 << 21 if points.is_cuda else 1 << 17
+
+
 def probe_bilinear_q(fn: Callable, points: torch.Tensor, conn: torch.Tensor, element_type: str, order: int,
-                     point_data: Dict, element_data: Dict, scalar_data: Dict, chunk: int = 1 << 19):
+                     point_data: Dict, element_data: Dict, scalar_data: Dict, chunk: Optional[int] = None):
     """Per-element moments of a bilinear integrand whose coefficients vary in
     space, for ``tm_asm_p1_moments``, or ``None``.
 
@@ -263,7 +311,8 @@ def probe_bilinear_q(fn: Callable, points: torch.Tensor, conn: torch.Tensor, ele
     ``U = (u, gradu)`` and ``V = (v, gradv)``: ``D(q)`` is ``forward`` at the
     unit vectors (``(d+1)^2`` vectorised evaluations over all points), and
     three random ``(U, V)`` pairs must reproduce ``U^T D V`` at every point to
-    1e-12.  Returns ``(moments [E, (nb+d)^2] float64, symmetric)``."""
+    1e-12.  The moments are one product of ``D`` with a constant map.  Returns
+    ``(moments [E, (nb+d)^2] float64, symmetric)``."""
     names = list(inspect.signature(fn).parameters)
     if not (set(names) & {"u", "gradu"} and set(names) & {"v", "gradv"}):
         return None
@@ -271,45 +320,48 @@ def probe_bilinear_q(fn: Callable, points: torch.Tensor, conn: torch.Tensor, ele
     dd = d + 1
     dt, dev = torch.float64, points.device
     w_np, q_np = quadrature(element_type, order)
-    w = torch.as_tensor(w_np, dtype=dt, device=dev)
+    T = torch.as_tensor(_bilinear_moment_map(w_np, shape_val(element_type, q_np), d), dtype=dt, device=dev)
     gen = torch.Generator().manual_seed(20240602)
     checks = [(torch.rand(dd, generator=gen, dtype=dt) - 0.5, torch.rand(dd, generator=gen, dtype=dt) - 0.5)
               for _ in range(3)]
+    I = torch.eye(dd, dtype=dt)
     out, symmetric = [], True
     try:
-        for c, ed in _chunks(conn, element_data, chunk):
+        for c, ed in _chunks(conn, element_data, _chunk_for(points, chunk)):
             names, phi, args, dims = _coefficient_args(fn, points, c, element_type, order, point_data, ed,
                                                        scalar_data)
             E, Q = c.shape[0], phi.shape[0]
 
-            def basis(U, V):
+            def at(U, V):
                 U, V = U.to(dev), V.to(dev)
-                return {"u": U[0], "gradu": U[1:], "v": V[0], "gradv": V[1:]}
+                return _eval_points(fn, names, args, dims, {"u": U[0], "gradu": U[1:], "v": V[0], "gradv": V[1:]},
+                                    E, Q)
 
-            I = torch.eye(dd, dtype=dt)
-            D = torch.empty((E, Q, dd, dd), dtype=dt, device=dev)
+            cols = []
             for i in range(dd):
                 for j in range(dd):
-                    f = _eval_points(fn, names, args, dims, basis(I[i], I[j]), E, Q)
+                    f = at(I[i], I[j])
                     if f is None:
                         return None
-                    D[:, :, i, j] = f
-            if not torch.isfinite(D).all():
+                    cols.append(f.to(dt))
+            # [E, dd * dd, Q]: each element's quadrature values stay contiguous
+            # (stacking on the last axis writes every 16th double)
+            D = torch.stack(cols, 1)
+            dmax = D.abs().amax(dim=(0, 2))                                   # per entry of D; NaN propagates
+            if not torch.isfinite(dmax).all():
                 return None
             for U, V in checks:
-                f = _eval_points(fn, names, args, dims, basis(U, V), E, Q)
-                Ud, Vd = U.to(dev), V.to(dev)
-                ref = torch.einsum("i,eqij,j->eq", Ud, D, Vd)
-                bound = torch.einsum("i,eqij,j->eq", Ud.abs(), D.abs(), Vd.abs())
-                if f is None or ((f.to(dt) - ref).abs() > _TOL * bound.max().clamp_min(1e-300)).any():
+                f = at(U, V)
+                uv = torch.outer(U, V).reshape(-1).to(dev)
+                # elementwise products and a sum over the entries: a matrix-vector
+                # product of this tall, narrow shape runs far below memory speed
+                ref = (D * uv[:, None]).sum(1)
+                bound = (dmax * uv.abs()).sum().clamp_min(1e-300)
+                if f is None or ((f.to(dt) - ref).abs() > _TOL * bound).any():
                     return None
-            symmetric = symmetric and bool(torch.equal(D, D.transpose(-1, -2)))
-            ph = phi.to(dt)
-            MM = torch.einsum("q,qa,qb,eq->eab", w, ph, ph, D[..., 0, 0])
-            P = torch.einsum("q,qa,eqj->eaj", w, ph, D[..., 0, 1:])
-            Qm = torch.einsum("q,eqj,qb->ebj", w, D[..., 1:, 0], ph)
-            S = torch.einsum("q,eqij->eij", w, D[..., 1:, 1:])
-            out.append(torch.cat([MM.reshape(E, -1), P.reshape(E, -1), Qm.reshape(E, -1), S.reshape(E, -1)], 1))
+            D4 = D.view(E, dd, dd, Q)
+            symmetric = symmetric and bool(torch.equal(D4, D4.transpose(1, 2)))
+            out.append(D.reshape(E, -1) @ T)
     except ValueError:
         raise
     except Exception:
@@ -318,53 +370,58 @@ def probe_bilinear_q(fn: Callable, points: torch.Tensor, conn: torch.Tensor, ele
 
 
 def probe_linear_q(fn: Callable, points: torch.Tensor, conn: torch.Tensor, element_type: str, order: int,
-                   point_data: Dict, scalar_data: Dict, chunk: int = 1 << 19):
+                   point_data: Dict, scalar_data: Dict, chunk: Optional[int] = None):
     """Per-element moments of a linear integrand ``F0(q) v + F1(q) . gradv``
     with spatially varying ``F0``, ``F1`` (``node_assembler.py:242-409``; e.g.
     ``func_node_assembler``, ``builtin.py:929-969``) for
     ``tm_asm_p1_load_moments``, or ``None``: ``F0`` and ``F1`` are ``forward``
     at ``(v, gradv) = (1, 0)`` and ``(0, e_j)``, checked on three random pairs
-    to 1e-12.  Returns ``moments [E, nb + d]`` float64."""
+    to 1e-12.  Returns ``moments [E, nb + d]`` float64
+    (``L0[a] = sum_q w_q phi_a F0``, ``L1[j] = sum_q w_q F1_j``)."""
     names = list(inspect.signature(fn).parameters)
     if not set(names) & {"v", "gradv"} or set(names) & {"u", "gradu"}:
         return None
     d = points.shape[1]
     dt, dev = torch.float64, points.device
-    w_np, _ = quadrature(element_type, order)
-    w = torch.as_tensor(w_np, dtype=dt, device=dev)
+    w_np, q_np = quadrature(element_type, order)
+    ph_np = shape_val(element_type, q_np)
+    Q, nb = ph_np.shape
+    T_np = np.zeros((d + 1, Q, nb + d))                                    # for F [E, d + 1, Q]
+    T_np[0, :, :nb] = w_np[:, None] * ph_np
+    for j in range(d):
+        T_np[1 + j, :, nb + j] = w_np
+    T = torch.as_tensor(T_np.reshape((d + 1) * Q, nb + d), dtype=dt, device=dev)
     gen = torch.Generator().manual_seed(20240603)
     checks = [torch.rand(d + 1, generator=gen, dtype=dt) - 0.5 for _ in range(3)]
+    I = torch.eye(d + 1, dtype=dt)
     out = []
     try:
-        for c, _ in _chunks(conn, {}, chunk):
+        for c, _ in _chunks(conn, {}, _chunk_for(points, chunk)):
             names, phi, args, dims = _coefficient_args(fn, points, c, element_type, order, point_data, {},
                                                        scalar_data)
-            E, Q = c.shape[0], phi.shape[0]
+            E = c.shape[0]
 
-            def basis(V):
+            def at(V):
                 V = V.to(dev)
-                return {"v": V[0], "gradv": V[1:]}
+                return _eval_points(fn, names, args, dims, {"v": V[0], "gradv": V[1:]}, E, Q)
 
-            I = torch.eye(d + 1, dtype=dt)
-            F = torch.empty((E, Q, d + 1), dtype=dt, device=dev)
+            cols = []
             for j in range(d + 1):
-                f = _eval_points(fn, names, args, dims, basis(I[j]), E, Q)
+                f = at(I[j])
                 if f is None:
                     return None
-                F[:, :, j] = f
-            if not torch.isfinite(F).all():
+                cols.append(f.to(dt))
+            F = torch.stack(cols, 1)                                          # [E, d + 1, Q]
+            fmax = F.abs().amax(dim=(0, 2))
+            if not torch.isfinite(fmax).all():
                 return None
             for V in checks:
-                f = _eval_points(fn, names, args, dims, basis(V), E, Q)
+                f = at(V)
                 Vd = V.to(dev)
-                ref = torch.einsum("eqj,j->eq", F, Vd)
-                bound = torch.einsum("eqj,j->eq", F.abs(), Vd.abs())
-                if f is None or ((f.to(dt) - ref).abs() > _TOL * bound.max().clamp_min(1e-300)).any():
+                bound = (fmax * Vd.abs()).sum().clamp_min(1e-300)
+                if f is None or ((f.to(dt) - (F * Vd[:, None]).sum(1)).abs() > _TOL * bound).any():
                     return None
-            ph = phi.to(dt)
-            L0 = torch.einsum("q,qa,eq->ea", w, ph, F[..., 0])
-            L1 = torch.einsum("q,eqj->ej", w, F[..., 1:])
-            out.append(torch.cat([L0, L1], 1))
+            out.append(F.reshape(E, -1) @ T)
     except ValueError:
         raise
     except Exception:

@@ -25,6 +25,12 @@ from .topology import P1Topology, topology_for
 
 
 class NodeAssembler(nn.Module):
+    #: Deterministic mode of the recognised-load kernel (``c * v``, ``f * v``):
+    #: the entries of each row are rounded to a power-of-two grid 2^-44 below
+    #: the row's largest entry, so the FP64 sums are exact and the result is
+    #: bit-identical from call to call whatever the order of the GPU atomics.
+    deterministic: bool = False
+
     def __init__(self, topology: P1Topology, elements: Dict[str, torch.Tensor], points: torch.Tensor,
                  quadrature_order: int = 2, *args, **kwargs):
         super().__init__()
@@ -70,6 +76,8 @@ class NodeAssembler(nn.Module):
         fdt = _lib.dtype_code(f) if f is not None else _lib.F32
         if f is not None:
             f = f.contiguous()
+        if self.deterministic:
+            form |= _lib.ASM_DETERMINISTIC
         _lib.check(_lib.lib().tm_asm_p1_load(form, self.topology.nb, self.topology.dim, plan.struct,
                                              _lib.ptr(points64), _lib.ptr(f), fdt, float(c), self._quad,
                                              _lib.ptr(out), _lib.stream_of(dev)))
@@ -112,7 +120,7 @@ class NodeAssembler(nn.Module):
                     out = torch.empty(self.n_points, dtype=torch.float64, device=dev)
                     _lib.check(_lib.lib().tm_asm_p1_load_moments(
                         self.topology.nb, self.topology.dim, plan.struct, _lib.ptr(p64), _lib.ptr(mom),
-                        _lib.ptr(out), _lib.stream_of(dev)))
+                        _lib.ptr(out), _lib.ASM_DETERMINISTIC if self.deterministic else 0, _lib.stream_of(dev)))
                     return out
         if form is not None:
             kind, scale, key = form

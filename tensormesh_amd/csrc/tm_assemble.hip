@@ -254,6 +254,19 @@ __device__ __forceinline__ void row_grids(const Lds& L, int nown) {
   }
 }
 
+// Load vectors, deterministic mode: the row maxima (bit patterns in L.accb,
+// first pass) become grid exponents in L.ms, which no load kernel reads
+// otherwise, and L.accb is cleared for the second pass.
+template <int NT>
+__device__ __forceinline__ void load_grids(const Lds& L, int nown) {
+  for (int a = threadIdx.x; a < nown; a += NT) {
+    const unsigned long long bits = *(const unsigned long long*)(L.accb + a);
+    const int e = (int)((bits >> 52) & 0x7ffull) - 1023;
+    L.ms[a] = (uint32_t)(bits ? e + 1 - 44 : 0);
+    L.accb[a] = 0.0;
+  }
+}
+
 // The element loop of one block: each lane evaluates records r = tid, tid + NT,
 // ... (nxt = the lane's first record, already loaded) and adds the owned rows
 // of each element matrix into L.acc through `em` (ds_add_f64).
@@ -490,20 +503,11 @@ __device__ __forceinline__ int rec_orig(const uint4& rc, int a) { return (int)((
 // With grad phi = c / det (c = cofactor vectors) the element matrix is
 //   K_ab = |det| MM_ab + sgn(det) (P_a . c_b + c_a . Qm_b) + c_a^T S c_b / |det|.
 // The record's vertex order may differ from the element's (rec_orig maps it).
-template <int NB, typename OT, int NT>
-__global__ __launch_bounds__(NT) void k_asm_moments(PlanArgs p, const int32_t* __restrict__ rec_elem,
-                                                   const double* __restrict__ X, const double* __restrict__ mom,
-                                                   OT* __restrict__ values) {
+template <int NB, int NT, typename Emit>
+__device__ __forceinline__ void moments_records(const Lds& L, const PlanArgs& p, const int32_t* __restrict__ rec_elem,
+                                                const double* __restrict__ mom, int roff, int nrec, int nown,
+                                                const Emit& em) {
   constexpr int DIM = NB - 1, NM = (NB + DIM) * (NB + DIM);
-  extern __shared__ __align__(16) char smem[];
-  const int b = tmx::xcd_remap(blockIdx.x, p.nblk);
-  const int4 bi0 = *(const int4*)(p.blk + 8 * b);
-  const int4 bi1 = *(const int4*)(p.blk + 8 * b + 4);
-  const int hoff = bi0.x, nh = bi0.y, nown = bi0.z, roff = bi0.w;
-  const int nrec = bi1.x, runoff = bi1.y, nrun = bi1.z, nacc = bi1.w;
-  const Lds L = carve(smem, nacc, nh, nown);
-  stage<NT, DIM, float, false>(L, p.halo + hoff, p.meta + hoff, nacc, nh, nown, X, (const float*)nullptr);
-  __syncthreads();
   for (int r = threadIdx.x; r < nrec; r += NT) {
     const uint4 rc = p.recs[roff + r];
     const uint64_t lo = (uint64_t)rc.x | ((uint64_t)rc.y << 32);
@@ -550,9 +554,35 @@ __global__ __launch_bounds__(NT) void k_asm_moments(PlanArgs p, const int32_t* _
         }
         const double v = adet * MM[o[a] * NB + o[bb]] + sg * (pc + cq) + inv * gg;
         const int slot = bb == a ? meta_diag(ms[a]) : rec_rank<NB>(rc, a, bb);
-        atomicAdd(L.acc + base + slot, v);
+        em(L, l[a], L.acc + base + slot, v);
       }
     }
+  }
+}
+
+// DET: deterministic mode (MaxEmit / GridEmit, as k_asm_matrix).
+template <int NB, typename OT, int NT, bool DET>
+__global__ __launch_bounds__(NT) void k_asm_moments(PlanArgs p, const int32_t* __restrict__ rec_elem,
+                                                   const double* __restrict__ X, const double* __restrict__ mom,
+                                                   OT* __restrict__ values) {
+  constexpr int DIM = NB - 1;
+  extern __shared__ __align__(16) char smem[];
+  const int b = tmx::xcd_remap(blockIdx.x, p.nblk);
+  const int4 bi0 = *(const int4*)(p.blk + 8 * b);
+  const int4 bi1 = *(const int4*)(p.blk + 8 * b + 4);
+  const int hoff = bi0.x, nh = bi0.y, nown = bi0.z, roff = bi0.w;
+  const int nrec = bi1.x, runoff = bi1.y, nrun = bi1.z, nacc = bi1.w;
+  const Lds L = carve(smem, nacc, nh, nown);
+  stage<NT, DIM, float, false>(L, p.halo + hoff, p.meta + hoff, nacc, nh, nown, X, (const float*)nullptr);
+  __syncthreads();
+  if (DET) {
+    moments_records<NB, NT>(L, p, rec_elem, mom, roff, nrec, nown, MaxEmit{});
+    __syncthreads();
+    row_grids<NT>(L, nown);
+    __syncthreads();
+    moments_records<NB, NT>(L, p, rec_elem, mom, roff, nrec, nown, GridEmit{});
+  } else {
+    moments_records<NB, NT>(L, p, rec_elem, mom, roff, nrec, nown, AddEmit{});
   }
   __syncthreads();
   const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
@@ -566,19 +596,12 @@ __global__ __launch_bounds__(NT) void k_asm_moments(PlanArgs p, const int32_t* _
 // moments in the element's vertex order, NB + DIM doubles per element:
 //   L0[NB] = sum_q w_q phi_a(q) F0(q),  L1[DIM] = sum_q w_q F1(q)
 // so that b_a += |det| L0_a + sgn(det) c_a . L1.
-template <int NB, int NT>
-__global__ __launch_bounds__(NT) void k_asm_load_moments(PlanArgs p, const int32_t* __restrict__ rec_elem,
-                                                        const double* __restrict__ X,
-                                                        const double* __restrict__ mom, double* __restrict__ out) {
+template <int NB, int NT, typename Emit>
+__device__ __forceinline__ void load_moment_records(const Lds& L, const PlanArgs& p,
+                                                    const int32_t* __restrict__ rec_elem,
+                                                    const double* __restrict__ mom, int roff, int nrec, int nown,
+                                                    const Emit& em) {
   constexpr int DIM = NB - 1, NM = NB + DIM;
-  extern __shared__ __align__(16) char smem[];
-  const int b = tmx::xcd_remap(blockIdx.x, p.nblk);
-  const int4 bi0 = *(const int4*)(p.blk + 8 * b);
-  const int4 bi1 = *(const int4*)(p.blk + 8 * b + 4);
-  const int hoff = bi0.x, nh = bi0.y, nown = bi0.z, roff = bi0.w, nrec = bi1.x;
-  const Lds L = carve(smem, 0, nh, nown);
-  stage<NT, DIM, float, false>(L, p.halo + hoff, p.meta + hoff, 0, nh, nown, X, (const float*)nullptr);
-  __syncthreads();
   for (int r = threadIdx.x; r < nrec; r += NT) {
     const uint4 rc = p.recs[roff + r];
     const uint64_t lo = (uint64_t)rc.x | ((uint64_t)rc.y << 32);
@@ -599,18 +622,16 @@ __global__ __launch_bounds__(NT) void k_asm_load_moments(PlanArgs p, const int32
       double g = 0.0;
 #pragma unroll
       for (int j = 0; j < DIM; ++j) g += c[a][j] * m[NB + j];
-      atomicAdd(L.accb + l[a], adet * m[rec_orig(rc, a)] + sg * g);
+      em(l[a], adet * m[rec_orig(rc, a)] + sg * g);
     }
   }
-  __syncthreads();
-  for (int a = threadIdx.x; a < nown; a += NT) out[p.halo[hoff + a]] = L.accb[a];
 }
 
-// FORM: 0 constant load c, 1 nodal f.
-template <int NB, int FORM, typename CT, int NT>
-__global__ __launch_bounds__(NT) void k_asm_load(PlanArgs p, const double* __restrict__ X,
-                                                const CT* __restrict__ f, double cval, QuadArg q,
-                                                double* __restrict__ out) {
+// DET: deterministic mode, as k_asm_load.
+template <int NB, int NT, bool DET>
+__global__ __launch_bounds__(NT) void k_asm_load_moments(PlanArgs p, const int32_t* __restrict__ rec_elem,
+                                                        const double* __restrict__ X,
+                                                        const double* __restrict__ mom, double* __restrict__ out) {
   constexpr int DIM = NB - 1;
   extern __shared__ __align__(16) char smem[];
   const int b = tmx::xcd_remap(blockIdx.x, p.nblk);
@@ -618,8 +639,35 @@ __global__ __launch_bounds__(NT) void k_asm_load(PlanArgs p, const double* __res
   const int4 bi1 = *(const int4*)(p.blk + 8 * b + 4);
   const int hoff = bi0.x, nh = bi0.y, nown = bi0.z, roff = bi0.w, nrec = bi1.x;
   const Lds L = carve(smem, 0, nh, nown);
-  stage<NT, DIM, CT, FORM == 1>(L, p.halo + hoff, p.meta + hoff, 0, nh, nown, X, f);
+  stage<NT, DIM, float, false>(L, p.halo + hoff, p.meta + hoff, 0, nh, nown, X, (const float*)nullptr);
   __syncthreads();
+  if (DET) {
+    load_moment_records<NB, NT>(L, p, rec_elem, mom, roff, nrec, nown, [&](int row, double v) {
+      atomicMax((unsigned long long*)(L.accb + row), (unsigned long long)__double_as_longlong(fabs(v)));
+    });
+    __syncthreads();
+    load_grids<NT>(L, nown);
+    __syncthreads();
+    load_moment_records<NB, NT>(L, p, rec_elem, mom, roff, nrec, nown, [&](int row, double v) {
+      const int k = (int)L.ms[row];
+      atomicAdd(L.accb + row, ldexp(rint(ldexp(v, -k)), k));
+    });
+  } else {
+    load_moment_records<NB, NT>(L, p, rec_elem, mom, roff, nrec, nown,
+                                [&](int row, double v) { atomicAdd(L.accb + row, v); });
+  }
+  __syncthreads();
+  for (int a = threadIdx.x; a < nown; a += NT) out[p.halo[hoff + a]] = L.accb[a];
+}
+
+// FORM: 0 constant load c, 1 nodal f.
+// The records of one block for k_asm_load: c * int phi_a (FORM 0) or
+// int (sum_c f_c phi_c) phi_a (FORM 1) of every owned vertex through
+// em(row slot, value).
+template <int NB, int FORM, int NT, typename Emit>
+__device__ __forceinline__ void load_records(const Lds& L, const PlanArgs& p, int roff, int nrec, int nown,
+                                             double cval, const QuadArg& q, const Emit& em) {
+  constexpr int DIM = NB - 1;
   for (int r = threadIdx.x; r < nrec; r += NT) {
     const uint4 rc = p.recs[roff + r];
     const uint64_t lo = (uint64_t)rc.x | ((uint64_t)rc.y << 32);
@@ -646,8 +694,42 @@ __global__ __launch_bounds__(NT) void k_asm_load(PlanArgs p, const double* __res
 #pragma unroll
         for (int cc = 0; cc < NB; ++cc) v += q.M[a * NB + cc] * cf[cc];
       }
-      atomicAdd(L.accb + l[a], adet * v);
+      em(l[a], adet * v);
     }
+  }
+}
+
+// Load vectors.  DET: deterministic mode, the arithmetic of the matrix kernels'
+// MaxEmit / GridEmit: a first pass takes each owned row's largest |entry| in
+// L.accb, its grid exponent then replaces the row meta in L.ms (which no load
+// kernel reads) and the second pass sums the rounded entries in L.accb.
+template <int NB, int FORM, typename CT, int NT, bool DET>
+__global__ __launch_bounds__(NT) void k_asm_load(PlanArgs p, const double* __restrict__ X,
+                                                const CT* __restrict__ f, double cval, QuadArg q,
+                                                double* __restrict__ out) {
+  constexpr int DIM = NB - 1;
+  extern __shared__ __align__(16) char smem[];
+  const int b = tmx::xcd_remap(blockIdx.x, p.nblk);
+  const int4 bi0 = *(const int4*)(p.blk + 8 * b);
+  const int4 bi1 = *(const int4*)(p.blk + 8 * b + 4);
+  const int hoff = bi0.x, nh = bi0.y, nown = bi0.z, roff = bi0.w, nrec = bi1.x;
+  const Lds L = carve(smem, 0, nh, nown);
+  stage<NT, DIM, CT, FORM == 1>(L, p.halo + hoff, p.meta + hoff, 0, nh, nown, X, f);
+  __syncthreads();
+  if (DET) {
+    load_records<NB, FORM, NT>(L, p, roff, nrec, nown, cval, q, [&](int row, double v) {
+      atomicMax((unsigned long long*)(L.accb + row), (unsigned long long)__double_as_longlong(fabs(v)));
+    });
+    __syncthreads();
+    load_grids<NT>(L, nown);
+    __syncthreads();
+    load_records<NB, FORM, NT>(L, p, roff, nrec, nown, cval, q, [&](int row, double v) {
+      const int k = (int)L.ms[row];
+      atomicAdd(L.accb + row, ldexp(rint(ldexp(v, -k)), k));
+    });
+  } else {
+    load_records<NB, FORM, NT>(L, p, roff, nrec, nown, cval, q,
+                               [&](int row, double v) { atomicAdd(L.accb + row, v); });
   }
   __syncthreads();
   for (int a = threadIdx.x; a < nown; a += NT) out[p.halo[hoff + a]] = L.accb[a];
@@ -840,32 +922,65 @@ extern "C" int tm_asm_p1_general(int nb, int dim, const tm_plan_dev* plan, const
   TM_BY_THREADS(plan, (k_asm_general<4, double, NT, false>), st, p, coords, g, q, (double*)values);
 }
 
+template <bool DET>
+int dispatch_moments(int nb, int value_dtype, const tm_plan_dev* plan, hipStream_t st, const PlanArgs& p,
+                     const double* coords, const double* moments, void* values) {
+  const int32_t* re = plan->rec_elem;
+  if (value_dtype == TM_F32) {
+    if (nb == 3) TM_BY_THREADS(plan, (k_asm_moments<3, float, NT, DET>), st, p, re, coords, moments, (float*)values);
+    TM_BY_THREADS(plan, (k_asm_moments<4, float, NT, DET>), st, p, re, coords, moments, (float*)values);
+  }
+  if (nb == 3) TM_BY_THREADS(plan, (k_asm_moments<3, double, NT, DET>), st, p, re, coords, moments, (double*)values);
+  TM_BY_THREADS(plan, (k_asm_moments<4, double, NT, DET>), st, p, re, coords, moments, (double*)values);
+}
+
 extern "C" int tm_asm_p1_moments(int nb, int dim, const tm_plan_dev* plan, const double* coords,
-                                 const double* moments, void* values, int value_dtype, void* stream) {
+                                 const double* moments, void* values, int value_dtype, int flags, void* stream) {
   TM_REQUIRE(plan && coords && moments && values && plan->rec_elem, TM_ERR_ARG, "null argument");
   TM_REQUIRE((nb == 3 && dim == 2) || (nb == 4 && dim == 3), TM_ERR_ARG,
              "only P1 triangles (nb=3, dim=2) and tetrahedra (nb=4, dim=3) are supported");
   TM_REQUIRE(value_dtype == TM_F32 || value_dtype == TM_F64, TM_ERR_ARG, "values must be f32 or f64");
+  TM_REQUIRE((flags & ~TM_ASM_DETERMINISTIC) == 0, TM_ERR_ARG, "unknown flags");
   hipStream_t st = (hipStream_t)stream;
   const PlanArgs p = plan_args(plan);
-  const int32_t* re = plan->rec_elem;
-  if (value_dtype == TM_F32) {
-    if (nb == 3) TM_BY_THREADS(plan, (k_asm_moments<3, float, NT>), st, p, re, coords, moments, (float*)values);
-    TM_BY_THREADS(plan, (k_asm_moments<4, float, NT>), st, p, re, coords, moments, (float*)values);
-  }
-  if (nb == 3) TM_BY_THREADS(plan, (k_asm_moments<3, double, NT>), st, p, re, coords, moments, (double*)values);
-  TM_BY_THREADS(plan, (k_asm_moments<4, double, NT>), st, p, re, coords, moments, (double*)values);
+  if (flags & TM_ASM_DETERMINISTIC) return dispatch_moments<true>(nb, value_dtype, plan, st, p, coords, moments, values);
+  return dispatch_moments<false>(nb, value_dtype, plan, st, p, coords, moments, values);
 }
 
 extern "C" int tm_asm_p1_load_moments(int nb, int dim, const tm_plan_dev* plan, const double* coords,
-                                      const double* moments, double* b, void* stream) {
+                                      const double* moments, double* b, int flags, void* stream) {
   TM_REQUIRE(plan && coords && moments && b && plan->rec_elem, TM_ERR_ARG, "null argument");
   TM_REQUIRE((nb == 3 && dim == 2) || (nb == 4 && dim == 3), TM_ERR_ARG, "only P1 simplices");
+  TM_REQUIRE((flags & ~TM_ASM_DETERMINISTIC) == 0, TM_ERR_ARG, "unknown flags");
   hipStream_t st = (hipStream_t)stream;
   const PlanArgs p = plan_args(plan);
+  const int32_t* re = plan->rec_elem;
+  if (flags & TM_ASM_DETERMINISTIC) {
+    if (nb == 3) return launch_plan_kernel(k_asm_load_moments<3, 256, true>, 256, plan, st, p, re, coords, moments, b);
+    return launch_plan_kernel(k_asm_load_moments<4, 256, true>, 256, plan, st, p, re, coords, moments, b);
+  }
+  if (nb == 3) return launch_plan_kernel(k_asm_load_moments<3, 256, false>, 256, plan, st, p, re, coords, moments, b);
+  return launch_plan_kernel(k_asm_load_moments<4, 256, false>, 256, plan, st, p, re, coords, moments, b);
+}
+
+template <bool DET>
+int dispatch_load(int form, int nb, const tm_plan_dev* plan, hipStream_t st, const PlanArgs& p,
+                  const double* coords, const void* f, int f_dtype, double c, const QuadArg& q, double* b) {
+  if (form == TM_LOAD_CONST) {
+    if (nb == 3)
+      return launch_plan_kernel(k_asm_load<3, 0, float, 256, DET>, 256, plan, st, p, coords, (const float*)nullptr, c,
+                                q, b);
+    return launch_plan_kernel(k_asm_load<4, 0, float, 256, DET>, 256, plan, st, p, coords, (const float*)nullptr, c, q,
+                              b);
+  }
+  if (f_dtype == TM_F32) {
+    if (nb == 3)
+      return launch_plan_kernel(k_asm_load<3, 1, float, 256, DET>, 256, plan, st, p, coords, (const float*)f, c, q, b);
+    return launch_plan_kernel(k_asm_load<4, 1, float, 256, DET>, 256, plan, st, p, coords, (const float*)f, c, q, b);
+  }
   if (nb == 3)
-    return launch_plan_kernel(k_asm_load_moments<3, 256>, 256, plan, st, p, plan->rec_elem, coords, moments, b);
-  return launch_plan_kernel(k_asm_load_moments<4, 256>, 256, plan, st, p, plan->rec_elem, coords, moments, b);
+    return launch_plan_kernel(k_asm_load<3, 1, double, 256, DET>, 256, plan, st, p, coords, (const double*)f, c, q, b);
+  return launch_plan_kernel(k_asm_load<4, 1, double, 256, DET>, 256, plan, st, p, coords, (const double*)f, c, q, b);
 }
 
 extern "C" int tm_asm_p1_load(int form, int nb, int dim, const tm_plan_dev* plan, const double* coords,
@@ -876,19 +991,13 @@ extern "C" int tm_asm_p1_load(int form, int nb, int dim, const tm_plan_dev* plan
   TM_REQUIRE(quad_is_symmetric(quad, nb), TM_ERR_ARG,
              "quadrature moments must be invariant under vertex relabelling");
   hipStream_t st = (hipStream_t)stream;
+  const bool det = (form & TM_ASM_DETERMINISTIC) != 0;
+  form &= ~TM_ASM_DETERMINISTIC;
+  TM_REQUIRE(form == TM_LOAD_CONST || (form == TM_LOAD_NODAL && f != nullptr), TM_ERR_ARG, "bad load form");
   QuadArg q = quad_arg(quad);
   PlanArgs p = plan_args(plan);
-  if (form == TM_LOAD_CONST) {
-    if (nb == 3) return launch_plan_kernel(k_asm_load<3, 0, float, 256>, 256, plan, st, p, coords, (const float*)nullptr, c, q, b);
-    return launch_plan_kernel(k_asm_load<4, 0, float, 256>, 256, plan, st, p, coords, (const float*)nullptr, c, q, b);
-  }
-  TM_REQUIRE(form == TM_LOAD_NODAL && f != nullptr, TM_ERR_ARG, "bad load form");
-  if (f_dtype == TM_F32) {
-    if (nb == 3) return launch_plan_kernel(k_asm_load<3, 1, float, 256>, 256, plan, st, p, coords, (const float*)f, c, q, b);
-    return launch_plan_kernel(k_asm_load<4, 1, float, 256>, 256, plan, st, p, coords, (const float*)f, c, q, b);
-  }
-  if (nb == 3) return launch_plan_kernel(k_asm_load<3, 1, double, 256>, 256, plan, st, p, coords, (const double*)f, c, q, b);
-  return launch_plan_kernel(k_asm_load<4, 1, double, 256>, 256, plan, st, p, coords, (const double*)f, c, q, b);
+  if (det) return dispatch_load<true>(form, nb, plan, st, p, coords, f, f_dtype, c, q, b);
+  return dispatch_load<false>(form, nb, plan, st, p, coords, f, f_dtype, c, q, b);
 }
 
 extern "C" int tm_asm_p1_kappa_adjoint(int nb, int dim, const tm_plan_dev* plan, const double* coords,

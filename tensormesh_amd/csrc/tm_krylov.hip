@@ -294,21 +294,28 @@ int grid_for(int64_t n, int per_block) {
 struct Poller {
   int check_every;
   int chunk;
+  int64_t maxiter;
   int it_prev = -1;
   double rel_prev = 0.0;
-  explicit Poller(int ce) : check_every(ce), chunk(ce > 0 ? ce : 16) {}
+  Poller(int ce, int64_t mi) : check_every(ce), chunk(cap(ce > 0 ? ce : 16, 0, mi)), maxiter(mi) {}
+  // launches past maxiter only set the status: never queue more than are left
+  static int cap(int c, int it, int64_t mi) {
+    if (mi <= 0) return c;
+    const int64_t left = mi - it;
+    return left < 1 ? 1 : (left < c ? (int)left : c);
+  }
   int next(int it, double rel, double target) {
-    if (check_every > 0) return chunk;
-    int c = chunk < 256 ? 2 * chunk : 256;
-    if (it_prev >= 0 && it > it_prev && rel > 0.0 && rel_prev > 0.0 && rel < rel_prev && target > 0.0) {
+    int c = check_every > 0 ? check_every : (chunk < 256 ? 2 * chunk : 256);
+    if (check_every <= 0 && it_prev >= 0 && it > it_prev && rel > 0.0 && rel_prev > 0.0 && rel < rel_prev &&
+        target > 0.0) {
       const double rate = log(rel / rel_prev) / (double)(it - it_prev);  // < 0
       const double rem = log(target / rel) / rate;                       // iterations left
       c = rem < 1.0 ? 1 : (rem > 256.0 ? 256 : (int)ceil(rem));
     }
     it_prev = it;
     rel_prev = rel;
-    chunk = c;
-    return c;
+    if (check_every <= 0) chunk = c;
+    return cap(c, it, maxiter);
   }
 };
 
@@ -333,7 +340,7 @@ int pcg_run(int64_t nrows, const int32_t* rp, const int32_t* ci, const VT* v, co
   static_assert(sizeof(h) == sizeof(PcgScalars) + sizeof(PcgFlags), "PCG state layout");
   PcgFlags& hf = h.f;
   PcgScalars& hs = h.s;
-  Poller poll(check_every);
+  Poller poll(check_every, maxiter);
   int chunk = poll.chunk;
   for (;;) {
     TM_HIP(hipMemcpyAsync(&h, w.s, sizeof(h), hipMemcpyDeviceToHost, st));
@@ -629,7 +636,7 @@ int bc_run(int64_t n, const int32_t* rp, const int32_t* ci, const VT* v, const d
   static_assert(sizeof(h) == sizeof(BcScalars) + sizeof(PcgFlags), "BiCGSTAB state layout");
   PcgFlags& hf = h.f;
   BcScalars& hs = h.s;
-  Poller poll(check_every);
+  Poller poll(check_every, maxiter);
   int chunk = poll.chunk;
   int restarts = 0;
   for (;;) {
@@ -646,8 +653,8 @@ int bc_run(int64_t n, const int32_t* rp, const int32_t* ci, const VT* v, const d
       TM_HIP(hipGetLastError());
       TM_HIP(hipMemcpyAsync(&h, w.sc, sizeof(h), hipMemcpyDeviceToHost, st));
       TM_HIP(hipStreamSynchronize(st));
-      poll = Poller(check_every);  // the recursion restarts from the true residual
-      chunk = poll.chunk;
+      poll = Poller(check_every, maxiter);  // the recursion restarts from the true residual
+      chunk = Poller::cap(poll.chunk, hf.iter, maxiter);
     } else if (hf.iter > 0) {
       chunk = poll.next(hf.iter, hs.relres, hs.bb > 0.0 ? sqrt(hs.tol2 / hs.bb) : 0.0);
     }
