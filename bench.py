@@ -273,6 +273,7 @@ def hyperelastic_leg(n, log_fn):
     Et = torch.tensor(E_np, dtype=torch.float32, device=dev)
     model = tm.NeoHookeanModel.from_mesh(mesh, E=Et, nu=0.3).to(dev)
     lin = tm.LinearElasticityElementAssembler.from_assembler(model, E=Et, nu=0.3).to(dev)
+    lin.value_dtype = torch.float32
     z = torch.tensor(pts[:, 2], device=dev)
     fixed = (z == 0) | (z == 1)
     u0 = torch.zeros((len(pts), 3), dtype=torch.float64, device=dev)
@@ -329,6 +330,7 @@ def reference_poisson_leg(sizes, log_fn):
         t0 = time.perf_counter()
         mesh = tm.kuhn_cube(n)
         asm = tm.LaplaceElementAssembler.from_mesh(mesh).to(dev)
+        asm.value_dtype = torch.float32
         load = tm.const_node_assembler(1.0).from_mesh(mesh).to(dev)
         cond = tm.Condenser(mesh.boundary_mask.to(dev))
         asm.topology.device_plan(dev)
@@ -379,6 +381,7 @@ def reference_elasticity_leg(sizes, log_fn):
         t0 = time.perf_counter()
         mesh = tm.kuhn_cube(n)
         asm = tm.LinearElasticityElementAssembler.from_mesh(mesh, E=1.0, nu=0.3).to(dev)
+        asm.value_dtype = torch.float32
         load = tm.const_node_assembler(1.0).from_mesh(mesh).to(dev)
         free = mesh.points[:, 2].to(dev) > 0.0
         cond = BlockCondenser(asm, free)
@@ -432,6 +435,7 @@ def convection_diffusion_leg(m, log_fn):
             return gradu @ gradv + (beta.to(gradu.dtype) @ gradu) * v
 
     asm = ConvDiff.from_mesh(mesh).to(dev)
+    asm.value_dtype = torch.float32
     # deterministic assembly: the matrix repeats bit for bit, so does BiCGSTAB's iteration count
     asm.deterministic = True
     load = tm.const_node_assembler(1.0).from_mesh(mesh).to(dev)
@@ -505,6 +509,7 @@ def forms_leg(mesh, asm, pd, t_default, log_fn):
             return (1.0 + x[0] * x[1]) * (gradu @ gradv) + x[2] * (gradu[0] * gradv[1])
 
     vd = VaryingDiffusion.from_assembler(asm)
+    vd.value_dtype = torch.float32
     pts = vd.points.to(dev)
     conn = vd.elements["tetra"].to(dev)
     t_call = _event_time(lambda: vd(), 5, dev)
@@ -584,6 +589,7 @@ def main():
     asm = tm.KappaLaplaceElementAssembler.from_mesh(mesh)
     t_pattern = time.perf_counter() - t0
     asm = asm.to(dev)
+    asm.value_dtype = torch.float32           # FP32 values, FP64 geometry and accumulation
     t0 = time.perf_counter()
     plan = asm.topology.device_plan(dev)
     t_plan = time.perf_counter() - t0

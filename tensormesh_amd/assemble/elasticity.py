@@ -126,7 +126,7 @@ class LinearElasticityElementAssembler(ElementAssembler):
         else:
             Et, Edt, E0 = None, _lib.F64, float(E)
         p64 = points.detach().to(dev, torch.float64).contiguous()
-        kv = torch.empty(self.topology.nnz * 9, dtype=value_dtype or self.value_dtype, device=dev)
+        kv = torch.empty(self.topology.nnz * 9, dtype=value_dtype or self._out_dtype(points), device=dev)
         plan = self.topology.device_plan(dev, "vector3", max_rows=LIN_PLAN_ROWS, lds_budget=LIN_PLAN_LDS)
         _lib.check(_lib.lib().tm_asm_linear_elastic(plan.struct, _lib.ptr(p64), _lib.ptr(Et), Edt, E0,
                                                      float(self.nu), _lib.ptr(kv), _lib.dtype_code(kv),

@@ -9,7 +9,8 @@ for parameters, ``edges`` ``[2, nnz]`` in canonical CSR order.
 
 On GPU tensors a recognised ``forward`` (see :mod:`.forms`) runs the
 owner-computes HIP kernel: FP64 geometry, FP64 on-chip accumulation, values
-stored as ``value_dtype`` (float32 unless set otherwise).  A ``forward`` that
+stored in the points' dtype as the reference's are (``self.type(points.dtype)``,
+``element_assembler.py:321``), or in ``value_dtype`` when it is set.  A ``forward`` that
 is bilinear in ``(u, gradu)`` and ``(v, gradv)`` with coefficients varying in
 space (``x``, point data and their gradients, element data) is probed at the
 quadrature points and runs the moments kernel.  Other forms, inputs that
@@ -52,7 +53,8 @@ class _KappaLaplace(torch.autograd.Function):
 
 
 class ElementAssembler(nn.Module):
-    value_dtype: torch.dtype = torch.float32
+    # dtype of the stored values: None follows the points (float32 or float64)
+    value_dtype: Optional[torch.dtype] = None
     # bit-identical values on every call (TM_ASM_DETERMINISTIC: entries rounded
     # to a per-row power-of-two grid, at most 2^-44 of the row's largest entry,
     # so the on-chip sums are exact and order-independent)
@@ -118,6 +120,11 @@ class ElementAssembler(nn.Module):
         return cls(obj.topology, elements, obj.points, obj.quadrature_order, *args, **kwargs)
 
     # ---- kernels --------------------------------------------------------------
+    def _out_dtype(self, points: torch.Tensor) -> torch.dtype:
+        if self.value_dtype is not None:
+            return self.value_dtype
+        return points.dtype if points.dtype in (torch.float32, torch.float64) else torch.float32
+
     def _plan(self, device):
         return self.topology.device_plan(device)
 
@@ -229,7 +236,7 @@ class ElementAssembler(nn.Module):
             if q is not None:
                 mom, sym = q
                 values = self._launch_moments(points.detach().to(torch.float64).contiguous(), mom,
-                                              self.value_dtype)
+                                              self._out_dtype(points))
                 e = self.topology.edges_on(dev)
                 return SparseMatrix._from_csr(values, e[0], e[1], (self.n_points, self.n_points),
                                               self.csr(dev), symmetric=sym)
@@ -237,18 +244,18 @@ class ElementAssembler(nn.Module):
             kind, scale, key = form
             p64 = points.detach().to(torch.float64).contiguous()
             if kind == _lib.FORM_GENERAL:
-                values = self._launch_general(p64, scale, self.value_dtype)
+                values = self._launch_general(p64, scale, self._out_dtype(points))
                 e = self.topology.edges_on(dev)
                 return SparseMatrix._from_csr(values, e[0], e[1], (self.n_points, self.n_points),
                                               self.csr(dev), symmetric=bool(np.array_equal(scale, scale.T)))
             if kind == _lib.FORM_KAPPA_LAPLACE:
                 coef = point_data[key].to(dev)
                 if coef.requires_grad:
-                    values = _KappaLaplace.apply(coef, self, p64, scale, self.value_dtype)
+                    values = _KappaLaplace.apply(coef, self, p64, scale, self._out_dtype(points))
                 else:
-                    values = self._launch_matrix(kind, p64, coef, scale, self.value_dtype)
+                    values = self._launch_matrix(kind, p64, coef, scale, self._out_dtype(points))
             else:
-                values = self._launch_matrix(kind, p64, None, scale, self.value_dtype)
+                values = self._launch_matrix(kind, p64, None, scale, self._out_dtype(points))
             e = self.topology.edges_on(dev)
             return SparseMatrix._from_csr(values, e[0], e[1], (self.n_points, self.n_points),
                                           self.csr(dev), symmetric=True)

@@ -308,9 +308,17 @@ struct Poller {
     int c = check_every > 0 ? check_every : (chunk < 256 ? 2 * chunk : 256);
     if (check_every <= 0 && it_prev >= 0 && it > it_prev && rel > 0.0 && rel_prev > 0.0 && rel < rel_prev &&
         target > 0.0) {
+      // Krylov convergence speeds up, so the rate since the last read
+      // overestimates the iterations left: queue half of them (at least 16),
+      // all of them once 16 or fewer remain.  On recorded Jacobi-PCG residual
+      // histories this leaves a few queued iterations past convergence per
+      // solve instead of tens, for a few more status reads.
       const double rate = log(rel / rel_prev) / (double)(it - it_prev);  // < 0
       const double rem = log(target / rel) / rate;                       // iterations left
-      c = rem < 1.0 ? 1 : (rem > 256.0 ? 256 : (int)ceil(rem));
+      if (rem <= 16.0)
+        c = rem < 1.0 ? 1 : (int)ceil(rem);
+      else
+        c = rem > 512.0 ? 256 : (int)fmax(16.0, ceil(0.5 * rem));
     }
     it_prev = it;
     rel_prev = rel;

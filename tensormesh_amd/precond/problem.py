@@ -33,6 +33,7 @@ class PoissonCube:
         self.mesh = Mesh(torch.from_numpy(pts), {"tetra": torch.from_numpy(conn)},
                          {"is_boundary": torch.from_numpy(bnd)})
         self.asm = KappaLaplaceElementAssembler.from_mesh(self.mesh).to(self.device)
+        self.asm.value_dtype = torch.float32              # the systems the network was trained on
         self.load = const_node_assembler(1.0).from_mesh(self.mesh).to(self.device)()
         self.cond = Condenser(self.mesh.boundary_mask.to(self.device))
         self.inner = torch.from_numpy(~bnd).to(self.device)

@@ -104,6 +104,9 @@ def test_spatially_varying_form_runs_the_moments_kernel(cuda):
     u, v, pe = op.pattern_and_slots(conn, len(pts))
     ref = oa.reduce(pe, oa.element_matrices(pts, conn, et, "kappa_laplace", kappa=1.0 + pts[:, 0]), len(u))
     K = asm(m.points.to(cuda))
+    assert K.values.dtype == m.points.dtype                  # the reference's dtype rule
+    asm.value_dtype = torch.float32
+    K = asm(m.points.to(cuda))
     assert K.values.dtype == torch.float32
     scale = np.zeros(len(pts))
     np.maximum.at(scale, u, np.abs(ref))
