@@ -183,25 +183,48 @@ template <int NT, int DIM, typename CT, bool COEF>
 __device__ __forceinline__ void stage(const Lds& L, const int32_t* __restrict__ halo, const uint32_t* __restrict__ meta,
                                       int nacc, int nh, int nown, const double* __restrict__ X,
                                       const CT* __restrict__ coef) {
-  for (int i = threadIdx.x; i < nacc; i += NT) L.acc[i] = 0.0;
-  for (int h = threadIdx.x; h < nh; h += NT) {
-    const int64_t g = halo[h];
-    double2 n0 = make_double2(0.0, 0.0), n1 = make_double2(0.0, 0.0);
-    if (g >= 0) {
-      const double cf = COEF ? ld(coef, g) : 0.0;
-      if (DIM == 3) {
-        n0 = make_double2(X[g * 3], X[g * 3 + 1]);
-        n1 = make_double2(X[g * 3 + 2], cf);
-      } else {
-        n0 = make_double2(X[g * 2], X[g * 2 + 1]);
-        n1 = make_double2(cf, 0.0);
-      }
-    }
-    L.p0[h] = n0;
-    L.p1[h] = n1;
-    L.ms[h] = meta[h];
+  // A block holds at most 1024 nodes, so a lane stages at most K of them.  All
+  // of a lane's halo ids are loaded first, then all of its node gathers, so a
+  // block waits for two dependent memory round trips, not two per NT nodes.
+  // The loads are unconditional (indices clamped into the block's halo range,
+  // empty slots pointed at node 0, then masked): a branch between them would
+  // make the compiler wait for each before the next is issued.
+  constexpr int K = (1024 + NT - 1) / NT;
+  int32_t g[K];
+  uint32_t mh[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int h = min((int)threadIdx.x + k * NT, nh - 1);
+    g[k] = halo[h];
+    mh[k] = meta[h];
   }
+  double2 n0[K], n1[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const bool ok = g[k] >= 0;
+    const int64_t gi = ok ? g[k] : 0;
+    const double cf = COEF ? ld(coef, gi) : 0.0;
+    if (DIM == 3) {
+      const double x = X[gi * 3], y = X[gi * 3 + 1], z = X[gi * 3 + 2];
+      n0[k] = make_double2(ok ? x : 0.0, ok ? y : 0.0);
+      n1[k] = make_double2(ok ? z : 0.0, ok ? cf : 0.0);
+    } else {
+      const double x = X[gi * 2], y = X[gi * 2 + 1];
+      n0[k] = make_double2(ok ? x : 0.0, ok ? y : 0.0);
+      n1[k] = make_double2(ok ? cf : 0.0, 0.0);
+    }
+  }
+  for (int i = threadIdx.x; i < nacc; i += NT) L.acc[i] = 0.0;
   for (int a = threadIdx.x; a < nown; a += NT) L.accb[a] = 0.0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int h = threadIdx.x + k * NT;
+    if (h < nh) {
+      L.p0[h] = n0[k];
+      L.p1[h] = n1[k];
+      L.ms[h] = mh[k];
+    }
+  }
 }
 
 // Sets the diagonal of every owned row to minus the sum of its off-diagonal
@@ -264,6 +287,19 @@ __device__ __forceinline__ void load_grids(const Lds& L, int nown) {
     const int e = (int)((bits >> 52) & 0x7ffull) - 1023;
     L.ms[a] = (uint32_t)(bits ? e + 1 - 44 : 0);
     L.accb[a] = 0.0;
+  }
+}
+
+// Row stores of a block: wave w writes runs w, w + NT/64, ... of consecutive
+// CSR values from the LDS accumulators (the run index is wave-uniform, so its
+// descriptor comes through the scalar cache).
+template <int NT, typename OT>
+__device__ __forceinline__ void store_runs(const Lds& L, const PlanArgs& p, int runoff, int nrun,
+                                           OT* __restrict__ values) {
+  const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  for (int k = w; k < nrun; k += NT / kWave) {
+    const int4 ru = p.runs[runoff + k];
+    for (int t = lane; t < ru.z; t += kWave) values[(int64_t)ru.x + t] = (OT)L.acc[ru.y + t];
   }
 }
 
@@ -381,12 +417,7 @@ __global__ __launch_bounds__(NT) void k_asm_matrix(PlanArgs p, const double* __r
     rowsum_diagonal<NT>(L, nown);
     __syncthreads();
   }
-
-  const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-  for (int k = w; k < nrun; k += NT / kWave) {
-    const int4 ru = p.runs[runoff + k];
-    for (int t = lane; t < ru.z; t += kWave) values[(int64_t)ru.x + t] = (OT)L.acc[ru.y + t];
-  }
+  store_runs<NT>(L, p, runoff, nrun, values);
 }
 
 // Integrand U_a^T D V_b with U = (phi, grad phi) of the row (u) basis
@@ -481,11 +512,7 @@ __global__ __launch_bounds__(NT) void k_asm_general(PlanArgs p, const double* __
     general_records<NB, NT>(L, p, roff, nrec, nown, D, q, AddEmit{});
   }
   __syncthreads();
-  const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-  for (int k = w; k < nrun; k += NT / kWave) {
-    const int4 ru = p.runs[runoff + k];
-    for (int t = lane; t < ru.z; t += kWave) values[(int64_t)ru.x + t] = (OT)L.acc[ru.y + t];
-  }
+  store_runs<NT>(L, p, runoff, nrun, values);
 }
 
 // Element-local vertex index of record vertex a (record bits 112-119).
@@ -585,11 +612,7 @@ __global__ __launch_bounds__(NT) void k_asm_moments(PlanArgs p, const int32_t* _
     moments_records<NB, NT>(L, p, rec_elem, mom, roff, nrec, nown, AddEmit{});
   }
   __syncthreads();
-  const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-  for (int k = w; k < nrun; k += NT / kWave) {
-    const int4 ru = p.runs[runoff + k];
-    for (int t = lane; t < ru.z; t += kWave) values[(int64_t)ru.x + t] = (OT)L.acc[ru.y + t];
-  }
+  store_runs<NT>(L, p, runoff, nrun, values);
 }
 
 // Spatially varying linear forms F0(q) v + F1(q) . grad v with per-element
