@@ -303,17 +303,23 @@ __device__ __forceinline__ void store_runs(const Lds& L, const PlanArgs& p, int 
   }
 }
 
+__device__ __forceinline__ uint4 first_record(const PlanArgs& p, int roff, int nrec, int off) {
+  const int r = (int)threadIdx.x + off;
+  return r < nrec ? p.recs[roff + r] : make_uint4(0, 0, 0, 0);
+}
+
 // The element loop of one block: each lane evaluates records r = tid, tid + NT,
 // ... (nxt = the lane's first record, already loaded) and adds the owned rows
 // of each element matrix into L.acc through `em` (ds_add_f64).
 template <int NB, int FORM, int NT, typename Emit>
 __device__ __forceinline__ void asm_records(const Lds& L, const PlanArgs& p, int roff, int nrec, int nown, uint4 nxt,
-                                            double scale, const QuadArg& q, const Emit& em) {
+                                            uint4 nxt2, double scale, const QuadArg& q, const Emit& em) {
   constexpr int DIM = NB - 1;
   constexpr bool ROWSUM = FORM != 2;
   for (int r = threadIdx.x; r < nrec; r += NT) {
     const uint4 rc = nxt;
-    if (r + NT < nrec) nxt = p.recs[roff + r + NT];  // prefetch the next record
+    nxt = nxt2;
+    if (r + 2 * NT < nrec) nxt2 = p.recs[roff + r + 2 * NT];  // records two iterations ahead
     const uint64_t lo = (uint64_t)rc.x | ((uint64_t)rc.y << 32);
     int l[NB];
     double x[NB][DIM], cf[NB];
@@ -397,20 +403,20 @@ __global__ __launch_bounds__(NT) void k_asm_matrix(PlanArgs p, const double* __r
   const int hoff = bi0.x, nh = bi0.y, nown = bi0.z, roff = bi0.w;
   const int nrec = bi1.x, runoff = bi1.y, nrun = bi1.z, nacc = bi1.w;
   const Lds L = carve(smem, nacc, nh, nown);
-  // the first record is independent of the staged nodes: load it first
-  uint4 nxt = threadIdx.x < nrec ? p.recs[roff + threadIdx.x] : make_uint4(0, 0, 0, 0);
+  // a lane's first two records are independent of the staged nodes: load them first
+  uint4 nxt = first_record(p, roff, nrec, 0), nxt2 = first_record(p, roff, nrec, NT);
   stage<NT, DIM, CT, FORM == 1>(L, p.halo + hoff, p.meta + hoff, nacc, nh, nown, X, coef);
   __syncthreads();
 
   if (DET) {
-    asm_records<NB, FORM, NT>(L, p, roff, nrec, nown, nxt, scale, q, MaxEmit{});
+    asm_records<NB, FORM, NT>(L, p, roff, nrec, nown, nxt, nxt2, scale, q, MaxEmit{});
     __syncthreads();
     row_grids<NT>(L, nown);
     __syncthreads();
-    nxt = threadIdx.x < nrec ? p.recs[roff + threadIdx.x] : make_uint4(0, 0, 0, 0);
-    asm_records<NB, FORM, NT>(L, p, roff, nrec, nown, nxt, scale, q, GridEmit{});
+    asm_records<NB, FORM, NT>(L, p, roff, nrec, nown, first_record(p, roff, nrec, 0),
+                              first_record(p, roff, nrec, NT), scale, q, GridEmit{});
   } else {
-    asm_records<NB, FORM, NT>(L, p, roff, nrec, nown, nxt, scale, q, AddEmit{});
+    asm_records<NB, FORM, NT>(L, p, roff, nrec, nown, nxt, nxt2, scale, q, AddEmit{});
   }
   __syncthreads();
   if (ROWSUM) {
