@@ -183,13 +183,14 @@ template <int NT, int DIM, typename CT, bool COEF>
 __device__ __forceinline__ void stage(const Lds& L, const int32_t* __restrict__ halo, const uint32_t* __restrict__ meta,
                                       int nacc, int nh, int nown, const double* __restrict__ X,
                                       const CT* __restrict__ coef) {
-  // A block holds at most 1024 nodes, so a lane stages at most K of them.  All
-  // of a lane's halo ids are loaded first, then all of its node gathers, so a
-  // block waits for two dependent memory round trips, not two per NT nodes.
-  // The loads are unconditional (indices clamped into the block's halo range,
-  // empty slots pointed at node 0, then masked): a branch between them would
-  // make the compiler wait for each before the next is issued.
-  constexpr int K = (1024 + NT - 1) / NT;
+  // A lane stages nodes tid, tid + NT, ...: the first K of them with all halo
+  // ids loaded first, then all node gathers, so a block waits for two
+  // dependent memory round trips, not two per NT nodes.  The loads are
+  // unconditional (indices clamped into the block's halo range, empty slots
+  // pointed at node 0, then masked): a branch between them would make the
+  // compiler wait for each before the next is issued.  K covers 512 nodes;
+  // larger halos (rare: at most 1024) stage the rest node by node.
+  constexpr int K = (512 + NT - 1) / NT;
   int32_t g[K];
   uint32_t mh[K];
 #pragma unroll
@@ -224,6 +225,23 @@ __device__ __forceinline__ void stage(const Lds& L, const int32_t* __restrict__ 
       L.p1[h] = n1[k];
       L.ms[h] = mh[k];
     }
+  }
+  for (int h = threadIdx.x + K * NT; h < nh; h += NT) {
+    const int64_t gh = halo[h];
+    double2 a = make_double2(0.0, 0.0), c = make_double2(0.0, 0.0);
+    if (gh >= 0) {
+      const double cf = COEF ? ld(coef, gh) : 0.0;
+      if (DIM == 3) {
+        a = make_double2(X[gh * 3], X[gh * 3 + 1]);
+        c = make_double2(X[gh * 3 + 2], cf);
+      } else {
+        a = make_double2(X[gh * 2], X[gh * 2 + 1]);
+        c = make_double2(cf, 0.0);
+      }
+    }
+    L.p0[h] = a;
+    L.p1[h] = c;
+    L.ms[h] = meta[h];
   }
 }
 
