@@ -214,7 +214,8 @@ def batched_leg(n_mesh: int, log_fn):
     bp.kappa.copy_(torch.tensor(np.concatenate([m[3] for m in ms]), dtype=torch.float32, device=dev))
     t_setup = time.perf_counter() - t0
     log_fn(f"batched: {n_mesh} meshes, {bp.n_elements} elements, {bp.n_inner} unknowns ({t_setup:.1f}s set-up)")
-    # two workgroups per CU measured 1.2-14x slower at this size: not searched here
+    # two workgroups per CU measured 1.2-14x slower at this size (the grid barrier has twice the
+    # arrivals): not searched here; the tuner's GPU test covers that candidate
     space = {"rows_per_tile": (1024, 2048, 4096), "lanes": (4, 8), "blocks_per_cu": (1,)}
     log_fn("batched: autotuning " + ", ".join(f"{k} {v}" for k, v in space.items()))
     cfg = autotune_batched(bp, space=space, use_cache=False, maxiter=10000, verbose=True)
