@@ -37,7 +37,7 @@ extern "C" {
 #define TM_FORM_KAPPA_LAPLACE 1 /* kappa(x) grad u . grad v, kappa nodal        */
 #define TM_FORM_MASS 2          /* scale * u v                                  */
 /* OR-ed into the form of tm_asm_p1_matrix / tm_asm_p1_load, or the flags of
- * tm_asm_p1_general / tm_asm_p1_moments / tm_asm_p1_load_moments:
+ * tm_asm_p1_general / tm_asm_p1_moments / tm_asm_p1_load_moments / tm_facet_load:
  * deterministic assembly.  Every element-matrix entry of a row is rounded to
  * a multiple of a power of two 2^k per row (2^(k+44) above the row's largest
  * |entry|, so each entry moves by at most 2^-44 of that maximum) before it is
@@ -154,10 +154,14 @@ typedef struct {
  * order; 0 for the opposite vertex), q_w the facet rule's weights (summing to
  * 1 on the unit interval, 1/2 on the reference triangle), F0 [n_facets][n_q][H]
  * and F1 [n_facets][n_q][H][dim] f64 (F1 may be NULL), H <= 3, n_q <= 16.
- * b [N*H] f64 is accumulated into (FP64 atomics), not overwritten. */
+ * b [N*H] f64 is accumulated into (FP64 atomics), not overwritten.  flags:
+ * TM_ASM_DETERMINISTIC (each entry's contributions rounded to a power-of-two
+ * grid below its largest one, so the sums do not depend on the order of the
+ * atomics); that mode needs scratch, N*H doubles zeroed by the caller
+ * (NULL otherwise). */
 int tm_facet_load(int nb, int dim, int64_t n_facets, const double* coords, const int64_t* cells,
                   const int32_t* opposite, int n_q, const double* q_bary, const double* q_w, int H,
-                  const double* F0, const double* F1, double* b, void* stream);
+                  const double* F0, const double* F1, double* b, int flags, double* scratch, void* stream);
 
 /* Stiffness / mass values in the canonical CSR order (ElementAssembler.__call__,
  * element_assembler.py:243-466, for the forms above).  coords [N, dim] f64;

@@ -152,7 +152,7 @@ _SIGS = {
     "tm_asm_p1_load_moments": (c_int, [c_int, c_int, ctypes.POINTER(PlanDev), c_void_p, c_void_p, c_void_p,
                                        c_int, c_void_p]),
     "tm_facet_load": (c_int, [c_int, c_int, c_i64, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int,
-                              c_void_p, c_void_p, c_void_p, c_void_p]),
+                              c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     "tm_pcg_noalloc_max_rows": (c_i64, []),
     "tm_pcg_jacobi_noalloc": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
                                       c_double, c_double, c_i64, c_int, ctypes.POINTER(c_i64),

@@ -53,6 +53,11 @@ def select_facets(elements: torch.Tensor, boundary_mask: torch.Tensor):
 
 
 class FacetAssembler(nn.Module):
+    #: Deterministic mode of the facet kernel: each entry's contributions are
+    #: rounded to a power-of-two grid 2^-44 below the entry's largest one, so
+    #: the FP64 sums are exact and repeat bit for bit whatever the atomics' order.
+    deterministic: bool = False
+
     def __init__(self, cells: torch.Tensor, opposite: torch.Tensor, points: torch.Tensor, element_type: str,
                  quadrature_order: int = 2, *args, **kwargs):
         super().__init__()
@@ -203,11 +208,13 @@ class FacetAssembler(nn.Module):
                 p64 = points.detach().to(torch.float64).contiguous()
                 bary64 = bary.detach().to(torch.float64).contiguous()
                 w64 = w.detach().to(torch.float64).contiguous()
+                scratch = torch.zeros_like(b) if self.deterministic else None
                 _lib.check(_lib.lib().tm_facet_load(
                     ELEMENT_NB[self.element_types[0]], self.dimension, self.n_facets, _lib.ptr(p64),
                     _lib.ptr(self.cells), _lib.ptr(self.opposite), int(w64.numel()), _lib.ptr(bary64),
-                    _lib.ptr(w64), int(H), _lib.ptr(F0), _lib.ptr(F1), _lib.ptr(b), _lib.stream_of(dev)))
-                return b
+                    _lib.ptr(w64), int(H), _lib.ptr(F0), _lib.ptr(F1), _lib.ptr(b),
+                    _lib.ASM_DETERMINISTIC if self.deterministic else 0, _lib.ptr(scratch), _lib.stream_of(dev)))
+                return b if points.dtype != torch.float32 else b.to(torch.float32)
         args, dims = self._arguments(names, points, point_data, phi, grad)
         out = torch.vmap(torch.vmap(fn))(*args)                             # [F, Q, nb, ...]
         integ = torch.einsum("fqb...,fq->fb...", out, fxw.to(out.dtype))

@@ -5,7 +5,8 @@ Drop-in for ``tensormesh.NodeAssembler`` (``tensormesh/assemble/node_assembler.p
 ``from_assembler``, ``__call__(points=None, func=None, point_data=None,
 scalar_data=None, batch_size=1) -> Tensor[N * H]`` (``:242-409``).  On GPU
 tensors the recognised loads ``c * v`` and ``f * v`` (nodal ``f``) run the
-owner-computes HIP kernel (FP64 result); other loads linear in ``(v, gradv)``
+owner-computes HIP kernel (FP64 accumulation, returned in the points' dtype as the
+reference's result is); other loads linear in ``(v, gradv)``
 (coefficients from ``x``, point data and their gradients, as
 ``func_node_assembler`` builds) run the moments kernel on per-element
 quadrature moments; anything else runs the generic path.
@@ -22,6 +23,13 @@ from ..element import ELEMENT_DIM
 from ..mesh import BufferDict, Mesh
 from . import forms
 from .topology import P1Topology, topology_for
+
+
+def _in_dtype(b: torch.Tensor, points: torch.Tensor) -> torch.Tensor:
+    """The kernels accumulate in FP64; the result takes the points' floating dtype
+    (``self.type(points.dtype)``, ``node_assembler.py:290``)."""
+    dt = points.dtype if points.dtype in (torch.float32, torch.float64) else torch.float64
+    return b if b.dtype == dt else b.to(dt)
 
 
 class NodeAssembler(nn.Module):
@@ -121,7 +129,7 @@ class NodeAssembler(nn.Module):
                     _lib.check(_lib.lib().tm_asm_p1_load_moments(
                         self.topology.nb, self.topology.dim, plan.struct, _lib.ptr(p64), _lib.ptr(mom),
                         _lib.ptr(out), _lib.ASM_DETERMINISTIC if self.deterministic else 0, _lib.stream_of(dev)))
-                    return out
+                    return _in_dtype(out, points)
         if form is not None:
             kind, scale, key = form
             p64 = points.detach().to(torch.float64).contiguous()
@@ -131,7 +139,7 @@ class NodeAssembler(nn.Module):
                 b = self._launch(kind, p64, point_data[key].to(dev), 1.0)
                 if scale != 1.0:
                     b = b * scale
-            return b
+            return _in_dtype(b, points)
         integ = forms.generic_element_integrals(fn, points, conn.to(dev), etype, self.quadrature_order,
                                                 point_data, {}, scalar_data, bilinear=False)
         out = torch.zeros((self.n_points,) + tuple(integ.shape[2:]), dtype=integ.dtype, device=dev)

@@ -25,13 +25,18 @@ struct FacetRule {
   int nq;
 };
 
-template <int NB>
+// PASS 0: plain FP64 atomics into b.  Deterministic mode: PASS 1 takes the
+// largest |contribution| of every entry of b into rmax (bit patterns; non-
+// negative doubles order like them), PASS 2 adds each contribution rounded to
+// a multiple of 2^k with 2^(k+44) above that maximum: sums of up to 512 such
+// terms are exact, so b does not depend on the order of the atomics.
+template <int NB, int PASS>
 __global__ __launch_bounds__(kFacetThreads) void k_facet_load(int64_t nf, const double* __restrict__ X,
                                                              const int64_t* __restrict__ cells,
                                                              const int32_t* __restrict__ opp, FacetRule rule,
                                                              int H, const double* __restrict__ F0,
                                                              const double* __restrict__ F1,
-                                                             double* __restrict__ b) {
+                                                             double* __restrict__ b, double* __restrict__ rmax) {
   constexpr int DIM = NB - 1;
   const int64_t f = blockIdx.x * (int64_t)kFacetThreads + threadIdx.x;
   if (f >= nf) return;
@@ -142,15 +147,42 @@ __global__ __launch_bounds__(kFacetThreads) void k_facet_load(int64_t nf, const 
 #pragma unroll
   for (int a = 0; a < NB; ++a)
 #pragma unroll
-    for (int h = 0; h < kMaxH; ++h)
-      if (h < H) atomicAdd(b + node[a] * H + h, acc[a][h]);
+    for (int h = 0; h < kMaxH; ++h) {
+      if (h >= H) continue;
+      const int64_t i = node[a] * H + h;
+      if (PASS == 0) {
+        atomicAdd(b + i, acc[a][h]);
+      } else if (PASS == 1) {
+        atomicMax((unsigned long long*)(rmax + i), (unsigned long long)__double_as_longlong(fabs(acc[a][h])));
+      } else {
+        const unsigned long long bits = (unsigned long long)__double_as_longlong(rmax[i]);
+        const int k = bits ? (int)((bits >> 52) & 0x7ffull) - 1023 + 1 - 44 : 0;
+        atomicAdd(b + i, ldexp(rint(ldexp(acc[a][h], -k)), k));
+      }
+    }
 }
 
 }  // namespace
 
+template <int NB>
+void launch_facets(int pass, int grid, hipStream_t st, int64_t nf, const double* X, const int64_t* cells,
+                   const int32_t* opp, const FacetRule& rule, int H, const double* F0, const double* F1, double* b,
+                   double* rmax) {
+  if (pass == 0)
+    hipLaunchKernelGGL((k_facet_load<NB, 0>), dim3(grid), dim3(kFacetThreads), 0, st, nf, X, cells, opp, rule, H, F0,
+                       F1, b, rmax);
+  else if (pass == 1)
+    hipLaunchKernelGGL((k_facet_load<NB, 1>), dim3(grid), dim3(kFacetThreads), 0, st, nf, X, cells, opp, rule, H, F0,
+                       F1, b, rmax);
+  else
+    hipLaunchKernelGGL((k_facet_load<NB, 2>), dim3(grid), dim3(kFacetThreads), 0, st, nf, X, cells, opp, rule, H, F0,
+                       F1, b, rmax);
+}
+
 extern "C" int tm_facet_load(int nb, int dim, int64_t n_facets, const double* coords, const int64_t* cells,
                              const int32_t* opposite, int n_q, const double* q_bary, const double* q_w, int H,
-                             const double* F0, const double* F1, double* b, void* stream) {
+                             const double* F0, const double* F1, double* b, int flags, double* scratch,
+                             void* stream) {
   TM_REQUIRE((nb == 3 && dim == 2) || (nb == 4 && dim == 3), TM_ERR_ARG,
              "only P1 triangles (nb=3, dim=2) and tetrahedra (nb=4, dim=3) are supported");
   TM_REQUIRE(n_facets >= 0 && n_q >= 1 && n_q <= kMaxQ && H >= 1 && H <= kMaxH, TM_ERR_ARG,
@@ -163,14 +195,17 @@ extern "C" int tm_facet_load(int nb, int dim, int64_t n_facets, const double* co
     rule.w[q] = q_w[q];
     for (int k = 0; k < dim; ++k) rule.bary[q][k] = q_bary[q * dim + k];
   }
+  TM_REQUIRE((flags & ~TM_ASM_DETERMINISTIC) == 0, TM_ERR_ARG, "unknown flags");
+  const bool det = (flags & TM_ASM_DETERMINISTIC) != 0;
+  TM_REQUIRE(!det || scratch, TM_ERR_ARG, "the deterministic mode needs a zeroed scratch array of N*H doubles");
   hipStream_t st = (hipStream_t)stream;
   const int grid = (int)((n_facets + kFacetThreads - 1) / kFacetThreads);
-  if (nb == 3)
-    hipLaunchKernelGGL(k_facet_load<3>, dim3(grid), dim3(kFacetThreads), 0, st, n_facets, coords, cells, opposite,
-                       rule, H, F0, F1, b);
-  else
-    hipLaunchKernelGGL(k_facet_load<4>, dim3(grid), dim3(kFacetThreads), 0, st, n_facets, coords, cells, opposite,
-                       rule, H, F0, F1, b);
-  TM_HIP(hipGetLastError());
+  for (int pass = det ? 1 : 0; pass <= (det ? 2 : 0); ++pass) {
+    if (nb == 3)
+      launch_facets<3>(pass, grid, st, n_facets, coords, cells, opposite, rule, H, F0, F1, b, scratch);
+    else
+      launch_facets<4>(pass, grid, st, n_facets, coords, cells, opposite, rule, H, F0, F1, b, scratch);
+    TM_HIP(hipGetLastError());
+  }
   return TM_OK;
 }

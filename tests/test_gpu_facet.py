@@ -66,3 +66,28 @@ def test_facet_kernel_matches_oracle(cuda, et, side):
 
         b = Const.from_assembler(asm)().reshape(-1, d).sum(0).cpu().numpy()
         assert np.abs(b - np.array([1.0, -2.0, 0.5][:d])).max() <= 1e-12
+
+
+@pytest.mark.parametrize("et", ["triangle", "tetra"])
+def test_facet_deterministic_mode(cuda, et):
+    """Deterministic facet loads: bit-identical repeats, at the oracle, within
+    rounding of the default mode; points in FP32 give an FP32 result."""
+    print(f"case facet deterministic {et}", flush=True)
+    pts, conn, bnd = _case(et)
+    mesh = tm.Mesh(torch.from_numpy(pts), {et: torch.from_numpy(conn)}, {"is_boundary": torch.from_numpy(bnd)})
+
+    class Robin(tm.FacetAssembler):
+        def forward(self, v, x):
+            return (torch.cos(3.0 * x[0]) + x[-1]) * v
+
+    asm = Robin.from_mesh(mesh).to(cuda)
+    asm.deterministic = True
+    runs = [asm().clone() for _ in range(4)]
+    assert all(torch.equal(runs[0], r) for r in runs[1:])
+    ref = of.load_vector(pts, conn, et, bnd, lambda v, gv, at: (np.cos(3.0 * at["x"][:, :1]) + at["x"][:, -1:]) * v)
+    got = runs[0].cpu().numpy()
+    assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+    asm.deterministic = False
+    assert np.abs(asm().cpu().numpy() - got).max() <= 1e-13 * np.abs(ref).max()
+    b32 = Robin.from_mesh(mesh).to(cuda).type(torch.float32)()
+    assert b32.dtype == torch.float32 and np.abs(b32.double().cpu().numpy() - ref).max() <= 1e-6 * np.abs(ref).max()
