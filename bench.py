@@ -695,8 +695,11 @@ def main():
         torch.cuda.synchronize()
         t_solve = ev0.elapsed_time(ev1) / 1e3
         true_rel = ((fi.double() - Ki @ xs).norm() / fi.double().norm()).item()
-        b_iter = 8 * nnzi + 4 * (ni + 1) + 96 * ni
-        out["solve"] = {"method": "jacobi-pcg", "n_inner": ni, "nnz_inner": nnzi, "iterations": it,
+        from tensormesh_amd.sparse.ops import col_delta16
+        d16 = col_delta16(csr) is not None      # int16 column offsets: 6 instead of 8 bytes per entry
+        b_iter = (6 if d16 else 8) * nnzi + 4 * (ni + 1) + 96 * ni
+        out["solve"] = {"method": "jacobi-pcg", "columns": "int16 offsets" if d16 else "int32",
+                        "n_inner": ni, "nnz_inner": nnzi, "iterations": it,
                         "converged": ok, "relres_recursive": rel, "relres_true_fp32_matrix": true_rel,
                         "time_to_1e-8_s": round(t_solve, 4), "ms_per_iter": round(t_solve / max(it, 1) * 1e3, 4),
                         "pcg_iter_GBps": round(b_iter / (t_solve / max(it, 1)) / 1e9, 1),

@@ -303,6 +303,18 @@ int tm_bicgstab_jacobi(int64_t n, int64_t nnz, const int32_t* row_ptr, const int
 int tm_spmv_bsr3(int64_t n_block_rows, int64_t nnz, const int32_t* row_ptr, const int32_t* col_idx,
                  const void* values, int value_dtype, const double* x, double* y, void* stream);
 
+/* Compact column format for the scalar Jacobi-PCG (no counterpart in the
+ * reference, which streams int64 indices): col_delta[k] = col_idx[k] - row as
+ * int16 for every entry of rows 0..n-1; *fits (device int) is set to 1, then
+ * cleared if some entry lies more than 32767 from the diagonal (col_delta is
+ * then unusable).  Stream-ordered; the caller reads *fits. */
+int tm_csr_delta16(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, int16_t* col_delta, int* fits,
+                   void* stream);
+/* tm_pcg_jacobi with the columns given as tm_csr_delta16 offsets: the same
+ * iteration (bit-identical iterates), 2 bytes less per matrix entry and SpMV. */
+int tm_pcg_jacobi_d16(int64_t n, const int32_t* row_ptr, const int16_t* col_delta, const void* values,
+                      int value_dtype, const double* b, double* x, void* work, double rtol, double atol,
+                      int64_t maxiter, int check_every, int64_t* iters, double* relres, void* stream);
 /* Jacobi-PCG on a 3x3 block-CSR matrix (point Jacobi on the block diagonals);
  * semantics as tm_pcg_jacobi with n = 3*n_block_rows unknowns; work must hold
  * tm_pcg_work_size(3*n_block_rows) bytes. */

@@ -147,6 +147,10 @@ _SIGS = {
     "tm_pcg_jacobi": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                               c_double, c_double, c_i64, c_int, ctypes.POINTER(c_i64),
                               ctypes.POINTER(c_double), c_void_p]),
+    "tm_csr_delta16": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "tm_pcg_jacobi_d16": (c_int, [c_i64, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                                  c_double, c_double, c_i64, c_int, ctypes.POINTER(c_i64),
+                                  ctypes.POINTER(c_double), c_void_p]),
     "tm_asm_p1_moments": (c_int, [c_int, c_int, ctypes.POINTER(PlanDev), c_void_p, c_void_p, c_void_p, c_int,
                                   c_int, c_void_p]),
     "tm_asm_p1_load_moments": (c_int, [c_int, c_int, ctypes.POINTER(PlanDev), c_void_p, c_void_p, c_void_p,

@@ -24,6 +24,7 @@
 namespace {
 
 using tmx::kWave;
+using tmx::col_of;
 using tmx::row_accum;
 using tmx::row_chunk;
 using tmx::RowChunk;
@@ -74,9 +75,9 @@ PcgWork carve_work(void* work, int64_t n) {
 constexpr int P_PQ = 0, P_RZ = 1, P_RR = 2, P_BB = 3;
 
 // y = A x; each S-lane group handles R rows per trip (rows row, row + stride, ...).
-template <typename VT, int S, int BS, int R>
+template <typename VT, int S, int BS, int R, typename CI = int32_t>
 __global__ __launch_bounds__(kThreads) void k_spmv(int64_t nrows, const int32_t* __restrict__ rp,
-                                                  const int32_t* __restrict__ ci, const VT* __restrict__ v,
+                                                  const CI* __restrict__ ci, const VT* __restrict__ v,
                                                   const double* __restrict__ x, double* __restrict__ y) {
   const int sl = threadIdx.x % S;
   constexpr int64_t stride = kThreads / S;
@@ -88,7 +89,8 @@ __global__ __launch_bounds__(kThreads) void k_spmv(int64_t nrows, const int32_t*
     for (int r = 0; r < R; ++r) {
 #pragma unroll
       for (int i = 0; i < BS; ++i) o[r][i] = 0.0;
-      if (row + r * stride < rc.r1) row_accum<VT, S, BS>(rp, ci, v, x, row + r * stride, sl, lim, o[r]);
+      if (row + r * stride < rc.r1)
+        row_accum<VT, S, BS, (S <= 4 ? 4 : 2), CI>(rp, ci, v, x, row + r * stride, sl, lim, o[r]);
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -113,9 +115,9 @@ __device__ __forceinline__ void write_parts(double* part, const int (&which)[NS]
   tmx::write_parts<kThreads, NS>(part, kMaxBlocks, which, loc);
 }
 
-template <typename VT, int BS>
+template <typename VT, int BS, typename CI = int32_t>
 __global__ __launch_bounds__(kThreads) void k_diag_inv(int64_t nrows, const int32_t* __restrict__ rp,
-                                                      const int32_t* __restrict__ ci, const VT* __restrict__ v,
+                                                      const CI* __restrict__ ci, const VT* __restrict__ v,
                                                       double* __restrict__ dinv) {
   for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < nrows; i += (int64_t)gridDim.x * kThreads) {
     double d[BS];
@@ -123,7 +125,7 @@ __global__ __launch_bounds__(kThreads) void k_diag_inv(int64_t nrows, const int3
     for (int c = 0; c < BS; ++c) d[c] = 1.0;  // missing diagonal -> 1 (torch_solve.py:24-45)
     bool found = false;
     for (int k = rp[i]; k < rp[i + 1]; ++k)
-      if (ci[k] == i) {
+      if (col_of(ci, i, k) == i) {
 #pragma unroll
         for (int c = 0; c < BS; ++c) {
           const double vk = (double)v[(int64_t)k * BS * BS + c * BS + c];
@@ -174,9 +176,9 @@ __global__ __launch_bounds__(kThreads) void k_pcg_ctl0(PcgWork w, int nparts, do
 }
 
 // K1: q = A p, partial (p, q)
-template <typename VT, int S, int BS>
+template <typename VT, int S, int BS, typename CI = int32_t>
 __global__ __launch_bounds__(kThreads) void k_pcg_spmv(int64_t nrows, const int32_t* __restrict__ rp,
-                                                      const int32_t* __restrict__ ci, const VT* __restrict__ v,
+                                                      const CI* __restrict__ ci, const VT* __restrict__ v,
                                                       PcgWork w) {
   if (blockIdx.x == 0 && threadIdx.x == 0) w.f->step = 0;
   if (w.f->status != 0) return;
@@ -192,7 +194,8 @@ __global__ __launch_bounds__(kThreads) void k_pcg_spmv(int64_t nrows, const int3
     for (int r = 0; r < R; ++r) {
 #pragma unroll
       for (int i = 0; i < BS; ++i) o[r][i] = 0.0;
-      if (row + r * stride < rc.r1) row_accum<VT, S, BS>(rp, ci, v, w.p, row + r * stride, sl, lim, o[r]);
+      if (row + r * stride < rc.r1)
+        row_accum<VT, S, BS, (S <= 4 ? 4 : 2), CI>(rp, ci, v, w.p, row + r * stride, sl, lim, o[r]);
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -327,16 +330,16 @@ struct Poller {
   }
 };
 
-template <typename VT, int S, int BS>
-int pcg_run(int64_t nrows, const int32_t* rp, const int32_t* ci, const VT* v, const double* b, double* x, PcgWork w,
+template <typename VT, int S, int BS, typename CI = int32_t>
+int pcg_run(int64_t nrows, const int32_t* rp, const CI* ci, const VT* v, const double* b, double* x, PcgWork w,
             double rtol, double atol, int64_t maxiter, int check_every, int64_t* iters, double* relres,
             hipStream_t st) {
   const int64_t n = nrows * BS;
   // one grid size for every PCG kernel: the partial arrays have G entries
   const int G = grid_for(n, kThreads * 2);
-  hipLaunchKernelGGL((k_diag_inv<VT, BS>), dim3(grid_for(nrows, kThreads)), dim3(kThreads), 0, st, nrows, rp, ci, v,
-                     w.dinv);
-  hipLaunchKernelGGL((k_spmv<VT, S, BS, 2>), dim3(G), dim3(kThreads), 0, st, nrows, rp, ci, v, x, w.q);
+  hipLaunchKernelGGL((k_diag_inv<VT, BS, CI>), dim3(grid_for(nrows, kThreads)), dim3(kThreads), 0, st, nrows, rp, ci,
+                     v, w.dinv);
+  hipLaunchKernelGGL((k_spmv<VT, S, BS, 2, CI>), dim3(G), dim3(kThreads), 0, st, nrows, rp, ci, v, x, w.q);
   hipLaunchKernelGGL(k_pcg_init, dim3(G), dim3(kThreads), 0, st, n, b, w);
   hipLaunchKernelGGL(k_pcg_ctl0, dim3(1), dim3(kThreads), 0, st, w, G, rtol * rtol, atol * atol, (long long)maxiter);
   TM_HIP(hipGetLastError());
@@ -356,7 +359,7 @@ int pcg_run(int64_t nrows, const int32_t* rp, const int32_t* ci, const VT* v, co
     if (hf.status != 0) break;
     if (hf.iter > 0) chunk = poll.next(hf.iter, hs.relres, hs.bb > 0.0 ? sqrt(hs.tol2 / hs.bb) : 0.0);
     for (int k = 0; k < chunk; ++k) {
-      hipLaunchKernelGGL((k_pcg_spmv<VT, S, BS>), dim3(G), dim3(kThreads), 0, st, nrows, rp, ci, v, w);
+      hipLaunchKernelGGL((k_pcg_spmv<VT, S, BS, CI>), dim3(G), dim3(kThreads), 0, st, nrows, rp, ci, v, w);
       hipLaunchKernelGGL(k_pcg_update, dim3(G), dim3(kThreads), 0, st, n, w, G);
       hipLaunchKernelGGL(k_pcg_dir, dim3(G), dim3(kThreads), 0, st, n, x, w, G);
     }
@@ -408,16 +411,34 @@ int spmv_dispatch(int64_t n, int64_t nnz, const int32_t* rp, const int32_t* ci, 
   return TM_OK;
 }
 
-template <typename VT, int BS>
-int pcg_dispatch(int64_t n, int64_t nnz, const int32_t* rp, const int32_t* ci, const VT* v, const double* b,
+template <typename VT, int BS, typename CI = int32_t>
+int pcg_dispatch(int64_t n, int64_t nnz, const int32_t* rp, const CI* ci, const VT* v, const double* b,
                  double* x, PcgWork w, double rtol, double atol, int64_t maxiter, int check_every,
                  int64_t* iters, double* relres, hipStream_t st) {
   switch (lanes_for(n, nnz)) {
-    case 2: return pcg_run<VT, 2, BS>(n, rp, ci, v, b, x, w, rtol, atol, maxiter, check_every, iters, relres, st);
-    case 4: return pcg_run<VT, 4, BS>(n, rp, ci, v, b, x, w, rtol, atol, maxiter, check_every, iters, relres, st);
-    case 8: return pcg_run<VT, 8, BS>(n, rp, ci, v, b, x, w, rtol, atol, maxiter, check_every, iters, relres, st);
-    case 16: return pcg_run<VT, 16, BS>(n, rp, ci, v, b, x, w, rtol, atol, maxiter, check_every, iters, relres, st);
-    default: return pcg_run<VT, 32, BS>(n, rp, ci, v, b, x, w, rtol, atol, maxiter, check_every, iters, relres, st);
+    case 2: return pcg_run<VT, 2, BS, CI>(n, rp, ci, v, b, x, w, rtol, atol, maxiter, check_every, iters, relres, st);
+    case 4: return pcg_run<VT, 4, BS, CI>(n, rp, ci, v, b, x, w, rtol, atol, maxiter, check_every, iters, relres, st);
+    case 8: return pcg_run<VT, 8, BS, CI>(n, rp, ci, v, b, x, w, rtol, atol, maxiter, check_every, iters, relres, st);
+    case 16:
+      return pcg_run<VT, 16, BS, CI>(n, rp, ci, v, b, x, w, rtol, atol, maxiter, check_every, iters, relres, st);
+    default:
+      return pcg_run<VT, 32, BS, CI>(n, rp, ci, v, b, x, w, rtol, atol, maxiter, check_every, iters, relres, st);
+  }
+}
+
+// Column deltas col - row as int16; fits[0] is cleared when an entry lies
+// more than 32767 from the diagonal (the caller sets it to 1 first).
+__global__ __launch_bounds__(kThreads) void k_delta16(int64_t nrows, const int32_t* __restrict__ rp,
+                                                     const int32_t* __restrict__ ci, int16_t* __restrict__ cd,
+                                                     int* __restrict__ fits) {
+  for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < nrows; i += (int64_t)gridDim.x * kThreads) {
+    bool ok = true;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+      const int64_t d = (int64_t)ci[k] - i;
+      ok = ok && d >= -32767 && d <= 32767;
+      cd[k] = (int16_t)(ok ? d : 0);
+    }
+    if (!ok) atomicAnd(fits, 0);
   }
 }
 
@@ -767,6 +788,34 @@ extern "C" int tm_pcg_jacobi(int64_t n, const int32_t* row_ptr, const int32_t* c
                                   check_every, iters, relres, st);
   return pcg_dispatch<double, 1>(n, nnz_h, row_ptr, col_idx, (const double*)values, b, x, w, rtol, atol, maxiter,
                                  check_every, iters, relres, st);
+}
+
+extern "C" int tm_csr_delta16(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, int16_t* col_delta,
+                              int* fits, void* stream) {
+  TM_REQUIRE(n > 0 && row_ptr && col_idx && col_delta && fits, TM_ERR_ARG, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  const int one = 1;
+  TM_HIP(hipMemcpyAsync(fits, &one, sizeof(int), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_delta16, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, st, n, row_ptr, col_idx, col_delta,
+                     fits);
+  TM_HIP(hipGetLastError());
+  return TM_OK;
+}
+
+extern "C" int tm_pcg_jacobi_d16(int64_t n, const int32_t* row_ptr, const int16_t* col_delta, const void* values,
+                                 int value_dtype, const double* b, double* x, void* work, double rtol, double atol,
+                                 int64_t maxiter, int check_every, int64_t* iters, double* relres, void* stream) {
+  TM_REQUIRE(n > 0 && row_ptr && col_delta && values && b && x && work, TM_ERR_ARG, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  int32_t nnz_h = 0;
+  TM_HIP(hipMemcpyAsync(&nnz_h, row_ptr + n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  TM_HIP(hipStreamSynchronize(st));
+  PcgWork w = carve_work(work, n);
+  if (value_dtype == TM_F32)
+    return pcg_dispatch<float, 1, int16_t>(n, nnz_h, row_ptr, col_delta, (const float*)values, b, x, w, rtol, atol,
+                                           maxiter, check_every, iters, relres, st);
+  return pcg_dispatch<double, 1, int16_t>(n, nnz_h, row_ptr, col_delta, (const double*)values, b, x, w, rtol, atol,
+                                          maxiter, check_every, iters, relres, st);
 }
 
 extern "C" int tm_pcg_jacobi_bsr3(int64_t n_block_rows, const int32_t* row_ptr, const int32_t* col_idx,

@@ -8,6 +8,15 @@
 namespace tmx {
 
 typedef int32_t i32x4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef int16_t i16x4u __attribute__((ext_vector_type(4), aligned(2)));
+
+// Column index of CSR entry k of row `row`: ci is either the int32 column
+// array or an int16 array of column - row (a compact copy for matrices whose
+// entries all lie within 32767 of the diagonal, 2 bytes per entry less).
+__device__ __forceinline__ int64_t col_of(const int32_t* __restrict__ ci, int64_t, int k) { return ci[k]; }
+__device__ __forceinline__ int64_t col_of(const int16_t* __restrict__ ci, int64_t row, int k) {
+  return row + ci[k];
+}
 typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));
 typedef double f64x2u __attribute__((ext_vector_type(2), aligned(8)));
 
@@ -29,8 +38,8 @@ typedef double f64x2u __attribute__((ext_vector_type(2), aligned(8)));
 // so a lane keeps U independent chains in flight.  FP32 3x3 blocks read their
 // 36 bytes as two 16-byte loads and one 4-byte load, and the three x entries
 // as one 16-byte and one 8-byte load (same products, same order).
-template <typename VT, int S, int BS, int U = (S <= 4 ? 4 : 2)>
-__device__ __forceinline__ void row_accum(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+template <typename VT, int S, int BS, int U = (S <= 4 ? 4 : 2), typename CI = int32_t>
+__device__ __forceinline__ void row_accum(const int32_t* __restrict__ rp, const CI* __restrict__ ci,
                                           const VT* __restrict__ v, const double* __restrict__ x, int64_t row,
                                           int sl, int lim, double (&out)[BS]) {
   const int b = rp[row], e = rp[row + 1];
@@ -39,14 +48,19 @@ __device__ __forceinline__ void row_accum(const int32_t* __restrict__ rp, const 
       int c[4];
       float a[4];
       if (k0 + 4 <= lim) {
-        const i32x4u cv = *(const i32x4u*)(ci + k0);
+        if constexpr (std::is_same<CI, int16_t>::value) {
+          const i16x4u cv = *(const i16x4u*)(ci + k0);  // one 8-byte load
+          c[0] = (int)row + cv.x, c[1] = (int)row + cv.y, c[2] = (int)row + cv.z, c[3] = (int)row + cv.w;
+        } else {
+          const i32x4u cv = *(const i32x4u*)(ci + k0);
+          c[0] = cv.x, c[1] = cv.y, c[2] = cv.z, c[3] = cv.w;
+        }
         const f32x4u av = *(const f32x4u*)(v + k0);
-        c[0] = cv.x, c[1] = cv.y, c[2] = cv.z, c[3] = cv.w;
         a[0] = av.x, a[1] = av.y, a[2] = av.z, a[3] = av.w;
       } else {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          c[u] = k0 + u < e ? ci[k0 + u] : 0;
+          c[u] = k0 + u < e ? (int)col_of(ci, row, k0 + u) : 0;
           a[u] = k0 + u < e ? v[k0 + u] : 0.0f;
         }
       }
@@ -66,7 +80,7 @@ __device__ __forceinline__ void row_accum(const int32_t* __restrict__ rp, const 
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         ok[u] = k0 + u * S < e;
-        c[u] = ok[u] ? ci[k0 + u * S] : 0;
+        c[u] = ok[u] ? col_of(ci, row, k0 + u * S) : 0;
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -97,7 +111,7 @@ __device__ __forceinline__ void row_accum(const int32_t* __restrict__ rp, const 
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       ok[u] = k0 + u * S < e;
-      c[u] = ok[u] ? ci[k0 + u * S] : 0;
+      c[u] = ok[u] ? col_of(ci, row, k0 + u * S) : 0;
     }
 #pragma unroll
     for (int u = 0; u < U; ++u)

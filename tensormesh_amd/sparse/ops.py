@@ -11,7 +11,7 @@ from __future__ import annotations
 
 import ctypes
 import warnings
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 from typing import Optional
 
 import torch
@@ -26,10 +26,30 @@ class CSR:
     col_idx: torch.Tensor
     n_rows: int
     n_cols: int
+    # int16 copy of col - row (built on first use by the scalar solver; False when
+    # some entry lies more than 32767 from the diagonal)
+    col_delta: Optional[object] = field(default=None, repr=False, compare=False)
 
     @property
     def nnz(self) -> int:
         return int(self.col_idx.numel())
+
+
+#: Scalar Jacobi-PCG reads column indices as int16 offsets from the diagonal when
+#: every entry of the matrix fits (2 bytes per entry less to stream per SpMV).
+COL_DELTA16 = True
+
+
+def col_delta16(csr: CSR) -> Optional[torch.Tensor]:
+    """The int16 ``col - row`` array of ``csr`` (cached), or None if an entry does not fit."""
+    if csr.col_delta is None:
+        dev = csr.col_idx.device
+        cd = torch.empty(max(csr.nnz, 1), dtype=torch.int16, device=dev)
+        fits = torch.empty(1, dtype=torch.int32, device=dev)
+        _lib.check(_lib.lib().tm_csr_delta16(csr.n_rows, _lib.ptr(csr.row_ptr), _lib.ptr(csr.col_idx), _lib.ptr(cd),
+                                             _lib.ptr(fits), _lib.stream_of(dev)))
+        csr.col_delta = cd if int(fits.item()) == 1 else False
+    return csr.col_delta if csr.col_delta is not False else None
 
 
 def _spmv_raw(csr: CSR, values: torch.Tensor, x: torch.Tensor, block_size: int = 1) -> torch.Tensor:
@@ -101,8 +121,12 @@ def pcg_raw(csr: CSR, values: torch.Tensor, b: torch.Tensor, x0: Optional[torch.
     work = torch.empty(int(L.tm_pcg_work_size(n)) // 8 + 1, dtype=torch.float64, device=dev)
     it = ctypes.c_int64(0)
     rr = ctypes.c_double(0.0)
-    fn = L.tm_pcg_jacobi if block_size == 1 else L.tm_pcg_jacobi_bsr3
-    st = fn(csr.n_rows, _lib.ptr(csr.row_ptr), _lib.ptr(csr.col_idx), _lib.ptr(values.detach().contiguous()),
+    fn, cols = (L.tm_pcg_jacobi, csr.col_idx) if block_size == 1 else (L.tm_pcg_jacobi_bsr3, csr.col_idx)
+    if block_size == 1 and COL_DELTA16 and csr.n_rows > 0:
+        cd = col_delta16(csr)
+        if cd is not None:
+            fn, cols = L.tm_pcg_jacobi_d16, cd
+    st = fn(csr.n_rows, _lib.ptr(csr.row_ptr), _lib.ptr(cols), _lib.ptr(values.detach().contiguous()),
                          _lib.dtype_code(values), _lib.ptr(b64), _lib.ptr(x), _lib.ptr(work), float(rtol),
                          float(atol), int(maxiter), int(check_every), ctypes.byref(it), ctypes.byref(rr),
                          _lib.stream_of(dev))
