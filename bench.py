@@ -687,7 +687,10 @@ def main():
         torch.cuda.synchronize()
         t_spmv = ev0.elapsed_time(ev1) / 1e3 / 20
         b_spmv = 4 * (ni + 1) + 8 * nnzi + 16 * ni
-        pcg_raw(csr, Ki.values, fi, rtol=1e-8, maxiter=50)           # warm-up (kernels, allocator)
+        import warnings
+        with warnings.catch_warnings():                              # the warm-up stops early on purpose
+            warnings.simplefilter("ignore")
+            pcg_raw(csr, Ki.values, fi, rtol=1e-8, maxiter=50)       # warm-up (kernels, allocator)
         torch.cuda.synchronize()
         ev0.record(stream)
         xs, it, rel, ok = pcg_raw(csr, Ki.values, fi, rtol=1e-8, maxiter=20000)
