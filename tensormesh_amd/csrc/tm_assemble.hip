@@ -247,16 +247,26 @@ __device__ __forceinline__ void stage(const Lds& L, const int32_t* __restrict__ 
 
 // Sets the diagonal of every owned row to minus the sum of its off-diagonal
 // entries (exact for forms whose element rows sum to zero: sum_b c_b = 0).
+// A row's entries are read 16 at a time, all reads issued before the first
+// add (one LDS round trip per 16 entries instead of one per entry); the adds
+// run in entry order, so the sum is the same as a plain loop's.
 template <int NT>
 __device__ __forceinline__ void rowsum_diagonal(const Lds& L, int nown) {
+  constexpr int C = 16;
   for (int a = threadIdx.x; a < nown; a += NT) {
     const uint32_t m = L.ms[a];
     const int b0 = meta_off(m), len = meta_len(m);
     if (len == 0) continue;
-    const int d = b0 + meta_diag(m);
+    const int d = meta_diag(m);
     double s = 0.0;
-    for (int k = b0; k < b0 + len; ++k) s += (k == d) ? 0.0 : L.acc[k];
-    L.acc[d] = -s;
+    for (int k0 = 0; k0 < len; k0 += C) {
+      double v[C];
+#pragma unroll
+      for (int k = 0; k < C; ++k) v[k] = L.acc[b0 + min(k0 + k, len - 1)];
+#pragma unroll
+      for (int k = 0; k < C; ++k) s += (k0 + k < len && k0 + k != d) ? v[k] : 0.0;
+    }
+    L.acc[b0 + d] = -s;
   }
 }
 
@@ -309,14 +319,32 @@ __device__ __forceinline__ void load_grids(const Lds& L, int nown) {
 }
 
 // Row stores of a block: wave w writes runs w, w + NT/64, ... of consecutive
-// CSR values from the LDS accumulators (the run index is wave-uniform, so its
-// descriptor comes through the scalar cache).
+// CSR values from the LDS accumulators.  Lane j of wave w fetches the
+// descriptor of the wave's j-th run (run_descriptors, one vector load issued
+// ahead of use); the store loop takes it from that lane (readlane: a
+// wave-uniform value), so no run waits for a load of its own.  Runs beyond
+// the wave's 64th (blocks of more than 64 NT/64 runs) load their descriptor
+// in the loop.
+template <int NT>
+__device__ __forceinline__ int4 run_descriptors(const PlanArgs& p, int runoff, int nrun) {
+  const int k = (int)(threadIdx.x / kWave) + (NT / kWave) * (int)(threadIdx.x % kWave);
+  return k < nrun ? p.runs[runoff + k] : make_int4(0, 0, 0, 0);
+}
 template <int NT, typename OT>
-__device__ __forceinline__ void store_runs(const Lds& L, const PlanArgs& p, int runoff, int nrun,
+__device__ __forceinline__ void store_runs(const Lds& L, const PlanArgs& p, int runoff, int nrun, const int4& dsc,
                                            OT* __restrict__ values) {
   const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-  for (int k = w; k < nrun; k += NT / kWave) {
-    const int4 ru = p.runs[runoff + k];
+  for (int j = 0;; ++j) {
+    const int k = w + (NT / kWave) * j;
+    if (k >= nrun) break;
+    int4 ru;
+    if (j < kWave) {
+      ru.x = __builtin_amdgcn_readlane(dsc.x, j);
+      ru.y = __builtin_amdgcn_readlane(dsc.y, j);
+      ru.z = __builtin_amdgcn_readlane(dsc.z, j);
+    } else {
+      ru = p.runs[runoff + k];
+    }
     for (int t = lane; t < ru.z; t += kWave) values[(int64_t)ru.x + t] = (OT)L.acc[ru.y + t];
   }
 }
@@ -326,18 +354,56 @@ __device__ __forceinline__ uint4 first_record(const PlanArgs& p, int roff, int n
   return r < nrec ? p.recs[roff + r] : make_uint4(0, 0, 0, 0);
 }
 
+// A block's records read through a raw buffer resource whose range ends after
+// the block's last record: a load beyond it returns zeros (the buffer unit's
+// range check), so loads ahead need no branch, and a record's address is a
+// 32-bit offset from a wave-uniform base.
+constexpr int kRawBufferWord3 = 0x00020000;  // gfx9 descriptor word 3: 32-bit data format, no swizzle
+struct RecBuf {
+  __amdgpu_buffer_rsrc_t rs;
+  __device__ __forceinline__ RecBuf(const PlanArgs& p, int roff, int nrec)
+      : rs(__builtin_amdgcn_make_buffer_rsrc((void*)(p.recs + roff), 0, nrec * 16, kRawBufferWord3)) {}
+  __device__ __forceinline__ uint4 operator[](int r) const {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, r * 16, 0, 0);
+    return make_uint4(v[0], v[1], v[2], v[3]);
+  }
+};
+
+// Owned rows of one element matrix when record vertices [0, K) are owned and
+// the others are not (records list their owned vertices first): each pair
+// (a, b), a < b, with a owned is formed once and added to row a, and to row b
+// when b is owned too.
+template <int NB, int K, int FORM, typename Emit>
+__device__ __forceinline__ void emit_owned(const Lds& L, const uint4& rc, const int (&l)[NB], const uint32_t (&m)[NB],
+                                           const double (&c)[NB][NB - 1], double s, const QuadArg& q,
+                                           const Emit& em) {
+  constexpr int DIM = NB - 1;
+  if (FORM == 2) {
+#pragma unroll
+    for (int a = 0; a < K; ++a) em(L, l[a], L.acc + meta_off(m[a]) + meta_diag(m[a]), s * q.M[a * NB + a]);
+  }
+#pragma unroll
+  for (int a = 0; a < K; ++a)
+#pragma unroll
+    for (int bb = a + 1; bb < NB; ++bb) {
+      const double kab = FORM == 2 ? s * q.M[a * NB + bb] : s * dotc<DIM>(c[a], c[bb]);
+      em(L, l[a], L.acc + meta_off(m[a]) + rec_rank<NB>(rc, a, bb), kab);
+      if (bb < K) em(L, l[bb], L.acc + meta_off(m[bb]) + rec_rank<NB>(rc, bb, a), kab);
+    }
+}
+
 // The element loop of one block: each lane evaluates records r = tid, tid + NT,
 // ... (nxt = the lane's first record, already loaded) and adds the owned rows
 // of each element matrix into L.acc through `em` (ds_add_f64).
 template <int NB, int FORM, int NT, typename Emit>
-__device__ __forceinline__ void asm_records(const Lds& L, const PlanArgs& p, int roff, int nrec, int nown, uint4 nxt,
+__device__ __forceinline__ void asm_records(const Lds& L, const RecBuf& rb, int nrec, int nown, uint4 nxt,
                                             uint4 nxt2, double scale, const QuadArg& q, const Emit& em) {
   constexpr int DIM = NB - 1;
   constexpr bool ROWSUM = FORM != 2;
   for (int r = threadIdx.x; r < nrec; r += NT) {
     const uint4 rc = nxt;
     nxt = nxt2;
-    if (r + 2 * NT < nrec) nxt2 = p.recs[roff + r + 2 * NT];  // records two iterations ahead
+    nxt2 = rb[r + 2 * NT];  // records two iterations ahead (zeros past the block's end)
     const uint64_t lo = (uint64_t)rc.x | ((uint64_t)rc.y << 32);
     int l[NB];
     double x[NB][DIM], cf[NB];
@@ -365,27 +431,26 @@ __device__ __forceinline__ void asm_records(const Lds& L, const PlanArgs& p, int
       s = scale * adet;
     }
     bool own[NB];
-    bool all = true;
+    int nk = 0;
 #pragma unroll
     for (int a = 0; a < NB; ++a) {
       own[a] = l[a] < nown;
-      all = all && own[a];
+      nk += own[a] ? 1 : 0;
     }
-    if (__all(all)) {
-      // every vertex owned in every lane: no branches (records are sorted so
-      // that the block's interior elements form whole waves)
-      if (!ROWSUM) {
-#pragma unroll
-        for (int a = 0; a < NB; ++a) em(L, l[a], L.acc + meta_off(m[a]) + meta_diag(m[a]), s * q.M[a * NB + a]);
+    // The plan groups a block's records by owned count, so most waves share
+    // one count: they run a branch-free instance that forms only the pairs
+    // with an owned vertex.
+    const int k0 = __builtin_amdgcn_readfirstlane(nk);
+    if (__all(nk == k0)) {
+      if (k0 == NB) {
+        emit_owned<NB, NB, FORM>(L, rc, l, m, c, s, q, em);
+      } else if (k0 == 1) {
+        emit_owned<NB, 1, FORM>(L, rc, l, m, c, s, q, em);
+      } else if (k0 == 2) {
+        emit_owned<NB, 2, FORM>(L, rc, l, m, c, s, q, em);
+      } else if (NB == 4 && k0 == 3) {
+        emit_owned<NB, (NB == 4 ? 3 : 1), FORM>(L, rc, l, m, c, s, q, em);
       }
-#pragma unroll
-      for (int a = 0; a < NB; ++a)
-#pragma unroll
-        for (int bb = a + 1; bb < NB; ++bb) {
-          const double kab = FORM == 2 ? s * q.M[a * NB + bb] : s * dotc<DIM>(c[a], c[bb]);
-          em(L, l[a], L.acc + meta_off(m[a]) + rec_rank<NB>(rc, a, bb), kab);
-          em(L, l[bb], L.acc + meta_off(m[bb]) + rec_rank<NB>(rc, bb, a), kab);
-        }
       continue;
     }
     if (!ROWSUM) {
@@ -408,8 +473,10 @@ __device__ __forceinline__ void asm_records(const Lds& L, const PlanArgs& p, int
 
 // FORM: 0 laplace, 1 kappa-laplace (nodal coef), 2 mass; all multiplied by scale.
 // DET: deterministic mode (entries rounded to per-row grids, see MaxEmit / GridEmit).
+// At most 64 VGPRs (8 waves per SIMD): eight 20 KiB blocks per CU overlap their
+// phases only if all of their waves are resident.
 template <int NB, int FORM, typename CT, typename OT, int NT, bool DET>
-__global__ __launch_bounds__(NT) void k_asm_matrix(PlanArgs p, const double* __restrict__ X,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_asm_matrix(PlanArgs p, const double* __restrict__ X,
                                                   const CT* __restrict__ coef, double scale, QuadArg q,
                                                   OT* __restrict__ values) {
   constexpr int DIM = NB - 1;
@@ -422,26 +489,28 @@ __global__ __launch_bounds__(NT) void k_asm_matrix(PlanArgs p, const double* __r
   const int nrec = bi1.x, runoff = bi1.y, nrun = bi1.z, nacc = bi1.w;
   const Lds L = carve(smem, nacc, nh, nown);
   // a lane's first two records are independent of the staged nodes: load them first
-  uint4 nxt = first_record(p, roff, nrec, 0), nxt2 = first_record(p, roff, nrec, NT);
+  const RecBuf rb(p, roff, nrec);
+  const int t = threadIdx.x;
+  uint4 nxt = rb[t], nxt2 = rb[t + NT];
   stage<NT, DIM, CT, FORM == 1>(L, p.halo + hoff, p.meta + hoff, nacc, nh, nown, X, coef);
   __syncthreads();
 
   if (DET) {
-    asm_records<NB, FORM, NT>(L, p, roff, nrec, nown, nxt, nxt2, scale, q, MaxEmit{});
+    asm_records<NB, FORM, NT>(L, rb, nrec, nown, nxt, nxt2, scale, q, MaxEmit{});
     __syncthreads();
     row_grids<NT>(L, nown);
     __syncthreads();
-    asm_records<NB, FORM, NT>(L, p, roff, nrec, nown, first_record(p, roff, nrec, 0),
-                              first_record(p, roff, nrec, NT), scale, q, GridEmit{});
+    asm_records<NB, FORM, NT>(L, rb, nrec, nown, rb[t], rb[t + NT], scale, q, GridEmit{});
   } else {
-    asm_records<NB, FORM, NT>(L, p, roff, nrec, nown, nxt, nxt2, scale, q, AddEmit{});
+    asm_records<NB, FORM, NT>(L, rb, nrec, nown, nxt, nxt2, scale, q, AddEmit{});
   }
+  const int4 dsc = run_descriptors<NT>(p, runoff, nrun);  // in flight during the row sums
   __syncthreads();
   if (ROWSUM) {
     rowsum_diagonal<NT>(L, nown);
     __syncthreads();
   }
-  store_runs<NT>(L, p, runoff, nrun, values);
+  store_runs<NT>(L, p, runoff, nrun, dsc, values);
 }
 
 // Integrand U_a^T D V_b with U = (phi, grad phi) of the row (u) basis
@@ -535,8 +604,9 @@ __global__ __launch_bounds__(NT) void k_asm_general(PlanArgs p, const double* __
   } else {
     general_records<NB, NT>(L, p, roff, nrec, nown, D, q, AddEmit{});
   }
+  const int4 dsc = run_descriptors<NT>(p, runoff, nrun);
   __syncthreads();
-  store_runs<NT>(L, p, runoff, nrun, values);
+  store_runs<NT>(L, p, runoff, nrun, dsc, values);
 }
 
 // Element-local vertex index of record vertex a (record bits 112-119).
@@ -635,8 +705,9 @@ __global__ __launch_bounds__(NT) void k_asm_moments(PlanArgs p, const int32_t* _
   } else {
     moments_records<NB, NT>(L, p, rec_elem, mom, roff, nrec, nown, AddEmit{});
   }
+  const int4 dsc = run_descriptors<NT>(p, runoff, nrun);
   __syncthreads();
-  store_runs<NT>(L, p, runoff, nrun, values);
+  store_runs<NT>(L, p, runoff, nrun, dsc, values);
 }
 
 // Spatially varying linear forms F0(q) v + F1(q) . grad v with per-element
