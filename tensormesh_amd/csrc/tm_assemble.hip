@@ -13,6 +13,8 @@
 //   3. writes each owned row once, in runs of consecutive rows whose CSR
 //      ranges are contiguous (coalesced stores, optionally narrowed to FP32).
 // No global atomics, no sort, every CSR entry written exactly once.
+#include <type_traits>
+
 #include "tm_device.h"
 
 namespace {
@@ -115,6 +117,19 @@ __device__ __forceinline__ double* node_coef_ptr(const Lds& L, int h) {
   return (double*)(L.p1 + h) + (DIM == 3 ? 1 : 0);
 }
 
+// Packed node word of the matrix kernels whose coefficient is FP32 or absent:
+// the second double of p1 holds the coefficient as FP32 (low word) and the
+// row meta (high word), so one 16-byte read brings both.
+__device__ __forceinline__ double pack_coef_meta(float cf, uint32_t m) {
+  return __hiloint2double((int)m, __float_as_int(cf));
+}
+__device__ __forceinline__ uint32_t packed_meta(const double2& n1) { return (uint32_t)__double2hiint(n1.y); }
+__device__ __forceinline__ double packed_coef(const double2& n1) { return (double)__int_as_float(__double2loint(n1.y)); }
+template <bool PK>
+__device__ __forceinline__ uint32_t row_meta(const Lds& L, int h) {
+  return PK ? ((const uint32_t*)(L.p1 + h))[3] : L.ms[h];
+}
+
 // Cofactor vectors c[b] (grad phi_b = c[b] / det) and det J of a P1 simplex.
 template <int DIM>
 __device__ __forceinline__ void p1_geometry(const double (&x)[DIM + 1][DIM], double (&c)[DIM + 1][DIM],
@@ -179,7 +194,8 @@ __device__ __forceinline__ int meta_len(uint32_t m) { return (int)(m >> 24); }
 
 // Stages the block's halo nodes (and optionally a nodal coefficient) into LDS
 // and zeroes the accumulators.  A slot with halo id -1 is empty.
-template <int NT, int DIM, typename CT, bool COEF>
+// PK: packed node words (pack_coef_meta; CT float or no coefficient), L.ms unused.
+template <int NT, int DIM, typename CT, bool COEF, bool PK = false>
 __device__ __forceinline__ void stage(const Lds& L, const int32_t* __restrict__ halo, const uint32_t* __restrict__ meta,
                                       int nacc, int nh, int nown, const double* __restrict__ X,
                                       const CT* __restrict__ coef) {
@@ -205,7 +221,18 @@ __device__ __forceinline__ void stage(const Lds& L, const int32_t* __restrict__ 
     const bool ok = g[k] >= 0;
     const int64_t gi = ok ? g[k] : 0;
     const double cf = COEF ? ld(coef, gi) : 0.0;
-    if (DIM == 3) {
+    if (PK) {
+      const double pc = pack_coef_meta(ok ? (float)cf : 0.0f, mh[k]);
+      if (DIM == 3) {
+        const double x = X[gi * 3], y = X[gi * 3 + 1], z = X[gi * 3 + 2];
+        n0[k] = make_double2(ok ? x : 0.0, ok ? y : 0.0);
+        n1[k] = make_double2(ok ? z : 0.0, pc);
+      } else {
+        const double x = X[gi * 2], y = X[gi * 2 + 1];
+        n0[k] = make_double2(ok ? x : 0.0, ok ? y : 0.0);
+        n1[k] = make_double2(0.0, pc);
+      }
+    } else if (DIM == 3) {
       const double x = X[gi * 3], y = X[gi * 3 + 1], z = X[gi * 3 + 2];
       n0[k] = make_double2(ok ? x : 0.0, ok ? y : 0.0);
       n1[k] = make_double2(ok ? z : 0.0, ok ? cf : 0.0);
@@ -215,7 +242,9 @@ __device__ __forceinline__ void stage(const Lds& L, const int32_t* __restrict__ 
       n1[k] = make_double2(ok ? cf : 0.0, 0.0);
     }
   }
-  for (int i = threadIdx.x; i < nacc; i += NT) L.acc[i] = 0.0;
+  // 16-byte zero stores (L.acc is 16-byte aligned; an odd last entry alone)
+  for (int i = threadIdx.x; i < nacc / 2; i += NT) ((double2*)L.acc)[i] = make_double2(0.0, 0.0);
+  if ((nacc & 1) && threadIdx.x == 0) L.acc[nacc - 1] = 0.0;
   for (int a = threadIdx.x; a < nown; a += NT) L.accb[a] = 0.0;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -223,14 +252,15 @@ __device__ __forceinline__ void stage(const Lds& L, const int32_t* __restrict__ 
     if (h < nh) {
       L.p0[h] = n0[k];
       L.p1[h] = n1[k];
-      L.ms[h] = mh[k];
+      if (!PK) L.ms[h] = mh[k];
     }
   }
   for (int h = threadIdx.x + K * NT; h < nh; h += NT) {
     const int64_t gh = halo[h];
     double2 a = make_double2(0.0, 0.0), c = make_double2(0.0, 0.0);
+    double cf = 0.0;
     if (gh >= 0) {
-      const double cf = COEF ? ld(coef, gh) : 0.0;
+      cf = COEF ? ld(coef, gh) : 0.0;
       if (DIM == 3) {
         a = make_double2(X[gh * 3], X[gh * 3 + 1]);
         c = make_double2(X[gh * 3 + 2], cf);
@@ -239,9 +269,10 @@ __device__ __forceinline__ void stage(const Lds& L, const int32_t* __restrict__ 
         c = make_double2(cf, 0.0);
       }
     }
+    if (PK) c = make_double2(DIM == 3 ? c.x : 0.0, pack_coef_meta((float)cf, meta[h]));
     L.p0[h] = a;
     L.p1[h] = c;
-    L.ms[h] = meta[h];
+    if (!PK) L.ms[h] = meta[h];
   }
 }
 
@@ -250,11 +281,11 @@ __device__ __forceinline__ void stage(const Lds& L, const int32_t* __restrict__ 
 // A row's entries are read 16 at a time, all reads issued before the first
 // add (one LDS round trip per 16 entries instead of one per entry); the adds
 // run in entry order, so the sum is the same as a plain loop's.
-template <int NT>
+template <int NT, bool PK = false>
 __device__ __forceinline__ void rowsum_diagonal(const Lds& L, int nown) {
   constexpr int C = 16;
   for (int a = threadIdx.x; a < nown; a += NT) {
-    const uint32_t m = L.ms[a];
+    const uint32_t m = row_meta<PK>(L, a);
     const int b0 = meta_off(m), len = meta_len(m);
     if (len == 0) continue;
     const int d = meta_diag(m);
@@ -395,7 +426,7 @@ __device__ __forceinline__ void emit_owned(const Lds& L, const uint4& rc, const 
 // The element loop of one block: each lane evaluates records r = tid, tid + NT,
 // ... (nxt = the lane's first record, already loaded) and adds the owned rows
 // of each element matrix into L.acc through `em` (ds_add_f64).
-template <int NB, int FORM, int NT, typename Emit>
+template <int NB, int FORM, int NT, bool PK, typename Emit>
 __device__ __forceinline__ void asm_records(const Lds& L, const RecBuf& rb, int nrec, int nown, uint4 nxt,
                                             uint4 nxt2, double scale, const QuadArg& q, const Emit& em) {
   constexpr int DIM = NB - 1;
@@ -412,9 +443,9 @@ __device__ __forceinline__ void asm_records(const Lds& L, const RecBuf& rb, int 
     for (int a = 0; a < NB; ++a) {
       l[a] = rec_local(lo, a);
       const double2 n0 = L.p0[l[a]], n1 = L.p1[l[a]];
-      m[a] = L.ms[l[a]];
+      m[a] = PK ? packed_meta(n1) : L.ms[l[a]];
       node_xyz<DIM>(n0, n1, x[a]);
-      cf[a] = node_coef<DIM>(n1);
+      cf[a] = PK ? packed_coef(n1) : node_coef<DIM>(n1);
     }
     double c[NB][DIM], det;
     p1_geometry<DIM>(x, c, det);
@@ -481,6 +512,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
                                                   OT* __restrict__ values) {
   constexpr int DIM = NB - 1;
   constexpr bool ROWSUM = FORM != 2;  // diagonal from the zero row sum
+  constexpr bool PK = FORM != 1 || std::is_same<CT, float>::value;  // packed node words
   extern __shared__ __align__(16) char smem[];
   const int b = tmx::xcd_remap(blockIdx.x, p.nblk);
   const int4 bi0 = *(const int4*)(p.blk + 8 * b);
@@ -492,22 +524,22 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
   const RecBuf rb(p, roff, nrec);
   const int t = threadIdx.x;
   uint4 nxt = rb[t], nxt2 = rb[t + NT];
-  stage<NT, DIM, CT, FORM == 1>(L, p.halo + hoff, p.meta + hoff, nacc, nh, nown, X, coef);
+  stage<NT, DIM, CT, FORM == 1, PK>(L, p.halo + hoff, p.meta + hoff, nacc, nh, nown, X, coef);
   __syncthreads();
 
   if (DET) {
-    asm_records<NB, FORM, NT>(L, rb, nrec, nown, nxt, nxt2, scale, q, MaxEmit{});
+    asm_records<NB, FORM, NT, PK>(L, rb, nrec, nown, nxt, nxt2, scale, q, MaxEmit{});
     __syncthreads();
     row_grids<NT>(L, nown);
     __syncthreads();
-    asm_records<NB, FORM, NT>(L, rb, nrec, nown, rb[t], rb[t + NT], scale, q, GridEmit{});
+    asm_records<NB, FORM, NT, PK>(L, rb, nrec, nown, rb[t], rb[t + NT], scale, q, GridEmit{});
   } else {
-    asm_records<NB, FORM, NT>(L, rb, nrec, nown, nxt, nxt2, scale, q, AddEmit{});
+    asm_records<NB, FORM, NT, PK>(L, rb, nrec, nown, nxt, nxt2, scale, q, AddEmit{});
   }
   const int4 dsc = run_descriptors<NT>(p, runoff, nrun);  // in flight during the row sums
   __syncthreads();
   if (ROWSUM) {
-    rowsum_diagonal<NT>(L, nown);
+    rowsum_diagonal<NT, PK>(L, nown);
     __syncthreads();
   }
   store_runs<NT>(L, p, runoff, nrun, dsc, values);
