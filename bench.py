@@ -592,10 +592,12 @@ def main():
     asm = asm.to(dev)
     asm.value_dtype = torch.float32           # FP32 values, FP64 geometry and accumulation
     t0 = time.perf_counter()
-    plan = asm.topology.device_plan(dev)
+    plan = asm.topology.device_plan(dev, edge=True)
     t_plan = time.perf_counter() - t0
     log(f"pattern {t_pattern:.1f}s (nnz {asm.nnz}), plan {t_plan:.1f}s "
-        f"({plan.n_blocks} blocks, {int(plan.info[2])} records, lds {int(plan.info[4])} B)")
+        f"({plan.n_blocks} blocks, {int(plan.info[2])} records, lds {int(plan.info[4])} B, "
+        f"edge records: {int(plan.edge_info[0]) * 4 if plan.has_edge else 16} B, "
+        f"{int(plan.edge_info[1]) if plan.has_edge else 0} mirrored entries)")
     kappa = torch.tensor(kappa_np, dtype=torch.float32, device=dev)
     pd = {"kappa": kappa}
     E, N, nnz, nb = len(conn), len(pts), asm.nnz, 4

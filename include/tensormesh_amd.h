@@ -115,6 +115,23 @@ int tm_plan_export(const tm_plan* plan, int32_t* blk, int32_t* halo, uint32_t* m
                    uint32_t* recs, int32_t* rec_elem, int32_t* runs);
 void tm_plan_destroy(tm_plan* plan);
 
+/* Edge records of an exported scalar plan (no reference counterpart; a
+ * compact form of the same projector P_E, element_assembler.py:863-884, for
+ * symmetric element matrices).  Each record lists its owned vertices first by
+ * ascending slot and carries one in-row rank per vertex pair (i, j), i < j,
+ * the rank of j in the row of i: tetrahedra 8 bytes when every such rank is
+ * below 16, else 12; triangles 8 bytes.  An edge with both ends owned is
+ * accumulated in the row of the smaller slot; mir[mir_ptr[b] .. mir_ptr[b+1])
+ * = src | dst << 16 (block accumulator indices) fills its transposed entry.
+ * info[0] words per record, [1] mirror entries, [2] max per block, [3] max rank. */
+typedef struct tm_plan_edge tm_plan_edge;
+int tm_plan_edge_build(int nb, int64_t n_blocks, const int32_t* blk, const uint32_t* meta,
+                       const uint32_t* recs, int n_threads, tm_plan_edge** out);
+int tm_plan_edge_info(const tm_plan_edge* plan, int64_t* info);
+/* recs[n_records * info[0]], mir_ptr[n_blocks + 1], mir[info[1]] */
+int tm_plan_edge_export(const tm_plan_edge* plan, uint32_t* recs, int32_t* mir_ptr, uint32_t* mir);
+void tm_plan_edge_destroy(tm_plan_edge* plan);
+
 /* -------------------------------------------------------------- device --- */
 
 /* Device view of an exported plan. */
@@ -129,6 +146,13 @@ typedef struct {
   int32_t lds_bytes;
   int32_t threads; /* workgroup size of the matrix kernel: 256, 512 or 1024 (0 = 256) */
   int32_t reserved;
+  /* optional edge records of the same plan (tm_plan_edge_export; NULL: none):
+   * tm_asm_p1_matrix uses them for its symmetric forms unless deterministic */
+  const uint32_t* erecs;
+  const int32_t* emir_ptr;
+  const uint32_t* emir;
+  int32_t erec_words; /* 2 or 3 */
+  int32_t reserved2;
 } tm_plan_dev;
 
 /* Reference-element moments of the quadrature rule: wsum = sum_q w_q,

@@ -135,7 +135,8 @@ class ElementAssembler(nn.Module):
 
     def _launch_matrix(self, form, points64, coef, scale, out_dtype) -> torch.Tensor:
         dev = points64.device
-        plan = self._plan(dev)
+        # symmetric forms read the plan's edge records (the deterministic mode keeps the 16-byte ones)
+        plan = self.topology.device_plan(dev, edge=not self.deterministic)
         values = torch.empty(self.topology.nnz, dtype=out_dtype, device=dev)
         cdt = _lib.dtype_code(coef) if coef is not None else _lib.F32
         if coef is not None:

@@ -69,8 +69,52 @@ class DevicePlan:
     def n_blocks(self) -> int:
         return int(self.info[0])
 
+    @property
+    def has_edge(self) -> bool:
+        return "erecs" in self.tensors
+
+    def attach_edge(self, host: Dict[str, np.ndarray], info: np.ndarray) -> None:
+        """Adds the plan's edge records (:func:`edge_plan`): the matrix kernel
+        of the symmetric scalar forms then reads them instead of the 16-byte
+        records."""
+        dev = self.tensors["blk"].device
+        for k in ("erecs", "emir_ptr", "emir"):
+            self.tensors[k] = torch.from_numpy(host[k]).to(dev)
+        self.edge_info = info
+        t = self.tensors
+        self.struct.erecs = t["erecs"].data_ptr()
+        self.struct.emir_ptr = t["emir_ptr"].data_ptr()
+        self.struct.emir = t["emir"].data_ptr()
+        self.struct.erec_words = int(info[0])
+
     def bytes(self) -> int:
         return sum(v.numel() * v.element_size() for v in self.tensors.values())
+
+
+def edge_plan(nb: int, arrs: Dict[str, np.ndarray], info: np.ndarray):
+    """Edge records and mirror lists of an exported scalar plan
+    (``tm_plan_edge_build``): ``({"erecs", "emir_ptr", "emir"}, info)`` with
+    info = (words per record, mirror entries, max per block, max rank)."""
+    L = _lib.lib()
+    nblk, nrec = int(info[0]), int(info[2])
+    h = ctypes.c_void_p()
+    _lib.check(L.tm_plan_edge_build(nb, nblk, _lib.ptr(arrs["blk"]), _lib.ptr(arrs["meta"]), _lib.ptr(arrs["recs"]),
+                                    0, ctypes.byref(h)))
+    try:
+        einfo = np.zeros(4, dtype=np.int64)
+        _lib.check(L.tm_plan_edge_info(h, _lib.ptr(einfo)))
+        out = dict(erecs=np.empty(nrec * int(einfo[0]), np.uint32), emir_ptr=np.empty(nblk + 1, np.int32),
+                   emir=np.empty(max(1, int(einfo[1])), np.uint32))
+        _lib.check(L.tm_plan_edge_export(h, _lib.ptr(out["erecs"]), _lib.ptr(out["emir_ptr"]), _lib.ptr(out["emir"])))
+    finally:
+        L.tm_plan_edge_destroy(h)
+    return out, einfo
+
+
+def edge_records_enabled() -> bool:
+    """Edge records for the symmetric matrix kernels (``TM_ASM_EDGE=0`` keeps
+    the 16-byte records, e.g. to compare the two)."""
+    return os.environ.get("TM_ASM_EDGE", "1") != "0"
 
 
 class P1Topology:
@@ -220,14 +264,20 @@ class P1Topology:
         return self._host_plans[key]
 
     def device_plan(self, device, kind: str = "scalar", max_rows: Optional[int] = None,
-                    lds_budget: Optional[int] = None) -> DevicePlan:
+                    lds_budget: Optional[int] = None, edge: bool = False) -> DevicePlan:
+        """Device copy of a plan; ``edge=True`` (scalar plans) also attaches
+        its edge records when :func:`edge_records_enabled`."""
         device = torch.device(device)
         rows0, budget0, threads = _default_plan_params(kind, self.dim)
         key = (device, kind, int(max_rows or rows0), int(lds_budget or budget0), threads)
         if key not in self._dev_plans:
             arrs, info = self.host_plan(kind, max_rows, lds_budget)
             self._dev_plans[key] = DevicePlan(arrs, info, device, threads)
-        return self._dev_plans[key]
+        plan = self._dev_plans[key]
+        if edge and kind == "scalar" and not plan.has_edge and edge_records_enabled():
+            arrs, info = self.host_plan(kind, max_rows, lds_budget)
+            plan.attach_edge(*edge_plan(self.nb, arrs, info))
+        return plan
 
     def quad(self, order: int) -> _lib.Quad:
         wsum, W, M = quadrature_moments(self.element_type, order)

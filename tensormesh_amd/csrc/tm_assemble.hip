@@ -13,6 +13,12 @@
 //   3. writes each owned row once, in runs of consecutive rows whose CSR
 //      ranges are contiguous (coalesced stores, optionally narrowed to FP32).
 // No global atomics, no sort, every CSR entry written exactly once.
+//
+// Symmetric scalar forms (Laplace, kappa-Laplace with an FP32 coefficient,
+// mass) read the plan's edge records instead when it has them: 8 bytes (12
+// for rows of more than 16 entries) carrying one rank per vertex pair, one
+// LDS atomic per element edge, and a per-block mirror list that fills the
+// transposed entries of edges whose two ends are both owned.
 #include <type_traits>
 
 #include "tm_device.h"
@@ -181,10 +187,11 @@ __device__ __forceinline__ double dotc(const double* a, const double* b) {
 
 // 1/x to full FP64 precision: hardware reciprocal estimate refined by two
 // Newton steps (each doubles the correct bits).
+template <int STEPS = 2>
 __device__ __forceinline__ double recip(double x) {
   double r = __builtin_amdgcn_rcp(x);
-  r = fma(r, fma(-x, r, 1.0), r);
-  r = fma(r, fma(-x, r, 1.0), r);
+#pragma unroll
+  for (int i = 0; i < STEPS; ++i) r = fma(r, fma(-x, r, 1.0), r);
   return r;
 }
 
@@ -221,15 +228,13 @@ __device__ __forceinline__ void stage(const Lds& L, const int32_t* __restrict__ 
     const bool ok = g[k] >= 0;
     const int64_t gi = ok ? g[k] : 0;
     const double cf = COEF ? ld(coef, gi) : 0.0;
-    if (PK) {
-      const double pc = pack_coef_meta(ok ? (float)cf : 0.0f, mh[k]);
+    if (PK) {  // an empty slot keeps node 0's data: no record or row reads it
+      const double pc = pack_coef_meta((float)cf, mh[k]);
       if (DIM == 3) {
-        const double x = X[gi * 3], y = X[gi * 3 + 1], z = X[gi * 3 + 2];
-        n0[k] = make_double2(ok ? x : 0.0, ok ? y : 0.0);
-        n1[k] = make_double2(ok ? z : 0.0, pc);
+        n0[k] = make_double2(X[gi * 3], X[gi * 3 + 1]);
+        n1[k] = make_double2(X[gi * 3 + 2], pc);
       } else {
-        const double x = X[gi * 2], y = X[gi * 2 + 1];
-        n0[k] = make_double2(ok ? x : 0.0, ok ? y : 0.0);
+        n0[k] = make_double2(X[gi * 2], X[gi * 2 + 1]);
         n1[k] = make_double2(0.0, pc);
       }
     } else if (DIM == 3) {
@@ -540,6 +545,203 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
   __syncthreads();
   if (ROWSUM) {
     rowsum_diagonal<NT, PK>(L, nown);
+    __syncthreads();
+  }
+  store_runs<NT>(L, p, runoff, nrun, dsc, values);
+}
+
+// ---- edge records (symmetric scalar forms) ---------------------------------
+//
+// A record of W 32-bit words: NB 10-bit LDS slots (owned vertices first, by
+// ascending slot), then one rank per vertex pair (i, j), i < j, in the order
+// (0,1) (0,2) (0,3) (1,2) (1,3) (2,3): the rank of vertex j in the row of
+// vertex i, which accumulates the pair's value when it is owned.  Tetrahedra:
+// ranks from bit 40, 4 bits (W = 2) or 6 bits (W = 3); triangles: 6 bits from
+// bit 32 (W = 2).  Every field lies inside one word.  An edge with both ends
+// owned is accumulated once, in the row of the smaller slot; the transposed
+// entry is copied from it after the element loop (the block's mirror list,
+// {src | dst << 16} accumulator indices).
+struct EdgeArgs {
+  const uint32_t* recs;
+  const int32_t* mir_ptr;
+  const uint32_t* mir;
+};
+
+template <int W>
+struct ERec {
+  uint32_t w[W];
+};
+
+template <int W>
+struct ERecBuf {
+  __amdgpu_buffer_rsrc_t rs;
+  __device__ __forceinline__ ERecBuf(const uint32_t* recs, int roff, int nrec)
+      : rs(__builtin_amdgcn_make_buffer_rsrc((void*)(recs + (int64_t)roff * W), 0, nrec * W * 4,
+                                             kRawBufferWord3)) {}
+  __device__ __forceinline__ ERec<W> operator[](int r) const {
+    ERec<W> e;
+    if constexpr (W == 2) {
+      const auto v = __builtin_amdgcn_raw_buffer_load_b64(rs, r * 8, 0, 0);
+      e.w[0] = v[0];
+      e.w[1] = v[1];
+    } else {
+      const auto v = __builtin_amdgcn_raw_buffer_load_b96(rs, r * 12, 0, 0);
+      e.w[0] = v[0];
+      e.w[1] = v[1];
+      e.w[2] = v[2];
+    }
+    return e;
+  }
+};
+
+template <int NB, int W>
+__device__ __forceinline__ int erec_slot(const ERec<W>& e, int a) {
+  if (NB == 4 && a == 3) return (int)(__builtin_amdgcn_alignbit(e.w[1], e.w[0], 30) & 1023u);
+  return (int)__builtin_amdgcn_ubfe(e.w[0], 10 * a, 10);
+}
+
+template <int NB, int W>
+__device__ __forceinline__ int erec_rank(const ERec<W>& e, int k) {
+  constexpr int R = (NB == 4 && W == 2) ? 4 : 6;
+  constexpr int base = NB == 4 ? 40 : 32;
+  const int bit = base + R * k;
+  return (int)__builtin_amdgcn_ubfe(e.w[bit / 32], bit % 32, R);
+}
+
+// Pairs (i, j), i < j, of an edge record with vertices [0, K) owned: each
+// pair with an owned first vertex adds its value once, to the row of i.
+template <int NB, int K, int FORM, int W>
+__device__ __forceinline__ void emit_edges(const Lds& L, const ERec<W>& e, const uint32_t (&m)[NB],
+                                           const double (&c)[NB][NB - 1], double s, const QuadArg& q) {
+  constexpr int DIM = NB - 1;
+  if (FORM == 2) {
+#pragma unroll
+    for (int a = 0; a < K; ++a) atomicAdd(L.acc + meta_off(m[a]) + meta_diag(m[a]), s * q.M[a * NB + a]);
+  }
+  int k = 0;
+#pragma unroll
+  for (int i = 0; i < NB; ++i)
+#pragma unroll
+    for (int j = i + 1; j < NB; ++j, ++k) {
+      if (i >= K) continue;
+      const double kij = FORM == 2 ? s * q.M[i * NB + j] : s * dotc<DIM>(c[i], c[j]);
+      atomicAdd(L.acc + meta_off(m[i]) + erec_rank<NB>(e, k), kij);
+    }
+}
+
+// LO: FP32 output values.  1/|det| then takes one Newton step (relative error
+// below 2^-40 from the hardware estimate) and the nodal coefficients are
+// summed in FP32 (the relabelling-invariant weights are all W[0]); both stay
+// far below the FP32 rounding of the stored values.  FP64 outputs keep two
+// Newton steps and the FP64 weighted sum.
+template <int NB, int FORM, int NT, int W, bool LO>
+__device__ __forceinline__ void edge_records(const Lds& L, const ERecBuf<W>& rb, int nrec, int nown, ERec<W> nxt,
+                                             ERec<W> nxt2, double scale, const QuadArg& q) {
+  constexpr int DIM = NB - 1;
+  const double sw = FORM == 0 ? scale * q.wsum : scale * q.W[0];
+  for (int r = threadIdx.x; r < nrec; r += NT) {
+    const ERec<W> e = nxt;
+    nxt = nxt2;
+    nxt2 = rb[r + 2 * NT];  // records two iterations ahead (zeros past the block's end)
+    int l[NB];
+    double x[NB][DIM];
+    float cf[NB];
+    uint32_t m[NB];
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+      l[a] = erec_slot<NB>(e, a);
+      const double2 n0 = L.p0[l[a]], n1 = L.p1[l[a]];
+      m[a] = packed_meta(n1);
+      node_xyz<DIM>(n0, n1, x[a]);
+      cf[a] = __int_as_float(__double2loint(n1.y));
+    }
+    double c[NB][DIM], det;
+    p1_geometry<DIM>(x, c, det);
+    const double adet = fabs(det);
+    double s;
+    if (FORM == 0) {
+      s = sw * recip<LO ? 1 : 2>(adet);
+    } else if (FORM == 1) {
+      double k;
+      if (LO) {
+        float ks = cf[0];
+#pragma unroll
+        for (int a = 1; a < NB; ++a) ks += cf[a];
+        k = (double)ks;
+      } else {
+        k = 0.0;
+#pragma unroll
+        for (int a = 0; a < NB; ++a) k += (double)cf[a];
+      }
+      s = sw * k * recip<LO ? 1 : 2>(adet);
+    } else {
+      s = scale * adet;
+    }
+    int nk = 0;
+#pragma unroll
+    for (int a = 0; a < NB; ++a) nk += l[a] < nown ? 1 : 0;
+    const int k0 = __builtin_amdgcn_readfirstlane(nk);
+    if (__all(nk == k0)) {
+      if (k0 == NB) {
+        emit_edges<NB, NB, FORM>(L, e, m, c, s, q);
+      } else if (k0 == 1) {
+        emit_edges<NB, 1, FORM>(L, e, m, c, s, q);
+      } else if (k0 == 2) {
+        emit_edges<NB, 2, FORM>(L, e, m, c, s, q);
+      } else if (NB == 4 && k0 == 3) {
+        emit_edges<NB, (NB == 4 ? 3 : 1), FORM>(L, e, m, c, s, q);
+      }
+      continue;
+    }
+    if (FORM == 2) {
+#pragma unroll
+      for (int a = 0; a < NB; ++a)
+        if (a < nk) atomicAdd(L.acc + meta_off(m[a]) + meta_diag(m[a]), s * q.M[a * NB + a]);
+    }
+    int k = 0;
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+      for (int j = i + 1; j < NB; ++j, ++k) {
+        if (i >= nk) continue;
+        const double kij = FORM == 2 ? s * q.M[i * NB + j] : s * dotc<DIM>(c[i], c[j]);
+        atomicAdd(L.acc + meta_off(m[i]) + erec_rank<NB>(e, k), kij);
+      }
+  }
+}
+
+// Symmetric scalar forms on edge records: as k_asm_matrix (packed node words),
+// plus the mirror copies between the element loop and the row sums.
+template <int NB, int FORM, typename OT, int NT, int W>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_asm_matrix_edge(
+    PlanArgs p, EdgeArgs ea, const double* __restrict__ X, const float* __restrict__ coef, double scale, QuadArg q,
+    OT* __restrict__ values) {
+  constexpr int DIM = NB - 1;
+  extern __shared__ __align__(16) char smem[];
+  const int b = tmx::xcd_remap(blockIdx.x, p.nblk);
+  const int4 bi0 = *(const int4*)(p.blk + 8 * b);
+  const int4 bi1 = *(const int4*)(p.blk + 8 * b + 4);
+  const int hoff = bi0.x, nh = bi0.y, nown = bi0.z, roff = bi0.w;
+  const int nrec = bi1.x, runoff = bi1.y, nrun = bi1.z, nacc = bi1.w;
+  const Lds L = carve(smem, nacc, nh, nown);
+  const ERecBuf<W> rb(ea.recs, roff, nrec);
+  const int t = threadIdx.x;
+  ERec<W> nxt = rb[t], nxt2 = rb[t + NT];
+  stage<NT, DIM, float, FORM == 1, true>(L, p.halo + hoff, p.meta + hoff, nacc, nh, nown, X, coef);
+  __syncthreads();
+  edge_records<NB, FORM, NT, W, std::is_same<OT, float>::value>(L, rb, nrec, nown, nxt, nxt2, scale, q);
+  const int moff = ea.mir_ptr[b], nmir = ea.mir_ptr[b + 1] - moff;
+  const uint32_t mv = t < nmir ? ea.mir[moff + t] : 0u;  // in flight during the barrier
+  const int4 dsc = run_descriptors<NT>(p, runoff, nrun);
+  __syncthreads();
+  if (t < nmir) L.acc[mv >> 16] = L.acc[mv & 0xffffu];
+  for (int i = t + NT; i < nmir; i += NT) {
+    const uint32_t v = ea.mir[moff + i];
+    L.acc[v >> 16] = L.acc[v & 0xffffu];
+  }
+  __syncthreads();
+  if (FORM != 2) {
+    rowsum_diagonal<NT, true>(L, nown);
     __syncthreads();
   }
   store_runs<NT>(L, p, runoff, nrun, dsc, values);
@@ -990,10 +1192,33 @@ int launch_plan_kernel(K kernel, int nt, const tm_plan_dev* plan, hipStream_t st
     { constexpr int NT = 256; return launch_plan_kernel(KERNEL_NT, NT, plan, __VA_ARGS__); }                   \
   } while (0)
 
+// Edge-record instances of the symmetric forms (coefficient absent or FP32).
+template <int NB, typename OT, int W>
+int dispatch_matrix_edge(int form, const tm_plan_dev* plan, const double* X, const float* coef, double scale,
+                         QuadArg q, OT* values, hipStream_t st) {
+  const PlanArgs p = plan_args(plan);
+  const EdgeArgs e{plan->erecs, plan->emir_ptr, plan->emir};
+  if (form == TM_FORM_LAPLACE)
+    TM_BY_THREADS(plan, (k_asm_matrix_edge<NB, 0, OT, NT, W>), st, p, e, X, coef, scale, q, values);
+  if (form == TM_FORM_MASS)
+    TM_BY_THREADS(plan, (k_asm_matrix_edge<NB, 2, OT, NT, W>), st, p, e, X, coef, scale, q, values);
+  TM_BY_THREADS(plan, (k_asm_matrix_edge<NB, 1, OT, NT, W>), st, p, e, X, coef, scale, q, values);
+}
+
 template <int NB, typename OT, bool DET>
 int dispatch_matrix(int form, const tm_plan_dev* plan, const double* X, const void* coef, int cdt,
                     double scale, QuadArg q, OT* values, hipStream_t st) {
   PlanArgs p = plan_args(plan);
+  const bool edge_ok = !DET && plan->erecs != nullptr && plan->emir_ptr != nullptr &&
+                       (form == TM_FORM_LAPLACE || form == TM_FORM_MASS ||
+                        (form == TM_FORM_KAPPA_LAPLACE && coef != nullptr && cdt == TM_F32));
+  if (edge_ok) {
+    TM_REQUIRE(plan->erec_words == 2 || (NB == 4 && plan->erec_words == 3), TM_ERR_ARG,
+               "edge records of %d words", plan->erec_words);
+    if (NB == 4 && plan->erec_words == 3)
+      return dispatch_matrix_edge<NB, OT, 3>(form, plan, X, (const float*)coef, scale, q, values, st);
+    return dispatch_matrix_edge<NB, OT, 2>(form, plan, X, (const float*)coef, scale, q, values, st);
+  }
   if (form == TM_FORM_LAPLACE)
     TM_BY_THREADS(plan, (k_asm_matrix<NB, 0, float, OT, NT, DET>), st, p, X, (const float*)nullptr, scale, q,
                   values);

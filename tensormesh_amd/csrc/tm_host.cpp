@@ -666,3 +666,162 @@ extern "C" int tm_plan_export(const tm_plan* p, int32_t* blk, int32_t* halo, uin
 }
 
 extern "C" void tm_plan_destroy(tm_plan* p) { delete p; }
+
+// ------------------------------------------------------------- edge plan ----
+//
+// Edge records for the symmetric scalar matrix kernels.  A symmetric element
+// matrix needs one value per vertex pair, so a block can accumulate each
+// undirected edge once: in the row of its owned end with the smaller LDS slot
+// (records list owned vertices first, by ascending slot), at the rank of the
+// other end in that row.  Each pair (i, j), i < j, then needs one rank field
+// instead of two, and the record shrinks from 16 to 8 bytes when every rank
+// fits 4 bits (12 bytes otherwise).  The transposed entry of an edge whose
+// two ends are both owned is filled after the element loop from a per-block
+// mirror list {src | dst << 16} of accumulator indices.
+
+struct tm_plan_edge {
+  std::vector<uint32_t> recs;
+  std::vector<int32_t> mir_ptr;
+  std::vector<uint32_t> mir;
+  int64_t info[4] = {0, 0, 0, 0};
+};
+
+namespace {
+
+// Rank of record vertex b in the row of owned record vertex a (16-byte record).
+inline uint32_t rec16_rank(const uint32_t* r, int nb, int a, int b) {
+  const int j = b - (b > a ? 1 : 0);
+  const int bit = 40 + 6 * (a * (nb - 1) + j);
+  const uint64_t lo = (uint64_t)r[0] | ((uint64_t)r[1] << 32), hi = (uint64_t)r[2] | ((uint64_t)r[3] << 32);
+  const uint64_t v = bit < 64 ? (lo >> bit) | (bit > 58 ? hi << (64 - bit) : 0) : hi >> (bit - 64);
+  return (uint32_t)(v & 63u);
+}
+
+}  // namespace
+
+extern "C" int tm_plan_edge_build(int nb, int64_t n_blocks, const int32_t* blk, const uint32_t* meta,
+                                  const uint32_t* recs, int n_threads, tm_plan_edge** out) {
+  TM_REQUIRE(out != nullptr, TM_ERR_ARG, "null output");
+  *out = nullptr;
+  TM_REQUIRE(nb == 3 || nb == 4, TM_ERR_ARG, "edge records cover P1 triangles and tetrahedra (nb=%d)", nb);
+  TM_REQUIRE(n_blocks >= 0 && (n_blocks == 0 || (blk && meta && recs)), TM_ERR_ARG, "null plan array");
+  const int npair = nb * (nb - 1) / 2;
+  int pi[6], pj[6];
+  for (int i = 0, k = 0; i < nb; ++i)
+    for (int j = i + 1; j < nb; ++j, ++k) {
+      pi[k] = i;
+      pj[k] = j;
+    }
+  // record width: tetrahedra take 4-bit ranks (8 bytes) when every rank an
+  // owned row needs is below 16, 6-bit ranks (12 bytes) otherwise
+  uint32_t max_rank = 0;
+  for (int64_t b = 0; b < n_blocks; ++b) {
+    const int32_t* bi = blk + 8 * b;
+    const int nown = bi[2];
+    for (int64_t r = bi[3]; r < (int64_t)bi[3] + bi[4]; ++r) {
+      const uint32_t* rc = recs + 4 * r;
+      const uint64_t lo = (uint64_t)rc[0] | ((uint64_t)rc[1] << 32);
+      for (int a = 0; a < nb; ++a) {
+        if ((int)((lo >> (10 * a)) & 1023u) >= nown) continue;
+        for (int c = 0; c < nb; ++c)
+          if (c != a) max_rank = std::max(max_rank, rec16_rank(rc, nb, a, c));
+      }
+    }
+  }
+  const int W = (nb == 4 && max_rank > 15) ? 3 : 2;
+  const int R = (nb == 4 && W == 2) ? 4 : 6;
+  const int rbase = nb == 4 ? 40 : 32;
+
+  auto* P = new tm_plan_edge();
+  int64_t nrec = n_blocks > 0 ? (int64_t)blk[8 * (n_blocks - 1) + 3] + blk[8 * (n_blocks - 1) + 4] : 0;
+  P->recs.assign((size_t)nrec * W, 0u);
+  std::vector<std::vector<uint32_t>> mirs((size_t)n_blocks);
+  std::atomic<int64_t> next{0};
+  const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(pick_threads(n_threads), std::max<int64_t>(1, n_blocks / 64)));
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; ++t) {
+    th.emplace_back([&] {
+      std::vector<uint64_t> keys;
+      for (;;) {
+        const int64_t b0 = next.fetch_add(64);
+        if (b0 >= n_blocks) break;
+        for (int64_t b = b0; b < std::min(n_blocks, b0 + 64); ++b) {
+          const int32_t* bi = blk + 8 * b;
+          const int hoff = bi[0], nown = bi[2];
+          keys.clear();
+          for (int64_t r = bi[3]; r < (int64_t)bi[3] + bi[4]; ++r) {
+            const uint32_t* rc = recs + 4 * r;
+            const uint64_t lo = (uint64_t)rc[0] | ((uint64_t)rc[1] << 32);
+            int sl[4], ord[4], no = 0;
+            for (int a = 0; a < nb; ++a) sl[a] = (int)((lo >> (10 * a)) & 1023u);
+            // owned vertices by ascending slot, then the others in record order
+            for (int a = 0; a < nb; ++a)
+              if (sl[a] < nown) ord[no++] = a;
+            std::sort(ord, ord + no, [&](int x, int y) { return sl[x] < sl[y]; });
+            for (int a = 0, k = no; a < nb; ++a)
+              if (sl[a] >= nown) ord[k++] = a;
+            uint32_t w[3] = {0, 0, 0};
+            uint64_t slots = 0;
+            for (int i = 0; i < nb; ++i) slots |= (uint64_t)sl[ord[i]] << (10 * i);
+            w[0] = (uint32_t)slots;
+            w[1] = (uint32_t)(slots >> 32);
+            for (int k = 0; k < npair; ++k) {
+              const int a = ord[pi[k]], c = ord[pj[k]];
+              if (pi[k] >= no) continue;  // neither end owned
+              const uint32_t rk = rec16_rank(rc, nb, a, c);
+              const int bit = rbase + R * k;
+              w[bit / 32] |= rk << (bit % 32);
+              if (pj[k] < no) {  // both ends owned: the transposed entry is mirrored
+                const uint32_t src = (meta[hoff + sl[a]] & 0xffffu) + rk;
+                const uint32_t dst = (meta[hoff + sl[c]] & 0xffffu) + rec16_rank(rc, nb, c, a);
+                keys.push_back(((uint64_t)dst << 32) | src);
+              }
+            }
+            for (int x = 0; x < W; ++x) P->recs[(size_t)r * W + x] = w[x];
+          }
+          std::sort(keys.begin(), keys.end());
+          keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+          auto& m = mirs[(size_t)b];
+          m.reserve(keys.size());
+          for (uint64_t k : keys) m.push_back((uint32_t)(k & 0xffffu) | ((uint32_t)(k >> 32) << 16));
+        }
+      }
+    });
+  }
+  for (auto& x : th) x.join();
+  P->mir_ptr.assign((size_t)n_blocks + 1, 0);
+  int64_t tot = 0, mx = 0;
+  for (int64_t b = 0; b < n_blocks; ++b) {
+    tot += (int64_t)mirs[(size_t)b].size();
+    mx = std::max<int64_t>(mx, (int64_t)mirs[(size_t)b].size());
+    if (tot >= (int64_t(1) << 31)) {
+      delete P;
+      TM_REQUIRE(false, TM_ERR_LIMIT, "mirror list too large for int32 offsets");
+    }
+    P->mir_ptr[(size_t)b + 1] = (int32_t)tot;
+  }
+  P->mir.reserve((size_t)tot);
+  for (auto& m : mirs) P->mir.insert(P->mir.end(), m.begin(), m.end());
+  P->info[0] = W;
+  P->info[1] = tot;
+  P->info[2] = mx;
+  P->info[3] = max_rank;
+  *out = P;
+  return TM_OK;
+}
+
+extern "C" int tm_plan_edge_info(const tm_plan_edge* p, int64_t* info) {
+  TM_REQUIRE(p && info, TM_ERR_ARG, "null edge plan");
+  std::memcpy(info, p->info, sizeof(p->info));
+  return TM_OK;
+}
+
+extern "C" int tm_plan_edge_export(const tm_plan_edge* p, uint32_t* recs, int32_t* mir_ptr, uint32_t* mir) {
+  TM_REQUIRE(p, TM_ERR_ARG, "null edge plan");
+  if (recs) std::memcpy(recs, p->recs.data(), p->recs.size() * 4);
+  if (mir_ptr) std::memcpy(mir_ptr, p->mir_ptr.data(), p->mir_ptr.size() * 4);
+  if (mir) std::memcpy(mir, p->mir.data(), p->mir.size() * 4);
+  return TM_OK;
+}
+
+extern "C" void tm_plan_edge_destroy(tm_plan_edge* p) { delete p; }

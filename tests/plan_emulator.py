@@ -97,3 +97,78 @@ def emulate_matrix(topo, arrs, element_matrix, det=False, shuffle=None):
             covered[vs:vs + ln] += 1
     assert (covered == 1).all(), "every CSR value must be written by exactly one run"
     return vals
+
+
+def decode_edge(words, nb):
+    """Edge record (2 or 3 words): record-order slots and ``rank(k)`` of pair
+    k in the order (0,1) (0,2) (0,3) (1,2) (1,3) (2,3)."""
+    w = [int(x) for x in words]
+    s = w[0] | (w[1] << 32)
+    loc = [(s >> (10 * a)) & 1023 for a in range(nb)]
+    R = 4 if (nb == 4 and len(w) == 2) else 6
+    base = 40 if nb == 4 else 32
+
+    def rank(k):
+        bit = base + R * k
+        return (w[bit // 32] >> (bit % 32)) & ((1 << R) - 1)
+    return loc, rank
+
+
+def emulate_edge_matrix(topo, arrs, earrs, words, element_matrix, rowsum=True):
+    """The edge-record kernel in float64: each pair (i, j), i < j, with an
+    owned first vertex adds K_ij once, to the row of i; the mirror list then
+    fills the transposed entries; ``rowsum`` forms the diagonal as minus the
+    row's off-diagonal sum (Laplace-type forms), otherwise the owned diagonal
+    entries are added directly (mass)."""
+    nb = topo.nb
+    blk = arrs["blk"].reshape(-1, 8)
+    erecs = earrs["erecs"].reshape(-1, words)
+    runs = arrs["runs"].reshape(-1, 4)
+    mptr, mir = earrs["emir_ptr"], earrs["emir"]
+    rp, ci = topo.row_ptr, topo.col_idx
+    vals = np.zeros(topo.nnz)
+    covered = np.zeros(topo.nnz, dtype=np.int64)
+    pairs = [(i, j) for i in range(nb) for j in range(i + 1, nb)]
+    for b in range(blk.shape[0]):
+        hoff, nh, nown, roff, nrec, runoff, nrun, nacc = (int(v) for v in blk[b])
+        halo = arrs["halo"][hoff:hoff + nh]
+        meta = arrs["meta"][hoff:hoff + nh].astype(np.int64)
+        acc = np.zeros(nacc)
+        hit = np.zeros(nacc, dtype=bool)
+        for r in range(nrec):
+            loc, rank = decode_edge(erecs[roff + r], nb)
+            assert max(loc) < nh
+            e = int(arrs["rec_elem"][roff + r])
+            assert sorted(halo[loc]) == sorted(topo.conn[e])
+            nk = sum(1 for s in loc if s < nown)
+            assert all(s < nown for s in loc[:nk]) and loc[:nk] == sorted(loc[:nk])
+            K = element_matrix(e, topo.points[halo[loc]])
+            if not rowsum:
+                for a in range(nk):
+                    m = int(meta[loc[a]])
+                    acc[(m & 0xFFFF) + ((m >> 16) & 0xFF)] += K[a, a]
+            for k, (i, j) in enumerate(pairs):
+                if i >= nk:
+                    continue
+                m = int(meta[loc[i]])
+                base, dr, ln = m & 0xFFFF, (m >> 16) & 0xFF, m >> 24
+                rk = rank(k)
+                assert rk < ln and rk != dr
+                assert ci[rp[halo[loc[i]]] + rk] == halo[loc[j]]
+                acc[base + rk] += K[i, j]
+                hit[base + rk] = True
+        for v in mir[mptr[b]:mptr[b + 1]]:
+            src, dst = int(v) & 0xFFFF, int(v) >> 16
+            assert hit[src] and not hit[dst] and dst < nacc
+            acc[dst] = acc[src]
+        if rowsum:
+            for a in range(nown):
+                m = int(meta[a])
+                base, dr, ln = m & 0xFFFF, (m >> 16) & 0xFF, m >> 24
+                acc[base + dr] = -(acc[base:base + ln].sum() - acc[base + dr])
+        for q in range(nrun):
+            vs, as_, ln, _ = (int(v) for v in runs[runoff + q])
+            vals[vs:vs + ln] = acc[as_:as_ + ln]
+            covered[vs:vs + ln] += 1
+    assert (covered == 1).all()
+    return vals

@@ -13,7 +13,8 @@ from oracle import pattern as op
 from tensormesh_amd import _lib
 from tensormesh_amd.assemble.topology import P1Topology
 from tensormesh_amd.mesh import kuhn_cube_arrays, structured_rectangle_arrays
-from tests.plan_emulator import emulate_matrix
+from tests.plan_emulator import emulate_edge_matrix, emulate_matrix
+from tensormesh_amd.assemble.topology import edge_plan
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ERR_ARG = -1  # TM_ERR_ARG
@@ -94,6 +95,53 @@ def test_block_plan_emulation_matches_oracle(name, arrays, et, rows):
         got = emulate_matrix(topo, arrs, _kernel_element_matrix(topo, form, kappa))
         ref = oa.reduce(pe, oa.element_matrices(pts, conn, et, form, kappa=kappa), len(u))
         assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def _mass_element_matrix(topo, order=2):
+    w, phi, _ = oa.rule(topo.element_type, order)
+    M = (w[:, None, None] * phi[:, :, None] * phi[:, None, :]).sum(0)
+
+    def f(e, x):
+        return abs(np.linalg.det(x[1:] - x[0])) * M
+    return f
+
+
+def _delaunay_tets(n, seed):
+    """Tetrahedra of a Delaunay mesh of n random points in the unit cube
+    (rows longer than 16 entries: the 12-byte edge records)."""
+    from scipy.spatial import Delaunay
+    pts = np.random.default_rng(seed).random((n, 3))
+    tri = Delaunay(pts)
+    conn = tri.simplices.astype(np.int64)
+    vol = np.abs(np.linalg.det(pts[conn[:, 1:]] - pts[conn[:, :1]]))
+    return pts, conn[vol > 1e-9 * vol.max()]
+
+
+@pytest.mark.parametrize("name,arrays,et", list(_meshes()) + [("delaunay", _delaunay_tets(300, 2) + (None,), "tetra")])
+@pytest.mark.parametrize("rows", [7, 33, 200])
+def test_edge_records_emulation_matches_oracle(name, arrays, et, rows):
+    """Edge records: one accumulation per vertex pair in the row of the owned
+    end with the smaller slot, mirrored entries, row-sum diagonals (Laplace,
+    kappa-Laplace) or direct diagonals (mass), against the float64 oracle."""
+    pts, conn, _ = arrays
+    used = np.unique(conn)
+    topo = P1Topology(pts, conn, len(pts), et)
+    arrs, info = topo.host_plan("scalar", max_rows=rows)
+    earrs, einfo = edge_plan(topo.nb, arrs, info)
+    words = int(einfo[0])
+    if name == "delaunay":
+        assert einfo[3] > 15 and words == 3
+    else:
+        assert words == 2
+    assert len(earrs["erecs"]) == words * int(info[2]) and earrs["emir_ptr"][-1] == einfo[1]
+    kappa = np.exp(np.random.default_rng(3).standard_normal(len(pts)))
+    u, v, pe = op.pattern_and_slots(conn, len(pts))
+    for form in ("laplace", "kappa_laplace", "mass"):
+        f = _mass_element_matrix(topo) if form == "mass" else _kernel_element_matrix(topo, form, kappa)
+        got = emulate_edge_matrix(topo, arrs, earrs, words, f, rowsum=form != "mass")
+        ref = oa.reduce(pe, oa.element_matrices(pts, conn, et, form, kappa=kappa), len(u))
+        assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max(), form
+    assert len(used) == len(pts)
 
 
 def test_block_plan_respects_lds_budget():
