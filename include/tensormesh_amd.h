@@ -118,18 +118,23 @@ void tm_plan_destroy(tm_plan* plan);
 /* Edge records of an exported scalar plan (no reference counterpart; a
  * compact form of the same projector P_E, element_assembler.py:863-884, for
  * symmetric element matrices).  Each record lists its owned vertices first by
- * ascending slot and carries one in-row rank per vertex pair (i, j), i < j,
- * the rank of j in the row of i: tetrahedra 8 bytes when every such rank is
- * below 16, else 12; triangles 8 bytes.  An edge with both ends owned is
- * accumulated in the row of the smaller slot; mir[mir_ptr[b] .. mir_ptr[b+1])
- * = src | dst << 16 (block accumulator indices) fills its transposed entry.
- * info[0] words per record, [1] mirror entries, [2] max per block, [3] max rank. */
+ * ascending slot, as LDS node words w(s) = s + (s & ~15), and carries one
+ * in-row rank per vertex pair (i, j), i < j, the rank of j in the row of i:
+ * tetrahedra 8 bytes (10-bit words, 4-bit ranks) when every such rank is
+ * below 16 and every halo at most 512 nodes, else 12 (11-bit words, 6-bit
+ * ranks); triangles 8 bytes (11-bit words, 6-bit ranks).  An edge with both
+ * ends owned is accumulated in the row of the smaller slot; mir[] = src | dst
+ * << 16 (block accumulator indices) fills its transposed entry.
+ * eblk[n_blocks*4] = {mirror offset, mirror count, e0 | e1 << 16, e2}: the
+ * ends of the block's record groups with nb, nb-1 and nb-2 owned vertices.
+ * info[0] words per record, [1] mirror entries, [2] max per block, [3] max
+ * rank, [4] LDS bytes per block of the edge kernel, [5] node-word bits. */
 typedef struct tm_plan_edge tm_plan_edge;
 int tm_plan_edge_build(int nb, int64_t n_blocks, const int32_t* blk, const uint32_t* meta,
                        const uint32_t* recs, int n_threads, tm_plan_edge** out);
 int tm_plan_edge_info(const tm_plan_edge* plan, int64_t* info);
-/* recs[n_records * info[0]], mir_ptr[n_blocks + 1], mir[info[1]] */
-int tm_plan_edge_export(const tm_plan_edge* plan, uint32_t* recs, int32_t* mir_ptr, uint32_t* mir);
+/* recs[n_records * info[0]], eblk[n_blocks * 4], mir[info[1]] */
+int tm_plan_edge_export(const tm_plan_edge* plan, uint32_t* recs, int32_t* eblk, uint32_t* mir);
 void tm_plan_edge_destroy(tm_plan_edge* plan);
 
 /* -------------------------------------------------------------- device --- */
@@ -149,10 +154,10 @@ typedef struct {
   /* optional edge records of the same plan (tm_plan_edge_export; NULL: none):
    * tm_asm_p1_matrix uses them for its symmetric forms unless deterministic */
   const uint32_t* erecs;
-  const int32_t* emir_ptr;
+  const int32_t* eblk;
   const uint32_t* emir;
   int32_t erec_words; /* 2 or 3 */
-  int32_t reserved2;
+  int32_t elds_bytes; /* tm_plan_edge_info info[4] */
 } tm_plan_dev;
 
 /* Reference-element moments of the quadrature rule: wsum = sum_q w_q,

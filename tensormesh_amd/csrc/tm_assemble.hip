@@ -552,20 +552,32 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
 
 // ---- edge records (symmetric scalar forms) ---------------------------------
 //
-// A record of W 32-bit words: NB 10-bit LDS slots (owned vertices first, by
+// A record of W 32-bit words: NB LDS node words (owned vertices first, by
 // ascending slot), then one rank per vertex pair (i, j), i < j, in the order
 // (0,1) (0,2) (0,3) (1,2) (1,3) (2,3): the rank of vertex j in the row of
 // vertex i, which accumulates the pair's value when it is owned.  Tetrahedra:
-// ranks from bit 40, 4 bits (W = 2) or 6 bits (W = 3); triangles: 6 bits from
-// bit 32 (W = 2).  Every field lies inside one word.  An edge with both ends
-// owned is accumulated once, in the row of the smaller slot; the transposed
-// entry is copied from it after the element loop (the block's mirror list,
-// {src | dst << 16} accumulator indices).
+// 10-bit node words and 4-bit ranks from bit 40 (W = 2), or 11-bit words and
+// 6-bit ranks at word 1 bits 12, 18, 24 and word 2 bits 0, 6, 12 (W = 3);
+// triangles: 11-bit words, 6-bit ranks at word 1 bits 12, 18, 24 (W = 2).  An
+// edge with both ends owned is accumulated once, in the row of the smaller
+// slot; the transposed entry is copied from it after the element loop (the
+// block's mirror list, {src | dst << 16} accumulator indices).
+//
+// LDS of a block: the accumulators, then the nodes in groups of 16 slots,
+// 512 bytes each: 16 first words {x, y} then 16 second words {z | coef,
+// packed coefficient and row meta}.  Slot s sits at node word w(s) = s +
+// (s & ~15) (its second word at w(s) + 16), which is what records carry: one
+// address serves both reads, and a quarter-wave's reads of slots that differ
+// mod 16 stay conflict-free.  The row meta's offset field holds the row's byte
+// offset in the accumulators (accumulators < 8192).  The block's records come
+// grouped by descending owned count; eblk gives the group ends.
 struct EdgeArgs {
   const uint32_t* recs;
-  const int32_t* mir_ptr;
+  const int32_t* eblk;
   const uint32_t* mir;
 };
+
+__device__ __forceinline__ int node_word(int s) { return s + (s & ~15); }
 
 template <int W>
 struct ERec {
@@ -594,29 +606,40 @@ struct ERecBuf {
   }
 };
 
+// LDS node word of record vertex a
 template <int NB, int W>
-__device__ __forceinline__ int erec_slot(const ERec<W>& e, int a) {
-  if (NB == 4 && a == 3) return (int)(__builtin_amdgcn_alignbit(e.w[1], e.w[0], 30) & 1023u);
-  return (int)__builtin_amdgcn_ubfe(e.w[0], 10 * a, 10);
+__device__ __forceinline__ uint32_t erec_word(const ERec<W>& e, int a) {
+  constexpr int SB = (NB == 4 && W == 2) ? 10 : 11;
+  const int bit = SB * a;
+  if (bit + SB <= 32) return __builtin_amdgcn_ubfe(e.w[0], bit, SB);
+  if (bit >= 32) return __builtin_amdgcn_ubfe(e.w[1], bit - 32, SB);
+  return __builtin_amdgcn_alignbit(e.w[1], e.w[0], bit) & ((1u << SB) - 1);
 }
 
+// byte offset (rank * 8) of pair k's rank field
 template <int NB, int W>
-__device__ __forceinline__ int erec_rank(const ERec<W>& e, int k) {
-  constexpr int R = (NB == 4 && W == 2) ? 4 : 6;
-  constexpr int base = NB == 4 ? 40 : 32;
-  const int bit = base + R * k;
-  return (int)__builtin_amdgcn_ubfe(e.w[bit / 32], bit % 32, R);
+__device__ __forceinline__ uint32_t erec_rank8(const ERec<W>& e, int k) {
+  constexpr bool R4 = NB == 4 && W == 2;
+  const int word = (R4 || k < 3) ? 1 : 2;
+  const int bit = R4 ? 8 + 4 * k : (k < 3 ? 12 + 6 * k : 6 * (k - 3));
+  return __builtin_amdgcn_ubfe(e.w[word], bit, R4 ? 4 : 6) << 3;
+}
+
+__device__ __forceinline__ double* acc_at(const Lds& L, uint32_t byte_off) {
+  return (double*)((char*)L.acc + byte_off);
 }
 
 // Pairs (i, j), i < j, of an edge record with vertices [0, K) owned: each
-// pair with an owned first vertex adds its value once, to the row of i.
+// pair with an owned first vertex adds its value once, to the row of i
+// (rb[i]: the row's byte offset in the accumulators).
 template <int NB, int K, int FORM, int W>
-__device__ __forceinline__ void emit_edges(const Lds& L, const ERec<W>& e, const uint32_t (&m)[NB],
-                                           const double (&c)[NB][NB - 1], double s, const QuadArg& q) {
+__device__ __forceinline__ void emit_edges(const Lds& L, const ERec<W>& e, const uint32_t (&rb)[NB],
+                                           const uint32_t (&m)[NB], const double (&c)[NB][NB - 1], double s,
+                                           const QuadArg& q) {
   constexpr int DIM = NB - 1;
   if (FORM == 2) {
 #pragma unroll
-    for (int a = 0; a < K; ++a) atomicAdd(L.acc + meta_off(m[a]) + meta_diag(m[a]), s * q.M[a * NB + a]);
+    for (int a = 0; a < K; ++a) atomicAdd(acc_at(L, rb[a] + 8 * meta_diag(m[a])), s * q.M[a * NB + a]);
   }
   int k = 0;
 #pragma unroll
@@ -625,8 +648,65 @@ __device__ __forceinline__ void emit_edges(const Lds& L, const ERec<W>& e, const
     for (int j = i + 1; j < NB; ++j, ++k) {
       if (i >= K) continue;
       const double kij = FORM == 2 ? s * q.M[i * NB + j] : s * dotc<DIM>(c[i], c[j]);
-      atomicAdd(L.acc + meta_off(m[i]) + erec_rank<NB>(e, k), kij);
+      atomicAdd(acc_at(L, rb[i] + erec_rank8<NB>(e, k)), kij);
     }
+}
+
+// Stages the block's nodes in the edge layout (node word w(s) = {x, y}, w(s) +
+// 16 = {z | coef, packed coef and row meta with its offset in bytes}) and
+// zeroes the accumulators.  A lane stages slots tid, tid + NT, ... with all
+// of its first K halo ids and meta words loaded before any gather; empty
+// slots and lanes past the halo keep node 0's data or stay unwritten.
+template <int NT, int DIM, bool COEF>
+__device__ __forceinline__ void stage_edge(const Lds& L, double2* nodes, const int32_t* __restrict__ halo,
+                                           const uint32_t* __restrict__ meta, int nacc, int nh,
+                                           const double* __restrict__ X, const float* __restrict__ coef) {
+  constexpr int K = (256 + NT - 1) / NT;
+  int32_t g[K];
+  uint32_t mh[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int h = min((int)threadIdx.x + k * NT, nh - 1);
+    g[k] = halo[h];
+    mh[k] = meta[h];
+  }
+  double2 n0[K], n1[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int64_t gi = g[k] >= 0 ? g[k] : 0;
+    const float cf = COEF ? coef[gi] : 0.0f;
+    const uint32_t mb = (mh[k] & 0xffff0000u) | ((mh[k] & 0xffffu) << 3);
+    const double pc = pack_coef_meta(cf, mb);
+    if (DIM == 3) {
+      n0[k] = make_double2(X[gi * 3], X[gi * 3 + 1]);
+      n1[k] = make_double2(X[gi * 3 + 2], pc);
+    } else {
+      n0[k] = make_double2(X[gi * 2], X[gi * 2 + 1]);
+      n1[k] = make_double2(0.0, pc);
+    }
+  }
+  for (int i = threadIdx.x; i < nacc / 2; i += NT) ((double2*)L.acc)[i] = make_double2(0.0, 0.0);
+  if ((nacc & 1) && threadIdx.x == 0) L.acc[nacc - 1] = 0.0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int h = threadIdx.x + k * NT;
+    if (h < nh) {
+      nodes[node_word(h)] = n0[k];
+      nodes[node_word(h) + 16] = n1[k];
+    }
+  }
+  for (int h = threadIdx.x + K * NT; h < nh; h += NT) {
+    const int64_t gh = halo[h] >= 0 ? halo[h] : 0;
+    const uint32_t m = meta[h];
+    const double pc = pack_coef_meta(COEF ? coef[gh] : 0.0f, (m & 0xffff0000u) | ((m & 0xffffu) << 3));
+    nodes[node_word(h)] = DIM == 3 ? make_double2(X[gh * 3], X[gh * 3 + 1]) : make_double2(X[gh * 2], X[gh * 2 + 1]);
+    nodes[node_word(h) + 16] = make_double2(DIM == 3 ? X[gh * 3 + 2] : 0.0, pc);
+  }
+}
+
+// Owned count of the records of group g (0: NB owned, 1: NB-1, ...).
+__device__ __forceinline__ int rec_group(int r, int e0, int e1, int e2) {
+  return (r >= e0 ? 1 : 0) + (r >= e1 ? 1 : 0) + (r >= e2 ? 1 : 0);
 }
 
 // LO: FP32 output values.  1/|det| then takes one Newton step (relative error
@@ -635,23 +715,29 @@ __device__ __forceinline__ void emit_edges(const Lds& L, const ERec<W>& e, const
 // far below the FP32 rounding of the stored values.  FP64 outputs keep two
 // Newton steps and the FP64 weighted sum.
 template <int NB, int FORM, int NT, int W, bool LO>
-__device__ __forceinline__ void edge_records(const Lds& L, const ERecBuf<W>& rb, int nrec, int nown, ERec<W> nxt,
-                                             ERec<W> nxt2, double scale, const QuadArg& q) {
+__device__ __forceinline__ void edge_records(const Lds& L, const double2* nodes, const ERecBuf<W>& rb, int nrec,
+                                             int e0, int e1, int e2, ERec<W> nxt, ERec<W> nxt2, double scale,
+                                             const QuadArg& q) {
   constexpr int DIM = NB - 1;
   const double sw = FORM == 0 ? scale * q.wsum : scale * q.W[0];
-  for (int r = threadIdx.x; r < nrec; r += NT) {
+  const int lane = threadIdx.x % kWave;
+  // r0: the wave's first record of the iteration (wave-uniform, kept in SGPRs)
+  const int w0 = __builtin_amdgcn_readfirstlane((int)threadIdx.x - lane);
+  for (int r0 = w0; r0 < nrec; r0 += NT) {
+    const int r = r0 + lane;
     const ERec<W> e = nxt;
     nxt = nxt2;
     nxt2 = rb[r + 2 * NT];  // records two iterations ahead (zeros past the block's end)
-    int l[NB];
+    if (r >= nrec) continue;
     double x[NB][DIM];
     float cf[NB];
-    uint32_t m[NB];
+    uint32_t m[NB], rw[NB];
 #pragma unroll
     for (int a = 0; a < NB; ++a) {
-      l[a] = erec_slot<NB>(e, a);
-      const double2 n0 = L.p0[l[a]], n1 = L.p1[l[a]];
+      const double2* nd = nodes + erec_word<NB>(e, a);
+      const double2 n0 = nd[0], n1 = nd[16];
       m[a] = packed_meta(n1);
+      rw[a] = m[a] & 0xffffu;
       node_xyz<DIM>(n0, n1, x[a]);
       cf[a] = __int_as_float(__double2loint(n1.y));
     }
@@ -677,26 +763,27 @@ __device__ __forceinline__ void edge_records(const Lds& L, const ERecBuf<W>& rb,
     } else {
       s = scale * adet;
     }
-    int nk = 0;
-#pragma unroll
-    for (int a = 0; a < NB; ++a) nk += l[a] < nown ? 1 : 0;
-    const int k0 = __builtin_amdgcn_readfirstlane(nk);
-    if (__all(nk == k0)) {
-      if (k0 == NB) {
-        emit_edges<NB, NB, FORM>(L, e, m, c, s, q);
-      } else if (k0 == 1) {
-        emit_edges<NB, 1, FORM>(L, e, m, c, s, q);
-      } else if (k0 == 2) {
-        emit_edges<NB, 2, FORM>(L, e, m, c, s, q);
-      } else if (NB == 4 && k0 == 3) {
-        emit_edges<NB, (NB == 4 ? 3 : 1), FORM>(L, e, m, c, s, q);
+    // the wave's records [r0, last] lie in one owned-count group (no group end
+    // in (r0, last]): a branch-free instance, chosen on scalar registers
+    const int last = min(r0 + kWave - 1, nrec - 1);
+    const bool split = (r0 < e0 && e0 <= last) || (r0 < e1 && e1 <= last) || (r0 < e2 && e2 <= last);
+    if (!split) {
+      if (r0 < e0) {
+        emit_edges<NB, NB, FORM>(L, e, rw, m, c, s, q);
+      } else if (r0 < e1) {
+        emit_edges<NB, NB - 1, FORM>(L, e, rw, m, c, s, q);
+      } else if (r0 < e2) {
+        emit_edges<NB, NB - 2, FORM>(L, e, rw, m, c, s, q);
+      } else if (NB == 4) {
+        emit_edges<NB, (NB == 4 ? 1 : 0), FORM>(L, e, rw, m, c, s, q);
       }
       continue;
     }
+    const int nk = NB - rec_group(r, e0, e1, e2);
     if (FORM == 2) {
 #pragma unroll
       for (int a = 0; a < NB; ++a)
-        if (a < nk) atomicAdd(L.acc + meta_off(m[a]) + meta_diag(m[a]), s * q.M[a * NB + a]);
+        if (a < nk) atomicAdd(acc_at(L, rw[a] + 8 * meta_diag(m[a])), s * q.M[a * NB + a]);
     }
     int k = 0;
 #pragma unroll
@@ -705,13 +792,35 @@ __device__ __forceinline__ void edge_records(const Lds& L, const ERecBuf<W>& rb,
       for (int j = i + 1; j < NB; ++j, ++k) {
         if (i >= nk) continue;
         const double kij = FORM == 2 ? s * q.M[i * NB + j] : s * dotc<DIM>(c[i], c[j]);
-        atomicAdd(L.acc + meta_off(m[i]) + erec_rank<NB>(e, k), kij);
+        atomicAdd(acc_at(L, rw[i] + erec_rank8<NB>(e, k)), kij);
       }
   }
 }
 
-// Symmetric scalar forms on edge records: as k_asm_matrix (packed node words),
-// plus the mirror copies between the element loop and the row sums.
+// Row sums of the edge layout (the diagonal accumulators hold 0): every entry
+// of the row is summed, 16 reads at fixed offsets from the row start (entries
+// past the row end are read and dropped; they lie inside the block's LDS).
+template <int NT>
+__device__ __forceinline__ void rowsum_edge(const Lds& L, const double2* nodes, int nown) {
+  for (int a = threadIdx.x; a < nown; a += NT) {
+    const uint32_t m = (uint32_t)__double2hiint(nodes[node_word(a) + 16].y);
+    const int len = meta_len(m);
+    if (len == 0) continue;
+    const double* r = acc_at(L, m & 0xffffu);
+    double s = 0.0;
+    for (int k0 = 0; k0 < len; k0 += 16) {
+      double v[16];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) v[k] = r[k0 + k];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) s += (k0 + k < len) ? v[k] : 0.0;
+    }
+    *acc_at(L, (m & 0xffffu) + 8 * meta_diag(m)) = -s;
+  }
+}
+
+// Symmetric scalar forms on edge records: staging, element loop, mirror
+// copies, row sums (Laplace-type forms), row stores.
 template <int NB, int FORM, typename OT, int NT, int W>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_asm_matrix_edge(
     PlanArgs p, EdgeArgs ea, const double* __restrict__ X, const float* __restrict__ coef, double scale, QuadArg q,
@@ -721,16 +830,19 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
   const int b = tmx::xcd_remap(blockIdx.x, p.nblk);
   const int4 bi0 = *(const int4*)(p.blk + 8 * b);
   const int4 bi1 = *(const int4*)(p.blk + 8 * b + 4);
+  const int4 eb = *(const int4*)(ea.eblk + 4 * b);
   const int hoff = bi0.x, nh = bi0.y, nown = bi0.z, roff = bi0.w;
   const int nrec = bi1.x, runoff = bi1.y, nrun = bi1.z, nacc = bi1.w;
-  const Lds L = carve(smem, nacc, nh, nown);
+  const int moff = eb.x, nmir = eb.y, e0 = eb.z & 0xffff, e1 = (int)((uint32_t)eb.z >> 16), e2 = eb.w;
+  Lds L;
+  L.acc = (double*)smem;
+  double2* nodes = (double2*)(smem + a16(8 * (size_t)nacc));
   const ERecBuf<W> rb(ea.recs, roff, nrec);
   const int t = threadIdx.x;
   ERec<W> nxt = rb[t], nxt2 = rb[t + NT];
-  stage<NT, DIM, float, FORM == 1, true>(L, p.halo + hoff, p.meta + hoff, nacc, nh, nown, X, coef);
+  stage_edge<NT, DIM, FORM == 1>(L, nodes, p.halo + hoff, p.meta + hoff, nacc, nh, X, coef);
   __syncthreads();
-  edge_records<NB, FORM, NT, W, std::is_same<OT, float>::value>(L, rb, nrec, nown, nxt, nxt2, scale, q);
-  const int moff = ea.mir_ptr[b], nmir = ea.mir_ptr[b + 1] - moff;
+  edge_records<NB, FORM, NT, W, std::is_same<OT, float>::value>(L, nodes, rb, nrec, e0, e1, e2, nxt, nxt2, scale, q);
   const uint32_t mv = t < nmir ? ea.mir[moff + t] : 0u;  // in flight during the barrier
   const int4 dsc = run_descriptors<NT>(p, runoff, nrun);
   __syncthreads();
@@ -741,7 +853,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
   }
   __syncthreads();
   if (FORM != 2) {
-    rowsum_diagonal<NT, true>(L, nown);
+    rowsum_edge<NT>(L, nodes, nown);
     __syncthreads();
   }
   store_runs<NT>(L, p, runoff, nrun, dsc, values);
@@ -1197,19 +1309,22 @@ template <int NB, typename OT, int W>
 int dispatch_matrix_edge(int form, const tm_plan_dev* plan, const double* X, const float* coef, double scale,
                          QuadArg q, OT* values, hipStream_t st) {
   const PlanArgs p = plan_args(plan);
-  const EdgeArgs e{plan->erecs, plan->emir_ptr, plan->emir};
+  const EdgeArgs e{plan->erecs, plan->eblk, plan->emir};
+  TM_REQUIRE(plan->elds_bytes > 0 && plan->elds_bytes <= 65536, TM_ERR_ARG, "edge LDS size %d", plan->elds_bytes);
+  tm_plan_dev pe = *plan;  // the edge layout's LDS size
+  pe.lds_bytes = plan->elds_bytes;
   if (form == TM_FORM_LAPLACE)
-    TM_BY_THREADS(plan, (k_asm_matrix_edge<NB, 0, OT, NT, W>), st, p, e, X, coef, scale, q, values);
+    TM_BY_THREADS(&pe, (k_asm_matrix_edge<NB, 0, OT, NT, W>), st, p, e, X, coef, scale, q, values);
   if (form == TM_FORM_MASS)
-    TM_BY_THREADS(plan, (k_asm_matrix_edge<NB, 2, OT, NT, W>), st, p, e, X, coef, scale, q, values);
-  TM_BY_THREADS(plan, (k_asm_matrix_edge<NB, 1, OT, NT, W>), st, p, e, X, coef, scale, q, values);
+    TM_BY_THREADS(&pe, (k_asm_matrix_edge<NB, 2, OT, NT, W>), st, p, e, X, coef, scale, q, values);
+  TM_BY_THREADS(&pe, (k_asm_matrix_edge<NB, 1, OT, NT, W>), st, p, e, X, coef, scale, q, values);
 }
 
 template <int NB, typename OT, bool DET>
 int dispatch_matrix(int form, const tm_plan_dev* plan, const double* X, const void* coef, int cdt,
                     double scale, QuadArg q, OT* values, hipStream_t st) {
   PlanArgs p = plan_args(plan);
-  const bool edge_ok = !DET && plan->erecs != nullptr && plan->emir_ptr != nullptr &&
+  const bool edge_ok = !DET && plan->erecs != nullptr && plan->eblk != nullptr && plan->emir != nullptr &&
                        (form == TM_FORM_LAPLACE || form == TM_FORM_MASS ||
                         (form == TM_FORM_KAPPA_LAPLACE && coef != nullptr && cdt == TM_F32));
   if (edge_ok) {

@@ -675,15 +675,23 @@ extern "C" void tm_plan_destroy(tm_plan* p) { delete p; }
 // (records list owned vertices first, by ascending slot), at the rank of the
 // other end in that row.  Each pair (i, j), i < j, then needs one rank field
 // instead of two, and the record shrinks from 16 to 8 bytes when every rank
-// fits 4 bits (12 bytes otherwise).  The transposed entry of an edge whose
-// two ends are both owned is filled after the element loop from a per-block
-// mirror list {src | dst << 16} of accumulator indices.
+// fits 4 bits and every halo 512 nodes (12 bytes otherwise).  The transposed
+// entry of an edge whose two ends are both owned is filled after the element
+// loop from a per-block mirror list {src | dst << 16} of accumulator indices.
+//
+// Layout the edge kernel stages (and the records address): node slot s keeps
+// its two 16-byte words at LDS words w(s) and w(s) + 16, w(s) = s + (s & ~15),
+// i.e. groups of 16 slots hold 16 first words then 16 second words, so a
+// record carries w(s) and both reads share one address.  Per block:
+// eblk = {mirror offset, mirror count, e0 | e1 << 16, e2}, the ends of the
+// record groups with nb, nb-1 and nb-2 owned vertices (the plan sorts a
+// block's records by descending owned count).
 
 struct tm_plan_edge {
   std::vector<uint32_t> recs;
-  std::vector<int32_t> mir_ptr;
+  std::vector<int32_t> eblk;
   std::vector<uint32_t> mir;
-  int64_t info[4] = {0, 0, 0, 0};
+  int64_t info[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 namespace {
@@ -696,6 +704,8 @@ inline uint32_t rec16_rank(const uint32_t* r, int nb, int a, int b) {
   const uint64_t v = bit < 64 ? (lo >> bit) | (bit > 58 ? hi << (64 - bit) : 0) : hi >> (bit - 64);
   return (uint32_t)(v & 63u);
 }
+
+inline uint32_t node_word(uint32_t s) { return s + (s & ~15u); }
 
 }  // namespace
 
@@ -712,29 +722,57 @@ extern "C" int tm_plan_edge_build(int nb, int64_t n_blocks, const int32_t* blk, 
       pi[k] = i;
       pj[k] = j;
     }
-  // record width: tetrahedra take 4-bit ranks (8 bytes) when every rank an
-  // owned row needs is below 16, 6-bit ranks (12 bytes) otherwise
+  // record width: tetrahedra take 10-bit node words and 4-bit ranks (8 bytes)
+  // when every rank an owned row needs is below 16 and every halo has at most
+  // 512 nodes, 11-bit words and 6-bit ranks (12 bytes) otherwise; triangles
+  // 11-bit words and 6-bit ranks (8 bytes)
   uint32_t max_rank = 0;
+  int64_t max_nh = 0, max_lds = 0;
   for (int64_t b = 0; b < n_blocks; ++b) {
     const int32_t* bi = blk + 8 * b;
     const int nown = bi[2];
+    max_nh = std::max<int64_t>(max_nh, bi[1]);
+    TM_REQUIRE(bi[7] < 8192 && bi[4] < 65536, TM_ERR_LIMIT,
+               "block %lld: %d accumulators, %d records (edge records take < 8192, < 65536)", (long long)b, bi[7],
+               bi[4]);
+    max_lds = std::max<int64_t>(max_lds, ((8 * (int64_t)bi[7] + 15) & ~int64_t(15)) + 512 * (((int64_t)bi[1] + 15) / 16));
+    int prev = nb;
     for (int64_t r = bi[3]; r < (int64_t)bi[3] + bi[4]; ++r) {
       const uint32_t* rc = recs + 4 * r;
       const uint64_t lo = (uint64_t)rc[0] | ((uint64_t)rc[1] << 32);
+      int no = 0;
       for (int a = 0; a < nb; ++a) {
         if ((int)((lo >> (10 * a)) & 1023u) >= nown) continue;
+        ++no;
         for (int c = 0; c < nb; ++c)
           if (c != a) max_rank = std::max(max_rank, rec16_rank(rc, nb, a, c));
       }
+      TM_REQUIRE(no >= 1 && no <= prev, TM_ERR_ARG,
+                 "block %lld: records not grouped by descending owned count", (long long)b);
+      prev = no;
     }
   }
-  const int W = (nb == 4 && max_rank > 15) ? 3 : 2;
+  TM_REQUIRE(max_lds <= 65536, TM_ERR_LIMIT, "edge-record blocks need %lld bytes of LDS", (long long)max_lds);
+  const int W = (nb == 4 && (max_rank > 15 || max_nh > 512)) ? 3 : 2;
+  const int SB = (nb == 4 && W == 2) ? 10 : 11;
   const int R = (nb == 4 && W == 2) ? 4 : 6;
-  const int rbase = nb == 4 ? 40 : 32;
+  auto rank_pos = [&](int k, int& word, int& bit) {
+    if (nb == 4 && W == 2) {
+      word = 1;
+      bit = 8 + 4 * k;
+    } else if (k < 3) {
+      word = 1;
+      bit = 12 + 6 * k;
+    } else {
+      word = 2;
+      bit = 6 * (k - 3);
+    }
+  };
 
   auto* P = new tm_plan_edge();
   int64_t nrec = n_blocks > 0 ? (int64_t)blk[8 * (n_blocks - 1) + 3] + blk[8 * (n_blocks - 1) + 4] : 0;
   P->recs.assign((size_t)nrec * W, 0u);
+  P->eblk.assign((size_t)n_blocks * 4, 0);
   std::vector<std::vector<uint32_t>> mirs((size_t)n_blocks);
   std::atomic<int64_t> next{0};
   const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(pick_threads(n_threads), std::max<int64_t>(1, n_blocks / 64)));
@@ -748,6 +786,7 @@ extern "C" int tm_plan_edge_build(int nb, int64_t n_blocks, const int32_t* blk, 
         for (int64_t b = b0; b < std::min(n_blocks, b0 + 64); ++b) {
           const int32_t* bi = blk + 8 * b;
           const int hoff = bi[0], nown = bi[2];
+          int ge[3] = {0, 0, 0};  // records with at least nb, nb-1, nb-2 owned vertices
           keys.clear();
           for (int64_t r = bi[3]; r < (int64_t)bi[3] + bi[4]; ++r) {
             const uint32_t* rc = recs + 4 * r;
@@ -760,17 +799,19 @@ extern "C" int tm_plan_edge_build(int nb, int64_t n_blocks, const int32_t* blk, 
             std::sort(ord, ord + no, [&](int x, int y) { return sl[x] < sl[y]; });
             for (int a = 0, k = no; a < nb; ++a)
               if (sl[a] >= nown) ord[k++] = a;
+            for (int g = 0; g < 3; ++g) ge[g] += no >= nb - g ? 1 : 0;
             uint32_t w[3] = {0, 0, 0};
-            uint64_t slots = 0;
-            for (int i = 0; i < nb; ++i) slots |= (uint64_t)sl[ord[i]] << (10 * i);
-            w[0] = (uint32_t)slots;
-            w[1] = (uint32_t)(slots >> 32);
+            uint64_t words = 0;
+            for (int i = 0; i < nb; ++i) words |= (uint64_t)node_word((uint32_t)sl[ord[i]]) << (SB * i);
+            w[0] = (uint32_t)words;
+            w[1] = (uint32_t)(words >> 32);
             for (int k = 0; k < npair; ++k) {
               const int a = ord[pi[k]], c = ord[pj[k]];
               if (pi[k] >= no) continue;  // neither end owned
               const uint32_t rk = rec16_rank(rc, nb, a, c);
-              const int bit = rbase + R * k;
-              w[bit / 32] |= rk << (bit % 32);
+              int word, bit;
+              rank_pos(k, word, bit);
+              w[word] |= rk << bit;
               if (pj[k] < no) {  // both ends owned: the transposed entry is mirrored
                 const uint32_t src = (meta[hoff + sl[a]] & 0xffffu) + rk;
                 const uint32_t dst = (meta[hoff + sl[c]] & 0xffffu) + rec16_rank(rc, nb, c, a);
@@ -784,21 +825,24 @@ extern "C" int tm_plan_edge_build(int nb, int64_t n_blocks, const int32_t* blk, 
           auto& m = mirs[(size_t)b];
           m.reserve(keys.size());
           for (uint64_t k : keys) m.push_back((uint32_t)(k & 0xffffu) | ((uint32_t)(k >> 32) << 16));
+          int32_t* eb = P->eblk.data() + 4 * b;
+          eb[1] = (int32_t)keys.size();
+          eb[2] = ge[0] | (ge[1] << 16);
+          eb[3] = ge[2];
         }
       }
     });
   }
   for (auto& x : th) x.join();
-  P->mir_ptr.assign((size_t)n_blocks + 1, 0);
   int64_t tot = 0, mx = 0;
   for (int64_t b = 0; b < n_blocks; ++b) {
-    tot += (int64_t)mirs[(size_t)b].size();
-    mx = std::max<int64_t>(mx, (int64_t)mirs[(size_t)b].size());
-    if (tot >= (int64_t(1) << 31)) {
+    if (tot + (int64_t)mirs[(size_t)b].size() >= (int64_t(1) << 31)) {
       delete P;
       TM_REQUIRE(false, TM_ERR_LIMIT, "mirror list too large for int32 offsets");
     }
-    P->mir_ptr[(size_t)b + 1] = (int32_t)tot;
+    P->eblk[(size_t)b * 4] = (int32_t)tot;
+    tot += (int64_t)mirs[(size_t)b].size();
+    mx = std::max<int64_t>(mx, (int64_t)mirs[(size_t)b].size());
   }
   P->mir.reserve((size_t)tot);
   for (auto& m : mirs) P->mir.insert(P->mir.end(), m.begin(), m.end());
@@ -806,6 +850,8 @@ extern "C" int tm_plan_edge_build(int nb, int64_t n_blocks, const int32_t* blk, 
   P->info[1] = tot;
   P->info[2] = mx;
   P->info[3] = max_rank;
+  P->info[4] = max_lds;
+  P->info[5] = SB;
   *out = P;
   return TM_OK;
 }
@@ -816,10 +862,10 @@ extern "C" int tm_plan_edge_info(const tm_plan_edge* p, int64_t* info) {
   return TM_OK;
 }
 
-extern "C" int tm_plan_edge_export(const tm_plan_edge* p, uint32_t* recs, int32_t* mir_ptr, uint32_t* mir) {
+extern "C" int tm_plan_edge_export(const tm_plan_edge* p, uint32_t* recs, int32_t* eblk, uint32_t* mir) {
   TM_REQUIRE(p, TM_ERR_ARG, "null edge plan");
   if (recs) std::memcpy(recs, p->recs.data(), p->recs.size() * 4);
-  if (mir_ptr) std::memcpy(mir_ptr, p->mir_ptr.data(), p->mir_ptr.size() * 4);
+  if (eblk) std::memcpy(eblk, p->eblk.data(), p->eblk.size() * 4);
   if (mir) std::memcpy(mir, p->mir.data(), p->mir.size() * 4);
   return TM_OK;
 }

@@ -100,17 +100,25 @@ def emulate_matrix(topo, arrs, element_matrix, det=False, shuffle=None):
 
 
 def decode_edge(words, nb):
-    """Edge record (2 or 3 words): record-order slots and ``rank(k)`` of pair
-    k in the order (0,1) (0,2) (0,3) (1,2) (1,3) (2,3)."""
+    """Edge record (2 or 3 words): record-order slots (decoded from the LDS
+    node words w(s) = s + (s & ~15)) and ``rank(k)`` of pair k in the order
+    (0,1) (0,2) (0,3) (1,2) (1,3) (2,3)."""
     w = [int(x) for x in words]
+    W = len(w)
+    SB = 10 if (nb == 4 and W == 2) else 11
     s = w[0] | (w[1] << 32)
-    loc = [(s >> (10 * a)) & 1023 for a in range(nb)]
-    R = 4 if (nb == 4 and len(w) == 2) else 6
-    base = 40 if nb == 4 else 32
+    nw = [(s >> (SB * a)) & ((1 << SB) - 1) for a in range(nb)]
+    assert all(x & 16 == 0 for x in nw), "node words have bit 4 clear"
+    loc = [16 * (x >> 5) + (x & 15) for x in nw]
 
     def rank(k):
-        bit = base + R * k
-        return (w[bit // 32] >> (bit % 32)) & ((1 << R) - 1)
+        if nb == 4 and W == 2:
+            word, bit, R = 1, 8 + 4 * k, 4
+        elif k < 3:
+            word, bit, R = 1, 12 + 6 * k, 6
+        else:
+            word, bit, R = 2, 6 * (k - 3), 6
+        return (w[word] >> bit) & ((1 << R) - 1)
     return loc, rank
 
 
@@ -119,18 +127,25 @@ def emulate_edge_matrix(topo, arrs, earrs, words, element_matrix, rowsum=True):
     owned first vertex adds K_ij once, to the row of i; the mirror list then
     fills the transposed entries; ``rowsum`` forms the diagonal as minus the
     row's off-diagonal sum (Laplace-type forms), otherwise the owned diagonal
-    entries are added directly (mass)."""
+    entries are added directly (mass).  The owned count of each record must
+    match the block's group ends."""
     nb = topo.nb
     blk = arrs["blk"].reshape(-1, 8)
     erecs = earrs["erecs"].reshape(-1, words)
+    eblk = earrs["eblk"].reshape(-1, 4)
     runs = arrs["runs"].reshape(-1, 4)
-    mptr, mir = earrs["emir_ptr"], earrs["emir"]
+    mir = earrs["emir"]
     rp, ci = topo.row_ptr, topo.col_idx
     vals = np.zeros(topo.nnz)
     covered = np.zeros(topo.nnz, dtype=np.int64)
     pairs = [(i, j) for i in range(nb) for j in range(i + 1, nb)]
+    moff = 0
     for b in range(blk.shape[0]):
         hoff, nh, nown, roff, nrec, runoff, nrun, nacc = (int(v) for v in blk[b])
+        mo, mn, e01, e2 = (int(v) for v in eblk[b])
+        assert mo == moff
+        moff += mn
+        ends = [e01 & 0xFFFF, e01 >> 16, e2]
         halo = arrs["halo"][hoff:hoff + nh]
         meta = arrs["meta"][hoff:hoff + nh].astype(np.int64)
         acc = np.zeros(nacc)
@@ -141,6 +156,7 @@ def emulate_edge_matrix(topo, arrs, earrs, words, element_matrix, rowsum=True):
             e = int(arrs["rec_elem"][roff + r])
             assert sorted(halo[loc]) == sorted(topo.conn[e])
             nk = sum(1 for s in loc if s < nown)
+            assert nk == nb - sum(1 for x in ends if r >= x)
             assert all(s < nown for s in loc[:nk]) and loc[:nk] == sorted(loc[:nk])
             K = element_matrix(e, topo.points[halo[loc]])
             if not rowsum:
@@ -157,7 +173,7 @@ def emulate_edge_matrix(topo, arrs, earrs, words, element_matrix, rowsum=True):
                 assert ci[rp[halo[loc[i]]] + rk] == halo[loc[j]]
                 acc[base + rk] += K[i, j]
                 hit[base + rk] = True
-        for v in mir[mptr[b]:mptr[b + 1]]:
+        for v in mir[mo:mo + mn]:
             src, dst = int(v) & 0xFFFF, int(v) >> 16
             assert hit[src] and not hit[dst] and dst < nacc
             acc[dst] = acc[src]
