@@ -154,8 +154,10 @@ __device__ __forceinline__ double elem_E(const ET* E, double E0, int e) {
 template <typename ET>
 __device__ __forceinline__ void stage_v(const LdsV& L, const int32_t* halo, const uint32_t* meta, int nacc, int nh,
                                         int nown, const double* X, const double* U, bool zero_acc) {
-  if (zero_acc)
-    for (int i = threadIdx.x; i < 9 * nacc; i += kThreads) L.acc[i] = 0.0;
+  if (zero_acc) {  // 16-byte stores (9 nacc doubles; the region is 16-byte aligned)
+    for (int i = threadIdx.x; i < 9 * nacc / 2; i += kThreads) ((double2*)L.acc)[i] = make_double2(0.0, 0.0);
+    if (((9 * nacc) & 1) && threadIdx.x == 0) L.acc[9 * nacc - 1] = 0.0;
+  }
   for (int h = threadIdx.x; h < nh; h += kThreads) {
     const int64_t g = halo[h];
 #pragma unroll
@@ -187,12 +189,17 @@ __global__ __launch_bounds__(kThreads) void k_nh_assemble(PlanArgs p, const doub
   const int hoff = bi0.x, nh = bi0.y, nown = bi0.z, roff = bi0.w;
   const int nrec = bi1.x, runoff = bi1.y, nrun = bi1.z, nacc = bi1.w;
   const LdsV L = carve_v(smem, nacc, nh, nown);
+  // a lane's first record and its element's modulus do not depend on the
+  // staged nodes: their loads are issued first
+  const int t0 = threadIdx.x;
+  const uint4 rc0 = t0 < nrec ? p.recs[roff + t0] : make_uint4(0, 0, 0, 0);
+  const double E0v = (E && t0 < nrec) ? (double)E[p.rec_elem[roff + t0]] : E0;
   stage_v<ET>(L, p.halo + hoff, p.meta + hoff, nacc, nh, nown, X, LIN ? nullptr : U, WK);
   __syncthreads();
   const double mu_f = 1.0 / (2.0 * (1.0 + nu)), lam_f = nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
   double en = 0.0;
   for (int r = threadIdx.x; r < nrec; r += kThreads) {
-    const uint4 rc = p.recs[roff + r];
+    const uint4 rc = r == t0 ? rc0 : p.recs[roff + r];
     const uint64_t lo = (uint64_t)rc.x | ((uint64_t)rc.y << 32);
     const uint64_t hi = (uint64_t)rc.z | ((uint64_t)rc.w << 32);
     int l[4];
@@ -208,7 +215,7 @@ __global__ __launch_bounds__(kThreads) void k_nh_assemble(PlanArgs p, const doub
     }
     Kin k;
     kinematics<LIN>(x, u, k);
-    const double Ee = elem_E(E, E0, p.rec_elem[roff + r]);
+    const double Ee = r == t0 ? E0v : elem_E(E, E0, p.rec_elem[roff + r]);
     const double mu = mu_f * Ee, lam = lam_f * Ee;
     const double s = lam * k.lnJ - mu;
     if (energy) {
