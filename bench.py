@@ -283,8 +283,8 @@ def hyperelastic_leg(n, log_fn):
     stream = torch.cuda.current_stream(dev)
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 
-    def timed(fn, reps=10):
-        for _ in range(2):
+    def timed(fn, reps=20):
+        for _ in range(20):   # clocks ramp up over the first launches
             fn()
         torch.cuda.synchronize()
         ev0.record(stream)
