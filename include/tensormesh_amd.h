@@ -125,16 +125,19 @@ void tm_plan_destroy(tm_plan* plan);
  * ranks); triangles 8 bytes (11-bit words, 6-bit ranks).  An edge with both
  * ends owned is accumulated in the row of the smaller slot; mir[] = src | dst
  * << 16 (block accumulator indices) fills its transposed entry.
- * eblk[n_blocks*4] = {mirror offset, mirror count, e0 | e1 << 16, e2}: the
- * ends of the block's record groups with nb, nb-1 and nb-2 owned vertices.
- * info[0] words per record, [1] mirror entries, [2] max per block, [3] max
- * rank, [4] LDS bytes per block of the edge kernel, [5] node-word bits. */
+ * eblk[n_blocks*4] = {mirror offset, mirror count | e2 << 16, e0 | e1 << 16,
+ * row offset}: e0, e1, e2 end the block's record groups with nb, nb-1 and
+ * nb-2 owned vertices; rows[2 * (row offset + i)] = {first value index,
+ * accumulator byte offset | row length << 16} of the block's i-th owned row
+ * in accumulator order (from the plan's runs).  info[0] words per record,
+ * [1] mirror entries, [2] max per block, [3] max rank, [4] LDS bytes per
+ * block of the edge kernel, [5] node-word bits, [6] owned rows. */
 typedef struct tm_plan_edge tm_plan_edge;
 int tm_plan_edge_build(int nb, int64_t n_blocks, const int32_t* blk, const uint32_t* meta,
-                       const uint32_t* recs, int n_threads, tm_plan_edge** out);
+                       const uint32_t* recs, const int32_t* runs, int n_threads, tm_plan_edge** out);
 int tm_plan_edge_info(const tm_plan_edge* plan, int64_t* info);
-/* recs[n_records * info[0]], eblk[n_blocks * 4], mir[info[1]] */
-int tm_plan_edge_export(const tm_plan_edge* plan, uint32_t* recs, int32_t* eblk, uint32_t* mir);
+/* recs[n_records * info[0]], eblk[n_blocks * 4], mir[info[1]], rows[info[6] * 2] */
+int tm_plan_edge_export(const tm_plan_edge* plan, uint32_t* recs, int32_t* eblk, uint32_t* mir, int32_t* rows);
 void tm_plan_edge_destroy(tm_plan_edge* plan);
 
 /* -------------------------------------------------------------- device --- */
@@ -156,6 +159,7 @@ typedef struct {
   const uint32_t* erecs;
   const int32_t* eblk;
   const uint32_t* emir;
+  const int32_t* erows;
   int32_t erec_words; /* 2 or 3 */
   int32_t elds_bytes; /* tm_plan_edge_info info[4] */
 } tm_plan_dev;

@@ -32,7 +32,7 @@ class PlanDev(ctypes.Structure):
     _fields_ = [("blk", c_void_p), ("halo", c_void_p), ("meta", c_void_p), ("recs", c_void_p),
                 ("rec_elem", c_void_p), ("runs", c_void_p), ("n_blocks", ctypes.c_int32),
                 ("lds_bytes", ctypes.c_int32), ("threads", ctypes.c_int32), ("reserved", ctypes.c_int32),
-                ("erecs", c_void_p), ("eblk", c_void_p), ("emir", c_void_p),
+                ("erecs", c_void_p), ("eblk", c_void_p), ("emir", c_void_p), ("erows", c_void_p),
                 ("erec_words", ctypes.c_int32), ("elds_bytes", ctypes.c_int32)]
 
 
@@ -79,9 +79,9 @@ _SIGS = {
     "tm_plan_info": (c_int, [c_void_p, c_void_p]),
     "tm_plan_export": (c_int, [c_void_p] * 7),
     "tm_plan_destroy": (None, [c_void_p]),
-    "tm_plan_edge_build": (c_int, [c_int, c_i64, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+    "tm_plan_edge_build": (c_int, [c_int, c_i64, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "tm_plan_edge_info": (c_int, [c_void_p, c_void_p]),
-    "tm_plan_edge_export": (c_int, [c_void_p] * 4),
+    "tm_plan_edge_export": (c_int, [c_void_p] * 5),
     "tm_plan_edge_destroy": (None, [c_void_p]),
     "tm_asm_p1_matrix": (c_int, [c_int, c_int, c_int, ctypes.POINTER(PlanDev), c_void_p, c_void_p, c_int,
                                  c_double, ctypes.POINTER(Quad), c_void_p, c_int, c_void_p]),

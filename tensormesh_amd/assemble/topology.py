@@ -78,13 +78,14 @@ class DevicePlan:
         of the symmetric scalar forms then reads them instead of the 16-byte
         records."""
         dev = self.tensors["blk"].device
-        for k in ("erecs", "eblk", "emir"):
+        for k in ("erecs", "eblk", "emir", "erows"):
             self.tensors[k] = torch.from_numpy(host[k]).to(dev)
         self.edge_info = info
         t = self.tensors
         self.struct.erecs = t["erecs"].data_ptr()
         self.struct.eblk = t["eblk"].data_ptr()
         self.struct.emir = t["emir"].data_ptr()
+        self.struct.erows = t["erows"].data_ptr()
         self.struct.erec_words = int(info[0])
         self.struct.elds_bytes = int(info[4])
 
@@ -94,20 +95,22 @@ class DevicePlan:
 
 def edge_plan(nb: int, arrs: Dict[str, np.ndarray], info: np.ndarray):
     """Edge records and mirror lists of an exported scalar plan
-    (``tm_plan_edge_build``): ``({"erecs", "eblk", "emir"}, info)`` with
-    info = (words per record, mirror entries, max per block, max rank, LDS
-    bytes per block, node-word bits, 0, 0)."""
+    (``tm_plan_edge_build``): ``({"erecs", "eblk", "emir", "erows"}, info)``
+    with info = (words per record, mirror entries, max per block, max rank,
+    LDS bytes per block, node-word bits, owned rows, 0)."""
     L = _lib.lib()
     nblk, nrec = int(info[0]), int(info[2])
     h = ctypes.c_void_p()
     _lib.check(L.tm_plan_edge_build(nb, nblk, _lib.ptr(arrs["blk"]), _lib.ptr(arrs["meta"]), _lib.ptr(arrs["recs"]),
-                                    0, ctypes.byref(h)))
+                                    _lib.ptr(arrs["runs"]), 0, ctypes.byref(h)))
     try:
         einfo = np.zeros(8, dtype=np.int64)
         _lib.check(L.tm_plan_edge_info(h, _lib.ptr(einfo)))
         out = dict(erecs=np.empty(nrec * int(einfo[0]), np.uint32), eblk=np.empty(nblk * 4, np.int32),
-                   emir=np.empty(max(1, int(einfo[1])), np.uint32))
-        _lib.check(L.tm_plan_edge_export(h, _lib.ptr(out["erecs"]), _lib.ptr(out["eblk"]), _lib.ptr(out["emir"])))
+                   emir=np.empty(max(1, int(einfo[1])), np.uint32),
+                   erows=np.empty(max(1, 2 * int(einfo[6])), np.int32))
+        _lib.check(L.tm_plan_edge_export(h, _lib.ptr(out["erecs"]), _lib.ptr(out["eblk"]), _lib.ptr(out["emir"]),
+                                         _lib.ptr(out["erows"])))
     finally:
         L.tm_plan_edge_destroy(h)
     return out, einfo

@@ -570,11 +570,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
 // address serves both reads, and a quarter-wave's reads of slots that differ
 // mod 16 stay conflict-free.  The row meta's offset field holds the row's byte
 // offset in the accumulators (accumulators < 8192).  The block's records come
-// grouped by descending owned count; eblk gives the group ends.
+// grouped by descending owned count; eblk gives the group ends.  After the
+// nodes, the block's row table {first value index, byte offset | length << 16}
+// in accumulator order drives the stores (16 lanes per row).
 struct EdgeArgs {
   const uint32_t* recs;
   const int32_t* eblk;
   const uint32_t* mir;
+  const int2* rows;
 };
 
 __device__ __forceinline__ int node_word(int s) { return s + (s & ~15); }
@@ -658,10 +661,12 @@ __device__ __forceinline__ void emit_edges(const Lds& L, const ERec<W>& e, const
 // of its first K halo ids and meta words loaded before any gather; empty
 // slots and lanes past the halo keep node 0's data or stay unwritten.
 template <int NT, int DIM, bool COEF>
-__device__ __forceinline__ void stage_edge(const Lds& L, double2* nodes, const int32_t* __restrict__ halo,
-                                           const uint32_t* __restrict__ meta, int nacc, int nh,
-                                           const double* __restrict__ X, const float* __restrict__ coef) {
+__device__ __forceinline__ void stage_edge(const Lds& L, double2* nodes, int2* rt, const int32_t* __restrict__ halo,
+                                           const uint32_t* __restrict__ meta, const int2* __restrict__ rows,
+                                           int nacc, int nh, int nown, const double* __restrict__ X,
+                                           const float* __restrict__ coef) {
   constexpr int K = (256 + NT - 1) / NT;
+  const int2 rv = (int)threadIdx.x < nown ? rows[threadIdx.x] : make_int2(0, 0);
   int32_t g[K];
   uint32_t mh[K];
 #pragma unroll
@@ -695,6 +700,8 @@ __device__ __forceinline__ void stage_edge(const Lds& L, double2* nodes, const i
       nodes[node_word(h) + 16] = n1[k];
     }
   }
+  if ((int)threadIdx.x < nown) rt[threadIdx.x] = rv;
+  for (int a = threadIdx.x + NT; a < nown; a += NT) rt[a] = rows[a];
   for (int h = threadIdx.x + K * NT; h < nh; h += NT) {
     const int64_t gh = halo[h] >= 0 ? halo[h] : 0;
     const uint32_t m = meta[h];
@@ -819,6 +826,19 @@ __device__ __forceinline__ void rowsum_edge(const Lds& L, const double2* nodes, 
   }
 }
 
+// Row stores of the edge kernel: 16 lanes per row (rows of any length),
+// rows in accumulator order, so a wave's store covers four consecutive rows.
+template <int NT, typename OT>
+__device__ __forceinline__ void store_rows(const Lds& L, const int2* rt, int nown, OT* __restrict__ values) {
+  const int g = threadIdx.x >> 4, j = threadIdx.x & 15;
+  for (int i = g; i < nown; i += NT / 16) {
+    const int2 r = rt[i];
+    const int len = (int)((uint32_t)r.y >> 16);
+    const double* a = acc_at(L, (uint32_t)r.y & 0xffffu);
+    for (int k = j; k < len; k += 16) values[(int64_t)r.x + k] = (OT)a[k];
+  }
+}
+
 // Symmetric scalar forms on edge records: staging, element loop, mirror
 // copies, row sums (Laplace-type forms), row stores.
 template <int NB, int FORM, typename OT, int NT, int W>
@@ -832,19 +852,21 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
   const int4 bi1 = *(const int4*)(p.blk + 8 * b + 4);
   const int4 eb = *(const int4*)(ea.eblk + 4 * b);
   const int hoff = bi0.x, nh = bi0.y, nown = bi0.z, roff = bi0.w;
-  const int nrec = bi1.x, runoff = bi1.y, nrun = bi1.z, nacc = bi1.w;
-  const int moff = eb.x, nmir = eb.y, e0 = eb.z & 0xffff, e1 = (int)((uint32_t)eb.z >> 16), e2 = eb.w;
+  const int nrec = bi1.x, nacc = bi1.w;
+  const int moff = eb.x, nmir = eb.y & 0xffff, e2 = (int)((uint32_t)eb.y >> 16);
+  const int e0 = eb.z & 0xffff, e1 = (int)((uint32_t)eb.z >> 16), rowoff = eb.w;
   Lds L;
   L.acc = (double*)smem;
   double2* nodes = (double2*)(smem + a16(8 * (size_t)nacc));
+  int2* rt = (int2*)((char*)nodes + 512 * (size_t)((nh + 15) / 16));
   const ERecBuf<W> rb(ea.recs, roff, nrec);
   const int t = threadIdx.x;
   ERec<W> nxt = rb[t], nxt2 = rb[t + NT];
-  stage_edge<NT, DIM, FORM == 1>(L, nodes, p.halo + hoff, p.meta + hoff, nacc, nh, X, coef);
+  stage_edge<NT, DIM, FORM == 1>(L, nodes, rt, p.halo + hoff, p.meta + hoff, ea.rows + rowoff, nacc, nh, nown, X,
+                                 coef);
   __syncthreads();
   edge_records<NB, FORM, NT, W, std::is_same<OT, float>::value>(L, nodes, rb, nrec, e0, e1, e2, nxt, nxt2, scale, q);
   const uint32_t mv = t < nmir ? ea.mir[moff + t] : 0u;  // in flight during the barrier
-  const int4 dsc = run_descriptors<NT>(p, runoff, nrun);
   __syncthreads();
   if (t < nmir) L.acc[mv >> 16] = L.acc[mv & 0xffffu];
   for (int i = t + NT; i < nmir; i += NT) {
@@ -856,7 +878,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
     rowsum_edge<NT>(L, nodes, nown);
     __syncthreads();
   }
-  store_runs<NT>(L, p, runoff, nrun, dsc, values);
+  store_rows<NT>(L, rt, nown, values);
 }
 
 // Integrand U_a^T D V_b with U = (phi, grad phi) of the row (u) basis
@@ -1309,7 +1331,7 @@ template <int NB, typename OT, int W>
 int dispatch_matrix_edge(int form, const tm_plan_dev* plan, const double* X, const float* coef, double scale,
                          QuadArg q, OT* values, hipStream_t st) {
   const PlanArgs p = plan_args(plan);
-  const EdgeArgs e{plan->erecs, plan->eblk, plan->emir};
+  const EdgeArgs e{plan->erecs, plan->eblk, plan->emir, (const int2*)plan->erows};
   TM_REQUIRE(plan->elds_bytes > 0 && plan->elds_bytes <= 65536, TM_ERR_ARG, "edge LDS size %d", plan->elds_bytes);
   tm_plan_dev pe = *plan;  // the edge layout's LDS size
   pe.lds_bytes = plan->elds_bytes;
@@ -1325,6 +1347,7 @@ int dispatch_matrix(int form, const tm_plan_dev* plan, const double* X, const vo
                     double scale, QuadArg q, OT* values, hipStream_t st) {
   PlanArgs p = plan_args(plan);
   const bool edge_ok = !DET && plan->erecs != nullptr && plan->eblk != nullptr && plan->emir != nullptr &&
+                       plan->erows != nullptr &&
                        (form == TM_FORM_LAPLACE || form == TM_FORM_MASS ||
                         (form == TM_FORM_KAPPA_LAPLACE && coef != nullptr && cdt == TM_F32));
   if (edge_ok) {

@@ -683,14 +683,17 @@ extern "C" void tm_plan_destroy(tm_plan* p) { delete p; }
 // its two 16-byte words at LDS words w(s) and w(s) + 16, w(s) = s + (s & ~15),
 // i.e. groups of 16 slots hold 16 first words then 16 second words, so a
 // record carries w(s) and both reads share one address.  Per block:
-// eblk = {mirror offset, mirror count, e0 | e1 << 16, e2}, the ends of the
-// record groups with nb, nb-1 and nb-2 owned vertices (the plan sorts a
-// block's records by descending owned count).
+// eblk = {mirror offset, mirror count | e2 << 16, e0 | e1 << 16, row offset}:
+// e0, e1, e2 end the record groups with nb, nb-1 and nb-2 owned vertices (the
+// plan sorts a block's records by descending owned count); the block's owned
+// rows, in accumulator order, are rows[row offset + i] = {first value index,
+// accumulator byte offset | row length << 16}, for stores of 16 lanes per row.
 
 struct tm_plan_edge {
   std::vector<uint32_t> recs;
   std::vector<int32_t> eblk;
   std::vector<uint32_t> mir;
+  std::vector<int32_t> rows;
   int64_t info[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
@@ -710,11 +713,11 @@ inline uint32_t node_word(uint32_t s) { return s + (s & ~15u); }
 }  // namespace
 
 extern "C" int tm_plan_edge_build(int nb, int64_t n_blocks, const int32_t* blk, const uint32_t* meta,
-                                  const uint32_t* recs, int n_threads, tm_plan_edge** out) {
+                                  const uint32_t* recs, const int32_t* runs, int n_threads, tm_plan_edge** out) {
   TM_REQUIRE(out != nullptr, TM_ERR_ARG, "null output");
   *out = nullptr;
   TM_REQUIRE(nb == 3 || nb == 4, TM_ERR_ARG, "edge records cover P1 triangles and tetrahedra (nb=%d)", nb);
-  TM_REQUIRE(n_blocks >= 0 && (n_blocks == 0 || (blk && meta && recs)), TM_ERR_ARG, "null plan array");
+  TM_REQUIRE(n_blocks >= 0 && (n_blocks == 0 || (blk && meta && recs && runs)), TM_ERR_ARG, "null plan array");
   const int npair = nb * (nb - 1) / 2;
   int pi[6], pj[6];
   for (int i = 0, k = 0; i < nb; ++i)
@@ -735,7 +738,8 @@ extern "C" int tm_plan_edge_build(int nb, int64_t n_blocks, const int32_t* blk, 
     TM_REQUIRE(bi[7] < 8192 && bi[4] < 65536, TM_ERR_LIMIT,
                "block %lld: %d accumulators, %d records (edge records take < 8192, < 65536)", (long long)b, bi[7],
                bi[4]);
-    max_lds = std::max<int64_t>(max_lds, ((8 * (int64_t)bi[7] + 15) & ~int64_t(15)) + 512 * (((int64_t)bi[1] + 15) / 16));
+    max_lds = std::max<int64_t>(max_lds, ((8 * (int64_t)bi[7] + 15) & ~int64_t(15)) + 512 * (((int64_t)bi[1] + 15) / 16) +
+                                             8 * (int64_t)bi[2]);
     int prev = nb;
     for (int64_t r = bi[3]; r < (int64_t)bi[3] + bi[4]; ++r) {
       const uint32_t* rc = recs + 4 * r;
@@ -753,6 +757,9 @@ extern "C" int tm_plan_edge_build(int nb, int64_t n_blocks, const int32_t* blk, 
     }
   }
   TM_REQUIRE(max_lds <= 65536, TM_ERR_LIMIT, "edge-record blocks need %lld bytes of LDS", (long long)max_lds);
+  int64_t all_rows = 0;
+  for (int64_t b = 0; b < n_blocks; ++b) all_rows += blk[8 * b + 2];
+  TM_REQUIRE(all_rows < (int64_t(1) << 30), TM_ERR_LIMIT, "too many owned rows");
   const int W = (nb == 4 && (max_rank > 15 || max_nh > 512)) ? 3 : 2;
   const int SB = (nb == 4 && W == 2) ? 10 : 11;
   const int R = (nb == 4 && W == 2) ? 4 : 6;
@@ -773,6 +780,13 @@ extern "C" int tm_plan_edge_build(int nb, int64_t n_blocks, const int32_t* blk, 
   int64_t nrec = n_blocks > 0 ? (int64_t)blk[8 * (n_blocks - 1) + 3] + blk[8 * (n_blocks - 1) + 4] : 0;
   P->recs.assign((size_t)nrec * W, 0u);
   P->eblk.assign((size_t)n_blocks * 4, 0);
+  // owned-row table: blocks' rows in accumulator order (the runs' order)
+  int64_t nrows = 0;
+  for (int64_t b = 0; b < n_blocks; ++b) {
+    P->eblk[(size_t)b * 4 + 3] = (int32_t)nrows;
+    nrows += blk[8 * b + 2];
+  }
+  P->rows.assign((size_t)nrows * 2, 0);
   std::vector<std::vector<uint32_t>> mirs((size_t)n_blocks);
   std::atomic<int64_t> next{0};
   const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(pick_threads(n_threads), std::max<int64_t>(1, n_blocks / 64)));
@@ -826,9 +840,26 @@ extern "C" int tm_plan_edge_build(int nb, int64_t n_blocks, const int32_t* blk, 
           m.reserve(keys.size());
           for (uint64_t k : keys) m.push_back((uint32_t)(k & 0xffffu) | ((uint32_t)(k >> 32) << 16));
           int32_t* eb = P->eblk.data() + 4 * b;
-          eb[1] = (int32_t)keys.size();
+          eb[1] = (int32_t)keys.size() | (ge[2] << 16);
           eb[2] = ge[0] | (ge[1] << 16);
-          eb[3] = ge[2];
+          // rows by accumulator offset: each run's rows are consecutive in both
+          // the accumulators and the values
+          std::vector<std::pair<uint32_t, uint32_t>> ro;  // (acc offset, length) of owned slots
+          for (int a = 0; a < nown; ++a) {
+            const uint32_t m = meta[hoff + a];
+            ro.push_back({m & 0xffffu, m >> 24});
+          }
+          std::sort(ro.begin(), ro.end());
+          int32_t* rw = P->rows.data() + 2 * (size_t)eb[3];
+          const int32_t* ru = runs + 4 * (int64_t)bi[5];
+          int q = 0;
+          for (int i = 0; i < nown; ++i) {
+            const uint32_t off = ro[i].first, len = ro[i].second;
+            while (q < bi[6] && (uint32_t)(ru[4 * q + 1] + ru[4 * q + 2]) <= off && len > 0) ++q;
+            const int32_t vs = (len > 0 && q < bi[6]) ? ru[4 * q] + (int32_t)(off - (uint32_t)ru[4 * q + 1]) : 0;
+            rw[2 * i] = vs;
+            rw[2 * i + 1] = (int32_t)((off * 8) | (len << 16));
+          }
         }
       }
     });
@@ -852,6 +883,7 @@ extern "C" int tm_plan_edge_build(int nb, int64_t n_blocks, const int32_t* blk, 
   P->info[3] = max_rank;
   P->info[4] = max_lds;
   P->info[5] = SB;
+  P->info[6] = nrows;
   *out = P;
   return TM_OK;
 }
@@ -862,11 +894,13 @@ extern "C" int tm_plan_edge_info(const tm_plan_edge* p, int64_t* info) {
   return TM_OK;
 }
 
-extern "C" int tm_plan_edge_export(const tm_plan_edge* p, uint32_t* recs, int32_t* eblk, uint32_t* mir) {
+extern "C" int tm_plan_edge_export(const tm_plan_edge* p, uint32_t* recs, int32_t* eblk, uint32_t* mir,
+                                   int32_t* rows) {
   TM_REQUIRE(p, TM_ERR_ARG, "null edge plan");
   if (recs) std::memcpy(recs, p->recs.data(), p->recs.size() * 4);
   if (eblk) std::memcpy(eblk, p->eblk.data(), p->eblk.size() * 4);
   if (mir) std::memcpy(mir, p->mir.data(), p->mir.size() * 4);
+  if (rows) std::memcpy(rows, p->rows.data(), p->rows.size() * 4);
   return TM_OK;
 }
 

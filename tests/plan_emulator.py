@@ -128,12 +128,13 @@ def emulate_edge_matrix(topo, arrs, earrs, words, element_matrix, rowsum=True):
     fills the transposed entries; ``rowsum`` forms the diagonal as minus the
     row's off-diagonal sum (Laplace-type forms), otherwise the owned diagonal
     entries are added directly (mass).  The owned count of each record must
-    match the block's group ends."""
+    match the block's group ends; rows are stored from the block's row table
+    (16 lanes per row in the kernel), which must cover each value once."""
     nb = topo.nb
     blk = arrs["blk"].reshape(-1, 8)
     erecs = earrs["erecs"].reshape(-1, words)
     eblk = earrs["eblk"].reshape(-1, 4)
-    runs = arrs["runs"].reshape(-1, 4)
+    erows = earrs["erows"]
     mir = earrs["emir"]
     rp, ci = topo.row_ptr, topo.col_idx
     vals = np.zeros(topo.nnz)
@@ -142,7 +143,8 @@ def emulate_edge_matrix(topo, arrs, earrs, words, element_matrix, rowsum=True):
     moff = 0
     for b in range(blk.shape[0]):
         hoff, nh, nown, roff, nrec, runoff, nrun, nacc = (int(v) for v in blk[b])
-        mo, mn, e01, e2 = (int(v) for v in eblk[b])
+        mo, mne2, e01, rowoff = (int(v) for v in eblk[b])
+        mn, e2 = mne2 & 0xFFFF, mne2 >> 16
         assert mo == moff
         moff += mn
         ends = [e01 & 0xFFFF, e01 >> 16, e2]
@@ -182,9 +184,11 @@ def emulate_edge_matrix(topo, arrs, earrs, words, element_matrix, rowsum=True):
                 m = int(meta[a])
                 base, dr, ln = m & 0xFFFF, (m >> 16) & 0xFF, m >> 24
                 acc[base + dr] = -(acc[base:base + ln].sum() - acc[base + dr])
-        for q in range(nrun):
-            vs, as_, ln, _ = (int(v) for v in runs[runoff + q])
-            vals[vs:vs + ln] = acc[as_:as_ + ln]
+        for i in range(nown):
+            vs, ol = int(erows[2 * (rowoff + i)]), int(erows[2 * (rowoff + i) + 1])
+            off, ln = (ol & 0xFFFF) // 8, ol >> 16
+            assert (ol & 0xFFFF) % 8 == 0 and off + ln <= nacc
+            vals[vs:vs + ln] = acc[off:off + ln]
             covered[vs:vs + ln] += 1
     assert (covered == 1).all()
     return vals

@@ -38,12 +38,12 @@ def _without_edge(asm, pd, dev):
     """The same assembler on the 16-byte records (edge fields cleared)."""
     plan = asm.topology.device_plan(dev, edge=True)
     st = plan.struct
-    saved = (st.erecs, st.eblk, st.emir)
-    st.erecs = st.eblk = st.emir = None
+    saved = (st.erecs, st.eblk, st.emir, st.erows)
+    st.erecs = st.eblk = st.emir = st.erows = None
     try:
         return asm(point_data=pd)
     finally:
-        st.erecs, st.eblk, st.emir = saved
+        st.erecs, st.eblk, st.emir, st.erows = saved
 
 
 @pytest.mark.parametrize("kind,words", [("tri", 2), ("tet", 2), ("delaunay", 3)])
