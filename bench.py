@@ -91,10 +91,10 @@ def _cpu_phases(n: int, threads: int, repeats: int, solve: str, pcg_iters: int =
 
 
 def cpu_baseline(n_cpu: int, n_single: int = 48, repeats: int = 3):
-    """The reference pipeline restated in float64 torch on the host CPU
-    (BASELINE plan): all threads of the process's affinity set on the n_cpu^3
-    grid, and one thread on the bounded n_single^3 sample; each phase timed
-    separately."""
+    """The reference pipeline restated in float64 torch on the host CPU: all
+    threads of the process's affinity set on the n_cpu^3 grid, and one thread
+    on the bounded n_single^3 sample; each phase (pattern, assembly,
+    condensation, Jacobi-PCG) timed separately."""
     import torch
     k = len(os.sched_getaffinity(0))
     prev = torch.get_num_threads()
