@@ -723,8 +723,7 @@ __device__ __forceinline__ int rec_group(int r, int e0, int e1, int e2) {
 // Newton steps and the FP64 weighted sum.
 template <int NB, int FORM, int NT, int W, bool LO>
 __device__ __forceinline__ void edge_records(const Lds& L, const double2* nodes, const ERecBuf<W>& rb, int nrec,
-                                             int e0, int e1, int e2, ERec<W> nxt, ERec<W> nxt2, double scale,
-                                             const QuadArg& q) {
+                                             int e0, int e1, int e2, ERec<W> nxt, double scale, const QuadArg& q) {
   constexpr int DIM = NB - 1;
   const double sw = FORM == 0 ? scale * q.wsum : scale * q.W[0];
   const int lane = threadIdx.x % kWave;
@@ -733,8 +732,7 @@ __device__ __forceinline__ void edge_records(const Lds& L, const double2* nodes,
   for (int r0 = w0; r0 < nrec; r0 += NT) {
     const int r = r0 + lane;
     const ERec<W> e = nxt;
-    nxt = nxt2;
-    nxt2 = rb[r + 2 * NT];  // records two iterations ahead (zeros past the block's end)
+    nxt = rb[r + NT];
     if (r >= nrec) continue;
     double x[NB][DIM];
     float cf[NB];
@@ -804,25 +802,29 @@ __device__ __forceinline__ void edge_records(const Lds& L, const double2* nodes,
   }
 }
 
-// Row sums of the edge layout (the diagonal accumulators hold 0): every entry
-// of the row is summed, 16 reads at fixed offsets from the row start (entries
-// past the row end are read and dropped; they lie inside the block's LDS).
+// Row sums of the edge layout (the diagonal accumulators hold 0): four lanes
+// per row (a quad) each sum every fourth entry, then two DPP quad swaps add
+// the four partial sums; the quad's first lane writes minus the sum to the
+// diagonal.  Every lane of a quad takes the same rows, so the swaps always
+// read active lanes.
+template <int CTRL>
+__device__ __forceinline__ double quad_swap(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
 template <int NT>
 __device__ __forceinline__ void rowsum_edge(const Lds& L, const double2* nodes, int nown) {
-  for (int a = threadIdx.x; a < nown; a += NT) {
+  const int q = threadIdx.x & 3;
+  for (int a = threadIdx.x >> 2; a < nown; a += NT / 4) {
     const uint32_t m = (uint32_t)__double2hiint(nodes[node_word(a) + 16].y);
     const int len = meta_len(m);
-    if (len == 0) continue;
     const double* r = acc_at(L, m & 0xffffu);
     double s = 0.0;
-    for (int k0 = 0; k0 < len; k0 += 16) {
-      double v[16];
-#pragma unroll
-      for (int k = 0; k < 16; ++k) v[k] = r[k0 + k];
-#pragma unroll
-      for (int k = 0; k < 16; ++k) s += (k0 + k < len) ? v[k] : 0.0;
-    }
-    *acc_at(L, (m & 0xffffu) + 8 * meta_diag(m)) = -s;
+    for (int k = q; k < len; k += 4) s += r[k];
+    s += quad_swap<0xB1>(s);  // quad_perm [1,0,3,2]: lanes xor 1
+    s += quad_swap<0x4E>(s);  // quad_perm [2,3,0,1]: lanes xor 2
+    if (q == 0 && len > 0) *acc_at(L, (m & 0xffffu) + 8 * meta_diag(m)) = -s;
   }
 }
 
@@ -861,11 +863,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
   int2* rt = (int2*)((char*)nodes + 512 * (size_t)((nh + 15) / 16));
   const ERecBuf<W> rb(ea.recs, roff, nrec);
   const int t = threadIdx.x;
-  ERec<W> nxt = rb[t], nxt2 = rb[t + NT];
+  ERec<W> nxt = rb[t];  // a lane's first record loads before the staging
   stage_edge<NT, DIM, FORM == 1>(L, nodes, rt, p.halo + hoff, p.meta + hoff, ea.rows + rowoff, nacc, nh, nown, X,
                                  coef);
   __syncthreads();
-  edge_records<NB, FORM, NT, W, std::is_same<OT, float>::value>(L, nodes, rb, nrec, e0, e1, e2, nxt, nxt2, scale, q);
+  edge_records<NB, FORM, NT, W, std::is_same<OT, float>::value>(L, nodes, rb, nrec, e0, e1, e2, nxt, scale, q);
   const uint32_t mv = t < nmir ? ea.mir[moff + t] : 0u;  // in flight during the barrier
   __syncthreads();
   if (t < nmir) L.acc[mv >> 16] = L.acc[mv & 0xffffu];
