@@ -60,6 +60,7 @@ class DevicePlan:
     def __init__(self, host: Dict[str, np.ndarray], info: np.ndarray, device: torch.device, threads: int = 512):
         self.tensors = {k: torch.from_numpy(v).to(device) for k, v in host.items()}
         self.info = info
+        self.edge_refused: Optional[str] = None   # why edge records were not built (limits)
         t = self.tensors
         self.struct = _lib.PlanDev(
             t["blk"].data_ptr(), t["halo"].data_ptr(), t["meta"].data_ptr(), t["recs"].data_ptr(),
@@ -279,9 +280,16 @@ class P1Topology:
             arrs, info = self.host_plan(kind, max_rows, lds_budget)
             self._dev_plans[key] = DevicePlan(arrs, info, device, threads)
         plan = self._dev_plans[key]
-        if edge and kind == "scalar" and not plan.has_edge and edge_records_enabled():
+        if edge and kind == "scalar" and not plan.has_edge and edge_records_enabled() and not plan.edge_refused:
             arrs, info = self.host_plan(kind, max_rows, lds_budget)
-            plan.attach_edge(*edge_plan(self.nb, arrs, info))
+            try:
+                plan.attach_edge(*edge_plan(self.nb, arrs, info))
+            except _lib.TMError as err:
+                if err.code != _lib.TM_ERR_LIMIT:
+                    raise
+                # a block outside the edge records' limits (accumulators, records or LDS):
+                # the matrix kernel keeps the 16-byte records
+                plan.edge_refused = str(err)
         return plan
 
     def quad(self, order: int) -> _lib.Quad:

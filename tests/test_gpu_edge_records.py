@@ -84,3 +84,21 @@ def test_deterministic_mode_keeps_full_records(cuda):
     assert torch.equal(K1, K2)
     u, v, _ = op.pattern_and_slots(conn, len(pts))
     assert row_scaled_err(K1.double().cpu().numpy(), K0.double().cpu().numpy(), u, v) <= 2.5e-7
+
+
+def test_plan_beyond_edge_limits_keeps_full_records(cuda, monkeypatch):
+    """One block of 729 rows (more than 8191 accumulators): the edge records
+    are refused, the matrix kernel runs the 16-byte records (102 KiB of LDS)
+    and matches the oracle."""
+    monkeypatch.setenv("TM_ASM_MAX_ROWS", "1000")
+    monkeypatch.setenv("TM_ASM_LDS_BUDGET", str(150 * 1024))
+    pts, conn, bnd = kuhn_cube_arrays(8)
+    m = Mesh(torch.from_numpy(pts), {"tetra": torch.from_numpy(conn)}, {"is_boundary": torch.from_numpy(bnd)})
+    asm = tm.LaplaceElementAssembler.from_mesh(m).to(cuda)
+    asm.value_dtype = torch.float32
+    K = asm()
+    plan = asm.topology.device_plan(cuda, edge=True)
+    assert plan.n_blocks == 1 and not plan.has_edge and plan.edge_refused
+    u, v, pe = op.pattern_and_slots(conn, len(pts))
+    ref = oa.reduce(pe, oa.element_matrices(pts, conn, "tetra", "laplace"), len(u))
+    assert row_scaled_err(K.values.double().cpu().numpy(), ref, u, v) <= 1e-6

@@ -145,6 +145,18 @@ def test_edge_records_emulation_matches_oracle(name, arrays, et, rows):
     assert len(used) == len(pts)
 
 
+def test_edge_records_refused_beyond_limits_fall_back():
+    """A plan whose blocks exceed the edge records' limits (here an LDS budget
+    that lets a block's edge layout pass 64 KiB) keeps the 16-byte records:
+    the builder reports TM_ERR_LIMIT and the device plan records why."""
+    pts, conn, _ = kuhn_cube_arrays(12)
+    topo = P1Topology(pts, conn, len(pts), "tetra")
+    arrs, info = topo.host_plan("scalar", max_rows=100_000, lds_budget=1 << 20)
+    with pytest.raises(_lib.TMError) as ei:
+        edge_plan(topo.nb, arrs, info)
+    assert ei.value.code == _lib.TM_ERR_LIMIT
+
+
 def test_block_plan_respects_lds_budget():
     pts, conn, _ = kuhn_cube_arrays(6)
     topo = P1Topology(pts, conn, len(pts), "tetra")
