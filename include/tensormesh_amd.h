@@ -383,7 +383,10 @@ int tm_asm_linear_elastic(const tm_plan_dev* plan, const double* coords, const v
 
 /* Matrix-free tangent action Y = K(u) lam (f64 [N*3], every entry written) and,
  * when dE != NULL, dE[e] += lam . dR/dE_e (caller zeroes dE): the adjoint of
- * the residual assembly w.r.t. nodal displacements and per-element E. */
+ * the residual assembly w.r.t. nodal displacements and per-element E.  The
+ * kernel keeps no CSR accumulators: it stages 72 bytes per halo node and 28
+ * per owned row, so plan may be built with entry_bytes 0, node_bytes 80,
+ * own_bytes 32 (or be a plan of tm_asm_nh). */
 int tm_nh_apply(const tm_plan_dev* plan, const double* coords, const double* u, const void* E_elem, int E_dtype,
                 double E0, double nu, const double* lam, double* Y, double* dE, void* stream);
 

@@ -117,12 +117,12 @@ class NeoHookeanModel(ElementAssembler):
 
     def _vplan(self, dev, assemble: bool = False):
         """3x3-block plan: the tangent / residual / energy kernel runs fastest on 40-row blocks
-        (four per CU; 64^3 tangent + residual 228.9 vs 272.6 us), the adjoint action on the
-        default 64-row blocks (165 vs 217 us)."""
+        (four per CU; 64^3 tangent + residual 228.9 vs 272.6 us); the matrix-free tangent action
+        keeps no CSR accumulators and takes the accumulator-free plan (64 rows in 20 KiB)."""
         assert self.topology.nb == 4, "neo-Hookean kernels cover P1 tetrahedra"
         if assemble:
             return self.topology.device_plan(dev, "vector3", max_rows=40, lds_budget=40 * 1024 - 64)
-        return self.topology.device_plan(dev, "vector3")
+        return self.topology.device_plan(dev, "vector3a")
 
     def _resolve_E(self, E):
         E = self.E if E is None else E

@@ -24,8 +24,12 @@ from ..element import ELEMENT_DIM, ELEMENT_NB, quadrature_moments
 
 # Plan kinds: (accumulator bytes per CSR entry, LDS bytes per halo node,
 # LDS bytes per owned row).  "scalar" serves the matrix, load and adjoint
-# kernels of scalar problems.
-PLAN_KINDS = {"scalar": (8, 36, 8), "vector3": (72, 56, 32)}
+# kernels of scalar problems; "vector3" the 3x3-block kernels that accumulate
+# CSR blocks (neo-Hookean tangent and residual, linear-elastic stiffness);
+# "vector3a" the matrix-free tangent action, which stages coordinates,
+# displacements and the multiplier per halo node (72 bytes) and three doubles
+# per owned row, and keeps no CSR accumulators.
+PLAN_KINDS = {"scalar": (8, 36, 8), "vector3": (72, 56, 32), "vector3a": (0, 80, 32)}
 
 
 def _default_plan_params(kind: str, dim: int) -> Tuple[int, int, int]:
@@ -39,11 +43,17 @@ def _default_plan_params(kind: str, dim: int) -> Tuple[int, int, int]:
     different phases (gathering their nodes, evaluating elements, writing
     rows) overlap, which outweighs the 9 % more element records.
 
-    3x3 blocks (64^3 neo-Hookean): 64 rows in 64 KiB, the adjoint action's
-    best (165 vs 217 us at 40 rows).  The tangent / residual and the
-    linear-elastic stiffness ask for 40-row plans explicitly (faster there).
+    3x3 blocks (64^3 neo-Hookean): 64 rows in 64 KiB; the tangent / residual
+    and the linear-elastic stiffness ask for 40-row plans explicitly (faster
+    there).  The matrix-free tangent action without CSR accumulators: 64 rows
+    in 20 KiB (129.9 us at 64^3, against 153.9 us on the 64-row 3x3-block plan
+    it used before; 96 / 128 / 192 / 256 rows: 130.9 / 131.3 / 132.3 / 138.2 us:
+    its 178 VGPRs allow two workgroups per CU whatever their LDS, and larger
+    blocks leave fewer of them for the end of the grid).
     """
     threads = os.environ.get("TM_ASM_THREADS")
+    if kind == "vector3a":
+        return (int(os.environ.get("TM_ASM_MAX_ROWS_APPLY", 64)), 20 * 1024, 256)
     if kind == "vector3":
         return (int(os.environ.get("TM_ASM_MAX_ROWS_VEC", 64)),
                 int(os.environ.get("TM_ASM_LDS_BUDGET", 64 * 1024 - 64)), int(threads or 512))

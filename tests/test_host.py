@@ -168,6 +168,24 @@ def test_block_plan_respects_lds_budget():
     assert np.array_equal(np.sort(owned), np.arange(len(pts)))
 
 
+@pytest.mark.parametrize("n,rows", [(3, 64), (8, 64), (8, 256)])
+def test_accumulator_free_plan_covers_the_tangent_action(n, rows):
+    """The matrix-free tangent action (k_nh_apply) carves, per block, 16-byte
+    aligned regions of 24·nh (coordinates), 24·nh (displacements), 24·nown
+    (row sums), 4·nown (row meta) and then 24·nh (multiplier) bytes; the
+    accumulator-free plan it is launched with must cover that in every block,
+    and own every node exactly once."""
+    pts, conn, _ = kuhn_cube_arrays(n)
+    topo = P1Topology(pts, conn, len(pts), "tetra")
+    arrs, info = topo.host_plan("vector3a", max_rows=rows, lds_budget=20 * 1024 if rows == 64 else 64 * 1024 - 64)
+    blk = arrs["blk"].reshape(-1, 8).astype(np.int64)
+    nh, nown = blk[:, 1], blk[:, 2]
+    a16 = lambda x: (x + 15) // 16 * 16
+    need = a16(a16(24 * nh) + a16(24 * nh) + a16(24 * nown) + 4 * nown) + 24 * nh
+    assert (need <= info[4]).all() and (blk[:, 7] >= nown).all()
+    assert nown.sum() == len(pts) and nown.max() <= rows
+
+
 def test_plan_limit_error_is_reported():
     pts, conn, _ = kuhn_cube_arrays(2)
     topo = P1Topology(pts, conn, len(pts), "tetra")
