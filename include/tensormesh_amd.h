@@ -363,7 +363,9 @@ int tm_pcg_jacobi_bsr3(int64_t n_block_rows, const int32_t* row_ptr, const int32
  *   R[N*3]        residual (gradient of the energy w.r.t. u, f64),
  *   *energy       += total strain energy (f64, caller zeroes it).
  * NULL outputs are skipped.  plan must be built with entry_bytes 72,
- * node_bytes 56, own_bytes 32.  The reference computes the residual by
+ * node_bytes 56, own_bytes 32; without kvals the kernel keeps no CSR
+ * accumulators and a plan with entry_bytes 0, node_bytes 80, own_bytes 32
+ * (that of tm_nh_apply) serves as well.  The reference computes the residual by
  * autograd of ElementAssembler.energy (element_assembler.py:468-680) and has
  * no tangent kernel. */
 int tm_asm_nh(const tm_plan_dev* plan, const double* coords, const double* u, const void* E_elem, int E_dtype,

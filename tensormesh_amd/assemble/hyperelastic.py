@@ -116,9 +116,10 @@ class NeoHookeanModel(ElementAssembler):
         return c[1]
 
     def _vplan(self, dev, assemble: bool = False):
-        """3x3-block plan: the tangent / residual / energy kernel runs fastest on 40-row blocks
-        (four per CU; 64^3 tangent + residual 228.9 vs 272.6 us); the matrix-free tangent action
-        keeps no CSR accumulators and takes the accumulator-free plan (64 rows in 20 KiB)."""
+        """``assemble=True``: the 3x3-block plan of the tangent kernel, fastest on 40-row blocks
+        (four per CU; 64^3 tangent + residual 228.9 vs 272.6 us).  Otherwise the accumulator-free
+        plan (64 rows in 20 KiB) of the calls that keep no CSR accumulators: the matrix-free
+        tangent action, and residual or energy evaluations without the tangent."""
         assert self.topology.nb == 4, "neo-Hookean kernels cover P1 tetrahedra"
         if assemble:
             return self.topology.device_plan(dev, "vector3", max_rows=40, lds_budget=40 * 1024 - 64)
@@ -140,7 +141,8 @@ class NeoHookeanModel(ElementAssembler):
         kv = torch.empty(self.topology.nnz * 9, dtype=kdtype, device=dev) if want_k else None
         R = torch.empty((self.n_points, 3), dtype=torch.float64, device=dev) if want_r else None
         en = torch.zeros(1, dtype=torch.float64, device=dev) if want_energy else None
-        plan = self._vplan(dev, assemble=True)
+        # residual / energy without the tangent need no CSR accumulators
+        plan = self._vplan(dev, assemble=want_k)
         _lib.check(_lib.lib().tm_asm_nh(plan.struct, _lib.ptr(self._coords(dev)), _lib.ptr(u64), _lib.ptr(Et), Edt,
                                         E0, self.nu, _lib.ptr(kv), _lib.dtype_code(kv) if kv is not None else 0,
                                         _lib.ptr(R), _lib.ptr(en), _lib.stream_of(dev)))

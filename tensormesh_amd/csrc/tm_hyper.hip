@@ -188,7 +188,9 @@ __global__ __launch_bounds__(kThreads) void k_nh_assemble(PlanArgs p, const doub
   const int4 bi1 = *(const int4*)(p.blk + 8 * b + 4);
   const int hoff = bi0.x, nh = bi0.y, nown = bi0.z, roff = bi0.w;
   const int nrec = bi1.x, runoff = bi1.y, nrun = bi1.z, nacc = bi1.w;
-  const LdsV L = carve_v(smem, nacc, nh, nown);
+  // without the tangent no CSR accumulators are carved, so residual and
+  // energy calls also run on plans built without them
+  const LdsV L = carve_v(smem, WK ? nacc : 0, nh, nown);
   // a lane's first record and its element's modulus do not depend on the
   // staged nodes: their loads are issued first
   const int t0 = threadIdx.x;
