@@ -30,10 +30,12 @@ from .element_assembler import ElementAssembler
 from .hyperelastic import block_matrix
 
 
-# Block size of the stiffness plan: 40 rows in 40 KiB measured fastest for the u = 0 kernel
-# (216.3 us at 64^3 against 236.1 us with the 64-row plan that the neo-Hookean adjoint action
-# prefers; 32 / 48 rows: 224.2 / 222.2 us).  The same plan as the neo-Hookean tangent.
-LIN_PLAN_ROWS = 40
+# Block size of the stiffness plan: leaves of at most 56 rows in 40 KiB (four blocks per CU)
+# measured fastest for the u = 0 kernel at 64^3: 143.1 us against 170.9 / 146.0 / 147.9 /
+# 149.1 / 189.1 us for 40 / 54 / 58 / 60 / 62 rows.  The builder halves a leaf that overflows
+# the LDS budget, so on that mesh 56-row leaves become 28-row blocks, 40-row leaves 20-row
+# blocks and 62-row leaves mostly 15-16-row blocks.  The same plan as the neo-Hookean tangent.
+LIN_PLAN_ROWS = 56
 LIN_PLAN_LDS = 40 * 1024 - 64
 
 

@@ -116,13 +116,16 @@ class NeoHookeanModel(ElementAssembler):
         return c[1]
 
     def _vplan(self, dev, assemble: bool = False):
-        """``assemble=True``: the 3x3-block plan of the tangent kernel, fastest on 40-row blocks
-        (four per CU; 64^3 tangent + residual 228.9 vs 272.6 us).  Otherwise the accumulator-free
+        """``assemble=True``: the 3x3-block plan of the tangent kernel, fastest with bisection
+        leaves of at most 56 rows in 40 KiB (four blocks per CU; 64^3 tangent + residual 172.8 us
+        against 206.0 / 185.9 / 181.8 / 184.0 / 235.0 us for 40 / 54 / 58 / 60 / 62 rows; leaves
+        that overflow the budget are halved, so 56-row leaves give 28-row blocks there).
+        Otherwise the accumulator-free
         plan (64 rows in 20 KiB) of the calls that keep no CSR accumulators: the matrix-free
         tangent action, and residual or energy evaluations without the tangent."""
         assert self.topology.nb == 4, "neo-Hookean kernels cover P1 tetrahedra"
         if assemble:
-            return self.topology.device_plan(dev, "vector3", max_rows=40, lds_budget=40 * 1024 - 64)
+            return self.topology.device_plan(dev, "vector3", max_rows=56, lds_budget=40 * 1024 - 64)
         return self.topology.device_plan(dev, "vector3a")
 
     def _resolve_E(self, E):

@@ -44,8 +44,8 @@ def _default_plan_params(kind: str, dim: int) -> Tuple[int, int, int]:
     rows) overlap, which outweighs the 9 % more element records.
 
     3x3 blocks (64^3 neo-Hookean): 64 rows in 64 KiB; the tangent / residual
-    and the linear-elastic stiffness ask for 40-row plans explicitly (faster
-    there).  The matrix-free tangent action without CSR accumulators: 64 rows
+    and the linear-elastic stiffness ask for plans of at most 56 rows in
+    40 KiB explicitly (faster there).  The matrix-free tangent action without CSR accumulators: 64 rows
     in 20 KiB (129.9 us at 64^3, against 153.9 us on the 64-row 3x3-block plan
     it used before; 96 / 128 / 192 / 256 rows: 130.9 / 131.3 / 132.3 / 138.2 us:
     its 178 VGPRs allow two workgroups per CU whatever their LDS, and larger
