@@ -379,7 +379,8 @@ int tm_asm_nh(const tm_plan_dev* plan, const double* coords, const double* u, co
  * mu = E/(2(1+nu)), lam = E nu/((1+nu)(1-2nu)), E per element (E_elem f32/f64)
  * or the constant E0 (E_elem NULL).  Writes kvals[nnz*9] (f32 or f64) as 3x3
  * row-major blocks on the scalar pattern (from_block_coo order, row = a,
- * the u basis function; col = b).  Same plan as tm_asm_nh. */
+ * the u basis function; col = b).  plan: built like that of tm_asm_nh
+ * (entry_bytes 72, node_bytes 56, own_bytes 32), of any block size. */
 int tm_asm_linear_elastic(const tm_plan_dev* plan, const double* coords, const void* E_elem, int E_dtype, double E0,
                           double nu, void* kvals, int kv_dtype, void* stream);
 

@@ -30,13 +30,15 @@ from .element_assembler import ElementAssembler
 from .hyperelastic import block_matrix
 
 
-# Block size of the stiffness plan: leaves of at most 56 rows in 40 KiB (four blocks per CU)
-# measured fastest for the u = 0 kernel at 64^3: 143.1 us against 170.9 / 146.0 / 147.9 /
-# 149.1 / 189.1 us for 40 / 54 / 58 / 60 / 62 rows.  The builder halves a leaf that overflows
-# the LDS budget, so on that mesh 56-row leaves become 28-row blocks, 40-row leaves 20-row
-# blocks and 62-row leaves mostly 15-16-row blocks.  The same plan as the neo-Hookean tangent.
-LIN_PLAN_ROWS = 56
-LIN_PLAN_LDS = 40 * 1024 - 64
+# Block size of the stiffness plan: 22-row blocks in 32 KiB, five per CU (the 83-VGPR kernel
+# allows five waves per SIMD), measured fastest for the u = 0 kernel at 64^3: 143.6 us against
+# 150.4 us on the neo-Hookean tangent's plan (56-row leaves in 40 KiB, four blocks per CU; the
+# tangent's 126 VGPRs allow only four), 153.9 / 156.2 / 164.2 us for 46 / 44 / 48-row caps in
+# 32 KiB.  The builder halves a leaf that overflows the LDS budget, so a cap just above what
+# fits gives blocks of half the size (on that mesh 40-row leaves in 40 KiB become 20-row
+# blocks: 170.9 us).
+LIN_PLAN_ROWS = 22
+LIN_PLAN_LDS = 32 * 1024 - 64
 
 
 def lame(E, nu):
