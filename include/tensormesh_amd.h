@@ -153,7 +153,10 @@ typedef struct {
   int32_t n_blocks;
   int32_t lds_bytes;
   int32_t threads; /* workgroup size of the matrix kernel: 256, 512 or 1024 (0 = 256) */
-  int32_t reserved;
+  int32_t entry_bytes; /* tm_plan_build entry_bytes the plan was built with, -1 for a plan
+                        * built with 0 (no CSR accumulators), 0 = not stated: the 3x3-block
+                        * tangent and stiffness calls refuse a stated value below 72, whose
+                        * blocks have no room for their accumulators */
   /* optional edge records of the same plan (tm_plan_edge_export; NULL: none):
    * tm_asm_p1_matrix uses them for its symmetric forms unless deterministic */
   const uint32_t* erecs;

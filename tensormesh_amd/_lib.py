@@ -31,7 +31,7 @@ c_double = ctypes.c_double
 class PlanDev(ctypes.Structure):
     _fields_ = [("blk", c_void_p), ("halo", c_void_p), ("meta", c_void_p), ("recs", c_void_p),
                 ("rec_elem", c_void_p), ("runs", c_void_p), ("n_blocks", ctypes.c_int32),
-                ("lds_bytes", ctypes.c_int32), ("threads", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("lds_bytes", ctypes.c_int32), ("threads", ctypes.c_int32), ("entry_bytes", ctypes.c_int32),
                 ("erecs", c_void_p), ("eblk", c_void_p), ("emir", c_void_p), ("erows", c_void_p),
                 ("erec_words", ctypes.c_int32), ("elds_bytes", ctypes.c_int32)]
 

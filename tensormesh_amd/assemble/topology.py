@@ -67,14 +67,15 @@ def _default_plan_params(kind: str, dim: int) -> Tuple[int, int, int]:
 class DevicePlan:
     """Device copy of an exported block plan (keeps the tensors alive)."""
 
-    def __init__(self, host: Dict[str, np.ndarray], info: np.ndarray, device: torch.device, threads: int = 512):
+    def __init__(self, host: Dict[str, np.ndarray], info: np.ndarray, device: torch.device, threads: int = 512,
+                 entry_bytes: int = 0):
         self.tensors = {k: torch.from_numpy(v).to(device) for k, v in host.items()}
         self.info = info
         self.edge_refused: Optional[str] = None   # why edge records were not built (limits)
         t = self.tensors
         self.struct = _lib.PlanDev(
             t["blk"].data_ptr(), t["halo"].data_ptr(), t["meta"].data_ptr(), t["recs"].data_ptr(),
-            t["rec_elem"].data_ptr(), t["runs"].data_ptr(), int(info[0]), int(info[4]), int(threads), 0)
+            t["rec_elem"].data_ptr(), t["runs"].data_ptr(), int(info[0]), int(info[4]), int(threads), int(entry_bytes))
 
     @property
     def n_blocks(self) -> int:
@@ -288,7 +289,9 @@ class P1Topology:
         key = (device, kind, int(max_rows or rows0), int(lds_budget or budget0), threads)
         if key not in self._dev_plans:
             arrs, info = self.host_plan(kind, max_rows, lds_budget)
-            self._dev_plans[key] = DevicePlan(arrs, info, device, threads)
+            # the plan states its accumulator bytes per entry (-1: none), which the 3x3-block
+            # tangent and stiffness calls check
+            self._dev_plans[key] = DevicePlan(arrs, info, device, threads, PLAN_KINDS[kind][0] or -1)
         plan = self._dev_plans[key]
         if edge and kind == "scalar" and not plan.has_edge and edge_records_enabled() and not plan.edge_refused:
             arrs, info = self.host_plan(kind, max_rows, lds_budget)

@@ -434,6 +434,8 @@ extern "C" int tm_asm_nh(const tm_plan_dev* plan, const double* coords, const do
                          void* stream) {
   TM_REQUIRE(plan && coords && u, TM_ERR_ARG, "null argument");
   TM_REQUIRE(nu > -1.0 && nu < 0.5, TM_ERR_ARG, "Poisson ratio %g outside (-1, 0.5)", nu);
+  TM_REQUIRE(!kvals || plan->entry_bytes == 0 || plan->entry_bytes >= 72, TM_ERR_ARG,
+             "the tangent needs a plan with 72-byte accumulators (this one has %d)", plan->entry_bytes);
   hipStream_t st = (hipStream_t)stream;
   if (E_elem == nullptr || E_dtype == TM_F64) {
     const double* E = (const double*)E_elem;
@@ -450,6 +452,8 @@ extern "C" int tm_asm_linear_elastic(const tm_plan_dev* plan, const double* coor
   TM_REQUIRE(plan && coords && kvals, TM_ERR_ARG, "null argument");
   TM_REQUIRE(nu > -1.0 && nu < 0.5, TM_ERR_ARG, "Poisson ratio %g outside (-1, 0.5)", nu);
   TM_REQUIRE(kv_dtype == TM_F32 || kv_dtype == TM_F64, TM_ERR_ARG, "stiffness values must be f32 or f64");
+  TM_REQUIRE(plan->entry_bytes == 0 || plan->entry_bytes >= 72, TM_ERR_ARG,
+             "the stiffness needs a plan with 72-byte accumulators (this one has %d)", plan->entry_bytes);
   hipStream_t st = (hipStream_t)stream;
   const PlanArgs p = plan_args(plan);
   if (E_elem == nullptr || E_dtype == TM_F64) {

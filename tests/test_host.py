@@ -217,6 +217,26 @@ def test_assembly_rejects_label_dependent_quadrature():
     assert rc == 0
 
 
+def test_tangent_entry_points_refuse_accumulator_free_plans():
+    """A 3x3-block plan states its accumulator bytes per CSR entry: the tangent
+    and stiffness calls refuse one without room for 72-byte accumulators (it
+    would overrun the blocks' LDS), while residual-only calls accept it; the
+    checks run on the host, and an empty plan launches nothing."""
+    L = _lib.lib()
+    fake = ctypes.c_void_p(64)
+    plan = _lib.PlanDev()
+    for eb, ok in ((0, True), (72, True), (36, False), (-1, False), (0, True)):
+        plan.entry_bytes = eb
+        rc = L.tm_asm_nh(ctypes.byref(plan), fake, fake, None, _lib.F64, 1.0, 0.3, fake, _lib.F32, fake, None, None)
+        assert (rc == 0) == ok and (ok or b"72-byte" in L.tm_last_error())
+        rc = L.tm_asm_linear_elastic(ctypes.byref(plan), fake, None, _lib.F64, 1.0, 0.3, fake, _lib.F32, None)
+        assert (rc == 0) == ok
+        # residual without the tangent: any plan
+        assert L.tm_asm_nh(ctypes.byref(plan), fake, fake, None, _lib.F64, 1.0, 0.3, None, 0, fake, None, None) == 0
+    from tensormesh_amd.assemble.topology import PLAN_KINDS
+    assert PLAN_KINDS["vector3"][0] == 72 and PLAN_KINDS["vector3a"][0] == 0   # stated as 72 and -1
+
+
 @pytest.mark.parametrize("name,arrays,et", list(_meshes()))
 def test_deterministic_mode_is_order_independent(name, arrays, et):
     """The deterministic mode's arithmetic on the CPU: with entries rounded to
