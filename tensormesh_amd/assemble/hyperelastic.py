@@ -121,7 +121,7 @@ class NeoHookeanModel(ElementAssembler):
         against 206.0 / 185.9 / 181.8 / 184.0 / 235.0 us for 40 / 54 / 58 / 60 / 62 rows; leaves
         that overflow the budget are halved, so 56-row leaves give 28-row blocks there).
         Otherwise the accumulator-free
-        plan (64 rows in 20 KiB) of the calls that keep no CSR accumulators: the matrix-free
+        plan (80 rows in 24 KiB) of the calls that keep no CSR accumulators: the matrix-free
         tangent action, and residual or energy evaluations without the tangent."""
         assert self.topology.nb == 4, "neo-Hookean kernels cover P1 tetrahedra"
         if assemble:

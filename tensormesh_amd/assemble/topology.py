@@ -45,15 +45,16 @@ def _default_plan_params(kind: str, dim: int) -> Tuple[int, int, int]:
 
     3x3 blocks (64^3 neo-Hookean): 64 rows in 64 KiB; the tangent / residual
     and the linear-elastic stiffness ask for plans of at most 56 rows in
-    40 KiB explicitly (faster there).  The matrix-free tangent action without CSR accumulators: 64 rows
-    in 20 KiB (129.9 us at 64^3, against 153.9 us on the 64-row 3x3-block plan
-    it used before; 96 / 128 / 192 / 256 rows: 130.9 / 131.3 / 132.3 / 138.2 us:
-    its 178 VGPRs allow two workgroups per CU whatever their LDS, and larger
-    blocks leave fewer of them for the end of the grid).
+    40 KiB explicitly (faster there).  The matrix-free tangent action without
+    CSR accumulators: 80 rows in 24 KiB (128.5 us at 64^3; 40 / 48 / 56 / 64
+    rows in 14-20 KiB: 138.2 / 135.0 / 135.1 / 130.8 us; 96 / 128 / 192 / 256
+    rows: 130.9 / 131.3 / 132.3 / 138.2 us; 153.9 us on the 64-row 3x3-block
+    plan it used before): its 178 VGPRs allow two workgroups per CU whatever
+    their LDS, and larger blocks leave fewer of them for the end of the grid.
     """
     threads = os.environ.get("TM_ASM_THREADS")
     if kind == "vector3a":
-        return (int(os.environ.get("TM_ASM_MAX_ROWS_APPLY", 64)), 20 * 1024, 256)
+        return (int(os.environ.get("TM_ASM_MAX_ROWS_APPLY", 80)), 24 * 1024, 256)
     if kind == "vector3":
         return (int(os.environ.get("TM_ASM_MAX_ROWS_VEC", 64)),
                 int(os.environ.get("TM_ASM_LDS_BUDGET", 64 * 1024 - 64)), int(threads or 512))

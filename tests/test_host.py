@@ -168,7 +168,7 @@ def test_block_plan_respects_lds_budget():
     assert np.array_equal(np.sort(owned), np.arange(len(pts)))
 
 
-@pytest.mark.parametrize("n,rows", [(3, 64), (8, 64), (8, 256)])
+@pytest.mark.parametrize("n,rows", [(3, 80), (8, 64), (8, 80), (8, 256)])
 def test_accumulator_free_plan_covers_the_tangent_action(n, rows):
     """The matrix-free tangent action (k_nh_apply) carves, per block, 16-byte
     aligned regions of 24·nh (coordinates), 24·nh (displacements), 24·nown
@@ -177,7 +177,8 @@ def test_accumulator_free_plan_covers_the_tangent_action(n, rows):
     and own every node exactly once."""
     pts, conn, _ = kuhn_cube_arrays(n)
     topo = P1Topology(pts, conn, len(pts), "tetra")
-    arrs, info = topo.host_plan("vector3a", max_rows=rows, lds_budget=20 * 1024 if rows == 64 else 64 * 1024 - 64)
+    budget = {64: 20 * 1024, 80: 24 * 1024}.get(rows, 64 * 1024 - 64)
+    arrs, info = topo.host_plan("vector3a", max_rows=rows, lds_budget=budget)
     blk = arrs["blk"].reshape(-1, 8).astype(np.int64)
     nh, nown = blk[:, 1], blk[:, 2]
     a16 = lambda x: (x + 15) // 16 * 16
